@@ -1,0 +1,100 @@
+/* zrk_device.h - device-level C ABI of the batched reactor-chemistry path.
+ *
+ * The plugin API (zerork_cfd_plugin.h) takes host arrays; these entry points
+ * take device pointers that are already resident in HBM and a HIP stream, so
+ * a caller (or a test/benchmark) can run each stage of the path on its own.
+ * All arrays are FP64; T [K], P [Pa], Y mass fractions reactor-major
+ * Y[r*stride + j].  Return value: 0 on success, a negative code on invalid
+ * arguments, a positive hipError_t value on a HIP failure.
+ *
+ * Entry point            replaces (reference file:line)
+ *   zrk_mech_load        zerork::mechanism ctor (src/zerork/mechanism.cpp:17-66)
+ *                        + mechanism_cuda device mirror (mechanism_cuda.cu:14-144)
+ *   zrk_mech_info,       mechanism::getNumSpecies/Reactions/Steps, getMolWtSpc,
+ *   zrk_mech_species,    getSpeciesName, step participant maps
+ *   zrk_mech_molwt,      (src/zerork/mechanism.h:33-56)
+ *   zrk_mech_steps
+ *   zrk_rates_device     mechanism_cuda::getReactionRatesLimiter_CUDA_mr_dev
+ *                        (src/zerork/mechanism_cuda.cu:158-165) together with
+ *                        ReactorConstantPressureGPU::GetTimeDerivative
+ *                        (applications/cfd_plugin/reactor_constant_pressure_gpu.cu:181-244)
+ *   zrk_jacobian_device  ReactorNVectorSerialCuda analytic Jacobian
+ *                        (reactor_nvector_serial_cuda.cu:1455-1612; CPU twin
+ *                        reactor_nvector_serial.cpp:610-771)
+ *   zrk_integrate_device CvodeSolver::Integrate over a reactor batch
+ *                        (applications/cfd_plugin/solver_cvode.cpp:51-299,
+ *                        zerork_reactor_manager.cpp:909-1053), one BDF per reactor
+ */
+#ifndef ZRK_DEVICE_H
+#define ZRK_DEVICE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZRK_EXPORT __attribute__((visibility("default")))
+
+typedef struct zrk_mech_s* zrk_mech;
+
+/* Integrator options; names follow the plugin options of the same meaning
+ * (zerork_reactor_manager.cpp:50-101). */
+typedef struct {
+  double rel_tol;
+  double abs_tol;
+  double max_dt;
+  double nonlinear_convergence_coeff;
+  double step_limiter;
+  double reference_temperature;
+  double min_mass_fraction;
+  double cvode_retry_absolute_tolerance_adjustment;
+  double cvode_retry_relative_tolerance_adjustment;
+  int max_steps;
+  int cvode_num_retries;
+} zrk_bdf_options;
+
+ZRK_EXPORT void zrk_bdf_default_options(zrk_bdf_options* o);
+
+ZRK_EXPORT int zrk_mech_load(const char* mech_file, const char* therm_file, zrk_mech* out,
+                             char* err, int errlen);
+ZRK_EXPORT void zrk_mech_free(zrk_mech m);
+ZRK_EXPORT int zrk_mech_info(zrk_mech m, int* nsp, int* nrxn, int* nstep);
+ZRK_EXPORT int zrk_mech_species(zrk_mech m, int j, char* name, int len);
+ZRK_EXPORT int zrk_mech_molwt(zrk_mech m, double* mw);
+/* step_rxn/step_dir [nstep]; reac_ptr/prod_ptr [nstep+1]; reac_sp/prod_sp sized
+ * by the last pointer entries (pass NULL to query sizes via the ptr arrays). */
+ZRK_EXPORT int zrk_mech_steps(zrk_mech m, int* step_rxn, int* step_dir, int* reac_ptr,
+                              int* reac_sp, int* prod_ptr, int* prod_sp);
+
+/* Rates of progress (limiter step_limiter, as perf_net.cpp:190-275), species
+ * creation/destruction/net rates and the constant-pressure derivative
+ * ydot[r*(nsp+1) + k] ([dY_0/dt .. dY_{nsp-1}/dt, dT/dt]).  Any output may be
+ * NULL. */
+ZRK_EXPORT int zrk_rates_device(zrk_mech m, int n, const double* T, const double* P,
+                                const double* Y, int stride, double step_limiter, double* ydot,
+                                double* rop, double* create, double* destroy, double* net,
+                                void* stream);
+
+/* Reference analytic Jacobian J[r][row][col] (row-major, N = nsp+1, temperature
+ * last) at the states.  Requires nsp <= 63. */
+ZRK_EXPORT int zrk_jacobian_device(zrk_mech m, const zrk_bdf_options* o, int n, const double* T,
+                                   const double* P, const double* Y, int stride, double* J,
+                                   void* stream);
+
+/* Integrate n constant-pressure reactors over [0, dt]; T and Y updated in
+ * place.  nsteps[r] (optional) = CVode one-step calls, negative on failure
+ * (the plugin's COST output); stats[r*8] (optional) = nst nfe nje nsetups nni
+ * ncfn netf status.  Requires nsp <= 63.  Returns 0 or an error code; the
+ * number of failed reactors is available from nsteps. */
+ZRK_EXPORT int zrk_integrate_device(zrk_mech m, const zrk_bdf_options* o, int n, double dt,
+                                    double* T, const double* P, double* Y, int stride,
+                                    int* nsteps, int* stats, void* stream);
+
+/* Number of wavefronts the persistent integrator keeps resident (for
+ * roofline bookkeeping). */
+ZRK_EXPORT int zrk_integrator_resident_waves(zrk_mech m);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

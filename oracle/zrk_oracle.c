@@ -1,0 +1,1139 @@
+/* CPU FP64 oracle: plain-C restatement of the reference cfd_plugin path.
+ * TEST INFRASTRUCTURE ONLY (checker + CPU baseline); see zrk_oracle.h. */
+#include "zrk_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NIST_RU 8.314462618e3 /* src/zerork/constants.h:10 */
+#define P_ATM 1.01325e5       /* src/zerork/constants.h:12 */
+#define TLIMIT 1.0e4          /* reactor_constant_pressure_cpu.cpp:88 */
+
+void zrko_default_opts(zrko_opts* o) {
+  /* zerork_reactor_manager.cpp:50-101 defaults, BDF dense direct */
+  o->rtol = 1.0e-8;
+  o->atol = 1.0e-20;
+  o->max_dt = 0.05;
+  o->nlscoef = 0.05;
+  o->max_steps = 5000;
+  o->num_retries = 5;
+  o->retry_atol_adj = 0.1;
+  o->retry_rtol_adj = 1.0;
+  o->min_mass_fraction = 1.0e-30;
+  o->step_limiter = 1.0e22;
+  o->Tref = 1.0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* NASA-7 thermo: src/zerork/nasa_poly.cpp:50-172                              */
+/* ------------------------------------------------------------------------- */
+static void thermo_cp(const zrko_mech* m, double T, double* cp) {
+  for (int j = 0; j < m->nsp; ++j) {
+    const double* c = m->thermo + 16 * j;
+    const double* a = (T < c[0]) ? c + 1 : c + 8;
+    cp[j] = a[0] + T * (a[1] + T * (a[2] + T * (a[3] + T * a[4])));
+  }
+}
+
+static void thermo_h(const zrko_mech* m, double T, double* h) {
+  const double invT = 1.0 / T;
+  double hm[4];
+  hm[0] = T;
+  hm[1] = T * hm[0];
+  hm[2] = T * hm[1];
+  hm[3] = T * hm[2];
+  hm[0] *= 0.50000000000000000000;
+  hm[1] *= 0.33333333333333333333;
+  hm[2] *= 0.25000000000000000000;
+  hm[3] *= 0.20000000000000000000;
+  for (int j = 0; j < m->nsp; ++j) {
+    const double* c = m->thermo + 16 * j;
+    const double* a = (T < c[0]) ? c + 1 : c + 8;
+    h[j] = a[0] + a[1] * hm[0] + a[2] * hm[1] + a[3] * hm[2] + a[4] * hm[3] + a[5] * invT;
+  }
+}
+
+static void thermo_g(const zrko_mech* m, double T, double* g) {
+  const double invT = 1.0 / T;
+  double gm[5];
+  gm[0] = 1.0 - log(T);
+  gm[1] = T;
+  gm[2] = T * gm[1];
+  gm[3] = T * gm[2];
+  gm[4] = T * gm[3];
+  gm[1] *= -0.50000000000000000000;
+  gm[2] *= -0.16666666666666666667;
+  gm[3] *= -0.08333333333333333333;
+  gm[4] *= -0.05000000000000000000;
+  for (int j = 0; j < m->nsp; ++j) {
+    const double* c = m->thermo + 16 * j;
+    const double* a = (T < c[0]) ? c + 1 : c + 8;
+    g[j] = a[0] * gm[0] + a[1] * gm[1] + a[2] * gm[2] + a[3] * gm[3] + a[4] * gm[4] +
+           a[5] * invT - a[6];
+  }
+}
+
+void zrko_thermo(const zrko_mech* m, double T, double* cp_R, double* h_RT, double* g_RT) {
+  if (cp_R) thermo_cp(m, T, cp_R);
+  if (h_RT) thermo_h(m, T, h_RT);
+  if (g_RT) thermo_g(m, T, g_RT);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Rate constants: rate_const::updateK (rate_const.cpp:539-582) and its parts  */
+/* ------------------------------------------------------------------------- */
+static void rate_constants_ws(const zrko_mech* m, double T, const double* C, double* K,
+                              double* G) {
+  const int nstep = m->nstep;
+  for (int s = 0; s < nstep; ++s) K[s] = 0.0;
+  double Csum = 0.0;
+  for (int j = 0; j < m->nsp; ++j) Csum += C[j];
+  const double logT = log(T);
+  const double invT = 1.0 / T;
+  const double log_patm_rut = log(P_ATM / (NIST_RU * T));
+  /* Arrhenius (rate_const.cpp:669-687) */
+  for (int i = 0; i < m->n_arr; ++i)
+    K[m->arr_step[i]] = exp(m->arr_lnA[i] + m->arr_beta[i] * logT - m->arr_Ta[i] * invT);
+  /* reverse from equilibrium (rate_const.cpp:690-731) */
+  if (m->n_keq > 0) {
+    thermo_g(m, T, G);
+    for (int i = 0; i < m->n_keq; ++i) {
+      const int f = m->keq_fwd[i];
+      double sum = 0.0;
+      for (int k = m->prod_ptr[f]; k < m->prod_ptr[f + 1]; ++k) sum += G[m->prod_sp[k]];
+      for (int k = m->reac_ptr[f]; k < m->reac_ptr[f + 1]; ++k) sum -= G[m->reac_sp[k]];
+      const double arg = sum - m->keq_ndelta[i] * log_patm_rut;
+      K[m->keq_step[i]] = exp(arg) * K[f];
+    }
+  }
+  /* third body (rate_const.cpp:862-888) */
+  for (int i = 0; i < m->n_tb; ++i) {
+    double cm = Csum;
+    for (int k = m->tb_ptr[i]; k < m->tb_ptr[i + 1]; ++k) cm += C[m->tb_sp[k]] * m->tb_eff[k];
+    K[m->tb_fwd[i]] *= cm;
+    if (m->tb_rev[i] >= 0) K[m->tb_rev[i]] *= cm;
+  }
+  /* falloff (rate_const.cpp:890-1012; Lindemann and Troe) */
+  for (int i = 0; i < m->n_fo; ++i) {
+    const double* p = m->fo_param + 7 * i;
+    const double klow = exp(p[0] + p[1] * logT - p[2] * invT);
+    double cm;
+    if (m->fo_spc[i] >= 0) {
+      cm = C[m->fo_spc[i]];
+    } else {
+      cm = Csum;
+      for (int k = m->fo_ptr[i]; k < m->fo_ptr[i + 1]; ++k) cm += C[m->fo_sp[k]] * m->fo_eff[k];
+    }
+    double Pr = klow * cm / K[m->fo_fwd[i]];
+    if (Pr < 1.0e-300) Pr = 1.0e-300;
+    double lpr = log10(Pr);
+    double f = 1.0;
+    if (m->fo_type[i] == 1 || m->fo_type[i] == 2) {
+      double fc = 0.0;
+      if (p[4] != 0) fc += (1.0 - p[3]) * exp(-T / p[4]);
+      if (p[5] != 0) fc += p[3] * exp(-T / p[5]);
+      if (m->fo_type[i] == 2) fc += exp(-p[6] * invT);
+      if (fc < 1.0e-300) fc = 1.0e-300;
+      f = log10(fc);
+      const double n = 0.75 - 1.27 * f;
+      lpr -= (0.4 + 0.67 * f);
+      lpr = lpr / (n - 0.14 * lpr);
+      lpr *= lpr;
+      f /= (1.0 + lpr);
+      f = pow(10.0, f);
+    }
+    const double pc = f * Pr / (1.0 + Pr);
+    K[m->fo_fwd[i]] *= pc;
+    if (m->fo_rev[i] >= 0) K[m->fo_rev[i]] *= pc;
+  }
+}
+
+void zrko_rate_constants(const zrko_mech* m, double T, const double* C, double* K) {
+  double* G = (double*)malloc(sizeof(double) * (size_t)m->nsp);
+  rate_constants_ws(m, T, C, K, G);
+  free(G);
+}
+
+/* perf_net::calcRatesFromTC_StepLimiter (perf_net.cpp:190-275) */
+static void rates_ws(const zrko_mech* m, double T, const double* C, double L, double* rop,
+                     double* create, double* destroy, double* net, double* G) {
+  rate_constants_ws(m, T, C, rop, G);
+  if (0.0 < L && L < 1.0e300)
+    for (int s = 0; s < m->nstep; ++s) rop[s] *= L / (L + rop[s]);
+  for (int s = 0; s < m->nstep; ++s)
+    for (int k = m->reac_ptr[s]; k < m->reac_ptr[s + 1]; ++k) rop[s] *= C[m->reac_sp[k]];
+  memset(create, 0, sizeof(double) * (size_t)m->nsp);
+  memset(destroy, 0, sizeof(double) * (size_t)m->nsp);
+  for (int s = 0; s < m->nstep; ++s)
+    for (int k = m->reac_ptr[s]; k < m->reac_ptr[s + 1]; ++k) destroy[m->reac_sp[k]] += rop[s];
+  for (int s = 0; s < m->nstep; ++s)
+    for (int k = m->prod_ptr[s]; k < m->prod_ptr[s + 1]; ++k) create[m->prod_sp[k]] += rop[s];
+  for (int j = 0; j < m->nsp; ++j) net[j] = create[j] - destroy[j];
+}
+
+void zrko_rates(const zrko_mech* m, double T, const double* C, double L, double* rop,
+                double* create, double* destroy, double* net) {
+  double* G = (double*)malloc(sizeof(double) * (size_t)m->nsp);
+  rates_ws(m, T, C, L, rop, create, destroy, net, G);
+  free(G);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Constant-pressure reactor (reactor_constant_pressure_cpu.cpp)               */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  const zrko_mech* m;
+  const zrko_opts* o;
+  double P;
+  double inv_density; /* carried between calls like the reference member */
+  double cp_mix;
+  double *C, *h, *cpm, *G, *rop, *cr, *de, *net, *tmp1, *tmp2, *tmp3;
+} reactor_t;
+
+static void reactor_init(reactor_t* r, const zrko_mech* m, const zrko_opts* o, double P) {
+  const int nsp = m->nsp, N = nsp + 1;
+  r->m = m;
+  r->o = o;
+  r->P = P;
+  r->inv_density = 0.0;
+  r->cp_mix = 0.0;
+  double* w = (double*)calloc((size_t)(8 * nsp + m->nstep + 3 * N), sizeof(double));
+  r->C = w;
+  r->h = w + nsp;
+  r->cpm = w + 2 * nsp;
+  r->G = w + 3 * nsp;
+  r->cr = w + 4 * nsp;
+  r->de = w + 5 * nsp;
+  r->net = w + 6 * nsp;
+  /* 7*nsp: spare */
+  r->rop = w + 8 * nsp;
+  r->tmp1 = r->rop + m->nstep;
+  r->tmp2 = r->tmp1 + N;
+  r->tmp3 = r->tmp2 + N;
+}
+
+static void reactor_free(reactor_t* r) { free(r->C); }
+
+/* GetTimeDerivative (reactor_constant_pressure_cpu.cpp:68-133), no sources. */
+static int reactor_rhs(reactor_t* r, double t, const double* y, double* ydot) {
+  (void)t;
+  const zrko_mech* m = r->m;
+  const int nsp = m->nsp;
+  double T = y[nsp] * r->o->Tref;
+  if (T <= 0.0) return 1;
+  if (T > TLIMIT) T = TLIMIT;
+  double mws = 0.0;
+  for (int j = 0; j < nsp; ++j) mws += y[j] * (1.0 / m->mw[j]);
+  const double wmix = 1.0 / mws;
+  r->inv_density = 1.0 / (r->P * wmix / (NIST_RU * T));
+  thermo_h(m, T, r->h);
+  thermo_cp(m, T, r->cpm);
+  double cp = 0.0;
+  for (int j = 0; j < nsp; ++j) {
+    r->cpm[j] *= NIST_RU / m->mw[j];
+    cp += r->cpm[j] * y[j];
+  }
+  r->cp_mix = cp;
+  const double dens = 1.0 / r->inv_density;
+  for (int j = 0; j < nsp; ++j) r->C[j] = dens * (1.0 / m->mw[j]) * y[j];
+  rates_ws(m, T, r->C, r->o->step_limiter, r->rop, r->cr, r->de, r->net, r->G);
+  double esum = 0.0;
+  for (int j = 0; j < nsp; ++j) {
+    ydot[j] = (r->net[j] * m->mw[j]) * r->inv_density;
+    esum += r->h[j] * r->net[j] * r->inv_density;
+  }
+  const double dTdt = -esum * NIST_RU * y[nsp] / cp;
+  ydot[nsp] = dTdt + 0.0 / (cp * r->o->Tref);
+  return 0;
+}
+
+int zrko_rhs(const zrko_mech* m, const zrko_opts* o, double P, double t, const double* y,
+             double* ydot) {
+  reactor_t r;
+  reactor_init(&r, m, o, P);
+  int flag = reactor_rhs(&r, t, y, ydot);
+  reactor_free(&r);
+  return flag;
+}
+
+/* SetupJacobianSparse (reactor_nvector_serial.cpp:610-771) in dense column-major
+ * storage.  r->inv_density must hold the value of the last RHS call (at y). */
+static void reactor_jacobian(reactor_t* r, double t, const double* y, const double* fy,
+                             double* J) {
+  const zrko_mech* m = r->m;
+  const int nsp = m->nsp, N = nsp + 1;
+  const double mmf = r->o->min_mass_fraction;
+  double* t1 = r->tmp1;
+  double* t2 = r->tmp2;
+  double* t3 = r->tmp3;
+  for (int j = 0; j < nsp; ++j) {
+    t1[j] = y[j];
+    if (fabs(t1[j]) < mmf) t1[j] = (t1[j] < 0.0) ? -mmf : mmf;
+    t2[j] = m->mw[j] / t1[j] * r->inv_density;
+  }
+  t1[nsp] = y[nsp];
+  reactor_rhs(r, t, t1, t3);
+  memset(J, 0, sizeof(double) * (size_t)N * N);
+  for (int s = 0; s < m->nstep; ++s) {
+    for (int k = m->reac_ptr[s]; k < m->reac_ptr[s + 1]; ++k) {
+      const int col = m->reac_sp[k];
+      const double v = r->rop[s] * t2[col];
+      for (int q = m->reac_ptr[s]; q < m->reac_ptr[s + 1]; ++q) J[col * N + m->reac_sp[q]] -= v;
+      for (int q = m->prod_ptr[s]; q < m->prod_ptr[s + 1]; ++q) J[col * N + m->prod_sp[q]] += v;
+    }
+  }
+  /* d(Tdot)/dy_j */
+  const double RuTemp = NIST_RU * y[nsp];
+  for (int j = 0; j < nsp; ++j) {
+    double acc = 0.0;
+    for (int k = 0; k < nsp; ++k) acc += J[j * N + k] * r->h[k];
+    J[j * N + nsp] = acc * RuTemp;
+  }
+  for (int j = 0; j < nsp; ++j) J[j * N + nsp] += t3[nsp] * r->cpm[j] * m->mw[j];
+  const double mult = -1.0 / r->cp_mix;
+  for (int j = 0; j < nsp; ++j) J[j * N + nsp] *= (1.0 / m->mw[j]) * mult;
+  /* d(ydot)/dT by forward difference */
+  double dT = y[nsp] * sqrt(DBL_EPSILON);
+  t1[nsp] += dT;
+  dT = t1[nsp] - y[nsp];
+  const double inv = 1.0 / dT;
+  for (int j = 0; j < nsp; ++j) t1[j] = y[j];
+  reactor_rhs(r, t, t1, t3);
+  for (int k = 0; k < N; ++k) J[nsp * N + k] = (t3[k] - fy[k]) * inv;
+  /* concentration -> mass fraction columns */
+  for (int j = 0; j < nsp; ++j)
+    for (int k = 0; k < nsp; ++k) J[j * N + k] *= m->mw[k] * (1.0 / m->mw[j]);
+}
+
+void zrko_jacobian(const zrko_mech* m, const zrko_opts* o, double P, double t,
+                   const double* y, const double* fy, double* J) {
+  reactor_t r;
+  reactor_init(&r, m, o, P);
+  double* f = (double*)malloc(sizeof(double) * (size_t)(m->nsp + 1));
+  reactor_rhs(&r, t, y, f); /* sets the carried inverse density at y */
+  reactor_jacobian(&r, t, y, fy, J);
+  free(f);
+  reactor_free(&r);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Dense LU with partial pivoting (SUNDIALS sundials_dense.c denseGETRF/RS)     */
+/* ------------------------------------------------------------------------- */
+int zrko_dense_getrf(int n, double* a, int* p) {
+  for (int k = 0; k < n; ++k) {
+    double* ck = a + (size_t)k * n;
+    int l = k;
+    for (int i = k + 1; i < n; ++i)
+      if (fabs(ck[i]) > fabs(ck[l])) l = i;
+    p[k] = l;
+    if (ck[l] == 0.0) return k + 1;
+    if (l != k)
+      for (int i = 0; i < n; ++i) {
+        double* ci = a + (size_t)i * n;
+        double tmp = ci[l];
+        ci[l] = ci[k];
+        ci[k] = tmp;
+      }
+    const double mult = 1.0 / ck[k];
+    for (int i = k + 1; i < n; ++i) ck[i] *= mult;
+    for (int j = k + 1; j < n; ++j) {
+      double* cj = a + (size_t)j * n;
+      const double akj = cj[k];
+      if (akj != 0.0)
+        for (int i = k + 1; i < n; ++i) cj[i] -= akj * ck[i];
+    }
+  }
+  return 0;
+}
+
+void zrko_dense_getrs(int n, const double* a, const int* p, double* b) {
+  for (int k = 0; k < n; ++k) {
+    const int pk = p[k];
+    if (pk != k) {
+      double tmp = b[k];
+      b[k] = b[pk];
+      b[pk] = tmp;
+    }
+  }
+  for (int k = 0; k < n - 1; ++k) {
+    const double* ck = a + (size_t)k * n;
+    const double bk = b[k];
+    for (int i = k + 1; i < n; ++i) b[i] -= ck[i] * bk;
+  }
+  for (int k = n - 1; k > 0; --k) {
+    const double* ck = a + (size_t)k * n;
+    b[k] /= ck[k];
+    const double bk = b[k];
+    for (int i = 0; i < k; ++i) b[i] -= ck[i] * bk;
+  }
+  b[0] /= a[0];
+}
+
+/* ------------------------------------------------------------------------- */
+/* CVODE v5.8.0 BDF restatement (cvode.c, cvode_ls.c, sunnonlinsol_newton.c)   */
+/* with a dense direct linear solver.  Names follow CVODE's.                    */
+/* ------------------------------------------------------------------------- */
+#define QMAX 5
+#define L_MAX (QMAX + 1)
+#define HLB_FACTOR 100.0
+#define HUB_FACTOR 0.1
+#define H_BIAS 0.5
+#define MAX_ITERS 4
+#define THRESH 1.5
+#define ETAMX1 10000.0
+#define ETAMX2 10.0
+#define ETAMX3 10.0
+#define ETAMXF 0.2
+#define ETAMIN 0.1
+#define ETACF 0.25
+#define ADDON 1.0e-6
+#define BIAS1 6.0
+#define BIAS2 6.0
+#define BIAS3 10.0
+#define ONEPSM 1.000001
+#define SMALL_NST 10
+#define MXNCF 10
+#define MXNEF 7
+#define MXNEF1 3
+#define SMALL_NEF 2
+#define LONG_WAIT 10
+#define NLS_MAXCOR 3
+#define CRDOWN 0.3
+#define DGMAX_LSETUP 0.3
+#define RDIV 2.0
+#define MSBP 20
+#define CVLS_MSBJ 51
+#define CVLS_DGMAX 0.2
+
+enum { FIRST_CALL = 0, PREV_CONV_FAIL, PREV_ERR_FAIL };
+enum { CV_NO_FAILURES = 0, CV_FAIL_BAD_J, CV_FAIL_OTHER };
+enum { DO_ERROR_TEST = 100, PREDICT_AGAIN, TRY_AGAIN };
+enum { NLS_SUCCESS = 0, NLS_CONTINUE = 901, NLS_CONV_RECVR = 902, RHSFUNC_RECVR = 903 };
+#define CV_SUCCESS 0
+#define CV_TOO_MUCH_WORK (-1)
+#define CV_TOO_MUCH_ACC (-2)
+#define CV_ERR_FAILURE (-3)
+#define CV_CONV_FAILURE (-4)
+#define CV_REPTD_RHSFUNC_ERR (-10)
+#define CV_ILL_INPUT (-22)
+
+typedef struct {
+  reactor_t* r;
+  int N;
+  double* zn[L_MAX];
+  double *ewt, *acor, *tempv, *ftemp, *y, *atolv, *delta;
+  double *savedJ, *M;
+  int* piv;
+  double rtol;
+  double tn, h, hscale, hprime, next_h, eta, etamax, hmin, hmax_inv, hu, h0u;
+  double tau[L_MAX + 1], tq[6], l[L_MAX];
+  double rl1, gamma, gammap, gamrat, crate, delp, acnrm, nlscoef, saved_tq5, tolsf;
+  double etaq, etaqm1, etaqp1;
+  int q, qprime, next_q, qwait, L, qu, indx_acor;
+  long nst, nstlp, nstlj, nfe, nje, nsetups, nni, ncfn, netf, nscon;
+  int convfail, jcur, acnrmcur;
+} cvmem;
+
+static double wrms(const double* x, const double* w, int n) {
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double p = x[i] * w[i];
+    s += p * p;
+  }
+  return sqrt(s / n);
+}
+
+static int cv_ewt(cvmem* cv, const double* y) {
+  for (int i = 0; i < cv->N; ++i) {
+    const double v = cv->rtol * fabs(y[i]) + cv->atolv[i];
+    if (v <= 0.0) return -1;
+    cv->ewt[i] = 1.0 / v;
+  }
+  return 0;
+}
+
+static int cv_f(cvmem* cv, double t, const double* y, double* yd) {
+  cv->nfe++;
+  return reactor_rhs(cv->r, t, y, yd);
+}
+
+static void cv_reinit(cvmem* cv, double t0, const double* y0) {
+  memcpy(cv->zn[0], y0, sizeof(double) * (size_t)cv->N);
+  cv->tn = t0;
+  cv->q = 1;
+  cv->L = 2;
+  cv->qwait = cv->L;
+  cv->etamax = ETAMX1;
+  cv->qu = 0;
+  cv->hu = 0.0;
+  cv->tolsf = 1.0;
+  cv->nst = cv->nfe = cv->ncfn = cv->netf = cv->nni = cv->nsetups = 0;
+  cv->nstlp = 0;
+  cv->nstlj = 0;
+  cv->nje = 0;
+  cv->nscon = 0;
+  cv->hmin = 0.0;
+  cv->h0u = 0.0;
+  cv->next_h = 0.0;
+  cv->next_q = 0;
+  cv->indx_acor = QMAX;
+  cv->saved_tq5 = 0.0;
+  cv->crate = 1.0;
+  cv->delp = 0.0;
+  cv->gammap = 0.0;
+  cv->gamrat = 1.0;
+  for (int i = 0; i <= L_MAX; ++i) cv->tau[i] = 0.0;
+  for (int i = 0; i < 6; ++i) cv->tq[i] = 0.0;
+  for (int i = 0; i < L_MAX; ++i) cv->l[i] = 0.0;
+}
+
+static double cv_upper_bound_h0(cvmem* cv, double tdist) {
+  double hub_inv = 0.0;
+  for (int i = 0; i < cv->N; ++i) {
+    const double t1 = HUB_FACTOR * fabs(cv->zn[0][i]) + 1.0 / cv->ewt[i];
+    const double t2 = fabs(cv->zn[1][i]) / t1;
+    if (t2 > hub_inv) hub_inv = t2;
+  }
+  double hub = HUB_FACTOR * tdist;
+  if (hub * hub_inv > 1.0) hub = 1.0 / hub_inv;
+  return hub;
+}
+
+static int cv_ydd_norm(cvmem* cv, double hg, double* yddnrm) {
+  for (int i = 0; i < cv->N; ++i) cv->y[i] = hg * cv->zn[1][i] + cv->zn[0][i];
+  int rv = cv_f(cv, cv->tn + hg, cv->y, cv->tempv);
+  if (rv > 0) return 1;
+  for (int i = 0; i < cv->N; ++i) cv->tempv[i] = (cv->tempv[i] - cv->zn[1][i]) * (1.0 / hg);
+  *yddnrm = wrms(cv->tempv, cv->ewt, cv->N);
+  return 0;
+}
+
+static int cv_hin(cvmem* cv, double tout) {
+  const double tdiff = tout - cv->tn;
+  if (tdiff == 0.0) return -1;
+  const double tdist = fabs(tdiff);
+  const double tround = DBL_EPSILON * fmax(fabs(cv->tn), fabs(tout));
+  if (tdist < 2.0 * tround) return -1;
+  const double hlb = HLB_FACTOR * tround;
+  const double hub = cv_upper_bound_h0(cv, tdist);
+  double hg = sqrt(hlb * hub);
+  if (hub < hlb) {
+    cv->h = hg;
+    return 0;
+  }
+  double hnew = hg, hs = hg, yddnrm = 0.0;
+  for (int count1 = 1; count1 <= MAX_ITERS; ++count1) {
+    int hgOK = 0;
+    for (int count2 = 1; count2 <= MAX_ITERS; ++count2) {
+      if (cv_ydd_norm(cv, hg, &yddnrm) == 0) {
+        hgOK = 1;
+        break;
+      }
+      hg *= 0.2;
+    }
+    if (!hgOK) {
+      if (count1 <= 2) return -2;
+      hnew = hs;
+      break;
+    }
+    hs = hg;
+    hnew = (yddnrm * hub * hub > 2.0) ? sqrt(2.0 / yddnrm) : sqrt(hg * hub);
+    if (count1 == MAX_ITERS) break;
+    const double hrat = hnew / hg;
+    if (hrat > 0.5 && hrat < 2.0) break;
+    if (count1 > 1 && hrat > 2.0) {
+      hnew = hg;
+      break;
+    }
+    hg = hnew;
+  }
+  double h0 = H_BIAS * hnew;
+  if (h0 < hlb) h0 = hlb;
+  if (h0 > hub) h0 = hub;
+  cv->h = h0;
+  return 0;
+}
+
+static void cv_rescale(cvmem* cv) {
+  double f = cv->eta;
+  for (int j = 1; j <= cv->q; ++j) {
+    for (int i = 0; i < cv->N; ++i) cv->zn[j][i] *= f;
+    f *= cv->eta;
+  }
+  cv->h = cv->hscale * cv->eta;
+  cv->next_h = cv->h;
+  cv->hscale = cv->h;
+  cv->nscon = 0;
+}
+
+static void cv_increase_bdf(cvmem* cv) {
+  for (int i = 0; i <= QMAX; ++i) cv->l[i] = 0.0;
+  cv->l[2] = 1.0;
+  double alpha1 = 1.0, prod = 1.0, xiold = 1.0, alpha0 = -1.0;
+  double hsum = cv->hscale;
+  if (cv->q > 1) {
+    for (int j = 1; j < cv->q; ++j) {
+      hsum += cv->tau[j + 1];
+      const double xi = hsum / cv->hscale;
+      prod *= xi;
+      alpha0 -= 1.0 / (j + 1);
+      alpha1 += 1.0 / xi;
+      for (int i = j + 2; i >= 2; --i) cv->l[i] = cv->l[i] * xiold + cv->l[i - 1];
+      xiold = xi;
+    }
+  }
+  const double A1 = (-alpha0 - alpha1) / prod;
+  double* zL = cv->zn[cv->L];
+  const double* za = cv->zn[cv->indx_acor];
+  for (int i = 0; i < cv->N; ++i) zL[i] = A1 * za[i];
+  for (int j = 2; j <= cv->q; ++j)
+    for (int i = 0; i < cv->N; ++i) cv->zn[j][i] += cv->l[j] * zL[i];
+}
+
+static void cv_decrease_bdf(cvmem* cv) {
+  for (int i = 0; i <= QMAX; ++i) cv->l[i] = 0.0;
+  cv->l[2] = 1.0;
+  double hsum = 0.0;
+  for (int j = 1; j <= cv->q - 2; ++j) {
+    hsum += cv->tau[j];
+    const double xi = hsum / cv->hscale;
+    for (int i = j + 2; i >= 2; --i) cv->l[i] = cv->l[i] * xi + cv->l[i - 1];
+  }
+  for (int j = 2; j < cv->q; ++j)
+    for (int i = 0; i < cv->N; ++i) cv->zn[j][i] = -cv->l[j] * cv->zn[cv->q][i] + cv->zn[j][i];
+}
+
+static void cv_adjust_order(cvmem* cv, int dq) {
+  if (cv->q == 2 && dq != 1) return;
+  if (dq == 1)
+    cv_increase_bdf(cv);
+  else
+    cv_decrease_bdf(cv);
+}
+
+static void cv_adjust_params(cvmem* cv) {
+  if (cv->qprime != cv->q) {
+    cv_adjust_order(cv, cv->qprime - cv->q);
+    cv->q = cv->qprime;
+    cv->L = cv->q + 1;
+    cv->qwait = cv->L;
+  }
+  cv_rescale(cv);
+}
+
+static void cv_predict(cvmem* cv) {
+  cv->tn += cv->h;
+  for (int k = 1; k <= cv->q; ++k)
+    for (int j = cv->q; j >= k; --j)
+      for (int i = 0; i < cv->N; ++i) cv->zn[j - 1][i] += cv->zn[j][i];
+}
+
+static void cv_restore(cvmem* cv, double saved_t) {
+  cv->tn = saved_t;
+  for (int k = 1; k <= cv->q; ++k)
+    for (int j = cv->q; j >= k; --j)
+      for (int i = 0; i < cv->N; ++i) cv->zn[j - 1][i] -= cv->zn[j][i];
+}
+
+static void cv_set_tq_bdf(cvmem* cv, double hsum, double alpha0, double alpha0_hat,
+                          double xi_inv, double xistar_inv) {
+  const int q = cv->q;
+  const double A1 = 1.0 - alpha0_hat + alpha0;
+  const double A2 = 1.0 + q * A1;
+  cv->tq[2] = fabs(A1 / (alpha0 * A2));
+  cv->tq[5] = fabs(A2 * xistar_inv / (cv->l[q] * xi_inv));
+  if (cv->qwait == 1) {
+    if (q > 1) {
+      const double C = xistar_inv / cv->l[q];
+      const double A3 = alpha0 + 1.0 / q;
+      const double A4 = alpha0_hat + xi_inv;
+      const double Cpinv = (1.0 - A4 + A3) / A3;
+      cv->tq[1] = fabs(C * Cpinv);
+    } else {
+      cv->tq[1] = 1.0;
+    }
+    hsum += cv->tau[q];
+    xi_inv = cv->h / hsum;
+    const double A5 = alpha0 - (1.0 / (q + 1));
+    const double A6 = alpha0_hat - xi_inv;
+    const double Cppinv = (1.0 - A6 + A5) / A2;
+    cv->tq[3] = fabs(Cppinv / (xi_inv * (q + 2) * A5));
+  }
+  cv->tq[4] = cv->nlscoef / cv->tq[2];
+}
+
+static void cv_set(cvmem* cv) {
+  const int q = cv->q;
+  double alpha0, alpha0_hat, xi_inv, xistar_inv, hsum;
+  cv->l[0] = cv->l[1] = xi_inv = xistar_inv = 1.0;
+  for (int i = 2; i <= q; ++i) cv->l[i] = 0.0;
+  alpha0 = alpha0_hat = -1.0;
+  hsum = cv->h;
+  if (q > 1) {
+    for (int j = 2; j < q; ++j) {
+      hsum += cv->tau[j - 1];
+      xi_inv = cv->h / hsum;
+      alpha0 -= 1.0 / j;
+      for (int i = j; i >= 1; --i) cv->l[i] += cv->l[i - 1] * xi_inv;
+    }
+    alpha0 -= 1.0 / q;
+    xistar_inv = -cv->l[1] - alpha0;
+    hsum += cv->tau[q - 1];
+    xi_inv = cv->h / hsum;
+    alpha0_hat = -cv->l[1] - xi_inv;
+    for (int i = q; i >= 1; --i) cv->l[i] += cv->l[i - 1] * xistar_inv;
+  }
+  cv_set_tq_bdf(cv, hsum, alpha0, alpha0_hat, xi_inv, xistar_inv);
+  cv->rl1 = 1.0 / cv->l[1];
+  cv->gamma = cv->h * cv->rl1;
+  if (cv->nst == 0) cv->gammap = cv->gamma;
+  cv->gamrat = (cv->nst > 0) ? cv->gamma / cv->gammap : 1.0;
+}
+
+/* cvLsSetup + cvLsLinSys + SUNLinSolSetup_Dense; returns 0, >0 recoverable. */
+static int cv_lsetup(cvmem* cv, int convfail, const double* ypred, const double* fpred,
+                     int* jcur) {
+  const int N = cv->N;
+  const double dgamma = fabs(cv->gamma / cv->gammap - 1.0);
+  const int jbad = (cv->nst == 0) || (cv->nst >= cv->nstlj + CVLS_MSBJ) ||
+                   ((convfail == CV_FAIL_BAD_J) && (dgamma < CVLS_DGMAX)) ||
+                   (convfail == CV_FAIL_OTHER);
+  if (!jbad) {
+    *jcur = 0;
+    memcpy(cv->M, cv->savedJ, sizeof(double) * (size_t)N * N);
+  } else {
+    *jcur = 1;
+    cv->nje++;
+    cv->nstlj = cv->nst;
+    reactor_jacobian(cv->r, cv->tn, ypred, fpred, cv->M);
+    memcpy(cv->savedJ, cv->M, sizeof(double) * (size_t)N * N);
+  }
+  for (int j = 0; j < N; ++j) {
+    for (int i = 0; i < N; ++i) cv->M[j * N + i] *= -cv->gamma;
+    cv->M[j * N + j] += 1.0;
+  }
+  return zrko_dense_getrf(N, cv->M, cv->piv) != 0 ? 1 : 0;
+}
+
+/* cvNlsLSetup */
+static int cv_nls_lsetup(cvmem* cv, int jbad, int* jcur) {
+  if (jbad) cv->convfail = CV_FAIL_BAD_J;
+  int rv = cv_lsetup(cv, cv->convfail, cv->y, cv->ftemp, jcur);
+  cv->nsetups++;
+  cv->gamrat = 1.0;
+  cv->gammap = cv->gamma;
+  cv->crate = 1.0;
+  cv->nstlp = cv->nst;
+  return rv > 0 ? NLS_CONV_RECVR : NLS_SUCCESS;
+}
+
+/* cvNlsResidual */
+static int cv_nls_residual(cvmem* cv, const double* ycor, double* res) {
+  const int N = cv->N;
+  for (int i = 0; i < N; ++i) cv->y[i] = cv->zn[0][i] + ycor[i];
+  if (cv_f(cv, cv->tn, cv->y, cv->ftemp) > 0) return RHSFUNC_RECVR;
+  for (int i = 0; i < N; ++i) {
+    res[i] = cv->rl1 * cv->zn[1][i] + ycor[i];
+    res[i] = -cv->gamma * cv->ftemp[i] + res[i];
+  }
+  return NLS_SUCCESS;
+}
+
+/* cvNlsConvTest */
+static int cv_nls_convtest(cvmem* cv, int m, const double* ycor, const double* delta,
+                           double tol) {
+  const double del = wrms(delta, cv->ewt, cv->N);
+  if (m > 0) cv->crate = fmax(CRDOWN * cv->crate, del / cv->delp);
+  const double dcon = del * fmin(1.0, cv->crate) / tol;
+  if (dcon <= 1.0) {
+    cv->acnrm = (m == 0) ? del : wrms(ycor, cv->ewt, cv->N);
+    cv->acnrmcur = 1;
+    return NLS_SUCCESS;
+  }
+  if (m >= 1 && del > RDIV * cv->delp) return NLS_CONV_RECVR;
+  cv->delp = del;
+  return NLS_CONTINUE;
+}
+
+/* SUNNonlinSolSolve_Newton (sunnonlinsol_newton.c) with CVODE's callbacks */
+static int cv_newton(cvmem* cv, double* ycor, double tol, int callLSetup) {
+  const int N = cv->N;
+  double* delta = cv->delta;
+  int jbad = 0, jcur = 0, rv;
+  for (;;) {
+    rv = cv_nls_residual(cv, ycor, delta);
+    if (rv != NLS_SUCCESS) goto fail;
+    if (callLSetup) {
+      rv = cv_nls_lsetup(cv, jbad, &jcur);
+      cv->jcur = jcur;
+      if (rv != NLS_SUCCESS) goto fail;
+    }
+    {
+      int curiter = 0;
+      for (;;) {
+        cv->nni++;
+        for (int i = 0; i < N; ++i) delta[i] = -delta[i];
+        zrko_dense_getrs(N, cv->M, cv->piv, delta);
+        if (cv->gamrat != 1.0) {
+          const double s = 2.0 / (1.0 + cv->gamrat);
+          for (int i = 0; i < N; ++i) delta[i] *= s;
+        }
+        for (int i = 0; i < N; ++i) ycor[i] += delta[i];
+        rv = cv_nls_convtest(cv, curiter, ycor, delta, tol);
+        if (rv == NLS_SUCCESS) {
+          cv->jcur = 0;
+          return NLS_SUCCESS;
+        }
+        if (rv != NLS_CONTINUE) break;
+        curiter++;
+        if (curiter >= NLS_MAXCOR) {
+          rv = NLS_CONV_RECVR;
+          break;
+        }
+        rv = cv_nls_residual(cv, ycor, delta);
+        if (rv != NLS_SUCCESS) break;
+      }
+    }
+  fail:
+    if (rv > 0 && !cv->jcur) {
+      callLSetup = 1;
+      jbad = 1;
+      for (int i = 0; i < N; ++i) ycor[i] = 0.0;
+      continue;
+    }
+    return rv;
+  }
+}
+
+/* cvNls */
+static int cv_nls(cvmem* cv, int nflag) {
+  cv->convfail = (nflag == FIRST_CALL || nflag == PREV_ERR_FAIL) ? CV_NO_FAILURES : CV_FAIL_OTHER;
+  const int callSetup = (nflag == PREV_CONV_FAIL) || (nflag == PREV_ERR_FAIL) || (cv->nst == 0) ||
+                        (cv->nst >= cv->nstlp + MSBP) ||
+                        (fabs(cv->gamrat - 1.0) > DGMAX_LSETUP);
+  for (int i = 0; i < cv->N; ++i) cv->acor[i] = 0.0;
+  cv->acnrmcur = 0;
+  const int rv = cv_newton(cv, cv->acor, cv->tq[4], callSetup);
+  if (rv != NLS_SUCCESS) return rv;
+  for (int i = 0; i < cv->N; ++i) cv->y[i] = cv->zn[0][i] + cv->acor[i];
+  if (!cv->acnrmcur) cv->acnrm = wrms(cv->acor, cv->ewt, cv->N);
+  cv->jcur = 0;
+  return CV_SUCCESS;
+}
+
+static int cv_handle_nflag(cvmem* cv, int* nflagPtr, double saved_t, int* ncfPtr) {
+  const int nflag = *nflagPtr;
+  if (nflag == CV_SUCCESS) return DO_ERROR_TEST;
+  cv->ncfn++;
+  cv_restore(cv, saved_t);
+  (*ncfPtr)++;
+  cv->etamax = 1.0;
+  if (fabs(cv->h) <= cv->hmin * ONEPSM || *ncfPtr == MXNCF) {
+    if (nflag == NLS_CONV_RECVR) return CV_CONV_FAILURE;
+    return CV_REPTD_RHSFUNC_ERR;
+  }
+  cv->eta = fmax(ETACF, cv->hmin / fabs(cv->h));
+  *nflagPtr = PREV_CONV_FAIL;
+  cv_rescale(cv);
+  return PREDICT_AGAIN;
+}
+
+static int cv_do_error_test(cvmem* cv, int* nflagPtr, double saved_t, int* nefPtr,
+                            double* dsmPtr) {
+  const double dsm = cv->acnrm * cv->tq[2];
+  *dsmPtr = dsm;
+  if (dsm <= 1.0) return CV_SUCCESS;
+  (*nefPtr)++;
+  cv->netf++;
+  *nflagPtr = PREV_ERR_FAIL;
+  cv_restore(cv, saved_t);
+  if (fabs(cv->h) <= cv->hmin * ONEPSM || *nefPtr == MXNEF) return CV_ERR_FAILURE;
+  cv->etamax = 1.0;
+  if (*nefPtr <= MXNEF1) {
+    cv->eta = 1.0 / (pow(BIAS2 * dsm, 1.0 / cv->L) + ADDON);
+    cv->eta = fmax(ETAMIN, fmax(cv->eta, cv->hmin / fabs(cv->h)));
+    if (*nefPtr >= SMALL_NEF) cv->eta = fmin(cv->eta, ETAMXF);
+    cv_rescale(cv);
+    return TRY_AGAIN;
+  }
+  if (cv->q > 1) {
+    cv->eta = fmax(ETAMIN, cv->hmin / fabs(cv->h));
+    cv_adjust_order(cv, -1);
+    cv->L = cv->q;
+    cv->q--;
+    cv->qwait = cv->L;
+    cv_rescale(cv);
+    return TRY_AGAIN;
+  }
+  cv->eta = fmax(ETAMIN, cv->hmin / fabs(cv->h));
+  cv->h *= cv->eta;
+  cv->next_h = cv->h;
+  cv->hscale = cv->h;
+  cv->qwait = LONG_WAIT;
+  cv->nscon = 0;
+  if (cv_f(cv, cv->tn, cv->zn[0], cv->tempv) > 0) return CV_REPTD_RHSFUNC_ERR;
+  for (int i = 0; i < cv->N; ++i) cv->zn[1][i] = cv->h * cv->tempv[i];
+  return TRY_AGAIN;
+}
+
+static void cv_complete_step(cvmem* cv) {
+  cv->nst++;
+  cv->nscon++;
+  cv->hu = cv->h;
+  cv->qu = cv->q;
+  for (int i = cv->q; i >= 2; --i) cv->tau[i] = cv->tau[i - 1];
+  if (cv->q == 1 && cv->nst > 1) cv->tau[2] = cv->tau[1];
+  cv->tau[1] = cv->h;
+  for (int j = 0; j <= cv->q; ++j)
+    for (int i = 0; i < cv->N; ++i) cv->zn[j][i] += cv->l[j] * cv->acor[i];
+  cv->qwait--;
+  if (cv->qwait == 1 && cv->q != QMAX) {
+    memcpy(cv->zn[QMAX], cv->acor, sizeof(double) * (size_t)cv->N);
+    cv->saved_tq5 = cv->tq[5];
+    cv->indx_acor = QMAX;
+  }
+}
+
+static void cv_set_eta(cvmem* cv) {
+  if (cv->eta < THRESH) {
+    cv->eta = 1.0;
+    cv->hprime = cv->h;
+  } else {
+    cv->eta = fmin(cv->eta, cv->etamax);
+    cv->eta /= fmax(1.0, fabs(cv->h) * cv->hmax_inv * cv->eta);
+    cv->hprime = cv->h * cv->eta;
+    if (cv->qprime < cv->q) cv->nscon = 0;
+  }
+}
+
+static double cv_etaqm1(cvmem* cv) {
+  cv->etaqm1 = 0.0;
+  if (cv->q > 1) {
+    const double ddn = wrms(cv->zn[cv->q], cv->ewt, cv->N) * cv->tq[1];
+    cv->etaqm1 = 1.0 / (pow(BIAS1 * ddn, 1.0 / cv->q) + ADDON);
+  }
+  return cv->etaqm1;
+}
+
+static double cv_etaqp1(cvmem* cv) {
+  cv->etaqp1 = 0.0;
+  if (cv->q != QMAX) {
+    if (cv->saved_tq5 == 0.0) return cv->etaqp1;
+    double hr = 1.0; /* SUNRpowerI: repeated product */
+    for (int i = 0; i < cv->L; ++i) hr *= cv->h / cv->tau[2];
+    const double cquot = (cv->tq[5] / cv->saved_tq5) * hr;
+    for (int i = 0; i < cv->N; ++i) cv->tempv[i] = -cquot * cv->zn[QMAX][i] + cv->acor[i];
+    const double dup = wrms(cv->tempv, cv->ewt, cv->N) * cv->tq[3];
+    cv->etaqp1 = 1.0 / (pow(BIAS3 * dup, 1.0 / (cv->L + 1)) + ADDON);
+  }
+  return cv->etaqp1;
+}
+
+static void cv_choose_eta(cvmem* cv) {
+  const double etam = fmax(cv->etaqm1, fmax(cv->etaq, cv->etaqp1));
+  if (etam < THRESH) {
+    cv->eta = 1.0;
+    cv->qprime = cv->q;
+    return;
+  }
+  if (etam == cv->etaqm1) {
+    cv->eta = cv->etaqm1;
+    cv->qprime = cv->q - 1;
+  } else if (etam == cv->etaq) {
+    cv->eta = cv->etaq;
+    cv->qprime = cv->q;
+  } else {
+    cv->eta = cv->etaqp1;
+    cv->qprime = cv->q + 1;
+    memcpy(cv->zn[QMAX], cv->acor, sizeof(double) * (size_t)cv->N);
+  }
+}
+
+static void cv_prepare_next_step(cvmem* cv, double dsm) {
+  if (cv->etamax == 1.0) {
+    cv->qwait = (cv->qwait > 2) ? cv->qwait : 2;
+    cv->qprime = cv->q;
+    cv->hprime = cv->h;
+    cv->eta = 1.0;
+    return;
+  }
+  cv->etaq = 1.0 / (pow(BIAS2 * dsm, 1.0 / cv->L) + ADDON);
+  if (cv->qwait != 0) {
+    cv->eta = cv->etaq;
+    cv->qprime = cv->q;
+    cv_set_eta(cv);
+    return;
+  }
+  cv->qwait = 2;
+  cv->etaqm1 = cv_etaqm1(cv);
+  cv->etaqp1 = cv_etaqp1(cv);
+  cv_choose_eta(cv);
+  cv_set_eta(cv);
+}
+
+static int cv_step(cvmem* cv) {
+  const double saved_t = cv->tn;
+  int ncf = 0, nef = 0, nflag = FIRST_CALL;
+  double dsm = 0.0;
+  if (cv->nst > 0 && cv->hprime != cv->h) cv_adjust_params(cv);
+  for (;;) {
+    cv_predict(cv);
+    cv_set(cv);
+    nflag = cv_nls(cv, nflag);
+    const int kflag = cv_handle_nflag(cv, &nflag, saved_t, &ncf);
+    if (kflag == PREDICT_AGAIN) continue;
+    if (kflag != DO_ERROR_TEST) return kflag;
+    const int eflag = cv_do_error_test(cv, &nflag, saved_t, &nef, &dsm);
+    if (eflag == TRY_AGAIN) continue;
+    if (eflag != CV_SUCCESS) return eflag;
+    break;
+  }
+  cv_complete_step(cv);
+  cv_prepare_next_step(cv, dsm);
+  cv->etamax = (cv->nst <= SMALL_NST) ? ETAMX2 : ETAMX3;
+  for (int i = 0; i < cv->N; ++i) cv->acor[i] *= cv->tq[2];
+  return CV_SUCCESS;
+}
+
+/* CVode(..., CV_ONE_STEP): one internal step, initial setup on the first call. */
+static int cv_one_step(cvmem* cv, double tout, double* tret) {
+  if (cv->nst == 0) {
+    if (cv_ewt(cv, cv->zn[0]) != 0) return CV_ILL_INPUT;
+    if (cv_f(cv, cv->tn, cv->zn[0], cv->zn[1]) > 0) return CV_REPTD_RHSFUNC_ERR;
+    if (cv_hin(cv, tout) != 0) return CV_ILL_INPUT;
+    const double rh = fabs(cv->h) * cv->hmax_inv;
+    if (rh > 1.0) cv->h /= rh;
+    cv->hscale = cv->h;
+    cv->h0u = cv->h;
+    cv->hprime = cv->h;
+    for (int i = 0; i < cv->N; ++i) cv->zn[1][i] *= cv->h;
+  } else {
+    if (cv_ewt(cv, cv->zn[0]) != 0) return CV_ILL_INPUT;
+  }
+  const double nrm = wrms(cv->zn[0], cv->ewt, cv->N);
+  cv->tolsf = DBL_EPSILON * nrm;
+  if (cv->tolsf > 1.0) return CV_TOO_MUCH_ACC;
+  cv->tolsf = 1.0;
+  const int kflag = cv_step(cv);
+  *tret = cv->tn;
+  if (kflag != CV_SUCCESS) return kflag;
+  cv->next_q = cv->qprime;
+  cv->next_h = cv->hprime;
+  return CV_SUCCESS;
+}
+
+/* CVodeGetDky(t, k=0) */
+static void cv_dky0(cvmem* cv, double t, double* out) {
+  const double s = (t - cv->tn) / cv->h;
+  for (int i = 0; i < cv->N; ++i) out[i] = 0.0;
+  for (int j = cv->q; j >= 0; --j) {
+    double c = 1.0;
+    for (int i = 0; i < j; ++i) c *= s;
+    for (int i = 0; i < cv->N; ++i) out[i] += c * cv->zn[j][i];
+  }
+}
+
+/* CvodeSolver::Integrate (solver_cvode.cpp:51-299): one-step loop, retries,
+ * Dky interpolation at min(tcurr, end_time). */
+static int integrate_one(const zrko_mech* m, const zrko_opts* o, double P, double dt,
+                         double* y, zrko_stats* st) {
+  const int N = m->nsp + 1;
+  reactor_t r;
+  reactor_init(&r, m, o, P);
+  cvmem cv;
+  memset(&cv, 0, sizeof(cv));
+  cv.r = &r;
+  cv.N = N;
+  double* w = (double*)calloc((size_t)(L_MAX + 8) * N + 2 * (size_t)N * N, sizeof(double));
+  for (int j = 0; j < L_MAX; ++j) cv.zn[j] = w + (size_t)j * N;
+  cv.ewt = w + (size_t)L_MAX * N;
+  cv.acor = cv.ewt + N;
+  cv.tempv = cv.acor + N;
+  cv.ftemp = cv.tempv + N;
+  cv.y = cv.ftemp + N;
+  cv.atolv = cv.y + N;
+  cv.delta = cv.atolv + N;
+  cv.savedJ = cv.delta + 2 * N;
+  cv.M = cv.savedJ + (size_t)N * N;
+  cv.piv = (int*)calloc((size_t)N, sizeof(int));
+  for (int i = 0; i < N; ++i) cv.atolv[i] = o->atol; /* abstol_dens = 0 */
+  cv.rtol = o->rtol;
+  cv.nlscoef = o->nlscoef;
+  cv.hmax_inv = (o->max_dt > 0.0) ? 1.0 / o->max_dt : 0.0;
+  cv_reinit(&cv, 0.0, y);
+
+  long nsteps = 0, nst_total = 0, nfe_total = 0, nje_total = 0, nset_total = 0;
+  long nni_total = 0, ncfn_total = 0, netf_total = 0;
+  int num_tries = 0, flag = CV_SUCCESS;
+  double tcurr = 0.0;
+  while (tcurr < dt) {
+    flag = cv_one_step(&cv, dt, &tcurr);
+    nsteps += 1;
+    if (cv.nst >= o->max_steps) flag = CV_TOO_MUCH_WORK;
+    if (flag < 0) {
+      num_tries += 1;
+      for (int i = 0; i < N; ++i) cv.atolv[i] *= o->retry_atol_adj;
+      cv.rtol *= o->retry_rtol_adj;
+      nst_total += cv.nst; nfe_total += cv.nfe; nje_total += cv.nje; nset_total += cv.nsetups;
+      nni_total += cv.nni; ncfn_total += cv.ncfn; netf_total += cv.netf;
+      if (num_tries == o->num_retries) break;
+      double* y0 = cv.tempv; /* yout = zn[0] at tn */
+      memcpy(y0, cv.zn[0], sizeof(double) * (size_t)N);
+      double* ysave = (double*)malloc(sizeof(double) * (size_t)N);
+      memcpy(ysave, y0, sizeof(double) * (size_t)N);
+      cv_reinit(&cv, tcurr, ysave);
+      free(ysave);
+    }
+  }
+  if (flag == CV_SUCCESS) {
+    cv_dky0(&cv, tcurr < dt ? tcurr : dt, y);
+    nst_total += cv.nst; nfe_total += cv.nfe; nje_total += cv.nje; nset_total += cv.nsetups;
+    nni_total += cv.nni; ncfn_total += cv.ncfn; netf_total += cv.netf;
+  }
+  /* a failed reactor keeps its input state (zerork_reactor_manager.cpp:1123-1139) */
+  if (flag < 0) nsteps = (nsteps <= 0) ? -1 : -nsteps;
+  if (st) {
+    st->nsteps = (int)nsteps;
+    st->nst = (int)nst_total;
+    st->nfe = (int)nfe_total;
+    st->nje = (int)nje_total;
+    st->nsetups = (int)nset_total;
+    st->nni = (int)nni_total;
+    st->ncfn = (int)ncfn_total;
+    st->netf = (int)netf_total;
+    st->status = flag;
+  }
+  free(cv.piv);
+  free(w);
+  reactor_free(&r);
+  return flag;
+}
+
+int zrko_integrate_batch(const zrko_mech* m, const zrko_opts* o, int n, double dt, double* T,
+                         const double* P, double* mf, int stride, zrko_stats* stats,
+                         int nthreads) {
+  const int nsp = m->nsp;
+  int nfail = 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : nfail)
+  for (int k = 0; k < n; ++k) {
+    double* y = (double*)malloc(sizeof(double) * (size_t)(nsp + 1));
+    for (int j = 0; j < nsp; ++j) y[j] = mf[(size_t)k * stride + j];
+    y[nsp] = T[k] / o->Tref;
+    int flag = integrate_one(m, o, P[k], dt, y, stats ? stats + k : NULL);
+    for (int j = 0; j < nsp; ++j) mf[(size_t)k * stride + j] = y[j];
+    T[k] = y[nsp] * o->Tref;
+    if (flag < 0) nfail++;
+    free(y);
+  }
+  return nfail;
+}

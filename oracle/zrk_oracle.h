@@ -1,0 +1,110 @@
+/* CPU FP64 oracle: a plain-C restatement of the reference cfd_plugin path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Used by tests/ (as the checker), by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg.  The product
+ * (zero-rk_amd/) never links, loads or calls it.
+ *
+ * Each function cites the reference code it restates:
+ *   zrko_thermo            src/zerork/nasa_poly.cpp:50-172
+ *   zrko_rate_constants    src/zerork/rate_const.cpp:539-582 (updateK), 669-731,
+ *                          862-1012
+ *   zrko_rates             src/zerork/perf_net.cpp:190-275
+ *                          (calcRatesFromTC_StepLimiter)
+ *   zrko_rhs               applications/cfd_plugin/reactor_constant_pressure_cpu.cpp:68-133
+ *   zrko_jacobian          applications/cfd_plugin/reactor_nvector_serial.cpp:610-771
+ *   zrko_integrate         applications/cfd_plugin/solver_cvode.cpp:51-299 driving a
+ *                          restatement of SUNDIALS CVODE v5.8.0 BDF (dense direct
+ *                          Newton, the plugin's dense=1/iterative=0 mode), pinned
+ *                          by cmake/sundials5.CMakeLists.txt:8-9 in the reference.
+ */
+#ifndef ZRK_ORACLE_H
+#define ZRK_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int nsp, nrxn, nstep;
+  const double* mw;        /* [nsp] kg/kmol */
+  const double* thermo;    /* [nsp*16] Tmid, low a1..a7, high a1..a7 */
+  const int* reac_ptr;     /* [nstep+1] */
+  const int* reac_sp;
+  const int* prod_ptr;     /* [nstep+1] */
+  const int* prod_sp;
+  int n_arr;
+  const int* arr_step;
+  const double* arr_lnA;
+  const double* arr_beta;
+  const double* arr_Ta;
+  int n_keq;
+  const int* keq_step;
+  const int* keq_fwd;
+  const double* keq_ndelta;
+  int n_tb;
+  const int* tb_fwd;
+  const int* tb_rev;
+  const int* tb_ptr;
+  const int* tb_sp;
+  const double* tb_eff;    /* efficiency - 1 */
+  int n_fo;
+  const int* fo_fwd;
+  const int* fo_rev;
+  const int* fo_type;      /* 0 Lindemann, 1 Troe-3, 2 Troe-4 */
+  const int* fo_spc;       /* -1: mixture M */
+  const double* fo_param;  /* [n_fo*7] lnAlow, beta, Ta, alpha, T***, T*, T** */
+  const int* fo_ptr;
+  const int* fo_sp;
+  const double* fo_eff;
+} zrko_mech;
+
+typedef struct {
+  double rtol, atol;
+  double max_dt;
+  double nlscoef;          /* nonlinear_convergence_coeff */
+  int max_steps;
+  int num_retries;         /* cvode_num_retries */
+  double retry_atol_adj;   /* cvode_retry_absolute_tolerance_adjustment */
+  double retry_rtol_adj;   /* cvode_retry_relative_tolerance_adjustment */
+  double min_mass_fraction;
+  double step_limiter;
+  double Tref;             /* reference_temperature */
+} zrko_opts;
+
+typedef struct {
+  int nsteps;      /* CVode one-step calls (the plugin's cost output), <0 on failure */
+  int nst, nfe, nje, nsetups, nni, ncfn, netf;
+  int status;      /* 0 ok, <0 CVODE-style failure flag */
+} zrko_stats;
+
+void zrko_default_opts(zrko_opts* o);
+
+void zrko_thermo(const zrko_mech* m, double T, double* cp_R, double* h_RT, double* g_RT);
+/* K per step after Arrhenius, Keq, third-body and falloff (no limiter). */
+void zrko_rate_constants(const zrko_mech* m, double T, const double* C, double* K);
+void zrko_rates(const zrko_mech* m, double T, const double* C, double step_limiter,
+                double* rop, double* create, double* destroy, double* net);
+
+/* Constant-pressure reactor state y = [Y_0..Y_{nsp-1}, T/Tref] (solve_temperature). */
+int zrko_rhs(const zrko_mech* m, const zrko_opts* o, double P, double t, const double* y,
+             double* ydot);
+/* Reference analytic Jacobian (dense, column-major J[col*N+row]) at y with
+ * fy = f(y); reproduces the inverse-density carry-over of the reference. */
+void zrko_jacobian(const zrko_mech* m, const zrko_opts* o, double P, double t,
+                   const double* y, const double* fy, double* J);
+
+/* Integrate n constant-pressure reactors over [0, dt] (AoS: mf[k*stride+j]),
+ * updating T and mf in place; nthreads <= 0 uses OpenMP's default. */
+int zrko_integrate_batch(const zrko_mech* m, const zrko_opts* o, int n, double dt,
+                         double* T, const double* P, double* mf, int stride,
+                         zrko_stats* stats, int nthreads);
+
+/* Dense LU with partial pivoting (SUNDIALS denseGETRF/denseGETRS semantics),
+ * column-major; exposed for tests. Returns 0 or k+1 at a zero pivot. */
+int zrko_dense_getrf(int n, double* a, int* piv);
+void zrko_dense_getrs(int n, const double* a, const int* piv, double* b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,124 @@
+"""Writes the golden rate vectors under tests/golden/ (run in the build
+container, where /root/reference and oracle/_ref exist):
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+For each mechanism it evaluates the reference Zero-RK rate core
+(src/zerork + src/CKconverter compiled by oracle/Makefile into
+oracle/_ref/zrk_ref_probe, libc exp) on seeded states and stores inputs and
+outputs as .npz (no pickles).  Mechanism files used are copied next to the
+vectors: the synthetic ones are generated here (seeded), the others are the
+reference's own data files (data/mechanisms/hydrogen, dme, ideal).
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "zero-rk_amd"))
+from zerork_hip import mechgen  # noqa: E402
+
+REF = "/root/reference"
+PROBE = os.path.join(ROOT, "oracle", "_ref", "zrk_ref_probe")
+MECH_DIR = os.path.join(HERE, "mech")
+
+
+def run_probe(mech, therm, T, P, Y):
+    with tempfile.TemporaryDirectory() as td:
+        sp = os.path.join(td, "states.txt")
+        op = os.path.join(td, "out.txt")
+        with open(sp, "w") as f:
+            f.write(f"{len(T)} {Y.shape[1]}\n")
+            for i in range(len(T)):
+                f.write(f"{T[i]:.17g} {P[i]:.17g} " + " ".join(f"{v:.17g}" for v in Y[i]) + "\n")
+        subprocess.run([PROBE, mech, therm, sp, op], check=True, stdout=subprocess.DEVNULL)
+        lines = open(op).read().split("\n")
+    nsp, nrxn, nstep = map(int, lines[0].split())
+    mw = np.array(lines[1].split(), float)
+    step_rxn, reac, prod = [], [], []
+    for l in lines[2:2 + nstep]:
+        v = list(map(int, l.split()))
+        step_rxn.append(v[0])
+        nr = v[1]
+        reac.append(v[2:2 + nr])
+        np_ = v[2 + nr]
+        prod.append(v[3 + nr:3 + nr + np_])
+    names = ["cp_R", "h_RT", "g_RT", "conc", "kfwd", "krev", "rop", "create", "destroy", "net",
+             "rop_nolimit", "net_nolimit"]
+    out = {k: [] for k in names}
+    rest = lines[2 + nstep:]
+    i = 0
+    for _ in range(len(T)):
+        i += 1  # "T P" line
+        for k in names:
+            out[k].append(np.array(rest[i].split(), float))
+            i += 1
+    res = {k: np.array(v) for k, v in out.items()}
+    rp = np.zeros(nstep + 1, np.int32)
+    pp = np.zeros(nstep + 1, np.int32)
+    rp[1:] = np.cumsum([len(r) for r in reac])
+    pp[1:] = np.cumsum([len(p) for p in prod])
+    res.update(mw=mw, step_rxn=np.array(step_rxn, np.int32), reac_ptr=rp,
+               reac_sp=np.array([x for r in reac for x in r], np.int32), prod_ptr=pp,
+               prod_sp=np.array([x for p in prod for x in p], np.int32),
+               T=np.asarray(T, float), P=np.asarray(P, float), Y=np.asarray(Y, float))
+    return res
+
+
+def main():
+    if not os.path.exists(PROBE):
+        raise SystemExit("build the reference probe first: make -C oracle ref")
+    os.makedirs(MECH_DIR, exist_ok=True)
+    cases = []
+    for cfg in (1, 2, 3):
+        m = mechgen.config_mechanism(cfg)
+        mp, tp = m.write(MECH_DIR, f"synth_cfg{cfg}")
+        cases.append((f"synth_cfg{cfg}", mp, tp, m, cfg))
+    refdata = [
+        ("ref_h2", "data/mechanisms/hydrogen/h2_v1b_mech.txt",
+         "data/mechanisms/hydrogen/h2_v1a_therm.txt"),
+        ("ref_dme", "data/mechanisms/dme/dme_24_mech.txt", "data/mechanisms/dme/dme_24_therm.txt"),
+        ("ref_h2_no_falloff", "data/mechanisms/ideal/hydrogen_no_falloff.mech",
+         "data/mechanisms/hydrogen/h2_v1a_therm.txt"),
+        ("ref_troe_test", "data/mechanisms/ideal/troe_test.mech",
+         "data/mechanisms/ideal/const_specific_heat.therm"),
+    ]
+    for name, mrel, trel in refdata:
+        mp = os.path.join(MECH_DIR, name + ".mech")
+        tp = os.path.join(MECH_DIR, name + ".therm")
+        shutil.copyfile(os.path.join(REF, mrel), mp)
+        shutil.copyfile(os.path.join(REF, trel), tp)
+        os.chmod(mp, 0o644)
+        os.chmod(tp, 0o644)
+        cases.append((name, mp, tp, None, None))
+    sys.path.insert(0, ROOT)
+    from oracle import ckparse
+    for name, mp, tp, m, cfg in cases:
+        t = ckparse.build_tables(mp, tp)
+        nsp = t.nsp
+        seed = 4242 + sum(map(ord, name))
+        rng = np.random.default_rng(seed)
+        n = {"synth_cfg3": 6, "ref_dme": 6}.get(name, 16)
+        T = rng.uniform(700.0, 2400.0, n)
+        P = np.exp(rng.uniform(np.log(0.3), np.log(60.0), n)) * 101325.0
+        Y = rng.dirichlet(np.ones(nsp), n)
+        if cfg is not None:  # add states from the workload distribution (zeros present)
+            st = mechgen.config_states(cfg, m, 4, seed + 1)
+            T = np.concatenate([T, st.T])
+            P = np.concatenate([P, st.P])
+            Y = np.concatenate([Y, st.Y])
+        res = run_probe(mp, tp, T, P, Y)
+        res["seed"] = np.array(seed)
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **res)
+        print(f"{name}: nsp={nsp} nstep={len(res['step_rxn'])} states={len(T)}")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,132 @@
+// Per-reactor variable-order BDF (CVODE v5.8.0 algorithm) on one wavefront.
+//
+// Layout: lane j < nsp holds species j, lane nsp holds y_T = T/Tref (the
+// reference state order), lanes above are padding (identically zero, zero
+// error weight).  Every CVODE vector (Nordsieck history zn[0..5], ewt,
+// acor, ...) is one register per lane; CVODE's scalars are wave-uniform.
+//
+// Linear algebra: the analytic Jacobian
+// (applications/cfd_plugin/reactor_nvector_serial.cpp:610-771) is assembled
+// column by column into the wave's LDS and saved to HBM for reuse; the Newton
+// matrix M = I - gamma*J is inverted in registers by Gauss-Jordan elimination
+// without pivoting (row per lane, columns rotated one register per step so
+// every register index is a compile-time constant), and each Newton solve is
+// then an LDS matrix-vector product.  This is the explicit-inverse scheme of
+// the reference GPU path (interfaces/cublas_manager/cublas_manager.cu:73-91,
+// getrf/getri without pivoting + block-diagonal matvec); a zero pivot is a
+// recoverable setup failure, like a singular dense factorisation in CVODE.
+//
+// CVODE structure restated (names kept): cvStep, cvPredict, cvSet/cvSetBDF,
+// cvSetTqBDF, cvNls + SUNNonlinSol_Newton, cvLsSetup/cvLsLinSys, cvLsSolve
+// (with the 2/(1+gamrat) BDF correction), cvHandleNFlag, cvDoErrorTest,
+// cvCompleteStep, cvPrepareNextStep, cvChooseEta, cvSetEta, cvAdjustOrder,
+// cvRescale, cvRestore, cvHin, CVodeGetDky; the plugin's driver loop is
+// applications/cfd_plugin/solver_cvode.cpp:220-299.
+#pragma once
+
+#include <cfloat>
+
+#include "chem_device.h"
+
+namespace zrk {
+
+constexpr int kQmax = 5;
+
+// ---------------------------------------------------------------------------
+// Reference analytic Jacobian into LDS, column-major J[c*kNP + row], and a
+// copy to HBM (savedJ).  inv_rho_prev is the inverse density of the last RHS
+// call (at y), which the reference reuses for the 1/C factors
+// (reactor_nvector_serial.cpp:625-637).  Returns the inverse density of the
+// last RHS evaluated here (the reference's carried value).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s, double P,
+                                               const BdfOpts& o, double y, double fy,
+                                               double inv_rho_prev, double* __restrict__ Jl,
+                                               double* __restrict__ saved) {
+  const int lane = lane_id();
+  const int nsp = m.nsp;
+  const bool sp = lane < nsp;
+  const bool act = lane <= nsp;
+  const double mmf = o.min_mass_fraction;
+  double t1 = y;
+  if (sp && fabs(t1) < mmf) t1 = (t1 < 0.0) ? -mmf : mmf;
+  const double t2 = sp ? m.mw[lane] / t1 * inv_rho_prev : 0.0;
+  // d(ydot)/dT by forward difference at the unfloored state.  Evaluated
+  // first: it does not depend on the floored evaluation, and its inverse
+  // density is the one carried on (it is the reference's last call).
+  const double yT = bcast(y, nsp);
+  double dT = yT * sqrt(DBL_EPSILON);
+  const double Tp = yT + dT;
+  dT = Tp - yT;
+  const double t4in = (lane == nsp) ? Tp : (sp ? y : 0.0);
+  double t4;
+  RhsAux aux2;
+  rhs64(m, s, P, o.Tref, o.step_limiter, t4in, t4, aux2);
+  const double colT = act ? (t4 - fy) * (1.0 / dT) : 0.0;
+  // floored evaluation: rates of progress for every species term
+  double t3;
+  RhsAux aux;
+  rhs64(m, s, P, o.Tref, o.step_limiter, t1, t3, aux);
+  const double Tdot = bcast(t3, nsp);
+  const double RuTemp = kRu * yT;
+  const double mult = -1.0 / aux.cp_mix;
+  const double mwi = sp ? m.mw[lane] : 0.0;
+  const int* __restrict__ jp = m.jt_ptr;
+  const int* __restrict__ jt = m.jt_term;
+  for (int c = 0; c < nsp; ++c) {
+    const double t2c = bcast(t2, c);
+    double v = 0.0;
+    if (sp) {
+      const int b = jp[c * nsp + lane], e = jp[c * nsp + lane + 1];
+      for (int k = b; k < e; ++k) {
+        const int t = jt[k];
+        const double r = s.rop[(t < 0 ? -t : t) - 1];
+        v = (t < 0) ? fma(-r, t2c, v) : fma(r, t2c, v);
+      }
+    }
+    // d(Tdot)/dy_c from the concentration-space column
+    const double sc = wave_sum(sp ? v * aux.h : 0.0);
+    double val = sc * RuTemp;
+    val += Tdot * bcast(aux.cpm, c) * m.mw[c];
+    val *= m.inv_mw[c] * mult;
+    const double out = sp ? v * (mwi * m.inv_mw[c]) : (lane == nsp ? val : 0.0);
+    Jl[c * kNP + lane] = out;
+    saved[c * kNP + lane] = out;
+  }
+  Jl[nsp * kNP + lane] = colT;
+  saved[nsp * kNP + lane] = colT;
+  return aux2.inv_rho;
+}
+
+// Gauss-Jordan inverse (no pivoting) of the N x N leading block of a matrix
+// held one row per lane in R[0..63]; padding rows/columns must be identity.
+// After N steps register c holds logical column (c + N) mod 64 of the
+// inverse.  prow: kNP doubles of LDS for the published pivot row.
+// Returns false on a zero pivot.
+__device__ __forceinline__ bool gj_invert(double (&R)[kNP], int N, double* __restrict__ prow) {
+  const int lane = lane_id();
+  bool ok = true;
+  for (int k = 0; k < N; ++k) {
+    const double piv = bcast(R[0], k);
+    if (piv == 0.0) ok = false;
+    const double rp = 1.0 / piv;
+    if (lane == k) {
+#pragma unroll
+      for (int c = 1; c < kNP; ++c) R[c] *= rp;
+#pragma unroll
+      for (int c = 1; c < kNP; ++c) prow[c] = R[c];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const double l = (lane == k) ? 0.0 : R[0];
+    const double last = (lane == k) ? rp : -l * rp;
+#pragma unroll
+    for (int c = 1; c < kNP; ++c) R[c - 1] = fma(-l, prow[c], R[c]);
+    R[kNP - 1] = last;
+    __builtin_amdgcn_wave_barrier();
+  }
+  return ok;
+}
+
+}  // namespace zrk

@@ -1,0 +1,431 @@
+// The reference plugin API (include/zerork_cfd_plugin.h) over the HIP path.
+//
+// Option names, defaults and status codes follow
+// applications/cfd_plugin/zerork_reactor_manager.cpp:30-133 and
+// zerork_cfd_plugin.cpp:21-216.  A solve copies T/P/mf to HBM once, runs the
+// persistent BDF kernel over the whole batch (reactors handed out in
+// descending previous-cost order when sort_reactors is set, the reference's
+// cost sort of :522-524), copies back, and fills the aux outputs
+// (COST = CVode one-step calls, GPU = seconds per reactor of the batched
+// solve, TEMPERATURE_DELTA = T change above solve_temperature_threshold).
+// Reactors whose integration fails keep their input state and report a
+// negative cost; the call then returns ZERORK_STATUS_FAILED_SOLVE
+// (:1123-1139).  No C++ exception or exit() crosses the ABI.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/zerork_cfd_plugin.h"
+#include "zrk_internal.h"
+
+struct zerork_handle_impl {
+  std::map<std::string, int> iopt;
+  std::map<std::string, double> dopt;
+  std::map<std::string, std::string> sopt;
+  std::unique_ptr<zrk::MechHandle> mech;
+  bool tried_init = false;
+  double* dpdt = nullptr;
+  double* cost = nullptr;
+  double* gpu = nullptr;
+  double* y_src = nullptr;
+  double* e_src = nullptr;
+  double* ign = nullptr;
+  double* tdelta = nullptr;
+  zerork_callback_fn cb = nullptr;
+  void* cb_data = nullptr;
+  int* ids = nullptr;
+  hipStream_t stream = nullptr;
+  // device buffers
+  double* dT = nullptr;
+  double* dP = nullptr;
+  double* dY = nullptr;
+  int* dsteps = nullptr;
+  int* dorder = nullptr;
+  size_t cap_n = 0, cap_y = 0;
+
+  zerork_handle_impl() {
+    sopt["mech_filename"] = "mech.dat";
+    sopt["therm_filename"] = "therm.dat";
+    iopt["verbosity"] = 4;
+    iopt["sort_reactors"] = 1;
+    iopt["load_balance"] = 0;
+    iopt["load_balance_noise"] = 0;
+    iopt["load_balance_mem"] = 1;
+    iopt["reactor_weight_mult"] = 1;
+    iopt["dump_reactors"] = 0;
+    iopt["dump_failed_reactors"] = 0;
+    iopt["output_performance_log"] = 1;
+    iopt["max_steps"] = 5000;
+    iopt["dense"] = 0;
+    iopt["analytic"] = 1;
+    iopt["iterative"] = 1;
+    iopt["integrator"] = 0;
+    iopt["abstol_dens"] = 0;
+    iopt["cvode_num_retries"] = 5;
+    dopt["rel_tol"] = 1.0e-8;
+    dopt["abs_tol"] = 1.0e-20;
+    dopt["eps_lin"] = 1.0e-3;
+    dopt["nonlinear_convergence_coeff"] = 0.05;
+    dopt["preconditioner_threshold"] = 1.0e-3;
+    dopt["max_dt"] = 0.05;
+    dopt["cvode_retry_absolute_tolerance_adjustment"] = 0.1;
+    dopt["cvode_retry_relative_tolerance_adjustment"] = 1.0;
+    iopt["constant_volume"] = 1;
+    dopt["reference_temperature"] = 1.0;
+    dopt["delta_temperature_ignition"] = 0.0;
+    dopt["min_mass_fraction"] = 1.0e-30;
+    iopt["always_solve_temperature"] = 1;
+    dopt["solve_temperature_threshold"] = 2.0;
+    dopt["step_limiter"] = 1.0e22;
+    iopt["gpu"] = 1;
+    iopt["initial_gpu_multiplier"] = 8;
+    iopt["n_reactors_min"] = 128;
+    iopt["n_reactors_max"] = 1024;
+    sopt["reactor_timing_log_filename"] = "/dev/null";
+    sopt["mechanism_parsing_log_filename"] = "/dev/null";
+  }
+  ~zerork_handle_impl() {
+    if (dT) (void)hipFree(dT);
+    if (dP) (void)hipFree(dP);
+    if (dY) (void)hipFree(dY);
+    if (dsteps) (void)hipFree(dsteps);
+    if (dorder) (void)hipFree(dorder);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+zerork_status_t load_mechanism(zerork_handle h) {
+  auto m = std::make_unique<zrk::MechHandle>();
+  std::string err;
+  const std::string& mf = h->sopt["mech_filename"];
+  const std::string& tf = h->sopt["therm_filename"];
+  {
+    std::ifstream a(mf), b(tf);
+    if (!a || (!tf.empty() && !b)) return ZERORK_STATUS_INVALID_FILENAME;
+  }
+  if (!zrk::load_mechanism(mf, tf, &m->host, &err)) {
+    const std::string& logf = h->sopt["mechanism_parsing_log_filename"];
+    std::ofstream log(logf);
+    if (log) log << "mechanism parse failed: " << err << "\n";
+    return ZERORK_STATUS_FAILED_MECHANISM_PARSE;
+  }
+  h->mech = std::move(m);
+  return ZERORK_STATUS_SUCCESS;
+}
+
+// FinishInit (zerork_reactor_manager.cpp:573-651): lazy mechanism load, once.
+zerork_status_t finish_init(zerork_handle h) {
+  if (!h->tried_init) {
+    h->tried_init = true;
+    if (!h->mech) {
+      zerork_status_t f = load_mechanism(h);
+      if (f != ZERORK_STATUS_SUCCESS) return f;
+    }
+  } else if (!h->mech) {
+    return ZERORK_STATUS_FAILED_MECHANISM_PARSE;
+  }
+  return ZERORK_STATUS_SUCCESS;
+}
+
+template <class T>
+bool grow(T** p, size_t n) {
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  return hipMalloc((void**)p, sizeof(T) * std::max<size_t>(n, 1)) == hipSuccess;
+}
+
+std::string trim(const std::string& s) {
+  size_t a = s.find_first_not_of(" \t\r\n\"'");
+  if (a == std::string::npos) return "";
+  size_t b = s.find_last_not_of(" \t\r\n\"'");
+  return s.substr(a, b - a + 1);
+}
+
+// Flat "key: value" YAML as written for the reference's options file
+// (applications/cfd_plugin/generateZeroRKCFDPluginIFP.py keys).
+zerork_status_t read_options(zerork_handle h, const char* fn) {
+  std::ifstream f(fn);
+  if (!f) return ZERORK_STATUS_FAILED_OPTIONS_PARSE;
+  // YAML key -> internal option name where they differ
+  static const std::map<std::string, std::string> alias = {
+      {"absolute_tolerance", "abs_tol"},
+      {"relative_tolerance", "rel_tol"},
+      {"reactor_timing_log", "reactor_timing_log_filename"},
+      {"mechanism_parsing_log", "mechanism_parsing_log_filename"},
+      {"mech_file", "mech_filename"},
+      {"therm_file", "therm_filename"}};
+  std::string line;
+  while (std::getline(f, line)) {
+    auto hash = line.find('#');
+    if (hash != std::string::npos) line = line.substr(0, hash);
+    if (trim(line).empty() || trim(line) == "---" || trim(line) == "...") continue;
+    auto colon = line.find(':');
+    if (colon == std::string::npos) return ZERORK_STATUS_FAILED_OPTIONS_PARSE;
+    std::string key = trim(line.substr(0, colon));
+    std::string val = trim(line.substr(colon + 1));
+    auto it = alias.find(key);
+    if (it != alias.end()) key = it->second;
+    if (h->iopt.count(key)) {
+      char* end = nullptr;
+      long v = std::strtol(val.c_str(), &end, 10);
+      if (val.empty() || *end) {
+        if (val == "true" || val == "True") v = 1;
+        else if (val == "false" || val == "False") v = 0;
+        else return ZERORK_STATUS_FAILED_OPTIONS_PARSE;
+      }
+      h->iopt[key] = (int)v;
+    } else if (h->dopt.count(key)) {
+      char* end = nullptr;
+      double v = std::strtod(val.c_str(), &end);
+      if (val.empty() || *end) return ZERORK_STATUS_FAILED_OPTIONS_PARSE;
+      h->dopt[key] = v;
+    } else if (h->sopt.count(key)) {
+      h->sopt[key] = val;
+    } else {
+      // unknown keys are kept as strings, like an Optionable map entry
+      h->sopt[key] = val;
+    }
+  }
+  return ZERORK_STATUS_SUCCESS;
+}
+
+zerork_status_t solve(zerork_handle h, double dt, int n, double* T, double* P, double* mf) {
+  if (n < 0 || !T || !P || (n > 0 && !mf)) return ZERORK_STATUS_INVALID_POINTER;
+  if (n == 0) return ZERORK_STATUS_SUCCESS;
+  zrk::MechHandle* m = h->mech.get();
+  const int nsp = m->host.nsp;
+  const int stride = h->iopt.count("num_species_stride") ? h->iopt["num_species_stride"] : nsp;
+  if (stride < nsp) return ZERORK_STATUS_INVALID_POINTER;
+  if (h->iopt["constant_volume"] != 0) {
+    // the constant-volume reactor is not part of this build's HIP path
+    return ZERORK_STATUS_UNKNOWN_ERROR;
+  }
+  if (h->y_src || h->e_src || h->dpdt) {
+    bool any = false;
+    for (int k = 0; k < n && !any; ++k) any = (h->dpdt && h->dpdt[k] != 0.0) || (h->e_src && h->e_src[k] != 0.0);
+    if (h->y_src) any = true;
+    if (any) return ZERORK_STATUS_UNKNOWN_ERROR;  // source terms not on the HIP path
+  }
+  if (!h->stream && hipStreamCreate(&h->stream) != hipSuccess) return ZERORK_STATUS_UNKNOWN_ERROR;
+  if ((size_t)n > h->cap_n) {
+    if (!grow(&h->dT, n) || !grow(&h->dP, n) || !grow(&h->dsteps, n) || !grow(&h->dorder, n))
+      return ZERORK_STATUS_UNKNOWN_ERROR;
+    h->cap_n = n;
+  }
+  const size_t ny = (size_t)n * stride;
+  if (ny > h->cap_y) {
+    if (!grow(&h->dY, ny)) return ZERORK_STATUS_UNKNOWN_ERROR;
+    h->cap_y = ny;
+  }
+  std::vector<double> T0(T, T + n);
+  // work order: descending previous cost (LPT) when costs are supplied
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  const bool sorted = h->iopt["sort_reactors"] != 0 && h->cost != nullptr;
+  if (sorted)
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return h->cost[a] > h->cost[b]; });
+  hipStream_t st = h->stream;
+  bool ok = hipMemcpyAsync(h->dT, T, sizeof(double) * n, hipMemcpyHostToDevice, st) == hipSuccess;
+  ok = ok && hipMemcpyAsync(h->dP, P, sizeof(double) * n, hipMemcpyHostToDevice, st) == hipSuccess;
+  ok = ok && hipMemcpyAsync(h->dY, mf, sizeof(double) * ny, hipMemcpyHostToDevice, st) == hipSuccess;
+  if (sorted)
+    ok = ok && hipMemcpyAsync(h->dorder, order.data(), sizeof(int) * n, hipMemcpyHostToDevice,
+                              st) == hipSuccess;
+  if (!ok) return ZERORK_STATUS_UNKNOWN_ERROR;
+  zrk_bdf_options o;
+  zrk_bdf_default_options(&o);
+  o.rel_tol = h->dopt["rel_tol"];
+  o.abs_tol = h->dopt["abs_tol"];
+  o.max_dt = h->dopt["max_dt"];
+  o.nonlinear_convergence_coeff = h->dopt["nonlinear_convergence_coeff"];
+  o.step_limiter = h->dopt["step_limiter"];
+  o.reference_temperature = h->dopt["reference_temperature"];
+  o.min_mass_fraction = h->dopt["min_mass_fraction"];
+  o.cvode_retry_absolute_tolerance_adjustment = h->dopt["cvode_retry_absolute_tolerance_adjustment"];
+  o.cvode_retry_relative_tolerance_adjustment = h->dopt["cvode_retry_relative_tolerance_adjustment"];
+  o.max_steps = h->iopt["max_steps"];
+  o.cvode_num_retries = h->iopt["cvode_num_retries"];
+  const auto t0 = std::chrono::steady_clock::now();
+  int rc = zrk::integrate_device(m, o, n, dt, h->dT, h->dP, h->dY, stride, h->dsteps, nullptr,
+                                 sorted ? h->dorder : nullptr, st);
+  if (rc != 0) return ZERORK_STATUS_UNKNOWN_ERROR;
+  std::vector<int> steps(n);
+  ok = hipMemcpyAsync(T, h->dT, sizeof(double) * n, hipMemcpyDeviceToHost, st) == hipSuccess;
+  ok = ok && hipMemcpyAsync(mf, h->dY, sizeof(double) * ny, hipMemcpyDeviceToHost, st) == hipSuccess;
+  ok = ok && hipMemcpyAsync(steps.data(), h->dsteps, sizeof(int) * n, hipMemcpyDeviceToHost,
+                            st) == hipSuccess;
+  if (hipStreamSynchronize(st) != hipSuccess || !ok) return ZERORK_STATUS_UNKNOWN_ERROR;
+  const double secs =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  zerork_status_t flag = ZERORK_STATUS_SUCCESS;
+  const double thr = h->dopt["solve_temperature_threshold"];
+  for (int k = 0; k < n; ++k) {
+    if (steps[k] < 0) flag = ZERORK_STATUS_FAILED_SOLVE;
+    if (h->cost) h->cost[k] = steps[k];
+    if (h->gpu) h->gpu[k] = secs / n;
+    if (h->tdelta && steps[k] >= 0) {
+      double d = T[k] - T0[k];
+      h->tdelta[k] = d < thr ? 0.0 : d;
+    }
+  }
+  return flag;
+}
+
+}  // namespace
+
+extern "C" {
+
+zerork_handle zerork_reactor_init() {
+  try {
+    return new zerork_handle_impl();
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+zerork_status_t zerork_reactor_read_options_file(const char* options_filename, zerork_handle h) {
+  if (!h) return ZERORK_STATUS_INVALID_HANDLE;
+  if (!options_filename) return ZERORK_STATUS_INVALID_FILENAME;
+  try {
+    return read_options(h, options_filename);
+  } catch (...) {
+    return ZERORK_STATUS_FAILED_OPTIONS_PARSE;
+  }
+}
+
+zerork_status_t zerork_reactor_set_mechanism_files(const char* mech_file, const char* therm_file,
+                                                   zerork_handle h) {
+  if (!h) return ZERORK_STATUS_INVALID_HANDLE;
+  if (!mech_file || !therm_file) return ZERORK_STATUS_INVALID_FILENAME;
+  h->sopt["mech_filename"] = mech_file;
+  h->sopt["therm_filename"] = therm_file;
+  return ZERORK_STATUS_SUCCESS;
+}
+
+zerork_status_t zerork_reactor_load_mechanism(zerork_handle h) {
+  if (!h) return ZERORK_STATUS_INVALID_HANDLE;
+  try {
+    return load_mechanism(h);
+  } catch (...) {
+    return ZERORK_STATUS_UNKNOWN_ERROR;
+  }
+}
+
+zerork_status_t zerork_reactor_solve(const int n_cycle, const double time, const double dt,
+                                     const int n_reactors, double* T, double* P, double* mf,
+                                     zerork_handle h) {
+  (void)n_cycle;
+  (void)time;
+  if (!h) return ZERORK_STATUS_INVALID_HANDLE;
+  try {
+    zerork_status_t f = finish_init(h);
+    if (f != ZERORK_STATUS_SUCCESS) return f;
+    return solve(h, dt, n_reactors, T, P, mf);
+  } catch (...) {
+    return ZERORK_STATUS_UNKNOWN_ERROR;
+  }
+}
+
+zerork_status_t zerork_reactor_set_aux_field_pointer(zerork_field_t ft, double* p,
+                                                     zerork_handle h) {
+  if (!h) return ZERORK_STATUS_INVALID_HANDLE;
+  if (!p) return ZERORK_STATUS_INVALID_POINTER;
+  switch (ft) {
+    case ZERORK_FIELD_DPDT: h->dpdt = p; break;
+    case ZERORK_FIELD_COST: h->cost = p; break;
+    case ZERORK_FIELD_GPU: h->gpu = p; break;
+    case ZERORK_FIELD_Y_SRC: h->y_src = p; break;
+    case ZERORK_FIELD_E_SRC: h->e_src = p; break;
+    case ZERORK_FIELD_IGNITION_TIME: h->ign = p; break;
+    case ZERORK_FIELD_TEMPERATURE_DELTA: h->tdelta = p; break;
+    default: return ZERORK_STATUS_INVALID_FIELD_NAME;
+  }
+  return ZERORK_STATUS_SUCCESS;
+}
+
+zerork_status_t zerork_reactor_set_int_option(const char* name, int v, zerork_handle h) {
+  if (!h) return ZERORK_STATUS_INVALID_HANDLE;
+  if (!name) return ZERORK_STATUS_INVALID_POINTER;
+  h->iopt[name] = v;
+  return ZERORK_STATUS_SUCCESS;
+}
+
+zerork_status_t zerork_reactor_set_double_option(const char* name, double v, zerork_handle h) {
+  if (!h) return ZERORK_STATUS_INVALID_HANDLE;
+  if (!name) return ZERORK_STATUS_INVALID_POINTER;
+  h->dopt[name] = v;
+  return ZERORK_STATUS_SUCCESS;
+}
+
+zerork_status_t zerork_reactor_set_string_option(const char* name, const char* v,
+                                                 zerork_handle h) {
+  if (!h) return ZERORK_STATUS_INVALID_HANDLE;
+  if (!name || !v) return ZERORK_STATUS_INVALID_POINTER;
+  h->sopt[name] = v;
+  return ZERORK_STATUS_SUCCESS;
+}
+
+zerork_status_t zerork_reactor_get_int_option(const char* name, int* v, zerork_handle h) {
+  if (!h) return ZERORK_STATUS_INVALID_HANDLE;
+  if (!name || !v) return ZERORK_STATUS_INVALID_POINTER;
+  auto it = h->iopt.find(name);
+  if (it == h->iopt.end()) return ZERORK_STATUS_OPTION_NOT_DEFINED;
+  *v = it->second;
+  return ZERORK_STATUS_SUCCESS;
+}
+
+zerork_status_t zerork_reactor_get_double_option(const char* name, double* v, zerork_handle h) {
+  if (!h) return ZERORK_STATUS_INVALID_HANDLE;
+  if (!name || !v) return ZERORK_STATUS_INVALID_POINTER;
+  auto it = h->dopt.find(name);
+  if (it == h->dopt.end()) return ZERORK_STATUS_OPTION_NOT_DEFINED;
+  *v = it->second;
+  return ZERORK_STATUS_SUCCESS;
+}
+
+zerork_status_t zerork_reactor_get_string_option(const char* name, char** v, int len,
+                                                 zerork_handle h) {
+  if (!h) return ZERORK_STATUS_INVALID_HANDLE;
+  if (!name || !v || !*v) return ZERORK_STATUS_INVALID_POINTER;
+  auto it = h->sopt.find(name);
+  if (it == h->sopt.end()) return ZERORK_STATUS_OPTION_NOT_DEFINED;
+  std::strncpy(*v, it->second.c_str(), len);
+  return ZERORK_STATUS_SUCCESS;
+}
+
+zerork_status_t zerork_reactor_set_callback_fn(zerork_callback_fn fn, void* data, zerork_handle h) {
+  if (!h) return ZERORK_STATUS_INVALID_HANDLE;
+  if (!fn) return ZERORK_STATUS_INVALID_POINTER;
+  h->cb = fn;
+  h->cb_data = data;
+  return ZERORK_STATUS_SUCCESS;
+}
+
+zerork_status_t zerork_reactor_set_reactor_ids(int* ids, zerork_handle h) {
+  if (!h) return ZERORK_STATUS_INVALID_HANDLE;
+  if (!ids) return ZERORK_STATUS_INVALID_POINTER;
+  h->ids = ids;
+  return ZERORK_STATUS_SUCCESS;
+}
+
+zerork_status_t zerork_reactor_free(zerork_handle h) {
+  if (!h) return ZERORK_STATUS_INVALID_HANDLE;
+  delete h;
+  return ZERORK_STATUS_SUCCESS;
+}
+
+}  // extern "C"

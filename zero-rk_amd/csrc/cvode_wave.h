@@ -1,0 +1,709 @@
+// CVODE v5.8.0 BDF restated for one reactor per wavefront (see bdf_device.h).
+//
+// Small CVODE arrays (zn, tau, tq, l) are indexed only with compile-time
+// indices inside unrolled loops guarded by the wave-uniform order q, so they
+// stay in registers.  Return codes follow CVODE's.
+#pragma once
+
+#include "bdf_device.h"
+
+namespace zrk {
+
+// CVODE constants (cvode.c / cvode_impl.h / cvode_ls_impl.h / sunnonlinsol_newton)
+constexpr double kHlbFactor = 100.0, kHubFactor = 0.1, kHBias = 0.5;
+constexpr int kMaxIters = 4;
+constexpr double kThresh = 1.5, kEtamx1 = 10000.0, kEtamx2 = 10.0, kEtamx3 = 10.0;
+constexpr double kEtamxf = 0.2, kEtamin = 0.1, kEtacf = 0.25, kAddon = 1.0e-6;
+constexpr double kBias1 = 6.0, kBias2 = 6.0, kBias3 = 10.0, kOnepsm = 1.000001;
+constexpr int kSmallNst = 10, kMxncf = 10, kMxnef = 7, kMxnef1 = 3, kSmallNef = 2, kLongWait = 10;
+constexpr int kNlsMaxcor = 3;
+constexpr double kCrdown = 0.3, kDgmaxLsetup = 0.3, kRdiv = 2.0, kCvlsDgmax = 0.2;
+constexpr int kMsbp = 20, kCvlsMsbj = 51;
+
+enum { kFirstCall = 0, kPrevConvFail, kPrevErrFail };
+enum { kNoFailures = 0, kFailBadJ, kFailOther };
+enum { kDoErrorTest = 100, kPredictAgain, kTryAgain };
+enum { kNlsSuccess = 0, kNlsContinue = 901, kNlsConvRecvr = 902, kRhsRecvr = 903 };
+enum {
+  kCvSuccess = 0, kCvTooMuchWork = -1, kCvTooMuchAcc = -2, kCvErrFailure = -3,
+  kCvConvFailure = -4, kCvReptdRhsErr = -10, kCvIllInput = -22
+};
+
+struct CvWave {
+  DevMech mech;
+  RhsLds s;
+  BdfOpts opt;
+  double* savedJ;  // this wave's NP x NP column-major scratch in HBM
+  double* Mlds;    // LDS, NP x NP column-major: Jacobian workspace, then the
+                   // rotated inverse of the Newton matrix
+  double* prow;    // LDS [NP]: published pivot row
+  double* rvec;    // LDS [NP]: right-hand side of a solve
+  double P;
+  // lane vectors
+  double zn[kQmax + 1];
+  double ewt, acor, tempv, ftemp, y, atolv, delta;
+  bool active;  // species lane or temperature lane
+  // uniform scalars
+  double rtol, tn, h, hscale, hprime, eta, etamax, hmin, hmax_inv, hu;
+  double tau[kQmax + 2], tq[6], l[kQmax + 1];
+  double rl1, gamma, gammap, gamrat, crate, delp, acnrm, nlscoef, saved_tq5;
+  double etaq, etaqm1, etaqp1, inv_rho_last;
+  int q, qprime, qwait, L;
+  int nst, nstlp, nstlj, nfe, nje, nsetups, nni, ncfn, netf;
+  int convfail, jcur, acnrmcur;
+  double inv_nvar;
+
+  __device__ __forceinline__ double wrms(double x) const {
+    const double p = active ? x * ewt : 0.0;
+    return sqrt(wave_sum(p * p) * inv_nvar);
+  }
+
+  __device__ __forceinline__ int f(double yy, double& yd) {
+    nfe++;
+    RhsAux aux;
+    const int fl = rhs64(mech, s, P, opt.Tref, opt.step_limiter, yy, yd, aux);
+    if (!fl) inv_rho_last = aux.inv_rho;
+    if (!active) yd = 0.0;
+    return fl;
+  }
+
+  __device__ __forceinline__ int set_ewt(double yy) {
+    const double v = rtol * fabs(yy) + atolv;
+    const double bad = wave_max((active && v <= 0.0) ? 1.0 : 0.0);
+    ewt = active ? 1.0 / v : 0.0;
+    return bad > 0.0 ? -1 : 0;
+  }
+
+  __device__ __forceinline__ void reinit(double t0, double y0) {
+    zn[0] = y0;
+#pragma unroll
+    for (int j = 1; j <= kQmax; ++j) zn[j] = 0.0;
+    tn = t0;
+    q = 1;
+    L = 2;
+    qwait = L;
+    etamax = kEtamx1;
+    hu = 0.0;
+    nst = nfe = ncfn = netf = nni = nsetups = 0;
+    nstlp = nstlj = nje = 0;
+    hmin = 0.0;
+    saved_tq5 = 0.0;
+    crate = 1.0;
+    delp = 0.0;
+    gammap = 0.0;
+    gamrat = 1.0;
+    jcur = 0;
+#pragma unroll
+    for (int i = 0; i < kQmax + 2; ++i) tau[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) tq[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i <= kQmax; ++i) l[i] = 0.0;
+  }
+
+  // cvUpperBoundH0
+  __device__ __forceinline__ double upper_bound_h0(double tdist) {
+    const double t1 = kHubFactor * fabs(zn[0]) + (active ? 1.0 / ewt : 1.0);
+    const double t2 = active ? fabs(zn[1]) / t1 : 0.0;
+    const double hub_inv = wave_max(t2);
+    double hub = kHubFactor * tdist;
+    if (hub * hub_inv > 1.0) hub = 1.0 / hub_inv;
+    return hub;
+  }
+
+  // cvYddNorm
+  __device__ __forceinline__ int ydd_norm(double hg, double& yddnrm) {
+    y = hg * zn[1] + zn[0];
+    double fy;
+    if (f(y, fy) > 0) return 1;
+    tempv = (fy - zn[1]) * (1.0 / hg);
+    yddnrm = wrms(tempv);
+    return 0;
+  }
+
+  // cvHin
+  __device__ __forceinline__ int hin(double tout) {
+    const double tdiff = tout - tn;
+    if (tdiff == 0.0) return -1;
+    const double tdist = fabs(tdiff);
+    const double tround = DBL_EPSILON * fmax(fabs(tn), fabs(tout));
+    if (tdist < 2.0 * tround) return -1;
+    const double hlb = kHlbFactor * tround;
+    const double hub = upper_bound_h0(tdist);
+    double hg = sqrt(hlb * hub);
+    if (hub < hlb) {
+      h = hg;
+      return 0;
+    }
+    double hnew = hg, hs = hg, yddnrm = 0.0;
+    for (int count1 = 1; count1 <= kMaxIters; ++count1) {
+      bool hgOK = false;
+      for (int count2 = 1; count2 <= kMaxIters; ++count2) {
+        if (ydd_norm(hg, yddnrm) == 0) {
+          hgOK = true;
+          break;
+        }
+        hg *= 0.2;
+      }
+      if (!hgOK) {
+        if (count1 <= 2) return -2;
+        hnew = hs;
+        break;
+      }
+      hs = hg;
+      hnew = (yddnrm * hub * hub > 2.0) ? sqrt(2.0 / yddnrm) : sqrt(hg * hub);
+      if (count1 == kMaxIters) break;
+      const double hrat = hnew / hg;
+      if (hrat > 0.5 && hrat < 2.0) break;
+      if (count1 > 1 && hrat > 2.0) {
+        hnew = hg;
+        break;
+      }
+      hg = hnew;
+    }
+    double h0 = kHBias * hnew;
+    if (h0 < hlb) h0 = hlb;
+    if (h0 > hub) h0 = hub;
+    h = h0;
+    return 0;
+  }
+
+  // cvRescale
+  __device__ __forceinline__ void rescale() {
+    double fct = eta;
+#pragma unroll
+    for (int j = 1; j <= kQmax; ++j) {
+      if (j <= q) {
+        zn[j] *= fct;
+        fct *= eta;
+      }
+    }
+    h = hscale * eta;
+    hscale = h;
+  }
+
+  // cvIncreaseBDF
+  __device__ __forceinline__ void increase_bdf() {
+#pragma unroll
+    for (int i = 0; i <= kQmax; ++i) l[i] = 0.0;
+    l[2] = 1.0;
+    double alpha1 = 1.0, prod = 1.0, xiold = 1.0, alpha0 = -1.0;
+    double hsum = hscale;
+#pragma unroll
+    for (int j = 1; j < kQmax; ++j) {
+      if (j < q) {
+        hsum += tau[j + 1];
+        const double xi = hsum / hscale;
+        prod *= xi;
+        alpha0 -= 1.0 / (j + 1);
+        alpha1 += 1.0 / xi;
+#pragma unroll
+        for (int i = j + 2; i >= 2; --i)
+          if (i <= kQmax) l[i] = l[i] * xiold + l[i - 1];
+        xiold = xi;
+      }
+    }
+    const double A1 = (-alpha0 - alpha1) / prod;
+    const double zL = A1 * zn[kQmax];  // zn[indx_acor], indx_acor == qmax
+#pragma unroll
+    for (int j = 2; j <= kQmax; ++j)
+      if (j <= q) zn[j] += l[j] * zL;
+    // zn[L] = zL with L = q + 1 (q < qmax here)
+#pragma unroll
+    for (int j = 2; j <= kQmax; ++j)
+      if (j == q + 1) zn[j] = zL;
+  }
+
+  // cvDecreaseBDF
+  __device__ __forceinline__ void decrease_bdf() {
+#pragma unroll
+    for (int i = 0; i <= kQmax; ++i) l[i] = 0.0;
+    l[2] = 1.0;
+    double hsum = 0.0;
+#pragma unroll
+    for (int j = 1; j <= kQmax - 2; ++j) {
+      if (j <= q - 2) {
+        hsum += tau[j];
+        const double xi = hsum / hscale;
+#pragma unroll
+        for (int i = j + 2; i >= 2; --i) l[i] = l[i] * xi + l[i - 1];
+      }
+    }
+    double zq = 0.0;
+#pragma unroll
+    for (int j = 0; j <= kQmax; ++j)
+      if (j == q) zq = zn[j];
+#pragma unroll
+    for (int j = 2; j < kQmax; ++j)
+      if (j < q) zn[j] = -l[j] * zq + zn[j];
+  }
+
+  __device__ __forceinline__ void adjust_order(int dq) {
+    if (q == 2 && dq != 1) return;
+    if (dq == 1) increase_bdf();
+    else decrease_bdf();
+  }
+
+  __device__ __forceinline__ void adjust_params() {
+    if (qprime != q) {
+      adjust_order(qprime - q);
+      q = qprime;
+      L = q + 1;
+      qwait = L;
+    }
+    rescale();
+  }
+
+  // cvPredict
+  __device__ __forceinline__ void predict() {
+    tn += h;
+#pragma unroll
+    for (int k = 1; k <= kQmax; ++k)
+#pragma unroll
+      for (int j = kQmax; j >= k; --j)
+        if (k <= q && j <= q) zn[j - 1] += zn[j];
+  }
+
+  // cvRestore
+  __device__ __forceinline__ void restore(double saved_t) {
+    tn = saved_t;
+#pragma unroll
+    for (int k = 1; k <= kQmax; ++k)
+#pragma unroll
+      for (int j = kQmax; j >= k; --j)
+        if (k <= q && j <= q) zn[j - 1] -= zn[j];
+  }
+
+  // cvSetBDF + cvSetTqBDF + gamma bookkeeping (cvSet)
+  __device__ __forceinline__ void set() {
+    double alpha0, alpha0_hat, xi_inv, xistar_inv, hsum;
+    l[0] = l[1] = xi_inv = xistar_inv = 1.0;
+#pragma unroll
+    for (int i = 2; i <= kQmax; ++i) l[i] = 0.0;
+    alpha0 = alpha0_hat = -1.0;
+    hsum = h;
+    if (q > 1) {
+#pragma unroll
+      for (int j = 2; j < kQmax; ++j) {
+        if (j < q) {
+          hsum += tau[j - 1];
+          xi_inv = h / hsum;
+          alpha0 -= 1.0 / j;
+#pragma unroll
+          for (int i = j; i >= 1; --i) l[i] += l[i - 1] * xi_inv;
+        }
+      }
+      alpha0 -= 1.0 / q;
+      xistar_inv = -l[1] - alpha0;
+      double tq1m = 0.0;
+#pragma unroll
+      for (int j = 1; j <= kQmax; ++j)
+        if (j == q - 1) tq1m = tau[j];
+      hsum += tq1m;
+      xi_inv = h / hsum;
+      alpha0_hat = -l[1] - xi_inv;
+#pragma unroll
+      for (int i = kQmax; i >= 1; --i)
+        if (i <= q) l[i] += l[i - 1] * xistar_inv;
+    }
+    // cvSetTqBDF
+    double lq = 0.0, tauq = 0.0;
+#pragma unroll
+    for (int j = 1; j <= kQmax; ++j)
+      if (j == q) {
+        lq = l[j];
+        tauq = tau[j];
+      }
+    const double A1 = 1.0 - alpha0_hat + alpha0;
+    const double A2 = 1.0 + q * A1;
+    tq[2] = fabs(A1 / (alpha0 * A2));
+    tq[5] = fabs(A2 * xistar_inv / (lq * xi_inv));
+    if (qwait == 1) {
+      if (q > 1) {
+        const double C = xistar_inv / lq;
+        const double A3 = alpha0 + 1.0 / q;
+        const double A4 = alpha0_hat + xi_inv;
+        const double Cpinv = (1.0 - A4 + A3) / A3;
+        tq[1] = fabs(C * Cpinv);
+      } else {
+        tq[1] = 1.0;
+      }
+      hsum += tauq;
+      xi_inv = h / hsum;
+      const double A5 = alpha0 - (1.0 / (q + 1));
+      const double A6 = alpha0_hat - xi_inv;
+      const double Cppinv = (1.0 - A6 + A5) / A2;
+      tq[3] = fabs(Cppinv / (xi_inv * (q + 2) * A5));
+    }
+    tq[4] = nlscoef / tq[2];
+    rl1 = 1.0 / l[1];
+    gamma = h * rl1;
+    if (nst == 0) gammap = gamma;
+    gamrat = (nst > 0) ? gamma / gammap : 1.0;
+  }
+
+  // cvLsSetup/cvLsLinSys + inversion of M = I - gamma*J; >0 recoverable failure
+  __device__ __forceinline__ int lsetup(int cf, double ypred, double fpred, int& jc) {
+    const int nsp = mech.nsp;
+    const int lane = lane_id();
+    const double dgamma = fabs(gamma / gammap - 1.0);
+    const bool jbad = (nst == 0) || (nst >= nstlj + kCvlsMsbj) ||
+                      ((cf == kFailBadJ) && (dgamma < kCvlsDgmax)) || (cf == kFailOther);
+    double R[kNP];
+    if (!jbad) {
+      jc = 0;
+#pragma unroll
+      for (int c = 0; c < kNP; ++c) R[c] = savedJ[c * kNP + lane];
+    } else {
+      jc = 1;
+      nje++;
+      nstlj = nst;
+      inv_rho_last = jacobian_lds(mech, s, P, opt, ypred, fpred, inv_rho_last, Mlds, savedJ);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (int c = 0; c < kNP; ++c) R[c] = Mlds[c * kNP + lane];
+    }
+    const double ng = -gamma;
+#pragma unroll
+    for (int c = 0; c < kNP; ++c) {
+      double v = (c <= nsp && lane <= nsp) ? ng * R[c] : 0.0;
+      if (c == lane) v += 1.0;
+      R[c] = v;
+    }
+    const bool ok = gj_invert(R, nsp + 1, prow);
+#pragma unroll
+    for (int c = 0; c < kNP; ++c) Mlds[c * kNP + lane] = R[c];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    return ok ? 0 : 1;
+  }
+
+  // x = M^{-1} b with the rotated inverse in LDS
+  __device__ __forceinline__ double lsolve(double b) const {
+    const int lane = lane_id();
+    const int N = mech.nsp + 1;
+    rvec[lane] = b;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    double acc = 0.0;
+#pragma unroll 8
+    for (int c = 0; c < kNP; ++c) acc = fma(Mlds[c * kNP + lane], rvec[(c + N) & (kNP - 1)], acc);
+    __builtin_amdgcn_wave_barrier();
+    return acc;
+  }
+
+  // cvNlsLSetup
+  __device__ __forceinline__ int nls_lsetup(bool jbad, int& jc) {
+    if (jbad) convfail = kFailBadJ;
+    const int rv = lsetup(convfail, y, ftemp, jc);
+    nsetups++;
+    gamrat = 1.0;
+    gammap = gamma;
+    crate = 1.0;
+    nstlp = nst;
+    return rv > 0 ? kNlsConvRecvr : kNlsSuccess;
+  }
+
+  // cvNlsResidual
+  __device__ __forceinline__ int nls_residual(double ycor, double& res) {
+    y = zn[0] + ycor;
+    if (f(y, ftemp) > 0) return kRhsRecvr;
+    res = rl1 * zn[1] + ycor;
+    res = -gamma * ftemp + res;
+    return kNlsSuccess;
+  }
+
+  // cvNlsConvTest
+  __device__ __forceinline__ int nls_convtest(int mi, double ycor, double dlt, double tol) {
+    const double del = wrms(dlt);
+    if (mi > 0) crate = fmax(kCrdown * crate, del / delp);
+    const double dcon = del * fmin(1.0, crate) / tol;
+    if (dcon <= 1.0) {
+      acnrm = (mi == 0) ? del : wrms(ycor);
+      acnrmcur = 1;
+      return kNlsSuccess;
+    }
+    if (mi >= 1 && del > kRdiv * delp) return kNlsConvRecvr;
+    delp = del;
+    return kNlsContinue;
+  }
+
+  // SUNNonlinSolSolve_Newton
+  __device__ __forceinline__ int newton(double& ycor, double tol, bool callLSetup) {
+    bool jbad = false;
+    int jc = 0, rv;
+    for (;;) {
+      rv = nls_residual(ycor, delta);
+      if (rv == kNlsSuccess && callLSetup) {
+        rv = nls_lsetup(jbad, jc);
+        jcur = jc;
+      }
+      if (rv == kNlsSuccess) {
+        int curiter = 0;
+        for (;;) {
+          nni++;
+          double b = -delta;
+          b = lsolve(b);
+          if (gamrat != 1.0) b *= 2.0 / (1.0 + gamrat);
+          if (!active) b = 0.0;
+          delta = b;
+          ycor += delta;
+          rv = nls_convtest(curiter, ycor, delta, tol);
+          if (rv == kNlsSuccess) {
+            jcur = 0;
+            return kNlsSuccess;
+          }
+          if (rv != kNlsContinue) break;
+          curiter++;
+          if (curiter >= kNlsMaxcor) {
+            rv = kNlsConvRecvr;
+            break;
+          }
+          rv = nls_residual(ycor, delta);
+          if (rv != kNlsSuccess) break;
+        }
+      }
+      if (rv > 0 && !jcur) {
+        callLSetup = true;
+        jbad = true;
+        ycor = 0.0;
+        continue;
+      }
+      return rv;
+    }
+  }
+
+  // cvNls
+  __device__ __forceinline__ int nls(int nflag) {
+    convfail = (nflag == kFirstCall || nflag == kPrevErrFail) ? kNoFailures : kFailOther;
+    const bool callSetup = (nflag == kPrevConvFail) || (nflag == kPrevErrFail) || (nst == 0) ||
+                           (nst >= nstlp + kMsbp) || (fabs(gamrat - 1.0) > kDgmaxLsetup);
+    acor = 0.0;
+    acnrmcur = 0;
+    const int rv = newton(acor, tq[4], callSetup);
+    if (rv != kNlsSuccess) return rv;
+    y = zn[0] + acor;
+    if (!acnrmcur) acnrm = wrms(acor);
+    jcur = 0;
+    return kCvSuccess;
+  }
+
+  __device__ __forceinline__ int handle_nflag(int& nflag, double saved_t, int& ncf) {
+    if (nflag == kCvSuccess) return kDoErrorTest;
+    ncfn++;
+    restore(saved_t);
+    ncf++;
+    etamax = 1.0;
+    if (fabs(h) <= hmin * kOnepsm || ncf == kMxncf) {
+      if (nflag == kNlsConvRecvr) return kCvConvFailure;
+      return kCvReptdRhsErr;
+    }
+    eta = fmax(kEtacf, hmin / fabs(h));
+    nflag = kPrevConvFail;
+    rescale();
+    return kPredictAgain;
+  }
+
+  __device__ __forceinline__ int do_error_test(int& nflag, double saved_t, int& nef, double& dsm) {
+    dsm = acnrm * tq[2];
+    if (dsm <= 1.0) return kCvSuccess;
+    nef++;
+    netf++;
+    nflag = kPrevErrFail;
+    restore(saved_t);
+    if (fabs(h) <= hmin * kOnepsm || nef == kMxnef) return kCvErrFailure;
+    etamax = 1.0;
+    if (nef <= kMxnef1) {
+      eta = 1.0 / (pow(kBias2 * dsm, 1.0 / L) + kAddon);
+      eta = fmax(kEtamin, fmax(eta, hmin / fabs(h)));
+      if (nef >= kSmallNef) eta = fmin(eta, kEtamxf);
+      rescale();
+      return kTryAgain;
+    }
+    if (q > 1) {
+      eta = fmax(kEtamin, hmin / fabs(h));
+      adjust_order(-1);
+      L = q;
+      q--;
+      qwait = L;
+      rescale();
+      return kTryAgain;
+    }
+    eta = fmax(kEtamin, hmin / fabs(h));
+    h *= eta;
+    hscale = h;
+    qwait = kLongWait;
+    double fz;
+    if (f(zn[0], fz) > 0) return kCvReptdRhsErr;
+    zn[1] = h * fz;
+    return kTryAgain;
+  }
+
+  __device__ __forceinline__ void complete_step() {
+    nst++;
+    hu = h;
+#pragma unroll
+    for (int i = kQmax; i >= 2; --i)
+      if (i <= q) tau[i] = tau[i - 1];
+    if (q == 1 && nst > 1) tau[2] = tau[1];
+    tau[1] = h;
+#pragma unroll
+    for (int j = 0; j <= kQmax; ++j)
+      if (j <= q) zn[j] += l[j] * acor;
+    qwait--;
+    if (qwait == 1 && q != kQmax) {
+      zn[kQmax] = acor;
+      saved_tq5 = tq[5];
+    }
+  }
+
+  __device__ __forceinline__ void set_eta() {
+    if (eta < kThresh) {
+      eta = 1.0;
+      hprime = h;
+    } else {
+      eta = fmin(eta, etamax);
+      eta /= fmax(1.0, fabs(h) * hmax_inv * eta);
+      hprime = h * eta;
+    }
+  }
+
+  __device__ __forceinline__ double comp_etaqm1() {
+    etaqm1 = 0.0;
+    if (q > 1) {
+      double zq = 0.0;
+#pragma unroll
+      for (int j = 0; j <= kQmax; ++j)
+        if (j == q) zq = zn[j];
+      const double ddn = wrms(zq) * tq[1];
+      etaqm1 = 1.0 / (pow(kBias1 * ddn, 1.0 / q) + kAddon);
+    }
+    return etaqm1;
+  }
+
+  __device__ __forceinline__ double comp_etaqp1() {
+    etaqp1 = 0.0;
+    if (q != kQmax) {
+      if (saved_tq5 == 0.0) return etaqp1;
+      double hr = 1.0;  // SUNRpowerI(h/tau[2], L): repeated product
+      const double base = h / tau[2];
+      for (int i = 0; i < L; ++i) hr *= base;
+      const double cquot = (tq[5] / saved_tq5) * hr;
+      tempv = -cquot * zn[kQmax] + acor;
+      const double dup = wrms(tempv) * tq[3];
+      etaqp1 = 1.0 / (pow(kBias3 * dup, 1.0 / (L + 1)) + kAddon);
+    }
+    return etaqp1;
+  }
+
+  __device__ __forceinline__ void choose_eta() {
+    const double etam = fmax(etaqm1, fmax(etaq, etaqp1));
+    if (etam < kThresh) {
+      eta = 1.0;
+      qprime = q;
+      return;
+    }
+    if (etam == etaqm1) {
+      eta = etaqm1;
+      qprime = q - 1;
+    } else if (etam == etaq) {
+      eta = etaq;
+      qprime = q;
+    } else {
+      eta = etaqp1;
+      qprime = q + 1;
+      zn[kQmax] = acor;
+    }
+  }
+
+  __device__ __forceinline__ void prepare_next_step(double dsm) {
+    if (etamax == 1.0) {
+      qwait = qwait > 2 ? qwait : 2;
+      qprime = q;
+      hprime = h;
+      eta = 1.0;
+      return;
+    }
+    etaq = 1.0 / (pow(kBias2 * dsm, 1.0 / L) + kAddon);
+    if (qwait != 0) {
+      eta = etaq;
+      qprime = q;
+      set_eta();
+      return;
+    }
+    qwait = 2;
+    etaqm1 = comp_etaqm1();
+    etaqp1 = comp_etaqp1();
+    choose_eta();
+    set_eta();
+  }
+
+  // cvStep
+  __device__ __forceinline__ int step() {
+    const double saved_t = tn;
+    int ncf = 0, nef = 0, nflag = kFirstCall;
+    double dsm = 0.0;
+    if (nst > 0 && hprime != h) adjust_params();
+    for (;;) {
+      predict();
+      set();
+      nflag = nls(nflag);
+      const int kflag = handle_nflag(nflag, saved_t, ncf);
+      if (kflag == kPredictAgain) continue;
+      if (kflag != kDoErrorTest) return kflag;
+      const int eflag = do_error_test(nflag, saved_t, nef, dsm);
+      if (eflag == kTryAgain) continue;
+      if (eflag != kCvSuccess) return eflag;
+      break;
+    }
+    complete_step();
+    prepare_next_step(dsm);
+    etamax = (nst <= kSmallNst) ? kEtamx2 : kEtamx3;
+    acor *= tq[2];
+    return kCvSuccess;
+  }
+
+  // CVode(..., CV_ONE_STEP)
+  __device__ __forceinline__ int one_step(double tout, double& tret) {
+    if (nst == 0) {
+      if (set_ewt(zn[0]) != 0) return kCvIllInput;
+      double f0;
+      if (f(zn[0], f0) > 0) return kCvReptdRhsErr;
+      zn[1] = f0;
+      if (hin(tout) != 0) return kCvIllInput;
+      const double rh = fabs(h) * hmax_inv;
+      if (rh > 1.0) h /= rh;
+      hscale = h;
+      hprime = h;
+      zn[1] *= h;
+    } else {
+      if (set_ewt(zn[0]) != 0) return kCvIllInput;
+    }
+    const double tolsf = DBL_EPSILON * wrms(zn[0]);
+    if (tolsf > 1.0) return kCvTooMuchAcc;
+    const int kflag = step();
+    tret = tn;
+    return kflag;
+  }
+
+  // CVodeGetDky(t, 0)
+  __device__ __forceinline__ double dky0(double t) const {
+    const double sv = (t - tn) / h;
+    double out = 0.0;
+#pragma unroll
+    for (int j = kQmax; j >= 0; --j) {
+      if (j <= q) {
+        double c = 1.0;
+#pragma unroll
+        for (int i = 0; i < j; ++i) c *= sv;
+        out += c * zn[j];
+      }
+    }
+    return out;
+  }
+};
+
+}  // namespace zrk

@@ -1,0 +1,259 @@
+// gfx950 kernels of the batched reactor-chemistry path and their launchers.
+//
+//   rates_kernel     FP64 rates of progress + species production + const-P RHS
+//                    for a batch of reactor states (any mechanism size):
+//                    one wavefront per reactor, 4 per workgroup.
+//   jacobian_kernel  reference analytic Jacobian for nsp <= 63 (parity tests).
+//   bdf_kernel       persistent per-reactor BDF integrator (nsp <= 63): each
+//                    wavefront takes reactors from a device work counter until
+//                    the batch is exhausted.
+#include <hip/hip_runtime.h>
+
+#include "cvode_wave.h"
+#include "launch.h"
+
+namespace zrk {
+
+constexpr int kRatesWaves = 4;  // wavefronts per workgroup in rates_kernel
+
+// ---------------------------------------------------------------------------
+// rates_kernel: inputs T[n], P[n], Y[n*stride] (reactor-major AoS, as the
+// plugin API); outputs (each optional): ydot[n*(nsp+1)] ([Y.., T] order),
+// rop[n*nstep], create/destroy/net[n*nsp].
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kWave* kRatesWaves)
+rates_kernel(DevMech m, int n, const double* __restrict__ Tin, const double* __restrict__ Pin,
+             const double* __restrict__ Yin, int stride, double L, double* __restrict__ ydot,
+             double* __restrict__ rop_out, double* __restrict__ create_out,
+             double* __restrict__ destroy_out, double* __restrict__ net_out) {
+  extern __shared__ double lds[];
+  const int wave = threadIdx.x / kWave;
+  const int lane = lane_id();
+  const int nsp = m.nsp, nstep = m.nstep;
+  const int per_wave = 3 * nsp + nstep;
+  double* C = lds + wave * per_wave;
+  double* G = C + nsp;
+  double* H = G + nsp;
+  double* rop = H + nsp;
+  for (int r = blockIdx.x * kRatesWaves + wave; r < n; r += gridDim.x * kRatesWaves) {
+    const double* Y = Yin + (size_t)r * stride;
+    const double T = fmin(Tin[r], kTlimit);
+    const double P = Pin[r];
+    const TPow p = make_tpow(T);
+    double part = 0.0;
+    for (int j = lane; j < nsp; j += kWave) part += Y[j] * m.inv_mw[j];
+    const double wmix = 1.0 / wave_sum(part);
+    const double inv_rho = 1.0 / (P * wmix / (kRu * T));
+    const double rho_c = 1.0 / inv_rho;
+    double cps = 0.0, cs = 0.0;
+    for (int j = lane; j < nsp; j += kWave) {
+      double cp, h, g;
+      thermo_one(m.thermo + 16 * j, p, cp, h, g);
+      const double Cj = rho_c * m.inv_mw[j] * Y[j];
+      C[j] = Cj;
+      G[j] = g;
+      H[j] = h;
+      cps += cp * m.ru_mw[j] * Y[j];
+      cs += Cj;
+    }
+    const double cp_mix = wave_sum(cps);
+    const double Csum = wave_sum(cs);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    wave_rates(m, C, G, rop, p, Csum, L);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    double e = 0.0;
+    for (int j = lane; j < nsp; j += kWave) {
+      double cr, de;
+      species_cd(m, rop, j, cr, de);
+      const double net = cr - de;
+      if (create_out) create_out[(size_t)r * nsp + j] = cr;
+      if (destroy_out) destroy_out[(size_t)r * nsp + j] = de;
+      if (net_out) net_out[(size_t)r * nsp + j] = net;
+      if (ydot) ydot[(size_t)r * (nsp + 1) + j] = (net * m.mw[j]) * inv_rho;
+      e += H[j] * net * inv_rho;
+    }
+    const double esum = wave_sum(e);
+    if (ydot && lane == 0)
+      ydot[(size_t)r * (nsp + 1) + nsp] = -esum * kRu * Tin[r] / cp_mix + 0.0 / cp_mix;
+    if (rop_out)
+      for (int s = lane; s < nstep; s += kWave) rop_out[(size_t)r * nstep + s] = rop[s];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// jacobian_kernel (nsp <= 63): J[n][N][N] row-major, N = nsp+1, T last.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kWave)
+jacobian_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ Tin,
+                const double* __restrict__ Pin, const double* __restrict__ Yin, int stride,
+                double* __restrict__ Jout, double* __restrict__ jscratch) {
+  extern __shared__ double lds[];
+  const int lane = lane_id();
+  const int nsp = m.nsp, N = nsp + 1;
+  double* Jl = lds;
+  double* rl = lds + kNP * kNP;
+  RhsLds s{rl, rl + nsp, rl + 2 * nsp};
+  double* saved = jscratch + (size_t)blockIdx.x * kNP * kNP;
+  for (int r = blockIdx.x; r < n; r += gridDim.x) {
+    double y = 0.0;
+    if (lane < nsp) y = Yin[(size_t)r * stride + lane];
+    if (lane == nsp) y = Tin[r] / o.Tref;
+    const double P = Pin[r];
+    double fy;
+    RhsAux aux;
+    rhs64(m, s, P, o.Tref, o.step_limiter, y, fy, aux);
+    jacobian_lds(m, s, P, o, y, fy, aux.inv_rho, Jl, saved);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane <= nsp)
+      for (int c = 0; c < N; ++c) Jout[(size_t)r * N * N + (size_t)lane * N + c] = Jl[c * kNP + lane];
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// bdf_kernel (nsp <= 63): persistent; wavefronts pull reactors from *counter.
+// T[n], Y[n*stride] updated in place; nsteps[n] = plugin cost (CVode one-step
+// calls, negative on failure); stats[n*8] (optional): nst nfe nje nsetups nni
+// ncfn netf status.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kWave)
+bdf_kernel(DevMech m, BdfOpts o, int n, double dt, double* __restrict__ Tio,
+           const double* __restrict__ Pin, double* __restrict__ Yio, int stride,
+           int* __restrict__ nsteps_out, int* __restrict__ stats_out,
+           const int* __restrict__ order, unsigned int* __restrict__ counter,
+           double* __restrict__ jscratch) {
+  extern __shared__ double lds[];
+  const int lane = lane_id();
+  const int nsp = m.nsp;
+  CvWave cv;
+  cv.mech = m;
+  cv.opt = o;
+  cv.Mlds = lds;
+  cv.prow = lds + kNP * kNP;
+  cv.rvec = cv.prow + kNP;
+  double* rl = cv.rvec + kNP;
+  cv.s = RhsLds{rl, rl + nsp, rl + 2 * nsp};
+  cv.savedJ = jscratch + (size_t)blockIdx.x * kNP * kNP;
+  cv.active = lane <= nsp;
+  cv.inv_nvar = 1.0 / (double)(nsp + 1);
+  cv.nlscoef = o.nlscoef;
+  cv.hmax_inv = o.hmax_inv;
+  for (;;) {
+    unsigned int r = 0;
+    if (lane == 0) r = atomicAdd(counter, 1u);
+    r = (unsigned int)__builtin_amdgcn_readfirstlane((int)r);
+    if (r >= (unsigned int)n) break;
+    if (order) r = (unsigned int)order[r];
+    double y0 = 0.0;
+    if (lane < nsp) y0 = Yio[(size_t)r * stride + lane];
+    if (lane == nsp) y0 = Tio[r] / o.Tref;
+    cv.P = Pin[r];
+    cv.atolv = cv.active ? o.atol : 0.0;
+    cv.rtol = o.rtol;
+    cv.inv_rho_last = 0.0;
+    cv.reinit(0.0, y0);
+    int nsteps = 0, num_tries = 0, flag = kCvSuccess;
+    int tot[7] = {0, 0, 0, 0, 0, 0, 0};
+    double tcurr = 0.0;
+    while (tcurr < dt) {
+      flag = cv.one_step(dt, tcurr);
+      nsteps += 1;
+      if (cv.nst >= o.max_steps) flag = kCvTooMuchWork;
+      if (flag < 0) {
+        num_tries += 1;
+        cv.atolv *= o.retry_atol_adj;
+        cv.rtol *= o.retry_rtol_adj;
+        tot[0] += cv.nst; tot[1] += cv.nfe; tot[2] += cv.nje; tot[3] += cv.nsetups;
+        tot[4] += cv.nni; tot[5] += cv.ncfn; tot[6] += cv.netf;
+        if (num_tries == o.num_retries) break;
+        cv.reinit(tcurr, cv.zn[0]);
+      }
+    }
+    // failed reactors keep their input state (zerork_reactor_manager.cpp:1123-1139)
+    if (flag == kCvSuccess) {
+      const double yout = cv.dky0(tcurr < dt ? tcurr : dt);
+      tot[0] += cv.nst; tot[1] += cv.nfe; tot[2] += cv.nje; tot[3] += cv.nsetups;
+      tot[4] += cv.nni; tot[5] += cv.ncfn; tot[6] += cv.netf;
+      if (lane < nsp) Yio[(size_t)r * stride + lane] = yout;
+      if (lane == nsp) Tio[r] = yout * o.Tref;
+    } else {
+      nsteps = (nsteps <= 0) ? -1 : -nsteps;
+    }
+    if (lane == 0) {
+      if (nsteps_out) nsteps_out[r] = nsteps;
+      if (stats_out) {
+#pragma unroll
+        for (int k = 0; k < 7; ++k) stats_out[(size_t)r * 8 + k] = tot[k];
+        stats_out[(size_t)r * 8 + 7] = flag;
+      }
+    }
+  }
+}
+
+__global__ void fill_u32_kernel(unsigned int* p, unsigned int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_rates(const DevMech& m, int n, const double* T, const double* P,
+                        const double* Y, int stride, double L, double* ydot, double* rop,
+                        double* create, double* destroy, double* net, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const size_t lds = sizeof(double) * (size_t)kRatesWaves * (3 * m.nsp + m.nstep);
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  int blocks = (n + kRatesWaves - 1) / kRatesWaves;
+  hipLaunchKernelGGL(rates_kernel, dim3(blocks), dim3(kWave * kRatesWaves), lds, st, m, n, T,
+                     P, Y, stride, L, ydot, rop, create, destroy, net);
+  return hipGetLastError();
+}
+
+size_t rhs_lds_bytes(const DevMech& m) { return sizeof(double) * (size_t)(2 * m.nsp + m.nstep); }
+size_t bdf_lds_bytes(const DevMech& m) {
+  return sizeof(double) * (size_t)(kNP * kNP + 2 * kNP) + rhs_lds_bytes(m);
+}
+
+hipError_t launch_jacobian(const DevMech& m, const BdfOpts& o, int n, const double* T,
+                           const double* P, const double* Y, int stride, double* J,
+                           double* jscratch, int nblocks, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  if (m.nsp > kNP - 1) return hipErrorInvalidValue;
+  const int grid = nblocks < n ? nblocks : n;
+  const size_t lds = sizeof(double) * (size_t)kNP * kNP + rhs_lds_bytes(m);
+  hipLaunchKernelGGL(jacobian_kernel, dim3(grid), dim3(kWave), lds, st, m, o, n, T, P, Y,
+                     stride, J, jscratch);
+  return hipGetLastError();
+}
+
+int bdf_max_resident_waves(const DevMech& m) {
+  int dev = 0, ncu = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bdf_kernel, kWave,
+                                                   bdf_lds_bytes(m)) != hipSuccess)
+    return 0;
+  return ncu * (per_cu > 0 ? per_cu : 1);
+}
+
+hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, double* T,
+                      const double* P, double* Y, int stride, int* nsteps, int* stats,
+                      const int* order, unsigned int* counter, double* jscratch, int nwaves,
+                      hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  if (m.nsp > kNP - 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, counter, 0u);
+  const int grid = nwaves < n ? nwaves : n;
+  hipLaunchKernelGGL(bdf_kernel, dim3(grid), dim3(kWave), bdf_lds_bytes(m), st, m, o, n, dt, T,
+                     P, Y, stride, nsteps, stats, order, counter, jscratch);
+  return hipGetLastError();
+}
+
+}  // namespace zrk

@@ -1,0 +1,23 @@
+// Host-side launchers of the kernels in kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include "dev_mech.h"
+
+namespace zrk {
+
+hipError_t launch_rates(const DevMech& m, int n, const double* T, const double* P,
+                        const double* Y, int stride, double L, double* ydot, double* rop,
+                        double* create, double* destroy, double* net, hipStream_t st);
+hipError_t launch_jacobian(const DevMech& m, const BdfOpts& o, int n, const double* T,
+                           const double* P, const double* Y, int stride, double* J,
+                           double* jscratch, int nblocks, hipStream_t st);
+int bdf_max_resident_waves(const DevMech& m);
+size_t bdf_lds_bytes(const DevMech& m);
+hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, double* T,
+                      const double* P, double* Y, int stride, int* nsteps, int* stats,
+                      const int* order, unsigned int* counter, double* jscratch, int nwaves,
+                      hipStream_t st);
+
+}  // namespace zrk

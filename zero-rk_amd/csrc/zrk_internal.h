@@ -1,0 +1,35 @@
+// Internal handle types shared by the C-ABI layers.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <string>
+
+#include "../../include/zrk_device.h"
+#include "ck_mech.h"
+#include "dev_mech.h"
+
+namespace zrk {
+
+struct MechHandle {
+  Mechanism host;
+  DevMech dev{};
+  void* dev_blob = nullptr;
+  size_t dev_bytes = 0;
+  int device = 0;
+  int resident_waves = 0;
+  // integrator workspace
+  double* ws_jac = nullptr;
+  unsigned int* ws_counter = nullptr;
+  int ws_slots = 0;
+  ~MechHandle();
+};
+
+int build_device_mech(MechHandle* h, std::string* err);
+int ensure_device(MechHandle* h);
+int ensure_workspace(MechHandle* h, int nslots);
+int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt, double* T,
+                     const double* P, double* Y, int stride, int* nsteps, int* stats,
+                     const int* order, hipStream_t st);
+
+}  // namespace zrk

@@ -1,0 +1,279 @@
+"""ctypes bindings of libzerork_cfd_plugin.so (include/zerork_cfd_plugin.h and
+include/zrk_device.h).
+
+The library is the product: a missing or unloadable library raises here; there
+is no fallback path.  Device-level calls take device pointers (ints), e.g. from
+``torch.Tensor.data_ptr()``, and a HIP stream handle (``stream.cuda_stream``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "lib", "libzerork_cfd_plugin.so")
+
+_lib = None
+
+# zerork_status_t
+STATUS = {
+    0: "SUCCESS", 1: "INVALID_HANDLE", 2: "INVALID_FILENAME", 3: "INVALID_FIELD_NAME",
+    4: "INVALID_POINTER", 5: "FAILED_MECHANISM_PARSE", 6: "FAILED_OPTIONS_PARSE",
+    7: "FAILED_SOLVE", 8: "OPTION_NOT_DEFINED", 99: "UNKNOWN_ERROR",
+}
+FIELD = {"DPDT": 0, "COST": 1, "GPU": 2, "Y_SRC": 3, "E_SRC": 4, "IGNITION_TIME": 5,
+         "TEMPERATURE_DELTA": 6}
+
+# exported symbols, as declared in include/*.h
+PLUGIN_SYMBOLS = [
+    "zerork_reactor_init", "zerork_reactor_read_options_file",
+    "zerork_reactor_set_mechanism_files", "zerork_reactor_load_mechanism",
+    "zerork_reactor_solve", "zerork_reactor_set_aux_field_pointer",
+    "zerork_reactor_set_int_option", "zerork_reactor_set_double_option",
+    "zerork_reactor_set_string_option", "zerork_reactor_get_int_option",
+    "zerork_reactor_get_double_option", "zerork_reactor_get_string_option",
+    "zerork_reactor_set_callback_fn", "zerork_reactor_set_reactor_ids", "zerork_reactor_free",
+]
+DEVICE_SYMBOLS = [
+    "zrk_bdf_default_options", "zrk_mech_load", "zrk_mech_free", "zrk_mech_info",
+    "zrk_mech_species", "zrk_mech_molwt", "zrk_mech_steps", "zrk_rates_device",
+    "zrk_jacobian_device", "zrk_integrate_device", "zrk_integrator_resident_waves",
+]
+
+
+class BdfOptions(C.Structure):
+    _fields_ = [
+        ("rel_tol", C.c_double), ("abs_tol", C.c_double), ("max_dt", C.c_double),
+        ("nonlinear_convergence_coeff", C.c_double), ("step_limiter", C.c_double),
+        ("reference_temperature", C.c_double), ("min_mass_fraction", C.c_double),
+        ("cvode_retry_absolute_tolerance_adjustment", C.c_double),
+        ("cvode_retry_relative_tolerance_adjustment", C.c_double),
+        ("max_steps", C.c_int), ("cvode_num_retries", C.c_int),
+    ]
+
+
+def lib():
+    """Load the product library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C {PKG}` "
+                               "(or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L):
+    P, I, D = C.c_void_p, C.c_int, C.c_double
+    L.zerork_reactor_init.restype = P
+    L.zerork_reactor_init.argtypes = []
+    for name, args in [
+        ("zerork_reactor_read_options_file", [C.c_char_p, P]),
+        ("zerork_reactor_set_mechanism_files", [C.c_char_p, C.c_char_p, P]),
+        ("zerork_reactor_load_mechanism", [P]),
+        ("zerork_reactor_solve", [I, D, D, I, P, P, P, P]),
+        ("zerork_reactor_set_aux_field_pointer", [I, P, P]),
+        ("zerork_reactor_set_int_option", [C.c_char_p, I, P]),
+        ("zerork_reactor_set_double_option", [C.c_char_p, D, P]),
+        ("zerork_reactor_set_string_option", [C.c_char_p, C.c_char_p, P]),
+        ("zerork_reactor_get_int_option", [C.c_char_p, C.POINTER(I), P]),
+        ("zerork_reactor_get_double_option", [C.c_char_p, C.POINTER(D), P]),
+        ("zerork_reactor_get_string_option", [C.c_char_p, C.POINTER(C.c_char_p), I, P]),
+        ("zerork_reactor_set_callback_fn", [P, P, P]),
+        ("zerork_reactor_set_reactor_ids", [P, P]),
+        ("zerork_reactor_free", [P]),
+    ]:
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = I
+    L.zrk_bdf_default_options.argtypes = [C.POINTER(BdfOptions)]
+    L.zrk_bdf_default_options.restype = None
+    L.zrk_mech_load.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(P), C.c_char_p, I]
+    L.zrk_mech_load.restype = I
+    L.zrk_mech_free.argtypes = [P]
+    L.zrk_mech_free.restype = None
+    L.zrk_mech_info.argtypes = [P, C.POINTER(I), C.POINTER(I), C.POINTER(I)]
+    L.zrk_mech_info.restype = I
+    L.zrk_mech_species.argtypes = [P, I, C.c_char_p, I]
+    L.zrk_mech_species.restype = I
+    L.zrk_mech_molwt.argtypes = [P, P]
+    L.zrk_mech_molwt.restype = I
+    L.zrk_mech_steps.argtypes = [P, P, P, P, P, P, P]
+    L.zrk_mech_steps.restype = I
+    L.zrk_rates_device.argtypes = [P, I, P, P, P, I, D, P, P, P, P, P, P]
+    L.zrk_rates_device.restype = I
+    L.zrk_jacobian_device.argtypes = [P, C.POINTER(BdfOptions), I, P, P, P, I, P, P]
+    L.zrk_jacobian_device.restype = I
+    L.zrk_integrate_device.argtypes = [P, C.POINTER(BdfOptions), I, D, P, P, P, I, P, P, P]
+    L.zrk_integrate_device.restype = I
+    L.zrk_integrator_resident_waves.argtypes = [P]
+    L.zrk_integrator_resident_waves.restype = I
+
+
+def default_options(**kw) -> BdfOptions:
+    o = BdfOptions()
+    lib().zrk_bdf_default_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+class Mech:
+    """A mechanism loaded into the library (parsed on the host; uploaded to the
+    device on first device call)."""
+
+    def __init__(self, mech_file: str, therm_file: str | None = None):
+        L = lib()
+        h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        rc = L.zrk_mech_load(mech_file.encode(), (therm_file or "").encode(), C.byref(h), err, 512)
+        if rc != 0:
+            raise RuntimeError(f"zrk_mech_load failed ({rc}): {err.value.decode()}")
+        self.h = h
+        nsp, nrxn, nstep = C.c_int(), C.c_int(), C.c_int()
+        L.zrk_mech_info(h, C.byref(nsp), C.byref(nrxn), C.byref(nstep))
+        self.nsp, self.nrxn, self.nstep = nsp.value, nrxn.value, nstep.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().zrk_mech_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def species(self):
+        buf = C.create_string_buffer(64)
+        out = []
+        for j in range(self.nsp):
+            lib().zrk_mech_species(self.h, j, buf, 64)
+            out.append(buf.value.decode())
+        return out
+
+    def molwt(self):
+        import numpy as np
+        mw = np.zeros(self.nsp)
+        lib().zrk_mech_molwt(self.h, mw.ctypes.data)
+        return mw
+
+    def steps(self):
+        import numpy as np
+        ns = self.nstep
+        srx, sdir = np.zeros(ns, np.int32), np.zeros(ns, np.int32)
+        rp, pp = np.zeros(ns + 1, np.int32), np.zeros(ns + 1, np.int32)
+        lib().zrk_mech_steps(self.h, srx.ctypes.data, sdir.ctypes.data, rp.ctypes.data, None,
+                             pp.ctypes.data, None)
+        rs, ps = np.zeros(max(int(rp[-1]), 1), np.int32), np.zeros(max(int(pp[-1]), 1), np.int32)
+        lib().zrk_mech_steps(self.h, None, None, None, rs.ctypes.data, None, ps.ctypes.data)
+        return dict(step_rxn=srx, step_dir=sdir, reac_ptr=rp, reac_sp=rs[:rp[-1]],
+                    prod_ptr=pp, prod_sp=ps[:pp[-1]])
+
+    # -- device entry points ---------------------------------------------------
+    def rates(self, n, T, P, Y, stride, step_limiter=1.0e22, ydot=None, rop=None, create=None,
+              destroy=None, net=None, stream=0):
+        rc = lib().zrk_rates_device(self.h, n, _ptr(T), _ptr(P), _ptr(Y), stride, step_limiter,
+                                    _ptr(ydot), _ptr(rop), _ptr(create), _ptr(destroy), _ptr(net),
+                                    stream)
+        if rc != 0:
+            raise RuntimeError(f"zrk_rates_device failed ({rc})")
+
+    def jacobian(self, n, T, P, Y, stride, J, opts: BdfOptions | None = None, stream=0):
+        o = opts or default_options()
+        rc = lib().zrk_jacobian_device(self.h, C.byref(o), n, _ptr(T), _ptr(P), _ptr(Y), stride,
+                                       _ptr(J), stream)
+        if rc != 0:
+            raise RuntimeError(f"zrk_jacobian_device failed ({rc})")
+
+    def integrate(self, n, dt, T, P, Y, stride, opts: BdfOptions | None = None, nsteps=None,
+                  stats=None, stream=0):
+        o = opts or default_options()
+        rc = lib().zrk_integrate_device(self.h, C.byref(o), n, dt, _ptr(T), _ptr(P), _ptr(Y),
+                                        stride, _ptr(nsteps), _ptr(stats), stream)
+        if rc != 0:
+            raise RuntimeError(f"zrk_integrate_device failed ({rc})")
+
+    def resident_waves(self):
+        return lib().zrk_integrator_resident_waves(self.h)
+
+
+class Plugin:
+    """Thin Python view of the plugin C API (mirrors zerork_cfd_plugin_tester)."""
+
+    def __init__(self):
+        self.L = lib()
+        self.h = self.L.zerork_reactor_init()
+        if not self.h:
+            raise RuntimeError("zerork_reactor_init failed")
+        self._keep = []
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what}: {STATUS.get(rc, rc)}")
+        return rc
+
+    def set_mechanism_files(self, mech, therm):
+        return self.L.zerork_reactor_set_mechanism_files(mech.encode(), therm.encode(), self.h)
+
+    def load_mechanism(self):
+        return self.L.zerork_reactor_load_mechanism(self.h)
+
+    def read_options_file(self, path):
+        return self.L.zerork_reactor_read_options_file(path.encode(), self.h)
+
+    def set_int(self, k, v):
+        return self.L.zerork_reactor_set_int_option(k.encode(), int(v), self.h)
+
+    def set_double(self, k, v):
+        return self.L.zerork_reactor_set_double_option(k.encode(), float(v), self.h)
+
+    def set_string(self, k, v):
+        return self.L.zerork_reactor_set_string_option(k.encode(), v.encode(), self.h)
+
+    def get_int(self, k):
+        v = C.c_int()
+        rc = self.L.zerork_reactor_get_int_option(k.encode(), C.byref(v), self.h)
+        return rc, v.value
+
+    def get_double(self, k):
+        v = C.c_double()
+        rc = self.L.zerork_reactor_get_double_option(k.encode(), C.byref(v), self.h)
+        return rc, v.value
+
+    def get_string(self, k, n=256):
+        buf = C.create_string_buffer(n)
+        p = C.c_char_p(C.addressof(buf))
+        rc = self.L.zerork_reactor_get_string_option(k.encode(), C.byref(p), n, self.h)
+        return rc, buf.value.decode()
+
+    def set_aux(self, field, arr):
+        self._keep.append(arr)
+        return self.L.zerork_reactor_set_aux_field_pointer(FIELD[field], arr.ctypes.data, self.h)
+
+    def solve(self, n_cycle, time, dt, T, P, mf):
+        n = len(T)
+        return self.L.zerork_reactor_solve(n_cycle, time, dt, n, T.ctypes.data, P.ctypes.data,
+                                           mf.ctypes.data, self.h)
+
+    def free(self):
+        if self.h:
+            rc = self.L.zerork_reactor_free(self.h)
+            self.h = None
+            return rc
+        return 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

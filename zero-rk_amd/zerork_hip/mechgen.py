@@ -1,0 +1,376 @@
+"""Seeded synthetic CHEMKIN mechanism and reactor-state generators.
+
+The workloads of this project are synthetic chemistry: element-balanced random
+reaction sets written as ordinary CHEMKIN-II mechanism and 80-column NASA-7
+thermo files, so that every parser (this package's, the oracle's and the
+reference's CKReader, src/CKconverter/CKParser.cpp:861-1046 for thermo cards,
+:1094-1110 for units) reads the same text.  Every generator takes an explicit
+seed and is deterministic for it.
+
+Mechanism recipe (all reactions reversible, reverse rate from Keq):
+  * elementary bimolecular / third-body (+M) / Troe falloff (+M) mix,
+  * 1-2 reactants and 1-2 products (up to ``max_participants``), element-balanced,
+  * log10 A ~ U[8,14] (cgs), beta ~ U[-1,2], Ea ~ U[0,40] kcal/mol,
+  * third-body efficiencies ~ U[0.5,3],
+  * NASA-7 thermo perturbed from a constant cp ~ U[3.5,6] R (atoms ~2.5 R),
+    continuous in cp, h and s at Tmid = 1000 K.
+"""
+from __future__ import annotations
+
+import itertools
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+RU_CAL = 1.98720425864083  # cal/mol/K (= 8.314462618 / 4.184)
+
+# Known species: formula -> (H_f298 [kcal/mol], S298 [cal/mol/K]); values are
+# textbook-order thermochemistry used only to make equilibrium constants sane.
+_KNOWN = {
+    "H": (52.1, 27.4), "H2": (0.0, 31.2), "O": (59.6, 38.5), "O2": (0.0, 49.0),
+    "OH": (8.9, 43.9), "H2O": (-57.8, 45.1), "HO2": (3.0, 54.8),
+    "H2O2": (-32.5, 55.7), "O3": (34.1, 57.1), "N2": (0.0, 45.8),
+    "AR": (0.0, 37.0), "N": (113.0, 36.6), "NO": (21.6, 50.4),
+    "NO2": (8.1, 57.4), "N2O": (19.6, 52.6), "NH": (85.2, 43.3),
+    "NH2": (45.1, 46.5), "NH3": (-11.0, 46.0), "HNO": (25.6, 52.8),
+    "NNH": (59.6, 53.6), "HCN": (32.3, 48.2), "CN": (104.0, 48.4),
+    "NCO": (30.5, 55.5), "HNCO": (-28.0, 57.0), "C": (171.3, 37.8),
+    "CH": (142.5, 43.7), "CH2": (93.0, 46.3), "CH3": (35.0, 46.4),
+    "CH4": (-17.9, 44.5), "CO": (-26.4, 47.2), "CO2": (-94.1, 51.1),
+    "HCO": (10.4, 53.7), "CH2O": (-26.0, 52.3), "CH3O": (4.1, 54.6),
+    "CH2OH": (-3.9, 58.4), "CH3OH": (-48.1, 57.3), "C2H": (135.0, 49.6),
+    "C2H2": (54.2, 48.0), "C2H3": (71.0, 55.9), "C2H4": (12.5, 52.4),
+    "C2H5": (28.9, 60.1), "C2H6": (-20.0, 54.9), "HCCO": (42.4, 60.7),
+    "CH2CO": (-11.4, 57.8), "HCCOH": (20.0, 58.0), "CH3CHO": (-39.7, 63.2),
+    "CH2CHO": (3.0, 64.0), "C2H5OH": (-56.2, 67.5), "C3H8": (-25.0, 64.5),
+    "C3H6": (4.9, 63.8), "C3H4": (45.5, 58.3), "C3H3": (81.0, 61.5),
+    "C3H5": (40.0, 63.0), "NC3H7": (24.0, 69.0), "IC3H7": (21.5, 69.2),
+}
+
+ATOM_H = {"H": 52.1, "O": 59.6, "N": 113.0, "C": 171.3, "AR": 0.0}
+
+
+def parse_formula(name: str) -> dict:
+    """Element counts of a formula-like species name (C2H5OH, IC3H7, AR)."""
+    s = name.upper()
+    for pre in ("IC", "NC", "SC", "TC"):
+        if s.startswith(pre) and len(s) > 2 and s[2].isdigit():
+            s = s[1:]
+            break
+    out: dict = {}
+    i = 0
+    while i < len(s):
+        if s.startswith("AR", i):
+            el, i = "AR", i + 2
+        else:
+            el, i = s[i], i + 1
+        j = i
+        while j < len(s) and s[j].isdigit():
+            j += 1
+        n = int(s[i:j]) if j > i else 1
+        i = j
+        out[el] = out.get(el, 0) + n
+    return out
+
+
+@dataclass
+class Reaction:
+    reac: list
+    prod: list
+    kind: str  # "elem" | "tb" | "falloff"
+    A: float
+    beta: float
+    Ea: float  # cal/mol
+    eff: dict = field(default_factory=dict)
+    low: tuple = ()
+    troe: tuple = ()
+
+
+@dataclass
+class Mechanism:
+    elements: list
+    species: list
+    comp: list           # list of element-count dicts
+    thermo: list         # list of (low[7], high[7]) coefficient tuples
+    reactions: list
+    seed: int
+    majors: list
+
+    def mech_text(self) -> str:
+        L = [f"! synthetic mechanism, seed {self.seed}, {len(self.species)} species,"
+             f" {len(self.reactions)} reactions", "ELEMENTS"]
+        L.append(" ".join(self.elements))
+        L.append("END")
+        L.append("SPECIES")
+        for i in range(0, len(self.species), 8):
+            L.append(" ".join(self.species[i:i + 8]))
+        L.append("END")
+        L.append("REACTIONS CAL/MOLE")
+        for r in self.reactions:
+            lhs = " + ".join(r.reac)
+            rhs = " + ".join(r.prod)
+            if r.kind == "tb":
+                lhs += " + M"
+                rhs += " + M"
+            elif r.kind == "falloff":
+                lhs += " (+M)"
+                rhs += " (+M)"
+            L.append(f"{lhs} <=> {rhs}   {r.A:.10E} {r.beta:.6f} {r.Ea:.4f}")
+            if r.kind == "falloff":
+                L.append(f"   LOW / {r.low[0]:.10E} {r.low[1]:.6f} {r.low[2]:.4f} /")
+                L.append("   TROE / " + " ".join(f"{v:.6E}" for v in r.troe) + " /")
+            if r.eff:
+                L.append("   " + " ".join(f"{k}/{v:.4f}/" for k, v in r.eff.items()))
+        L.append("END")
+        return "\n".join(L) + "\n"
+
+    def therm_text(self) -> str:
+        L = ["THERMO ALL", "   300.000  1000.000  5000.000"]
+        for name, comp, (lo, hi) in zip(self.species, self.comp, self.thermo):
+            els = "".join(f"{e:<2s}{n:>3d}" for e, n in comp.items())
+            els = els.ljust(20)
+            l1 = f"{name:<16s}{'SYNTH':<8s}{els}G{300.0:10.3f}{5000.0:10.3f}{1000.0:8.2f}"
+            L.append(l1.ljust(79) + "1")
+            c = list(hi) + list(lo)
+            fmt = lambda v: f"{v:15.8E}"
+            L.append("".join(fmt(v) for v in c[0:5]).ljust(79) + "2")
+            L.append("".join(fmt(v) for v in c[5:10]).ljust(79) + "3")
+            L.append("".join(fmt(v) for v in c[10:14]).ljust(79) + "4")
+        L.append("END")
+        return "\n".join(L) + "\n"
+
+    def write(self, directory: str, stem: str) -> tuple:
+        os.makedirs(directory, exist_ok=True)
+        mp = os.path.join(directory, stem + ".mech")
+        tp = os.path.join(directory, stem + ".therm")
+        with open(mp, "w") as f:
+            f.write(self.mech_text())
+        with open(tp, "w") as f:
+            f.write(self.therm_text())
+        return mp, tp
+
+
+def _nasa_coeffs(rng, cp0: float, hf_kcal: float, s298: float, atom: bool):
+    """Low/high NASA-7 sets, continuous in cp, h, s at 1000 K."""
+    Tm, Tr = 1000.0, 298.15
+    if atom:
+        lo = [cp0, 0.0, 0.0, 0.0, 0.0]
+        hi = [cp0, 0.0, 0.0, 0.0, 0.0]
+    else:
+        lo = [cp0, rng.uniform(0.0, 1.5e-3), rng.uniform(-4e-7, 0.0), 0.0, 0.0]
+        hi = [0.0, rng.uniform(0.0, 6e-4), rng.uniform(-1e-7, 0.0), 0.0, 0.0]
+    cp = lambda a, T: a[0] + a[1] * T + a[2] * T**2 + a[3] * T**3 + a[4] * T**4
+    hint = lambda a, T: a[0] * T + a[1] * T**2 / 2 + a[2] * T**3 / 3 + a[3] * T**4 / 4 + a[4] * T**5 / 5
+    sint = lambda a, T: a[0] * math.log(T) + a[1] * T + a[2] * T**2 / 2 + a[3] * T**3 / 3 + a[4] * T**4 / 4
+    if not atom:
+        hi[0] = cp(lo, Tm) - (hi[1] * Tm + hi[2] * Tm**2)
+    a6lo = hf_kcal * 1000.0 / RU_CAL - hint(lo, Tr)
+    a7lo = s298 / RU_CAL - sint(lo, Tr)
+    a6hi = hint(lo, Tm) + a6lo - hint(hi, Tm)
+    a7hi = sint(lo, Tm) + a7lo - sint(hi, Tm)
+    return tuple(lo + [a6lo, a7lo]), tuple(hi + [a6hi, a7hi])
+
+
+def _species_pool(elements):
+    names = [n for n in _KNOWN if set(parse_formula(n)) <= set(elements)]
+    return names
+
+
+def _extra_species(rng, elements, n, taken):
+    """Formula species beyond the known table (CxHyOz(Nw)) with additive thermo."""
+    out = []
+    hasC, hasN = "C" in elements, "N" in elements
+    tries = 0
+    while len(out) < n and tries < 200000:
+        tries += 1
+        c = int(rng.integers(1, 13)) if hasC else 0
+        h = int(rng.integers(0, 2 * c + 3)) if hasC else int(rng.integers(1, 5))
+        o = int(rng.integers(0, 4))
+        w = int(rng.integers(0, 2)) if hasN and rng.random() < 0.15 else 0
+        if c + h + o + w < 2:
+            continue
+        name = (f"C{c}" if c > 1 else ("C" if c == 1 else "")) + \
+               (f"H{h}" if h > 1 else ("H" if h == 1 else "")) + \
+               (f"O{o}" if o > 1 else ("O" if o == 1 else "")) + \
+               (f"N{w}" if w > 1 else ("N" if w == 1 else ""))
+        if name in taken or name in out:
+            continue
+        out.append(name)
+    return out
+
+
+def _thermo_for(rng, name, comp):
+    natoms = sum(comp.values())
+    if name in _KNOWN:
+        hf, s = _KNOWN[name]
+        hf += rng.uniform(-2.0, 2.0)
+    else:
+        hf = sum(ATOM_H[e] * k for e, k in comp.items()) - 95.0 * (natoms - 1) + rng.uniform(-10, 10)
+        s = 25.0 + 9.0 * natoms + rng.uniform(-3, 3)
+    atom = natoms == 1
+    cp0 = rng.uniform(2.45, 2.55) if atom else rng.uniform(3.5, 6.0)
+    return _nasa_coeffs(rng, cp0, hf, s, atom)
+
+
+def generate(n_species: int, n_reactions: int, seed: int, elements=("H", "O", "N"),
+             frac_third_body: float = 0.25, frac_falloff: float = 0.15,
+             inert=("N2", "AR"), majors=None, max_participants: int = 2,
+             base=None) -> Mechanism:
+    """Seeded synthetic element-balanced CHEMKIN mechanism."""
+    rng = np.random.default_rng(seed)
+    elements = [e.upper() for e in elements]
+    pool = _species_pool(elements)
+    if base is not None:
+        chosen = [s for s in base]
+    else:
+        chosen = []
+    for s in pool:
+        if len(chosen) >= n_species:
+            break
+        if s not in chosen:
+            chosen.append(s)
+    if len(chosen) < n_species:
+        chosen += _extra_species(rng, elements, n_species - len(chosen), set(chosen))
+    species = chosen[:n_species]
+    comp = [parse_formula(s) for s in species]
+    # element order as in the ELEMENTS list
+    comp = [{e: c[e] for e in elements if e in c} for c in comp]
+    thermo = [_thermo_for(rng, s, c) for s, c in zip(species, comp)]
+
+    reactive = [i for i, s in enumerate(species) if s not in inert]
+    evec = np.array([[c.get(e, 0) for e in elements] for c in comp], dtype=np.int64)
+    groups: dict = {}
+    for size in range(1, max_participants + 1):
+        for ms in itertools.combinations_with_replacement(reactive, size):
+            key = tuple(evec[list(ms)].sum(axis=0))
+            groups.setdefault(key, []).append(ms)
+    # candidate pairs by reaction class
+    bimol, assoc = [], []  # (reac, prod)
+    for key, lst in groups.items():
+        if len(lst) < 2:
+            continue
+        for a, b in itertools.combinations(lst, 2):
+            if set(a) & set(b):
+                continue
+            if len(a) >= 2 and len(b) >= 2:
+                bimol.append((a, b))
+            elif (len(a) == 1) != (len(b) == 1):
+                assoc.append((a, b) if len(a) > len(b) else (b, a))
+    n_fo = int(round(frac_falloff * n_reactions))
+    n_tb = int(round(frac_third_body * n_reactions))
+    n_el = n_reactions - n_fo - n_tb
+    if len(bimol) < n_el or len(assoc) < n_fo + n_tb:
+        raise ValueError(f"species set too small: {len(bimol)} bimolecular, {len(assoc)} "
+                         f"association candidates for {n_el} + {n_tb + n_fo} reactions")
+    pick_el = rng.choice(len(bimol), size=n_el, replace=False)
+    pick_as = rng.choice(len(assoc), size=n_tb + n_fo, replace=False)
+    kinds = ["elem"] * n_el + ["tb"] * n_tb + ["falloff"] * n_fo
+    pairs = [bimol[i] for i in pick_el] + [assoc[i] for i in pick_as]
+    order = rng.permutation(n_reactions)
+    reactions = []
+    for idx in order:
+        (a, b), kind = pairs[idx], kinds[idx]
+        if kind != "elem" and rng.random() < 0.5:
+            a, b = b, a  # dissociation direction
+        A = 10.0 ** rng.uniform(8.0, 14.0)
+        beta = rng.uniform(-1.0, 2.0)
+        Ea = rng.uniform(0.0, 40.0) * 1000.0
+        r = Reaction([species[i] for i in a], [species[i] for i in b], kind, A, beta, Ea)
+        if kind in ("tb", "falloff"):
+            ne = int(rng.integers(1, 5))
+            cand = [s for s in species]
+            for j in rng.choice(len(cand), size=min(ne, len(cand)), replace=False):
+                r.eff[cand[j]] = float(rng.uniform(0.5, 3.0))
+        if kind == "falloff":
+            r.low = (A * 10.0 ** rng.uniform(3.0, 7.0), rng.uniform(-2.0, 1.0),
+                     rng.uniform(0.0, 40.0) * 1000.0)
+            alpha = rng.uniform(0.1, 0.9)
+            t3 = rng.uniform(50.0, 500.0)
+            t1 = rng.uniform(500.0, 5000.0)
+            if rng.random() < 0.5:
+                r.troe = (alpha, t3, t1, rng.uniform(1000.0, 10000.0))
+            else:
+                r.troe = (alpha, t3, t1)
+        reactions.append(r)
+    if majors is None:
+        majors = species[:4]
+    return Mechanism(elements, species, comp, thermo, reactions, seed, list(majors))
+
+
+# ----------------------------------------------------------------------------
+# Workload configurations (mechanism + reactor state distributions)
+# ----------------------------------------------------------------------------
+_C2_BASE = ["H2", "O2", "N2", "H2O", "CO", "CO2", "CH4", "AR"]
+
+
+def config_mechanism(cfg: int, seed: int | None = None) -> Mechanism:
+    if cfg == 1:
+        base = ["H2", "O2", "H2O", "OH", "H", "O", "HO2", "H2O2", "O3", "N2"]
+        return generate(10, 27, seed if seed is not None else 1001, ("H", "O", "N"),
+                        0.25, 0.15, inert=("N2",), majors=["H2", "O2", "H2O", "OH"],
+                        base=base)
+    if cfg == 2:
+        return generate(53, 325, seed if seed is not None else 1002,
+                        ("C", "H", "O", "N", "AR"), 0.25, 0.15, inert=("N2", "AR"),
+                        majors=_C2_BASE, base=_C2_BASE)
+    if cfg == 3:
+        return generate(160, 900, seed if seed is not None else 1003,
+                        ("C", "H", "O", "N", "AR"), 0.30, 0.10, inert=("N2", "AR"),
+                        majors=_C2_BASE + ["C2H6", "CH3OH"], base=_C2_BASE)
+    raise ValueError(f"unknown config {cfg}")
+
+
+@dataclass
+class States:
+    T: np.ndarray
+    P: np.ndarray
+    Y: np.ndarray   # [n_reactors, n_species] reactor-major (AoS), as the C API
+    seed: int
+
+
+def config_states(cfg: int, mech: Mechanism, n: int, seed: int) -> States:
+    """Seeded reactor states of a workload configuration."""
+    rng = np.random.default_rng(seed)
+    nsp = len(mech.species)
+    idx = {s: i for i, s in enumerate(mech.species)}
+    Y = np.zeros((n, nsp))
+    if cfg == 1:
+        T = rng.uniform(900.0, 1500.0, n)
+        P = rng.uniform(1.0, 20.0, n) * 101325.0
+        maj = [idx[s] for s in mech.majors]
+        frac = rng.uniform(0.3, 0.7, n)
+        Y[:, maj] = rng.dirichlet(np.ones(len(maj)), n) * frac[:, None]
+        Y[:, idx["N2"]] = 1.0 - frac
+    elif cfg == 2:
+        slow = rng.random(n) < 0.5
+        T = np.where(slow, rng.uniform(700.0, 950.0, n), rng.uniform(1050.0, 1400.0, n))
+        P = np.exp(rng.uniform(math.log(1.0), math.log(50.0), n)) * 101325.0
+        maj = [idx[s] for s in mech.majors]
+        Y[:, maj] = rng.dirichlet(0.5 * np.ones(len(maj)), n)
+    elif cfg == 3:
+        T = rng.uniform(800.0, 1600.0, n)
+        P = np.exp(rng.uniform(math.log(1.0), math.log(40.0), n)) * 101325.0
+        maj = [idx[s] for s in mech.majors]
+        Y[:, maj] = rng.dirichlet(0.5 * np.ones(len(maj)), n)
+    else:
+        raise ValueError(cfg)
+    # renormalise in extended precision so that sum(Y) = 1 to the last bit
+    Y = Y / Y.sum(axis=1, keepdims=True)
+    return States(T, P, np.ascontiguousarray(Y), seed)
+
+
+def random_dense_states(mech: Mechanism, n: int, seed: int, Tlo=800.0, Thi=2200.0,
+                        Plo=0.5, Phi=40.0) -> States:
+    """States with every species present (rate/Jacobian parity cases)."""
+    rng = np.random.default_rng(seed)
+    nsp = len(mech.species)
+    T = rng.uniform(Tlo, Thi, n)
+    P = np.exp(rng.uniform(math.log(Plo), math.log(Phi), n)) * 101325.0
+    Y = rng.dirichlet(np.ones(nsp), n)
+    return States(T, P, Y, seed)
+
+
+CONFIG_DT = {1: 1.0e-4, 2: 1.0e-4, 3: 1.0e-5}
+CONFIG_NR = {1: 1024, 2: 65536, 3: 16384}
