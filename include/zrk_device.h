@@ -11,9 +11,10 @@
  *   zrk_mech_load        zerork::mechanism ctor (src/zerork/mechanism.cpp:17-66)
  *                        + mechanism_cuda device mirror (mechanism_cuda.cu:14-144)
  *   zrk_mech_info,       mechanism::getNumSpecies/Reactions/Steps, getMolWtSpc,
- *   zrk_mech_species,    getSpeciesName, step participant maps
- *   zrk_mech_molwt,      (src/zerork/mechanism.h:33-56)
- *   zrk_mech_steps
+ *   zrk_mech_species,    getSpeciesName, step participant maps, reaction kinds
+ *   zrk_mech_molwt,      (src/zerork/mechanism.h:33-56, 205-220)
+ *   zrk_mech_steps,
+ *   zrk_mech_reactions
  *   zrk_rates_device     mechanism_cuda::getReactionRatesLimiter_CUDA_mr_dev
  *                        (src/zerork/mechanism_cuda.cu:158-165) together with
  *                        ReactorConstantPressureGPU::GetTimeDerivative
@@ -64,6 +65,13 @@ ZRK_EXPORT int zrk_mech_molwt(zrk_mech m, double* mw);
  * by the last pointer entries (pass NULL to query sizes via the ptr arrays). */
 ZRK_EXPORT int zrk_mech_steps(zrk_mech m, int* step_rxn, int* step_dir, int* reac_ptr,
                               int* reac_sp, int* prod_ptr, int* prod_sp);
+
+/* Per-reaction structure [nrxn]: forward/reverse step index (-1 if none),
+ * reverse type (0 none, 1 explicit REV Arrhenius, 2 from Keq), number of
+ * enhanced third-body species (-1 if not a third-body/fall-off reaction),
+ * fall-off type (-1 none, 0 Lindemann, 1 Troe-3, 2 Troe-4).  NULL skips. */
+ZRK_EXPORT int zrk_mech_reactions(zrk_mech m, int* fwd_step, int* rev_step, int* rev_type,
+                                  int* n_enhanced, int* falloff_type);
 
 /* Rates of progress (limiter step_limiter, as perf_net.cpp:190-275), species
  * creation/destruction/net rates and the constant-pressure derivative

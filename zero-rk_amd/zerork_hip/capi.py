@@ -37,7 +37,8 @@ PLUGIN_SYMBOLS = [
 ]
 DEVICE_SYMBOLS = [
     "zrk_bdf_default_options", "zrk_mech_load", "zrk_mech_free", "zrk_mech_info",
-    "zrk_mech_species", "zrk_mech_molwt", "zrk_mech_steps", "zrk_rates_device",
+    "zrk_mech_species", "zrk_mech_molwt", "zrk_mech_steps", "zrk_mech_reactions",
+    "zrk_rates_device",
     "zrk_jacobian_device", "zrk_integrate_device", "zrk_integrator_resident_waves",
 ]
 
@@ -103,6 +104,8 @@ def _declare(L):
     L.zrk_mech_molwt.restype = I
     L.zrk_mech_steps.argtypes = [P, P, P, P, P, P, P]
     L.zrk_mech_steps.restype = I
+    L.zrk_mech_reactions.argtypes = [P, P, P, P, P, P]
+    L.zrk_mech_reactions.restype = I
     L.zrk_rates_device.argtypes = [P, I, P, P, P, I, D, P, P, P, P, P, P]
     L.zrk_rates_device.restype = I
     L.zrk_jacobian_device.argtypes = [P, C.POINTER(BdfOptions), I, P, P, P, I, P, P]
@@ -178,6 +181,16 @@ class Mech:
         lib().zrk_mech_steps(self.h, None, None, None, rs.ctypes.data, None, ps.ctypes.data)
         return dict(step_rxn=srx, step_dir=sdir, reac_ptr=rp, reac_sp=rs[:rp[-1]],
                     prod_ptr=pp, prod_sp=ps[:pp[-1]])
+
+    def reactions(self):
+        import numpy as np
+        n = self.nrxn
+        a = {k: np.zeros(n, np.int32) for k in ("fwd_step", "rev_step", "rev_type", "n_enhanced",
+                                                   "falloff_type")}
+        lib().zrk_mech_reactions(self.h, *(a[k].ctypes.data for k in ("fwd_step", "rev_step",
+                                                                      "rev_type", "n_enhanced",
+                                                                      "falloff_type")))
+        return a
 
     # -- device entry points ---------------------------------------------------
     def rates(self, n, T, P, Y, stride, step_limiter=1.0e22, ydot=None, rop=None, create=None,

@@ -8,7 +8,9 @@ reference's CKReader, src/CKconverter/CKParser.cpp:861-1046 for thermo cards,
 seed and is deterministic for it.
 
 Mechanism recipe (all reactions reversible, reverse rate from Keq):
-  * elementary bimolecular / third-body (+M) / Troe falloff (+M) mix,
+  * elementary bimolecular / third-body (+M) / Troe falloff (+M) mix, each
+    reaction written in its exothermic direction (the random activation
+    energy applies downhill; the reverse gets Ea + dH through Keq),
   * 1-2 reactants and 1-2 products (up to ``max_participants``), element-balanced,
   * log10 A ~ U[8,14] (cgs), beta ~ U[-1,2], Ea ~ U[0,40] kcal/mol,
   * third-body efficiencies ~ U[0.5,3],
@@ -173,6 +175,14 @@ def _nasa_coeffs(rng, cp0: float, hf_kcal: float, s298: float, atom: bool):
     return tuple(lo + [a6lo, a7lo]), tuple(hi + [a6hi, a7hi])
 
 
+def _h298(low):
+    """h(298.15 K) in J/mol-equivalent units (R * T * h/RT) of a low-range set."""
+    T = 298.15
+    a = low
+    h_rt = a[0] + a[1] * T / 2 + a[2] * T**2 / 3 + a[3] * T**3 / 4 + a[4] * T**4 / 5 + a[5] / T
+    return h_rt * T
+
+
 def _species_pool(elements):
     names = [n for n in _KNOWN if set(parse_formula(n)) <= set(elements)]
     return names
@@ -264,16 +274,51 @@ def generate(n_species: int, n_reactions: int, seed: int, elements=("H", "O", "N
     if len(bimol) < n_el or len(assoc) < n_fo + n_tb:
         raise ValueError(f"species set too small: {len(bimol)} bimolecular, {len(assoc)} "
                          f"association candidates for {n_el} + {n_tb + n_fo} reactions")
-    pick_el = rng.choice(len(bimol), size=n_el, replace=False)
-    pick_as = rng.choice(len(assoc), size=n_tb + n_fo, replace=False)
-    kinds = ["elem"] * n_el + ["tb"] * n_tb + ["falloff"] * n_fo
-    pairs = [bimol[i] for i in pick_el] + [assoc[i] for i in pick_as]
+    # grow a connected network: a candidate is accepted only when one of its
+    # sides consists of species already reachable from the initial majors, so
+    # every reaction can fire from a mixture the reactors actually contain
+    major_idx = [species.index(x) for x in (majors or species[:4]) if x in species]
+    reach = set(major_idx)
+    want = {"elem": n_el, "tb": n_tb, "falloff": n_fo}
+    pools = {"elem": bimol, "tb": assoc, "falloff": assoc}
+    perms = {"elem": list(rng.permutation(len(bimol))), "assoc": list(rng.permutation(len(assoc)))}
+    used_assoc = set()
+    kinds, pairs = [], []
+    schedule = list(rng.permutation(["elem"] * n_el + ["tb"] * n_tb + ["falloff"] * n_fo))
+    for kind in schedule:
+        pool = pools[kind]
+        key = "elem" if kind == "elem" else "assoc"
+        perm = perms[key]
+        pick = None
+        for pos, ci in enumerate(perm):
+            if key == "assoc" and ci in used_assoc:
+                continue
+            a, b = pool[ci]
+            if set(a) <= reach or set(b) <= reach:
+                pick = pos
+                break
+        if pick is None:  # nothing reachable left: take the next candidate
+            pick = next(p for p, ci in enumerate(perm) if key != "assoc" or ci not in used_assoc)
+        ci = perm.pop(pick)
+        if key == "assoc":
+            used_assoc.add(ci)
+        a, b = pool[ci]
+        reach.update(a)
+        reach.update(b)
+        kinds.append(kind)
+        pairs.append((a, b))
+    want.clear()
     order = rng.permutation(n_reactions)
+    # standard enthalpy of formation at 298.15 K from the generated NASA-7 data
+    h298 = np.array([_h298(lo) for lo, _ in thermo])
     reactions = []
     for idx in order:
         (a, b), kind = pairs[idx], kinds[idx]
-        if kind != "elem" and rng.random() < 0.5:
-            a, b = b, a  # dissociation direction
+        # write every reaction in its exothermic direction: the random
+        # activation energy then belongs to the downhill direction and the
+        # reverse (from Keq) carries Ea + dH, as in physical mechanisms
+        if h298[list(b)].sum() > h298[list(a)].sum():
+            a, b = b, a
         A = 10.0 ** rng.uniform(8.0, 14.0)
         beta = rng.uniform(-1.0, 2.0)
         Ea = rng.uniform(0.0, 40.0) * 1000.0
@@ -312,7 +357,10 @@ def config_mechanism(cfg: int, seed: int | None = None) -> Mechanism:
                         0.25, 0.15, inert=("N2",), majors=["H2", "O2", "H2O", "OH"],
                         base=base)
     if cfg == 2:
-        return generate(53, 325, seed if seed is not None else 1002,
+        # seed 1056: chosen by a scan of seeds 1000-1059 (64 reactors each) as
+        # the mechanism whose cold mode (700-950 K) stays quiescent over dt
+        # while combustible hot-mode (1050-1400 K) reactors ignite within dt
+        return generate(53, 325, seed if seed is not None else 1056,
                         ("C", "H", "O", "N", "AR"), 0.25, 0.15, inert=("N2", "AR"),
                         majors=_C2_BASE, base=_C2_BASE)
     if cfg == 3:
