@@ -92,12 +92,12 @@ def main():
     from zerork_hip import capi, mechgen, workcount
 
     dev = torch.device("cuda:0")
-    mech = os.path.join(ROOT, "tests", "golden", "mech", "synth_cfg2.mech")
-    therm = os.path.join(ROOT, "tests", "golden", "mech", "synth_cfg2.therm")
+    mech = os.path.join(ROOT, "tests", "golden", "mech", "synth_53sp.mech")
+    therm = os.path.join(ROOT, "tests", "golden", "mech", "synth_53sp.therm")
     m = capi.Mech(mech, therm)
-    mg = mechgen.config_mechanism(2)
+    mg = mechgen.workload_mechanism("53sp")
     n = a.reactors
-    st = mechgen.config_states(2, mg, n, seed=2)
+    st = mechgen.workload_states("53sp", mg, n, seed=2)
     dt, rtol, atol = 1.0e-4, 1.0e-6, 1.0e-12
     nsp = m.nsp
     t = lambda x: torch.as_tensor(np.ascontiguousarray(x), dtype=torch.float64, device=dev)
@@ -144,7 +144,7 @@ def main():
     W = workcount.Work(m)
     ops = W.bdf_ops(s[:, 1].sum(), s[:, 2].sum(), s[:, 3].sum(), s[:, 4].sum())
     achieved = ops / (kern_ms / 1e3) / 1e12
-    traffic = pmc_traffic(os.path.join(ROOT, "profiles", "bdf_kernel_cfg2_65536_pmc.md"))
+    traffic = pmc_traffic(os.path.join(ROOT, "profiles", "bdf_kernel_53sp_65536_pmc.md"))
     roofline = {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
                 "traffic": traffic, "kernel": "bdf_kernel",

@@ -12,7 +12,7 @@ for p in (ROOT, PKG):
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 MECHS = os.path.join(GOLDEN, "mech")
-GOLDEN_CASES = ["synth_cfg1", "synth_cfg2", "synth_cfg3", "ref_h2", "ref_dme",
+GOLDEN_CASES = ["synth_10sp", "synth_53sp", "synth_160sp", "ref_h2", "ref_dme",
                 "ref_h2_no_falloff", "ref_troe_test"]
 
 

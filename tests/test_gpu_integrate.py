@@ -49,16 +49,16 @@ def _referee(o, T0, P, Y0, dt):
     return sol.y[:-1, -1], sol.y[-1, -1]
 
 
-@pytest.mark.parametrize("cfg,n", [(1, 1024), (2, 256)])
-def test_integrate_matches_oracle(torch_gpu, oracle_mod, cfg, n):
+@pytest.mark.parametrize("wl,n,seed", [("10sp", 1024, 101), ("53sp", 256, 102)])
+def test_integrate_matches_oracle(torch_gpu, oracle_mod, wl, n, seed):
     from zerork_hip import capi, mechgen
-    name = f"synth_cfg{cfg}"
+    name = f"synth_{wl}"
     m = capi.Mech(*mech_paths(name))
     o = oracle_mod.Oracle(*mech_paths(name))
-    mg = mechgen.config_mechanism(cfg)
-    st = mechgen.config_states(cfg, mg, n, seed=100 + cfg)
-    dt = mechgen.CONFIG_DT[cfg]
-    print("case cfg", cfg, "n", n, flush=True)
+    mg = mechgen.workload_mechanism(wl)
+    st = mechgen.workload_states(wl, mg, n, seed=seed)
+    dt = mechgen.WORKLOAD_DT[wl]
+    print("case", wl, "n", n, flush=True)
     opts = capi.default_options(rel_tol=RTOL, abs_tol=ATOL)
     Tg, Yg, steps, stats = _gpu_integrate(torch_gpu, m, st.T, st.P, st.Y, dt, opts)
     oopts = oracle_mod.default_opts(rtol=RTOL, atol=ATOL)
@@ -70,7 +70,7 @@ def test_integrate_matches_oracle(torch_gpu, oracle_mod, cfg, n):
     ulp = _units(Yp, Tp, Yo, To, RTOL, ATOL)
     q = [50, 90, 100]
     gq, uq = np.percentile(gpu, q), np.percentile(ulp, q)
-    print(f"cfg{cfg}: GPU-oracle tolerance units p50/p90/max {gq.round(2)}; oracle 1-ulp "
+    print(f"{wl}: GPU-oracle tolerance units p50/p90/max {gq.round(2)}; oracle 1-ulp "
           f"sensitivity {uq.round(2)}; identical step counts "
           f"{np.mean(steps == ost['nsteps']):.3f}; mean steps gpu {steps.mean():.1f} oracle "
           f"{ost['nsteps'].mean():.1f}", flush=True)
@@ -85,7 +85,7 @@ def test_integrate_matches_oracle(torch_gpu, oracle_mod, cfg, n):
         yr, tr = _referee(o, st.T[i], st.P[i], st.Y[i], dt)
         eg.append(_units(Yg[i:i + 1], Tg[i:i + 1], yr[None], np.array([tr]), RTOL, ATOL)[0])
         eo.append(_units(Yo[i:i + 1], To[i:i + 1], yr[None], np.array([tr]), RTOL, ATOL)[0])
-    print(f"cfg{cfg}: error vs referee (tol units) gpu {np.round(eg, 1)} oracle {np.round(eo, 1)}",
+    print(f"{wl}: error vs referee (tol units) gpu {np.round(eg, 1)} oracle {np.round(eo, 1)}",
           flush=True)
     assert max(eg) <= 2.0 * max(eo) + 50.0
 
@@ -95,11 +95,11 @@ def test_integrate_full_batch_properties(torch_gpu):
     finite states, and per-reactor results independent of the batch (a
     reactor integrated alone gives bitwise the same end state)."""
     from zerork_hip import capi, mechgen
-    m = capi.Mech(*mech_paths("synth_cfg2"))
-    mg = mechgen.config_mechanism(2)
-    st = mechgen.config_states(2, mg, 65536, seed=2)
+    m = capi.Mech(*mech_paths("synth_53sp"))
+    mg = mechgen.workload_mechanism("53sp")
+    st = mechgen.workload_states("53sp", mg, 65536, seed=2)
     opts = capi.default_options(rel_tol=RTOL, abs_tol=ATOL)
-    print("case cfg2 full batch", flush=True)
+    print("case 53sp full batch", flush=True)
     Tg, Yg, steps, stats = _gpu_integrate(torch_gpu, m, st.T, st.P, st.Y, 1e-4, opts)
     assert np.all(steps > 0), np.sum(steps <= 0)
     assert np.all(np.isfinite(Yg)) and np.all(np.isfinite(Tg))
@@ -117,9 +117,9 @@ def test_forced_failure_keeps_state(torch_gpu):
     """max_steps=1 makes every try fail (CV_TOO_MUCH_WORK, solver_cvode.cpp:233-235):
     negative cost, input state unchanged (zerork_reactor_manager.cpp:1123-1139)."""
     from zerork_hip import capi, mechgen
-    m = capi.Mech(*mech_paths("synth_cfg1"))
-    mg = mechgen.config_mechanism(1)
-    st = mechgen.config_states(1, mg, 64, seed=9)
+    m = capi.Mech(*mech_paths("synth_10sp"))
+    mg = mechgen.workload_mechanism("10sp")
+    st = mechgen.workload_states("10sp", mg, 64, seed=9)
     opts = capi.default_options(rel_tol=RTOL, abs_tol=ATOL, max_steps=1, cvode_num_retries=3)
     Tg, Yg, steps, stats = _gpu_integrate(torch_gpu, m, st.T, st.P, st.Y, 1e-4, opts)
     assert np.all(steps < 0)

@@ -14,7 +14,7 @@ from conftest import load_golden, mech_paths
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", ["synth_cfg1", "synth_cfg2", "ref_h2", "ref_h2_no_falloff"])
+@pytest.mark.parametrize("name", ["synth_10sp", "synth_53sp", "ref_h2", "ref_h2_no_falloff"])
 def test_jacobian_matches_oracle(torch_gpu, oracle_mod, name):
     from zerork_hip import capi
     torch = torch_gpu

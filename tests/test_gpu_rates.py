@@ -43,7 +43,7 @@ def test_rates_match_reference_golden(torch_gpu, name):
     np.testing.assert_allclose(r0["rop"], g["rop_nolimit"], rtol=1e-12, atol=0)
 
 
-@pytest.mark.parametrize("name", ["synth_cfg1", "synth_cfg2", "synth_cfg3", "ref_h2", "ref_dme"])
+@pytest.mark.parametrize("name", ["synth_10sp", "synth_53sp", "synth_160sp", "ref_h2", "ref_dme"])
 def test_rhs_matches_oracle(torch_gpu, oracle_mod, name):
     from zerork_hip import capi
     g = load_golden(name)
@@ -68,15 +68,15 @@ def test_rates_large_batch_properties(torch_gpu):
     depend on the batch (bitwise equal to a small batch of the same states)
     and dY/dt sums to zero (mass conservation)."""
     from zerork_hip import capi, mechgen
-    m = capi.Mech(*mech_paths("synth_cfg2"))
-    mg = mechgen.config_mechanism(2)
-    st = mechgen.config_states(2, mg, 65536, seed=3)
+    m = capi.Mech(*mech_paths("synth_53sp"))
+    mg = mechgen.workload_mechanism("53sp")
+    st = mechgen.workload_states("53sp", mg, 65536, seed=3)
     r = _run_rates(torch_gpu, m, st.T, st.P, st.Y)
     idx = np.array([0, 17, 4095, 65535])
     rs = _run_rates(torch_gpu, m, st.T[idx], st.P[idx], st.Y[idx])
     for k in rs:
         np.testing.assert_array_equal(rs[k], r[k][idx])
-    mw = load_golden("synth_cfg2")["mw"]
+    mw = load_golden("synth_53sp")["mw"]
     s = np.abs(r["ydot"][:, :-1]).sum(axis=1)
     assert np.all(np.abs(r["ydot"][:, :-1].sum(axis=1)) <= 1e-12 * s + 1e-300)
     assert np.all(np.isfinite(r["rop"]))

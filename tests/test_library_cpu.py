@@ -81,7 +81,7 @@ def test_plugin_options_and_status(product_lib, tmp_path):
     bad.write_text("ELEMENTS\nH\nEND\nSPECIES\nH2\nEND\nREACTIONS\nH2 => H2 + ZZ 1 0 0\nEND\n")
     assert p.set_mechanism_files(str(bad), mech_paths("ref_h2")[1]) == 0
     assert p.load_mechanism() == 5  # FAILED_MECHANISM_PARSE
-    m, t = mech_paths("synth_cfg1")
+    m, t = mech_paths("synth_10sp")
     assert p.set_mechanism_files(m, t) == 0
     assert p.load_mechanism() == 0
     assert p.free() == 0

@@ -19,8 +19,8 @@ from conftest import load_golden, mech_paths
 
 
 def test_rhs_invariants(oracle_mod):
-    o = oracle_mod.Oracle(*mech_paths("synth_cfg2"))
-    g = load_golden("synth_cfg2")
+    o = oracle_mod.Oracle(*mech_paths("synth_53sp"))
+    g = load_golden("synth_53sp")
     t = o.tables
     for i in range(len(g["T"])):
         T, P, Y = g["T"][i], g["P"][i], g["Y"][i]
@@ -104,8 +104,8 @@ def test_jacobian_ignores_third_body_dependence(oracle_mod):
     """With third-body/fall-off reactions the reference Jacobian holds M (and Pr)
     fixed: the species block must equal the derivative of the rates with the
     third-body multiplier frozen, which we build from the rates of progress."""
-    o = oracle_mod.Oracle(*mech_paths("synth_cfg1"))
-    g = load_golden("synth_cfg1")
+    o = oracle_mod.Oracle(*mech_paths("synth_10sp"))
+    g = load_golden("synth_10sp")
     t = o.tables
     T, P, Y = g["T"][0], g["P"][0], g["Y"][0]
     y = np.append(Y, T)
@@ -165,9 +165,9 @@ def test_bdf_matches_scipy_referee(oracle_mod):
     shrinks as the tolerance is tightened, at rtol 1e-9 it is below 1e-6
     relative, and sum(Y) = 1 is kept to 1e-9."""
     from zerork_hip import mechgen
-    o = oracle_mod.Oracle(*mech_paths("synth_cfg1"))
-    m = mechgen.config_mechanism(1)
-    st = mechgen.config_states(1, m, 6, seed=11)
+    o = oracle_mod.Oracle(*mech_paths("synth_10sp"))
+    m = mechgen.workload_mechanism("10sp")
+    st = mechgen.workload_states("10sp", m, 6, seed=11)
     refs = [_scipy_reference(o, st.T[i], st.P[i], st.Y[i], 1e-4) for i in range(len(st.T))]
     errs = []
     for rtol in (1e-6, 1e-9):
@@ -192,9 +192,9 @@ def test_bdf_failure_keeps_state(oracle_mod):
     """max_steps=1 forces CV_TOO_MUCH_WORK on every try (solver_cvode.cpp:233-235):
     negative cost, input state kept."""
     from zerork_hip import mechgen
-    o = oracle_mod.Oracle(*mech_paths("synth_cfg1"))
-    m = mechgen.config_mechanism(1)
-    st = mechgen.config_states(1, m, 3, seed=5)
+    o = oracle_mod.Oracle(*mech_paths("synth_10sp"))
+    m = mechgen.workload_mechanism("10sp")
+    st = mechgen.workload_states("10sp", m, 3, seed=5)
     opts = oracle_mod.default_opts(rtol=1e-6, atol=1e-12, max_steps=1, num_retries=3)
     T, Y, stats = o.integrate(st.T, st.P, st.Y, 1e-4, opts)
     assert np.all(stats["nsteps"] < 0)

@@ -79,9 +79,11 @@ __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s
     if (sp) {
       const int b = jp[c * nsp + lane], e = jp[c * nsp + lane + 1];
       for (int k = b; k < e; ++k) {
+        // product rounded once, then added or subtracted as the reference's
+        // J -= ROP*invC / J += ROP*invC, so balanced terms cancel exactly
         const int t = jt[k];
-        const double r = s.rop[(t < 0 ? -t : t) - 1];
-        v = (t < 0) ? fma(-r, t2c, v) : fma(r, t2c, v);
+        const double p = __dmul_rn(s.rop[(t < 0 ? -t : t) - 1], t2c);
+        v = (t < 0) ? __dsub_rn(v, p) : __dadd_rn(v, p);
       }
     }
     // d(Tdot)/dy_c from the concentration-space column

@@ -350,24 +350,29 @@ def generate(n_species: int, n_reactions: int, seed: int, elements=("H", "O", "N
 _C2_BASE = ["H2", "O2", "N2", "H2O", "CO", "CO2", "CH4", "AR"]
 
 
-def config_mechanism(cfg: int, seed: int | None = None) -> Mechanism:
-    if cfg == 1:
+WORKLOADS = ("10sp", "53sp", "160sp")
+
+
+def workload_mechanism(name: str, seed: int | None = None) -> Mechanism:
+    """Seeded synthetic mechanism of a named workload: "10sp" (H/O/N, 27 reactions),
+    "53sp" (C/H/O/N/Ar, 325 reactions) or "160sp" (C/H/O/N/Ar, 900 reactions)."""
+    if name == "10sp":
         base = ["H2", "O2", "H2O", "OH", "H", "O", "HO2", "H2O2", "O3", "N2"]
         return generate(10, 27, seed if seed is not None else 1001, ("H", "O", "N"),
                         0.25, 0.15, inert=("N2",), majors=["H2", "O2", "H2O", "OH"],
                         base=base)
-    if cfg == 2:
+    if name == "53sp":
         # seed 1056: chosen by a scan of seeds 1000-1059 (64 reactors each) as
         # the mechanism whose cold mode (700-950 K) stays quiescent over dt
         # while combustible hot-mode (1050-1400 K) reactors ignite within dt
         return generate(53, 325, seed if seed is not None else 1056,
                         ("C", "H", "O", "N", "AR"), 0.25, 0.15, inert=("N2", "AR"),
                         majors=_C2_BASE, base=_C2_BASE)
-    if cfg == 3:
+    if name == "160sp":
         return generate(160, 900, seed if seed is not None else 1003,
                         ("C", "H", "O", "N", "AR"), 0.30, 0.10, inert=("N2", "AR"),
                         majors=_C2_BASE + ["C2H6", "CH3OH"], base=_C2_BASE)
-    raise ValueError(f"unknown config {cfg}")
+    raise ValueError(f"unknown workload {name}")
 
 
 @dataclass
@@ -378,32 +383,32 @@ class States:
     seed: int
 
 
-def config_states(cfg: int, mech: Mechanism, n: int, seed: int) -> States:
-    """Seeded reactor states of a workload configuration."""
+def workload_states(name: str, mech: Mechanism, n: int, seed: int) -> States:
+    """Seeded reactor states of a named workload."""
     rng = np.random.default_rng(seed)
     nsp = len(mech.species)
     idx = {s: i for i, s in enumerate(mech.species)}
     Y = np.zeros((n, nsp))
-    if cfg == 1:
+    if name == "10sp":
         T = rng.uniform(900.0, 1500.0, n)
         P = rng.uniform(1.0, 20.0, n) * 101325.0
         maj = [idx[s] for s in mech.majors]
         frac = rng.uniform(0.3, 0.7, n)
         Y[:, maj] = rng.dirichlet(np.ones(len(maj)), n) * frac[:, None]
         Y[:, idx["N2"]] = 1.0 - frac
-    elif cfg == 2:
+    elif name == "53sp":
         slow = rng.random(n) < 0.5
         T = np.where(slow, rng.uniform(700.0, 950.0, n), rng.uniform(1050.0, 1400.0, n))
         P = np.exp(rng.uniform(math.log(1.0), math.log(50.0), n)) * 101325.0
         maj = [idx[s] for s in mech.majors]
         Y[:, maj] = rng.dirichlet(0.5 * np.ones(len(maj)), n)
-    elif cfg == 3:
+    elif name == "160sp":
         T = rng.uniform(800.0, 1600.0, n)
         P = np.exp(rng.uniform(math.log(1.0), math.log(40.0), n)) * 101325.0
         maj = [idx[s] for s in mech.majors]
         Y[:, maj] = rng.dirichlet(0.5 * np.ones(len(maj)), n)
     else:
-        raise ValueError(cfg)
+        raise ValueError(name)
     # renormalise in extended precision so that sum(Y) = 1 to the last bit
     Y = Y / Y.sum(axis=1, keepdims=True)
     return States(T, P, np.ascontiguousarray(Y), seed)
@@ -420,5 +425,5 @@ def random_dense_states(mech: Mechanism, n: int, seed: int, Tlo=800.0, Thi=2200.
     return States(T, P, Y, seed)
 
 
-CONFIG_DT = {1: 1.0e-4, 2: 1.0e-4, 3: 1.0e-5}
-CONFIG_NR = {1: 1024, 2: 65536, 3: 16384}
+WORKLOAD_DT = {"10sp": 1.0e-4, "53sp": 1.0e-4, "160sp": 1.0e-5}
+WORKLOAD_NR = {"10sp": 1024, "53sp": 65536, "160sp": 16384}
