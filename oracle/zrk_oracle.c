@@ -279,14 +279,22 @@ static void reactor_jacobian(reactor_t* r, double t, const double* y, const doub
   t1[nsp] = y[nsp];
   reactor_rhs(r, t, t1, t3);
   memset(J, 0, sizeof(double) * (size_t)N * N);
-  for (int s = 0; s < m->nstep; ++s) {
+  /* every destruction term first, then every creation term, each list in
+   * (step, perturbed reactant, row) order: reactor_nvector_serial.cpp:146-172
+   * builds the lists in that order and :655-669 applies them one after the
+   * other, so an entry fed by both lists sums all its destructions first */
+  for (int s = 0; s < m->nstep; ++s)
     for (int k = m->reac_ptr[s]; k < m->reac_ptr[s + 1]; ++k) {
       const int col = m->reac_sp[k];
-      const double v = r->rop[s] * t2[col];
-      for (int q = m->reac_ptr[s]; q < m->reac_ptr[s + 1]; ++q) J[col * N + m->reac_sp[q]] -= v;
-      for (int q = m->prod_ptr[s]; q < m->prod_ptr[s + 1]; ++q) J[col * N + m->prod_sp[q]] += v;
+      for (int q = m->reac_ptr[s]; q < m->reac_ptr[s + 1]; ++q)
+        J[col * N + m->reac_sp[q]] -= r->rop[s] * t2[col];
     }
-  }
+  for (int s = 0; s < m->nstep; ++s)
+    for (int k = m->reac_ptr[s]; k < m->reac_ptr[s + 1]; ++k) {
+      const int col = m->reac_sp[k];
+      for (int q = m->prod_ptr[s]; q < m->prod_ptr[s + 1]; ++q)
+        J[col * N + m->prod_sp[q]] += r->rop[s] * t2[col];
+    }
   /* d(Tdot)/dy_j */
   const double RuTemp = NIST_RU * y[nsp];
   for (int j = 0; j < nsp; ++j) {

@@ -2,7 +2,8 @@
 C oracle's restatement of reactor_nvector_serial.cpp:610-771.
 
 Species columns and the temperature row: per entry 1e-10 of
-max(|J_ij|, 1e-3 * row max).  The temperature column is a forward finite
+max(|J_ij|, 1e-3 * row max, 1e-6 * column max) (the column floor covers entries
+whose terms cancel analytically and keep a round-off residue).  The temperature column is a forward finite
 difference (relative step sqrt(eps)), so it inherits round-off amplified by
 1/sqrt(eps): compared at 1e-6 of the column maximum.
 """
@@ -33,7 +34,8 @@ def test_jacobian_matches_oracle(torch_gpu, oracle_mod, name):
     for i in range(n):
         Jo = o.jacobian(g["P"][i], np.append(g["Y"][i], g["T"][i]))
         A, B = Jg[i][:, :nsp], Jo[:, :nsp]
-        scale = np.maximum(np.abs(B), 1e-3 * np.max(np.abs(B), axis=1, keepdims=True))
+        scale = np.maximum(np.maximum(np.abs(B), 1e-3 * np.max(np.abs(B), axis=1, keepdims=True)),
+                           1e-6 * np.max(np.abs(B), axis=0, keepdims=True))
         assert np.all(np.abs(A - B) <= 1e-10 * scale + 1e-300), (i, np.max(np.abs(A - B) / (scale + 1e-300)))
         cmax = np.max(np.abs(Jo[:, nsp]))
         assert np.all(np.abs(Jg[i][:, nsp] - Jo[:, nsp]) <= 1e-6 * cmax)

@@ -37,12 +37,18 @@ constexpr int kQmax = 5;
 // copy to HBM (savedJ).  inv_rho_prev is the inverse density of the last RHS
 // call (at y), which the reference reuses for the 1/C factors
 // (reactor_nvector_serial.cpp:625-637).  Returns the inverse density of the
-// last RHS evaluated here (the reference's carried value).
+// last RHS evaluated here (the reference's carried value).  t2l: kNP doubles
+// of LDS scratch.
+//
+// Species block: lane i walks row i's term list (sorted by column), so each
+// entry has one writer and the loads of the list are coalesced across lanes;
+// the temperature row and the MW_i/MW_j scaling follow column by column.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s, double P,
                                                const BdfOpts& o, double y, double fy,
                                                double inv_rho_prev, double* __restrict__ Jl,
-                                               double* __restrict__ saved) {
+                                               double* __restrict__ saved,
+                                               double* __restrict__ t2l) {
   const int lane = lane_id();
   const int nsp = m.nsp;
   const bool sp = lane < nsp;
@@ -51,6 +57,7 @@ __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s
   double t1 = y;
   if (sp && fabs(t1) < mmf) t1 = (t1 < 0.0) ? -mmf : mmf;
   const double t2 = sp ? m.mw[lane] / t1 * inv_rho_prev : 0.0;
+  t2l[lane] = t2;
   // d(ydot)/dT by forward difference at the unfloored state.  Evaluated
   // first: it does not depend on the floored evaluation, and its inverse
   // density is the one carried on (it is the reference's last call).
@@ -71,23 +78,46 @@ __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s
   const double RuTemp = kRu * yT;
   const double mult = -1.0 / aux.cp_mix;
   const double mwi = sp ? m.mw[lane] : 0.0;
-  const int* __restrict__ jp = m.jt_ptr;
-  const int* __restrict__ jt = m.jt_term;
-  for (int c = 0; c < nsp; ++c) {
-    const double t2c = bcast(t2, c);
+  for (int c = 0; c < nsp; ++c) Jl[c * kNP + lane] = 0.0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  {
+    // each product ROP*invC is rounded once, then added or subtracted as the
+    // reference's J -= ROP*invC / J += ROP*invC, so balanced terms cancel exactly
+    const uint4* __restrict__ jr = m.jr + lane;
+    const int nb = m.jr_blocks;
+    int cur = -1;
     double v = 0.0;
-    if (sp) {
-      const int b = jp[c * nsp + lane], e = jp[c * nsp + lane + 1];
-      for (int k = b; k < e; ++k) {
-        // product rounded once, then added or subtracted as the reference's
-        // J -= ROP*invC / J += ROP*invC, so balanced terms cancel exactly
-        const int t = jt[k];
-        const double p = __dmul_rn(s.rop[(t < 0 ? -t : t) - 1], t2c);
-        v = (t < 0) ? __dsub_rn(v, p) : __dadd_rn(v, p);
+    auto term = [&](unsigned t) {
+      const int col = (int)((t >> 16) & 0xFFu);
+      if (col != 0xFF) {
+        if (col != cur) {
+          cur = col;
+          v = 0.0;
+        }
+        const double p = __dmul_rn(s.rop[t & 0xFFFFu], t2l[col]);
+        v = (t & 0x80000000u) ? __dsub_rn(v, p) : __dadd_rn(v, p);
+        Jl[col * kNP + lane] = v;
       }
+    };
+    uint4 w = nb > 0 ? jr[0] : make_uint4(0, 0, 0, 0);
+    for (int b = 0; b < nb; ++b) {
+      const uint4 x = w;
+      if (b + 1 < nb) w = jr[(b + 1) * kWave];
+      term(x.x);
+      term(x.y);
+      term(x.z);
+      term(x.w);
     }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int c = 0; c < nsp; ++c) {
+    const double v = sp ? Jl[c * kNP + lane] : 0.0;
     // d(Tdot)/dy_c from the concentration-space column
-    const double sc = wave_sum(sp ? v * aux.h : 0.0);
+    const double sc = wave_sum(v * aux.h);
     double val = sc * RuTemp;
     val += Tdot * bcast(aux.cpm, c) * m.mw[c];
     val *= m.inv_mw[c] * mult;

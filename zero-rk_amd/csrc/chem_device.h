@@ -81,31 +81,74 @@ __device__ __forceinline__ TPow make_tpow(double T) {
   return p;
 }
 
-// NASA-7 cp/R, h/RT, g/RT of one species (coefficient row c[16]).
+// NASA-7 cp/R, h/RT, g/RT of one species (coefficient row c[16]).  Both
+// ranges are loaded before the T < Tmid choice so the loads issue together.
 __device__ __forceinline__ void thermo_one(const double* __restrict__ c, const TPow& p,
                                            double& cp, double& h, double& g) {
-  const double* a = (p.T < c[0]) ? c + 1 : c + 8;
-  const double a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3], a4 = a[4], a5 = a[5], a6 = a[6];
+  double lo[7], hi[7];
+  const double tmid = c[0];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    lo[k] = c[1 + k];
+    hi[k] = c[8 + k];
+  }
+  const bool low = p.T < tmid;
+  double a[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) a[k] = low ? lo[k] : hi[k];
   const double T = p.T;
-  cp = a0 + T * (a1 + T * (a2 + T * (a3 + T * a4)));
-  h = a0 + a1 * p.hm0 + a2 * p.hm1 + a3 * p.hm2 + a4 * p.hm3 + a5 * p.invT;
-  g = a0 * p.gm0 + a1 * p.gm1 + a2 * p.gm2 + a3 * p.gm3 + a4 * p.gm4 + a5 * p.invT - a6;
+  cp = a[0] + T * (a[1] + T * (a[2] + T * (a[3] + T * a[4])));
+  h = a[0] + a[1] * p.hm0 + a[2] * p.hm1 + a[3] * p.hm2 + a[4] * p.hm3 + a[5] * p.invT;
+  g = a[0] * p.gm0 + a[1] * p.gm1 + a[2] * p.gm2 + a[3] * p.gm3 + a[4] * p.gm4 + a[5] * p.invT -
+      a[6];
+}
+
+__device__ __forceinline__ int lo16(int w) { return (int)(short)(w & 0xFFFF); }
+__device__ __forceinline__ int hi16(int w) { return w >> 16; }
+
+// One reaction record as loaded from HBM (all fields issued together).
+struct RecLoad {
+  int4 meta, sp;
+  double2 c0, c1, c2;
+  int e0, e1;
+};
+
+__device__ __forceinline__ RecLoad load_rec(const DevMech& m, int r) {
+  RecLoad x;
+  x.meta = m.r_meta[r];
+  x.sp = m.r_sp[r];
+  x.c0 = m.r_coef[3 * r];
+  x.c1 = m.r_coef[3 * r + 1];
+  x.c2 = m.r_coef[3 * r + 2];
+  x.e0 = m.r_eptr[r];
+  x.e1 = m.r_eptr[r + 1];
+  return x;
 }
 
 // Rate constants, limiter and rates of progress of every step of one reactor.
-// C, G: species concentrations and g/RT in the wave's LDS; rop: [nstep] LDS.
+// C, G: species concentrations and g/RT in the wave's LDS; rop: [nstep+1] LDS
+// (slot nstep is the zero rate the padded gathers point at).
+// Lane l handles records l, l+64, ...; the next record is loaded while the
+// current one is evaluated.
 __device__ __forceinline__ void wave_rates(const DevMech& m, const double* __restrict__ C,
                                            const double* __restrict__ G, double* __restrict__ rop,
                                            const TPow& p, double Csum, double L) {
   const bool limit = (0.0 < L && L < 1.0e300);
-  for (int r = lane_id(); r < m.nrec; r += kWave) {
-    const int fl = m.r_flags[r];
-    double kf = exp(m.r_fA[r] + m.r_fB[r] * p.logT - m.r_fT[r] * p.invT);
+  const int nrec = m.nrec;
+  if (lane_id() == 0) rop[m.nstep] = 0.0;
+  int r = lane_id();
+  RecLoad nx;
+  if (r < nrec) nx = load_rec(m, r);
+  for (; r < nrec; r += kWave) {
+    const RecLoad x = nx;
+    if (r + kWave < nrec) nx = load_rec(m, r + kWave);
+    const int fl = x.meta.z;
+    const int re[kMaxPart] = {lo16(x.sp.x), hi16(x.sp.x), lo16(x.sp.y), hi16(x.sp.y)};
+    const int pr[kMaxPart] = {lo16(x.sp.z), hi16(x.sp.z), lo16(x.sp.w), hi16(x.sp.w)};
+    double kf = exp(x.c0.x + x.c0.y * p.logT - x.c1.x * p.invT);
     double kr = 0.0;
-    const int* re = m.r_reac + kMaxPart * r;
-    const int* pr = m.r_prod + kMaxPart * r;
     if (fl & kRecRevArr) {
-      kr = exp(m.r_rA[r] + m.r_rB[r] * p.logT - m.r_rT[r] * p.invT);
+      kr = exp(x.c1.y + x.c2.x * p.logT - x.c2.y * p.invT);
     } else if (fl & kRecRevKeq) {
       double s = 0.0;
 #pragma unroll
@@ -114,16 +157,16 @@ __device__ __forceinline__ void wave_rates(const DevMech& m, const double* __res
 #pragma unroll
       for (int k = 0; k < kMaxPart; ++k)
         if (re[k] >= 0) s -= G[re[k]];
-      kr = exp(s - m.r_rA[r] * p.log_patm_rut) * kf;
+      kr = exp(s - x.c1.y * p.log_patm_rut) * kf;
     }
     if (fl & (kRecThirdBody | kRecFalloff)) {
       double cm;
-      const int fs = m.r_fspc[r];
+      const int fs = x.meta.w;
       if (fs >= 0) {
         cm = C[fs];
       } else {
         cm = Csum;
-        for (int k = m.r_eptr[r]; k < m.r_eptr[r + 1]; ++k) cm += C[m.e_sp[k]] * m.e_eff[k];
+        for (int k = x.e0; k < x.e1; ++k) cm += C[m.e_sp[k]] * m.e_eff[k];
       }
       if (fl & kRecThirdBody) {
         kf *= cm;
@@ -161,8 +204,8 @@ __device__ __forceinline__ void wave_rates(const DevMech& m, const double* __res
 #pragma unroll
     for (int k = 0; k < kMaxPart; ++k)
       if (re[k] >= 0) kf *= C[re[k]];
-    rop[m.r_fstep[r]] = kf;
-    const int rs = m.r_rstep[r];
+    rop[x.meta.x] = kf;
+    const int rs = x.meta.y;
     if (rs >= 0) {
 #pragma unroll
       for (int k = 0; k < kMaxPart; ++k)
@@ -172,12 +215,43 @@ __device__ __forceinline__ void wave_rates(const DevMech& m, const double* __res
   }
 }
 
-// Creation and destruction rates of species j (gather, reference summation order).
+// Creation and destruction rates of species j (gathers in the reference's
+// summation order; padding entries add the zero rate, which is exact).
 __device__ __forceinline__ void species_cd(const DevMech& m, const double* __restrict__ rop, int j,
                                            double& create, double& destroy) {
+  const int jb = j >> 6, jl = j & 63;
+  const uint4* __restrict__ gd = m.g_d + (size_t)jb * m.gd_blocks * kWave + jl;
+  const uint4* __restrict__ gc = m.g_c + (size_t)jb * m.gc_blocks * kWave + jl;
   double d = 0.0, c = 0.0;
-  for (int k = m.d_ptr[j]; k < m.d_ptr[j + 1]; ++k) d += rop[m.d_step[k]];
-  for (int k = m.c_ptr[j]; k < m.c_ptr[j + 1]; ++k) c += rop[m.c_step[k]];
+  const int nd = m.gd_blocks, nc = m.gc_blocks;
+  const int nb = nd > nc ? nd : nc;
+  uint4 wd = nd > 0 ? gd[0] : make_uint4(0, 0, 0, 0);
+  uint4 wc = nc > 0 ? gc[0] : make_uint4(0, 0, 0, 0);
+  for (int b = 0; b < nb; ++b) {
+    const uint4 xd = wd, xc = wc;
+    if (b + 1 < nd) wd = gd[(b + 1) * kWave];
+    if (b + 1 < nc) wc = gc[(b + 1) * kWave];
+    if (b < nd) {
+      d += rop[xd.x & 0xFFFF];
+      d += rop[xd.x >> 16];
+      d += rop[xd.y & 0xFFFF];
+      d += rop[xd.y >> 16];
+      d += rop[xd.z & 0xFFFF];
+      d += rop[xd.z >> 16];
+      d += rop[xd.w & 0xFFFF];
+      d += rop[xd.w >> 16];
+    }
+    if (b < nc) {
+      c += rop[xc.x & 0xFFFF];
+      c += rop[xc.x >> 16];
+      c += rop[xc.y & 0xFFFF];
+      c += rop[xc.y >> 16];
+      c += rop[xc.z & 0xFFFF];
+      c += rop[xc.z >> 16];
+      c += rop[xc.w & 0xFFFF];
+      c += rop[xc.w >> 16];
+    }
+  }
   create = c;
   destroy = d;
 }

@@ -29,7 +29,20 @@ enum {
   kCvConvFailure = -4, kCvReptdRhsErr = -10, kCvIllInput = -22
 };
 
+// Optional per-phase cycle counters (build with -DZRK_PHASE_CYCLES): RHS,
+// Jacobian, Newton-matrix inverse and Newton solves, in shader clock cycles.
+#ifdef ZRK_PHASE_CYCLES
+#define ZRK_TIC(t) const long long t = (long long)__builtin_amdgcn_s_memtime()
+#define ZRK_TOC(t, i) cyc[i] += (long long)__builtin_amdgcn_s_memtime() - t
+#else
+#define ZRK_TIC(t)
+#define ZRK_TOC(t, i)
+#endif
+
 struct CvWave {
+#ifdef ZRK_PHASE_CYCLES
+  long long cyc[4];
+#endif
   DevMech mech;
   RhsLds s;
   BdfOpts opt;
@@ -61,7 +74,9 @@ struct CvWave {
   __device__ __forceinline__ int f(double yy, double& yd) {
     nfe++;
     RhsAux aux;
+    ZRK_TIC(t0);
     const int fl = rhs64(mech, s, P, opt.Tref, opt.step_limiter, yy, yd, aux);
+    ZRK_TOC(t0, 0);
     if (!fl) inv_rho_last = aux.inv_rho;
     if (!active) yd = 0.0;
     return fl;
@@ -358,13 +373,17 @@ struct CvWave {
       jc = 1;
       nje++;
       nstlj = nst;
-      inv_rho_last = jacobian_lds(mech, s, P, opt, ypred, fpred, inv_rho_last, Mlds, savedJ);
+      ZRK_TIC(t0);
+      inv_rho_last = jacobian_lds(mech, s, P, opt, ypred, fpred, inv_rho_last, Mlds, savedJ,
+                                  prow);
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
       for (int c = 0; c < kNP; ++c) R[c] = Mlds[c * kNP + lane];
+      ZRK_TOC(t0, 1);
     }
+    ZRK_TIC(t1);
     const double ng = -gamma;
 #pragma unroll
     for (int c = 0; c < kNP; ++c) {
@@ -377,11 +396,13 @@ struct CvWave {
     for (int c = 0; c < kNP; ++c) Mlds[c * kNP + lane] = R[c];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
+    ZRK_TOC(t1, 2);
     return ok ? 0 : 1;
   }
 
   // x = M^{-1} b with the rotated inverse in LDS
-  __device__ __forceinline__ double lsolve(double b) const {
+  __device__ __forceinline__ double lsolve(double b) {
+    ZRK_TIC(t0);
     const int lane = lane_id();
     const int N = mech.nsp + 1;
     rvec[lane] = b;
@@ -392,6 +413,7 @@ struct CvWave {
 #pragma unroll 8
     for (int c = 0; c < kNP; ++c) acc = fma(Mlds[c * kNP + lane], rvec[(c + N) & (kNP - 1)], acc);
     __builtin_amdgcn_wave_barrier();
+    ZRK_TOC(t0, 3);
     return acc;
   }
 

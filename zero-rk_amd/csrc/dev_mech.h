@@ -8,6 +8,8 @@
 
 #include <cstdint>
 
+#include <hip/hip_vector_types.h>
+
 namespace zrk {
 
 constexpr int kMaxPart = 4;  // max expanded reactants/products per step on device
@@ -31,38 +33,33 @@ enum RecFlags : int {
 
 struct DevMech {
   int nsp, nstep, nrxn, nrec;
-  int max_dlist, max_clist;  // longest per-species destroy/create list
+  int gd_blocks, gc_blocks;  // 8-entry blocks of the padded destroy/create gathers
+  int jr_blocks;             // 4-entry blocks of the per-row Jacobian term lists
   // species [nsp]
   const double* mw;
   const double* inv_mw;   // 1/mw
   const double* ru_mw;    // Ru/mw
   const double* thermo;   // [nsp*16]
   // reaction records [nrec]
-  const int* r_fstep;
-  const int* r_rstep;     // -1 if irreversible
-  const int* r_flags;
-  const double* r_fA;     // ln A, beta, Ta of the forward step
-  const double* r_fB;
-  const double* r_fT;
-  const double* r_rA;     // reverse: ln A, beta, Ta (REV) or ndelta in r_rA (Keq)
-  const double* r_rB;
-  const double* r_rT;
-  const int* r_reac;      // [nrec*kMaxPart], -1 padded (forward-direction reactants)
-  const int* r_prod;      // [nrec*kMaxPart]
+  const int4* r_meta;     // {forward step, reverse step or -1, flags, falloff species or -1}
+  const int4* r_sp;       // 16-bit species indices (-1 = none): {re0|re1, re2|re3, pr0|pr1, pr2|pr3}
+  const double2* r_coef;  // [nrec*3]: {fwd lnA, beta}, {fwd Ta, rev lnA}, {rev beta, rev Ta};
+                          // for a Keq reverse the reverse lnA slot holds ndelta
   const int* r_eptr;      // [nrec+1] enhanced third-body list
   const int* e_sp;
   const double* e_eff;    // efficiency - 1
-  const int* r_fspc;      // falloff single species or -1
   const double* r_fo;     // [nrec*7] falloff parameters
-  // per-species gather lists (step indices, multiplicity repeated, step order)
-  const int* d_ptr;       // [nsp+1]
-  const int* d_step;
-  const int* c_ptr;       // [nsp+1]
-  const int* c_step;
-  // Jacobian species-block terms: entry (row i, col c) -> terms
-  // jt_ptr[c*nsp + i] .. jt_ptr[c*nsp+i+1]; term = -(s+1) destruction, +(s+1) creation
-  const int* jt_ptr;
-  const int* jt_term;
+  // per-species gathers, padded to a common length so the 64 lanes of a
+  // species block read one contiguous line per block of 8 entries:
+  // g_d[(jb*gd_blocks + b)*64 + (j&63)] holds 8 x uint16 step indices of
+  // species j = 64*jb + (j&63), in step order; padding index nstep (rate 0)
+  const uint4* g_d;
+  const uint4* g_c;
+  // Jacobian species-block terms by row (nsp <= 63): jr[b*64 + row] holds 4
+  // terms, each bits 0-15 step, 16-23 column (0xFF = padding), bit 31 set for
+  // a destruction term; sorted by column, then destruction before creation,
+  // each in step order (the reference's accumulation order)
+  const uint4* jr;
 };
 
 }  // namespace zrk

@@ -30,7 +30,7 @@ rates_kernel(DevMech m, int n, const double* __restrict__ Tin, const double* __r
   const int wave = threadIdx.x / kWave;
   const int lane = lane_id();
   const int nsp = m.nsp, nstep = m.nstep;
-  const int per_wave = 3 * nsp + nstep;
+  const int per_wave = 3 * nsp + nstep + 1;
   double* C = lds + wave * per_wave;
   double* G = C + nsp;
   double* H = G + nsp;
@@ -95,7 +95,8 @@ jacobian_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ Tin,
   const int lane = lane_id();
   const int nsp = m.nsp, N = nsp + 1;
   double* Jl = lds;
-  double* rl = lds + kNP * kNP;
+  double* t2l = lds + kNP * kNP;
+  double* rl = t2l + kNP;
   RhsLds s{rl, rl + nsp, rl + 2 * nsp};
   double* saved = jscratch + (size_t)blockIdx.x * kNP * kNP;
   for (int r = blockIdx.x; r < n; r += gridDim.x) {
@@ -106,7 +107,7 @@ jacobian_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ Tin,
     double fy;
     RhsAux aux;
     rhs64(m, s, P, o.Tref, o.step_limiter, y, fy, aux);
-    jacobian_lds(m, s, P, o, y, fy, aux.inv_rho, Jl, saved);
+    jacobian_lds(m, s, P, o, y, fy, aux.inv_rho, Jl, saved, t2l);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -157,6 +158,9 @@ bdf_kernel(DevMech m, BdfOpts o, int n, double dt, double* __restrict__ Tio,
     cv.atolv = cv.active ? o.atol : 0.0;
     cv.rtol = o.rtol;
     cv.inv_rho_last = 0.0;
+#ifdef ZRK_PHASE_CYCLES
+    cv.cyc[0] = cv.cyc[1] = cv.cyc[2] = cv.cyc[3] = 0;
+#endif
     cv.reinit(0.0, y0);
     int nsteps = 0, num_tries = 0, flag = kCvSuccess;
     int tot[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -188,9 +192,18 @@ bdf_kernel(DevMech m, BdfOpts o, int n, double dt, double* __restrict__ Tio,
     if (lane == 0) {
       if (nsteps_out) nsteps_out[r] = nsteps;
       if (stats_out) {
+#ifdef ZRK_PHASE_CYCLES
+        // profiling layout: 16 ints per reactor, cycles / 16 in slots 8..11
+#pragma unroll
+        for (int k = 0; k < 7; ++k) stats_out[(size_t)r * 16 + k] = tot[k];
+        stats_out[(size_t)r * 16 + 7] = flag;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) stats_out[(size_t)r * 16 + 8 + k] = (int)(cv.cyc[k] >> 4);
+#else
 #pragma unroll
         for (int k = 0; k < 7; ++k) stats_out[(size_t)r * 8 + k] = tot[k];
         stats_out[(size_t)r * 8 + 7] = flag;
+#endif
       }
     }
   }
@@ -207,7 +220,7 @@ hipError_t launch_rates(const DevMech& m, int n, const double* T, const double* 
                         const double* Y, int stride, double L, double* ydot, double* rop,
                         double* create, double* destroy, double* net, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  const size_t lds = sizeof(double) * (size_t)kRatesWaves * (3 * m.nsp + m.nstep);
+  const size_t lds = sizeof(double) * (size_t)kRatesWaves * (3 * m.nsp + m.nstep + 1);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   int blocks = (n + kRatesWaves - 1) / kRatesWaves;
   hipLaunchKernelGGL(rates_kernel, dim3(blocks), dim3(kWave * kRatesWaves), lds, st, m, n, T,
@@ -215,7 +228,9 @@ hipError_t launch_rates(const DevMech& m, int n, const double* T, const double* 
   return hipGetLastError();
 }
 
-size_t rhs_lds_bytes(const DevMech& m) { return sizeof(double) * (size_t)(2 * m.nsp + m.nstep); }
+size_t rhs_lds_bytes(const DevMech& m) {
+  return sizeof(double) * (size_t)(2 * m.nsp + m.nstep + 1);
+}
 size_t bdf_lds_bytes(const DevMech& m) {
   return sizeof(double) * (size_t)(kNP * kNP + 2 * kNP) + rhs_lds_bytes(m);
 }
@@ -226,7 +241,7 @@ hipError_t launch_jacobian(const DevMech& m, const BdfOpts& o, int n, const doub
   if (n <= 0) return hipSuccess;
   if (m.nsp > kNP - 1) return hipErrorInvalidValue;
   const int grid = nblocks < n ? nblocks : n;
-  const size_t lds = sizeof(double) * (size_t)kNP * kNP + rhs_lds_bytes(m);
+  const size_t lds = sizeof(double) * (size_t)(kNP * kNP + kNP) + rhs_lds_bytes(m);
   hipLaunchKernelGGL(jacobian_kernel, dim3(grid), dim3(kWave), lds, st, m, o, n, T, P, Y,
                      stride, J, jscratch);
   return hipGetLastError();

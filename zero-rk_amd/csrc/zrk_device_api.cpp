@@ -49,96 +49,126 @@ int build_device_mech(MechHandle* h, std::string* err) {
       if (kind == pass) order.push_back(j);
     }
   const int nrec = (int)order.size();
-  std::vector<int> fstep(nrec), rstep(nrec), flags(nrec), reac(nrec * kMaxPart, -1),
-      prod(nrec * kMaxPart, -1), eptr(nrec + 1, 0), fspc(nrec, -1), esp;
-  std::vector<double> fA(nrec), fB(nrec), fT(nrec), rA(nrec, 0.0), rB(nrec, 0.0), rT(nrec, 0.0),
-      fo(nrec * 7, 0.0), eeff;
+  if (nstep >= 0xFFFF || nsp >= 0x7FFF) {
+    *err = "mechanism too large for 16-bit step/species indices";
+    return -1;
+  }
+  std::vector<int> meta((size_t)nrec * 4), spk((size_t)nrec * 4), eptr(nrec + 1, 0), esp;
+  std::vector<double> coef((size_t)nrec * 6, 0.0), fo((size_t)nrec * 7, 0.0), eeff;
+  auto pack16 = [](int a, int b) { return (int)((unsigned)(a & 0xFFFF) | ((unsigned)(b & 0xFFFF) << 16)); };
   for (int q = 0; q < nrec; ++q) {
     const int j = order[q];
     const auto& r = M.rxns[j];
     const int s = M.fwd_step[j];
-    fstep[q] = s;
-    rstep[q] = M.rev_step[j];
-    fA[q] = M.step_lnA[s];
-    fB[q] = M.step_beta[s];
-    fT[q] = M.step_Ta[s];
-    int fl = 0;
+    double* c = &coef[(size_t)q * 6];
+    c[0] = M.step_lnA[s];
+    c[1] = M.step_beta[s];
+    c[2] = M.step_Ta[s];
+    int fl = 0, fsp = -1;
     if (M.rev_type[j] == kRevArrhenius) {
       fl |= kRecRevArr;
       const int sr = M.rev_step[j];
-      rA[q] = M.step_lnA[sr];
-      rB[q] = M.step_beta[sr];
-      rT[q] = M.step_Ta[sr];
+      c[3] = M.step_lnA[sr];
+      c[4] = M.step_beta[sr];
+      c[5] = M.step_Ta[sr];
     } else if (M.rev_type[j] == kRevKeq) {
       fl |= kRecRevKeq;
-      rA[q] = M.keq_ndelta[j];
+      c[3] = M.keq_ndelta[j];
     }
     if (r.third_body) fl |= kRecThirdBody;
     if (r.falloff) {
       fl |= kRecFalloff;
       if (M.falloff_type[j] == kTroe3 || M.falloff_type[j] == kTroe4) fl |= kRecTroe;
       if (M.falloff_type[j] == kTroe4) fl |= kRecTroe4;
-      for (int k = 0; k < 7; ++k) fo[q * 7 + k] = M.falloff_param[(size_t)j * 7 + k];
-      fspc[q] = M.falloff_spc[j];
+      for (int k = 0; k < 7; ++k) fo[(size_t)q * 7 + k] = M.falloff_param[(size_t)j * 7 + k];
+      fsp = M.falloff_spc[j];
     }
-    flags[q] = fl;
+    meta[(size_t)q * 4 + 0] = s;
+    meta[(size_t)q * 4 + 1] = M.rev_step[j];
+    meta[(size_t)q * 4 + 2] = fl;
+    meta[(size_t)q * 4 + 3] = fsp;
     const int nr = M.reac_ptr[s + 1] - M.reac_ptr[s];
     const int np = M.prod_ptr[s + 1] - M.prod_ptr[s];
     if (nr > kMaxPart || np > kMaxPart) {
       *err = "reaction " + std::to_string(j + 1) + " has more than 4 expanded participants";
       return -1;
     }
-    for (int k = 0; k < nr; ++k) reac[q * kMaxPart + k] = M.reac_sp[M.reac_ptr[s] + k];
-    for (int k = 0; k < np; ++k) prod[q * kMaxPart + k] = M.prod_sp[M.prod_ptr[s] + k];
+    int re[kMaxPart] = {-1, -1, -1, -1}, pr[kMaxPart] = {-1, -1, -1, -1};
+    for (int k = 0; k < nr; ++k) re[k] = M.reac_sp[M.reac_ptr[s] + k];
+    for (int k = 0; k < np; ++k) pr[k] = M.prod_sp[M.prod_ptr[s] + k];
+    spk[(size_t)q * 4 + 0] = pack16(re[0], re[1]);
+    spk[(size_t)q * 4 + 1] = pack16(re[2], re[3]);
+    spk[(size_t)q * 4 + 2] = pack16(pr[0], pr[1]);
+    spk[(size_t)q * 4 + 3] = pack16(pr[2], pr[3]);
     for (int k = M.tb_sp_ptr[j]; k < M.tb_sp_ptr[j + 1]; ++k) {
       esp.push_back(M.tb_sp[k]);
       eeff.push_back(M.tb_eff[k]);
     }
     eptr[q + 1] = (int)esp.size();
   }
-  // per-species gather lists in step order (reference summation order)
+  // per-species gather lists in step order (reference summation order),
+  // padded with the zero-rate slot nstep to a common length per 64-species block
   std::vector<std::vector<int>> dl(nsp), cl(nsp);
   for (int s = 0; s < nstep; ++s) {
     for (int k = M.reac_ptr[s]; k < M.reac_ptr[s + 1]; ++k) dl[M.reac_sp[k]].push_back(s);
     for (int k = M.prod_ptr[s]; k < M.prod_ptr[s + 1]; ++k) cl[M.prod_sp[k]].push_back(s);
   }
-  std::vector<int> dptr(nsp + 1, 0), cptr(nsp + 1, 0), dstep, cstep;
-  int maxd = 0, maxc = 0;
+  size_t maxd = 0, maxc = 0;
   for (int j = 0; j < nsp; ++j) {
-    dstep.insert(dstep.end(), dl[j].begin(), dl[j].end());
-    cstep.insert(cstep.end(), cl[j].begin(), cl[j].end());
-    dptr[j + 1] = (int)dstep.size();
-    cptr[j + 1] = (int)cstep.size();
-    maxd = std::max(maxd, (int)dl[j].size());
-    maxc = std::max(maxc, (int)cl[j].size());
+    maxd = std::max(maxd, dl[j].size());
+    maxc = std::max(maxc, cl[j].size());
   }
-  // Jacobian species-block terms, entry (row i, col c): destruction terms
-  // first then creation terms, each in step order (the reference's order of
-  // accumulation, reactor_nvector_serial.cpp:149-176 and 655-669)
-  std::vector<std::vector<int>> jd((size_t)nsp * nsp), jc((size_t)nsp * nsp);
-  for (int s = 0; s < nstep; ++s)
-    for (int k = M.reac_ptr[s]; k < M.reac_ptr[s + 1]; ++k) {
-      const int c = M.reac_sp[k];
-      for (int q = M.reac_ptr[s]; q < M.reac_ptr[s + 1]; ++q)
-        jd[(size_t)c * nsp + M.reac_sp[q]].push_back(-(s + 1));
-      for (int q = M.prod_ptr[s]; q < M.prod_ptr[s + 1]; ++q)
-        jc[(size_t)c * nsp + M.prod_sp[q]].push_back(s + 1);
+  const int nsb = (nsp + 63) / 64;
+  const int gdb = (int)((maxd + 7) / 8), gcb = (int)((maxc + 7) / 8);
+  auto pad_gather = [&](const std::vector<std::vector<int>>& lists, int nb) {
+    std::vector<uint16_t> g((size_t)nsb * nb * 64 * 8, (uint16_t)nstep);
+    for (int j = 0; j < nsp; ++j)
+      for (size_t k = 0; k < lists[j].size(); ++k) {
+        const size_t blk = (size_t)(j / 64) * nb + k / 8;
+        g[(blk * 64 + (j % 64)) * 8 + k % 8] = (uint16_t)lists[j][k];
+      }
+    return g;
+  };
+  const std::vector<uint16_t> gd = pad_gather(dl, gdb), gc = pad_gather(cl, gcb);
+  // Jacobian species-block terms by row (only the nsp <= 63 kernels use them):
+  // entry (row i, col c) = destruction terms then creation terms, each in step
+  // order (the reference's order of accumulation, reactor_nvector_serial.cpp:149-176
+  // and 655-669); a row's entries are sorted by column
+  std::vector<uint32_t> jrt;
+  int jrb = 0;
+  if (nsp <= 63) {
+    std::vector<std::vector<int>> jd((size_t)nsp * nsp), jc((size_t)nsp * nsp);
+    for (int s = 0; s < nstep; ++s)
+      for (int k = M.reac_ptr[s]; k < M.reac_ptr[s + 1]; ++k) {
+        const int c = M.reac_sp[k];
+        for (int q = M.reac_ptr[s]; q < M.reac_ptr[s + 1]; ++q)
+          jd[(size_t)M.reac_sp[q] * nsp + c].push_back(s);
+        for (int q = M.prod_ptr[s]; q < M.prod_ptr[s + 1]; ++q)
+          jc[(size_t)M.prod_sp[q] * nsp + c].push_back(s);
+      }
+    std::vector<std::vector<uint32_t>> rows(nsp);
+    size_t maxr = 0;
+    for (int i = 0; i < nsp; ++i) {
+      for (int c = 0; c < nsp; ++c) {
+        const size_t e = (size_t)i * nsp + c;
+        for (int st : jd[e]) rows[i].push_back((uint32_t)st | ((uint32_t)c << 16) | 0x80000000u);
+        for (int st : jc[e]) rows[i].push_back((uint32_t)st | ((uint32_t)c << 16));
+      }
+      maxr = std::max(maxr, rows[i].size());
     }
-  std::vector<int> jptr((size_t)nsp * nsp + 1, 0), jterm;
-  for (size_t e = 0; e < (size_t)nsp * nsp; ++e) {
-    jterm.insert(jterm.end(), jd[e].begin(), jd[e].end());
-    jterm.insert(jterm.end(), jc[e].begin(), jc[e].end());
-    jptr[e + 1] = (int)jterm.size();
+    jrb = (int)((maxr + 3) / 4);
+    const uint32_t pad = (uint32_t)nstep | (0xFFu << 16);
+    jrt.assign((size_t)jrb * 64 * 4, pad);
+    for (int i = 0; i < nsp; ++i)
+      for (size_t k = 0; k < rows[i].size(); ++k) jrt[((k / 4) * 64 + i) * 4 + k % 4] = rows[i][k];
   }
+  if (jrt.empty()) jrt.assign(4, 0u);
 
   BlobBuilder b;
   const size_t o_mw = b.add(M.mw), o_imw = b.add(inv_mw), o_rmw = b.add(ru_mw),
-               o_th = b.add(M.thermo), o_fs = b.add(fstep), o_rs = b.add(rstep),
-               o_fl = b.add(flags), o_fA = b.add(fA), o_fB = b.add(fB), o_fT = b.add(fT),
-               o_rA = b.add(rA), o_rB = b.add(rB), o_rT = b.add(rT), o_re = b.add(reac),
-               o_pr = b.add(prod), o_ep = b.add(eptr), o_es = b.add(esp), o_ee = b.add(eeff),
-               o_fsp = b.add(fspc), o_fo = b.add(fo), o_dp = b.add(dptr), o_ds = b.add(dstep),
-               o_cp = b.add(cptr), o_cs = b.add(cstep), o_jp = b.add(jptr), o_jt = b.add(jterm);
+               o_th = b.add(M.thermo), o_meta = b.add(meta), o_sp = b.add(spk),
+               o_coef = b.add(coef), o_ep = b.add(eptr), o_es = b.add(esp), o_ee = b.add(eeff),
+               o_fo = b.add(fo), o_gd = b.add(gd), o_gc = b.add(gc), o_jr = b.add(jrt);
   void* base = nullptr;
   hipError_t e = hipMalloc(&base, b.bytes.size());
   if (e != hipSuccess) {
@@ -157,34 +187,23 @@ int build_device_mech(MechHandle* h, std::string* err) {
   d.nstep = nstep;
   d.nrxn = nrxn;
   d.nrec = nrec;
-  d.max_dlist = maxd;
-  d.max_clist = maxc;
+  d.gd_blocks = gdb;
+  d.gc_blocks = gcb;
+  d.jr_blocks = jrb;
   d.mw = (const double*)(p + o_mw);
   d.inv_mw = (const double*)(p + o_imw);
   d.ru_mw = (const double*)(p + o_rmw);
   d.thermo = (const double*)(p + o_th);
-  d.r_fstep = (const int*)(p + o_fs);
-  d.r_rstep = (const int*)(p + o_rs);
-  d.r_flags = (const int*)(p + o_fl);
-  d.r_fA = (const double*)(p + o_fA);
-  d.r_fB = (const double*)(p + o_fB);
-  d.r_fT = (const double*)(p + o_fT);
-  d.r_rA = (const double*)(p + o_rA);
-  d.r_rB = (const double*)(p + o_rB);
-  d.r_rT = (const double*)(p + o_rT);
-  d.r_reac = (const int*)(p + o_re);
-  d.r_prod = (const int*)(p + o_pr);
+  d.r_meta = (const int4*)(p + o_meta);
+  d.r_sp = (const int4*)(p + o_sp);
+  d.r_coef = (const double2*)(p + o_coef);
   d.r_eptr = (const int*)(p + o_ep);
   d.e_sp = (const int*)(p + o_es);
   d.e_eff = (const double*)(p + o_ee);
-  d.r_fspc = (const int*)(p + o_fsp);
   d.r_fo = (const double*)(p + o_fo);
-  d.d_ptr = (const int*)(p + o_dp);
-  d.d_step = (const int*)(p + o_ds);
-  d.c_ptr = (const int*)(p + o_cp);
-  d.c_step = (const int*)(p + o_cs);
-  d.jt_ptr = (const int*)(p + o_jp);
-  d.jt_term = (const int*)(p + o_jt);
+  d.g_d = (const uint4*)(p + o_gd);
+  d.g_c = (const uint4*)(p + o_gc);
+  d.jr = (const uint4*)(p + o_jr);
   h->dev_blob = base;
   h->dev_bytes = b.bytes.size();
   if (hipGetDevice(&h->device) != hipSuccess) h->device = 0;
