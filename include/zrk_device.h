@@ -51,6 +51,10 @@ typedef struct {
   double cvode_retry_relative_tolerance_adjustment;
   int max_steps;
   int cvode_num_retries;
+  /* 0: constant-pressure reactor (reactor_constant_pressure_cpu.cpp);
+   * 1: constant-volume reactor (reactor_constant_volume_cpu.cpp), the
+   * plugin's constant_volume option */
+  int constant_volume;
 } zrk_bdf_options;
 
 ZRK_EXPORT void zrk_bdf_default_options(zrk_bdf_options* o);
@@ -88,13 +92,13 @@ ZRK_EXPORT int zrk_jacobian_device(zrk_mech m, const zrk_bdf_options* o, int n, 
                                    const double* P, const double* Y, int stride, double* J,
                                    void* stream);
 
-/* Integrate n constant-pressure reactors over [0, dt]; T and Y updated in
- * place.  nsteps[r] (optional) = CVode one-step calls, negative on failure
+/* Integrate n reactors over [0, dt]; T and Y updated in place, and P at
+ * constant volume (P = Ru T / (v Wmix) of the end state).  nsteps[r] (optional) = CVode one-step calls, negative on failure
  * (the plugin's COST output); stats[r*8] (optional) = nst nfe nje nsetups nni
  * ncfn netf status.  Requires nsp <= 63.  Returns 0 or an error code; the
  * number of failed reactors is available from nsteps. */
 ZRK_EXPORT int zrk_integrate_device(zrk_mech m, const zrk_bdf_options* o, int n, double dt,
-                                    double* T, const double* P, double* Y, int stride,
+                                    double* T, double* P, double* Y, int stride,
                                     int* nsteps, int* stats, void* stream);
 
 /* Number of wavefronts the persistent integrator keeps resident (for

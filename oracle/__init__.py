@@ -64,7 +64,7 @@ class Opts(C.Structure):
                 ("nlscoef", C.c_double), ("max_steps", C.c_int), ("num_retries", C.c_int),
                 ("retry_atol_adj", C.c_double), ("retry_rtol_adj", C.c_double),
                 ("min_mass_fraction", C.c_double), ("step_limiter", C.c_double),
-                ("Tref", C.c_double)]
+                ("Tref", C.c_double), ("constant_volume", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -182,14 +182,16 @@ class Oracle:
         return J.reshape(N, N).T.copy()
 
     def integrate(self, T, P, Y, dt, opts: Opts | None = None, nthreads=0):
-        """Integrate reactors; returns (T_out, Y_out, stats structured array)."""
+        """Integrate reactors; returns (T_out, Y_out, stats), stats a dict of
+        per-reactor counters plus "P" (end pressure; changes at constant volume)."""
         o = opts or default_opts()
         T = np.array(T, dtype=np.float64, copy=True)
-        P = np.ascontiguousarray(P, dtype=np.float64)
+        P = np.array(P, dtype=np.float64, copy=True)
         Y = np.array(Y, dtype=np.float64, copy=True, order="C")
         n = len(T)
         st = (Stats * n)()
         self.L.zrko_integrate_batch(self.ptr(), C.byref(o), n, dt, _p(T), _p(P), _p(Y), Y.shape[1],
                                     C.cast(st, C.c_void_p), nthreads)
         stats = {f: np.array([getattr(s, f) for s in st]) for f, _ in Stats._fields_}
+        stats["P"] = P
         return T, Y, stats

@@ -27,6 +27,7 @@ void zrko_default_opts(zrko_opts* o) {
   o->min_mass_fraction = 1.0e-30;
   o->step_limiter = 1.0e22;
   o->Tref = 1.0;
+  o->constant_volume = 0;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -184,14 +185,16 @@ void zrko_rates(const zrko_mech* m, double T, const double* C, double L, double*
 }
 
 /* ------------------------------------------------------------------------- */
-/* Constant-pressure reactor (reactor_constant_pressure_cpu.cpp)               */
+/* Constant-pressure reactor (reactor_constant_pressure_cpu.cpp) and           */
+/* constant-volume reactor (reactor_constant_volume_cpu.cpp)                   */
 /* ------------------------------------------------------------------------- */
 typedef struct {
   const zrko_mech* m;
   const zrko_opts* o;
   double P;
   double inv_density; /* carried between calls like the reference member */
-  double cp_mix;
+  double v0;          /* constant volume: the fixed specific volume (1/density) */
+  double cp_mix;      /* mass cp (constant pressure) or cv (constant volume) */
   double *C, *h, *cpm, *G, *rop, *cr, *de, *net, *tmp1, *tmp2, *tmp3;
 } reactor_t;
 
@@ -201,6 +204,7 @@ static void reactor_init(reactor_t* r, const zrko_mech* m, const zrko_opts* o, d
   r->o = o;
   r->P = P;
   r->inv_density = 0.0;
+  r->v0 = 0.0;
   r->cp_mix = 0.0;
   double* w = (double*)calloc((size_t)(8 * nsp + m->nstep + 3 * N), sizeof(double));
   r->C = w;
@@ -219,22 +223,42 @@ static void reactor_init(reactor_t* r, const zrko_mech* m, const zrko_opts* o, d
 
 static void reactor_free(reactor_t* r) { free(r->C); }
 
-/* GetTimeDerivative (reactor_constant_pressure_cpu.cpp:68-133), no sources. */
+/* 1/density at (T, P, Y) (mechanism::getDensityFromTPY, mechanism.cpp:279-284) */
+static double inv_density_tpy(const zrko_mech* m, double T, double P, const double* y) {
+  double mws = 0.0;
+  for (int j = 0; j < m->nsp; ++j) mws += y[j] * (1.0 / m->mw[j]);
+  return 1.0 / (P * (1.0 / mws) / (NIST_RU * T));
+}
+
+/* GetTimeDerivative, no sources: constant pressure
+ * (reactor_constant_pressure_cpu.cpp:68-133) recomputes the density from P
+ * and uses h/RT and cp; constant volume (reactor_constant_volume_cpu.cpp:
+ * 66-130) keeps the initial density and uses u/RT = h/RT - 1 and
+ * cv_j = (cp_j/R - 1) R/MW_j (mechanism.cpp:381-386, 428-441). */
 static int reactor_rhs(reactor_t* r, double t, const double* y, double* ydot) {
   (void)t;
   const zrko_mech* m = r->m;
   const int nsp = m->nsp;
+  const int cvol = r->o->constant_volume;
   double T = y[nsp] * r->o->Tref;
   if (T <= 0.0) return 1;
   if (T > TLIMIT) T = TLIMIT;
-  double mws = 0.0;
-  for (int j = 0; j < nsp; ++j) mws += y[j] * (1.0 / m->mw[j]);
-  const double wmix = 1.0 / mws;
-  r->inv_density = 1.0 / (r->P * wmix / (NIST_RU * T));
+  if (cvol) {
+    r->inv_density = r->v0;
+  } else {
+    double mws = 0.0;
+    for (int j = 0; j < nsp; ++j) mws += y[j] * (1.0 / m->mw[j]);
+    const double wmix = 1.0 / mws;
+    r->inv_density = 1.0 / (r->P * wmix / (NIST_RU * T));
+  }
   thermo_h(m, T, r->h);
   thermo_cp(m, T, r->cpm);
   double cp = 0.0;
   for (int j = 0; j < nsp; ++j) {
+    if (cvol) {
+      r->h[j] -= 1.0;
+      r->cpm[j] -= 1.0;
+    }
     r->cpm[j] *= NIST_RU / m->mw[j];
     cp += r->cpm[j] * y[j];
   }
@@ -256,6 +280,7 @@ int zrko_rhs(const zrko_mech* m, const zrko_opts* o, double P, double t, const d
              double* ydot) {
   reactor_t r;
   reactor_init(&r, m, o, P);
+  if (o->constant_volume) r.v0 = inv_density_tpy(m, y[m->nsp] * o->Tref, P, y);
   int flag = reactor_rhs(&r, t, y, ydot);
   reactor_free(&r);
   return flag;
@@ -322,6 +347,7 @@ void zrko_jacobian(const zrko_mech* m, const zrko_opts* o, double P, double t,
                    const double* y, const double* fy, double* J) {
   reactor_t r;
   reactor_init(&r, m, o, P);
+  if (o->constant_volume) r.v0 = inv_density_tpy(m, y[m->nsp] * o->Tref, P, y);
   double* f = (double*)malloc(sizeof(double) * (size_t)(m->nsp + 1));
   reactor_rhs(&r, t, y, f); /* sets the carried inverse density at y */
   reactor_jacobian(&r, t, y, fy, J);
@@ -1048,11 +1074,13 @@ static void cv_dky0(cvmem* cv, double t, double* out) {
 
 /* CvodeSolver::Integrate (solver_cvode.cpp:51-299): one-step loop, retries,
  * Dky interpolation at min(tcurr, end_time). */
-static int integrate_one(const zrko_mech* m, const zrko_opts* o, double P, double dt,
+static int integrate_one(const zrko_mech* m, const zrko_opts* o, double* P, double dt,
                          double* y, zrko_stats* st) {
   const int N = m->nsp + 1;
   reactor_t r;
-  reactor_init(&r, m, o, P);
+  reactor_init(&r, m, o, *P);
+  /* ReactorConstantVolumeCPU::InitializeState (reactor_constant_volume_cpu.cpp:15-42) */
+  if (o->constant_volume) r.v0 = inv_density_tpy(m, y[m->nsp] * o->Tref, *P, y);
   cvmem cv;
   memset(&cv, 0, sizeof(cv));
   cv.r = &r;
@@ -1100,6 +1128,13 @@ static int integrate_one(const zrko_mech* m, const zrko_opts* o, double P, doubl
   }
   if (flag == CV_SUCCESS) {
     cv_dky0(&cv, tcurr < dt ? tcurr : dt, y);
+    /* GetState: constant volume reports P = Ru T / (v Wmix)
+     * (reactor_constant_volume_cpu.cpp:44-63, mechanism.cpp:292-297) */
+    if (o->constant_volume) {
+      double mws = 0.0;
+      for (int j = 0; j < m->nsp; ++j) mws += y[j] * (1.0 / m->mw[j]);
+      *P = NIST_RU * (y[m->nsp] * o->Tref) / (r.v0 * (1.0 / mws));
+    }
     nst_total += cv.nst; nfe_total += cv.nfe; nje_total += cv.nje; nset_total += cv.nsetups;
     nni_total += cv.nni; ncfn_total += cv.ncfn; netf_total += cv.netf;
   }
@@ -1123,7 +1158,7 @@ static int integrate_one(const zrko_mech* m, const zrko_opts* o, double P, doubl
 }
 
 int zrko_integrate_batch(const zrko_mech* m, const zrko_opts* o, int n, double dt, double* T,
-                         const double* P, double* mf, int stride, zrko_stats* stats,
+                         double* P, double* mf, int stride, zrko_stats* stats,
                          int nthreads) {
   const int nsp = m->nsp;
   int nfail = 0;
@@ -1137,7 +1172,7 @@ int zrko_integrate_batch(const zrko_mech* m, const zrko_opts* o, int n, double d
     double* y = (double*)malloc(sizeof(double) * (size_t)(nsp + 1));
     for (int j = 0; j < nsp; ++j) y[j] = mf[(size_t)k * stride + j];
     y[nsp] = T[k] / o->Tref;
-    int flag = integrate_one(m, o, P[k], dt, y, stats ? stats + k : NULL);
+    int flag = integrate_one(m, o, &P[k], dt, y, stats ? stats + k : NULL);
     for (int j = 0; j < nsp; ++j) mf[(size_t)k * stride + j] = y[j];
     T[k] = y[nsp] * o->Tref;
     if (flag < 0) nfail++;

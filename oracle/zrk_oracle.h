@@ -69,6 +69,7 @@ typedef struct {
   double min_mass_fraction;
   double step_limiter;
   double Tref;             /* reference_temperature */
+  int constant_volume;     /* 0: constant pressure, 1: constant volume */
 } zrko_opts;
 
 typedef struct {
@@ -85,7 +86,9 @@ void zrko_rate_constants(const zrko_mech* m, double T, const double* C, double* 
 void zrko_rates(const zrko_mech* m, double T, const double* C, double step_limiter,
                 double* rop, double* create, double* destroy, double* net);
 
-/* Constant-pressure reactor state y = [Y_0..Y_{nsp-1}, T/Tref] (solve_temperature). */
+/* Reactor state y = [Y_0..Y_{nsp-1}, T/Tref] (solve_temperature).  Constant
+ * pressure, or with o->constant_volume the constant-volume reactor whose
+ * volume is fixed by (P, y) itself. */
 int zrko_rhs(const zrko_mech* m, const zrko_opts* o, double P, double t, const double* y,
              double* ydot);
 /* Reference analytic Jacobian (dense, column-major J[col*N+row]) at y with
@@ -93,10 +96,10 @@ int zrko_rhs(const zrko_mech* m, const zrko_opts* o, double P, double t, const d
 void zrko_jacobian(const zrko_mech* m, const zrko_opts* o, double P, double t,
                    const double* y, const double* fy, double* J);
 
-/* Integrate n constant-pressure reactors over [0, dt] (AoS: mf[k*stride+j]),
- * updating T and mf in place; nthreads <= 0 uses OpenMP's default. */
+/* Integrate n reactors over [0, dt] (AoS: mf[k*stride+j]), updating T and mf
+ * in place (and P for constant volume); nthreads <= 0 uses OpenMP's default. */
 int zrko_integrate_batch(const zrko_mech* m, const zrko_opts* o, int n, double dt,
-                         double* T, const double* P, double* mf, int stride,
+                         double* T, double* P, double* mf, int stride,
                          zrko_stats* stats, int nthreads);
 
 /* Dense LU with partial pivoting (SUNDIALS denseGETRF/denseGETRS semantics),

@@ -1,5 +1,6 @@
 """GPU parity of the persistent BDF integrator (zrk_integrate_device) against the
-C oracle's CVODE restatement, on the workload distributions.
+C oracle's CVODE restatement, on the workload distributions, for the
+constant-pressure and the constant-volume reactor.
 
 Both run the same variable-order BDF with the same options.  An adaptive
 integrator's step sequence reacts to round-off: perturbing the oracle's
@@ -21,7 +22,7 @@ pytestmark = pytest.mark.gpu
 RTOL, ATOL = 1.0e-6, 1.0e-12
 
 
-def _gpu_integrate(torch, m, T, P, Y, dt, opts):
+def _gpu_integrate(torch, m, T, P, Y, dt, opts, with_p=False):
     dev = torch.device("cuda:0")
     n, nsp = Y.shape
     t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
@@ -30,8 +31,9 @@ def _gpu_integrate(torch, m, T, P, Y, dt, opts):
     stats = torch.zeros(n * 8, dtype=torch.int32, device=dev)
     m.integrate(n, dt, dT, dP, dY, nsp, opts, steps, stats)
     torch.cuda.synchronize()
-    return (dT.cpu().numpy(), dY.cpu().numpy(), steps.cpu().numpy(),
-            stats.cpu().numpy().reshape(n, 8))
+    out = (dT.cpu().numpy(), dY.cpu().numpy(), steps.cpu().numpy(),
+           stats.cpu().numpy().reshape(n, 8))
+    return out + (dP.cpu().numpy(),) if with_p else out
 
 
 def _units(Ya, Ta, Yb, Tb, rtol, atol):
@@ -41,16 +43,26 @@ def _units(Ya, Ta, Yb, Tb, rtol, atol):
     return np.maximum(eY.max(axis=1), eT)
 
 
-def _referee(o, T0, P, Y0, dt):
+def _referee(o, T0, P, Y0, dt, oopts=None):
     from scipy.integrate import solve_ivp
-    sol = solve_ivp(lambda t, y: o.rhs(P, y)[0], (0.0, dt), np.append(Y0, T0), method="LSODA",
+    if oopts is not None and oopts.constant_volume:
+        # constant volume: the RHS at the initial state's volume
+        rho0 = P / (8.314462618e3 * T0) / np.sum(Y0 / o.tables.mw)
+        f = lambda t, y: o.rhs(rho0 * 8.314462618e3 * y[-1] * np.sum(y[:-1] / o.tables.mw), y,
+                               oopts)[0]
+    else:
+        f = lambda t, y: o.rhs(P, y)[0]
+    sol = solve_ivp(f, (0.0, dt), np.append(Y0, T0), method="LSODA",
                     rtol=1e-11, atol=1e-22)
     assert sol.success
     return sol.y[:-1, -1], sol.y[-1, -1]
 
 
-@pytest.mark.parametrize("wl,n,seed", [("10sp", 1024, 101), ("53sp", 256, 102)])
-def test_integrate_matches_oracle(torch_gpu, oracle_mod, wl, n, seed):
+@pytest.mark.parametrize("wl,n,seed,cvol", [("10sp", 1024, 101, 0), ("53sp", 256, 102, 0),
+                                             ("10sp", 512, 103, 1), ("53sp", 128, 104, 1)])
+def test_integrate_matches_oracle(torch_gpu, oracle_mod, wl, n, seed, cvol):
+    """cvol=1: the constant-volume reactor (reactor_constant_volume_cpu.cpp);
+    its end pressure must also match the oracle's."""
     from zerork_hip import capi, mechgen
     name = f"synth_{wl}"
     m = capi.Mech(*mech_paths(name))
@@ -58,10 +70,10 @@ def test_integrate_matches_oracle(torch_gpu, oracle_mod, wl, n, seed):
     mg = mechgen.workload_mechanism(wl)
     st = mechgen.workload_states(wl, mg, n, seed=seed)
     dt = mechgen.WORKLOAD_DT[wl]
-    print("case", wl, "n", n, flush=True)
-    opts = capi.default_options(rel_tol=RTOL, abs_tol=ATOL)
-    Tg, Yg, steps, stats = _gpu_integrate(torch_gpu, m, st.T, st.P, st.Y, dt, opts)
-    oopts = oracle_mod.default_opts(rtol=RTOL, atol=ATOL)
+    print("case", wl, "n", n, "constant volume", cvol, flush=True)
+    opts = capi.default_options(rel_tol=RTOL, abs_tol=ATOL, constant_volume=cvol)
+    Tg, Yg, steps, stats, Pg = _gpu_integrate(torch_gpu, m, st.T, st.P, st.Y, dt, opts, True)
+    oopts = oracle_mod.default_opts(rtol=RTOL, atol=ATOL, constant_volume=cvol)
     To, Yo, ost = o.integrate(st.T, st.P, st.Y, dt, oopts)
     Tp, Yp, _ = o.integrate(np.nextafter(st.T, np.inf), st.P, st.Y, dt, oopts)
     assert np.all(ost["nsteps"] > 0)
@@ -78,11 +90,19 @@ def test_integrate_matches_oracle(torch_gpu, oracle_mod, wl, n, seed):
     assert gq[2] <= 4.0 * uq[2] + 10.0
     assert abs(steps.mean() / ost["nsteps"].mean() - 1.0) < 0.02
     assert np.all(np.abs(Yg.sum(axis=1) - 1.0) <= 1e-9)
+    if cvol:
+        # P = rho Ru T / W: the same tolerance-unit comparison on the pressure
+        pu = np.abs(Pg - ost["P"]) / (RTOL * ost["P"])
+        pp = np.abs(Tp - To) / (RTOL * To)
+        assert np.percentile(pu, 90) <= 2.0 * np.percentile(pp, 90) + 1.0
+        assert pu.max() <= 4.0 * max(uq[2], 1.0) + 10.0
+    else:
+        np.testing.assert_array_equal(Pg, st.P)
     # accuracy against an independent referee on a subset
     sub = np.linspace(0, n - 1, 6).astype(int)
     eg, eo = [], []
     for i in sub:
-        yr, tr = _referee(o, st.T[i], st.P[i], st.Y[i], dt)
+        yr, tr = _referee(o, st.T[i], st.P[i], st.Y[i], dt, oopts)
         eg.append(_units(Yg[i:i + 1], Tg[i:i + 1], yr[None], np.array([tr]), RTOL, ATOL)[0])
         eo.append(_units(Yo[i:i + 1], To[i:i + 1], yr[None], np.array([tr]), RTOL, ATOL)[0])
     print(f"{wl}: error vs referee (tol units) gpu {np.round(eg, 1)} oracle {np.round(eo, 1)}",

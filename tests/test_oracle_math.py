@@ -205,3 +205,75 @@ def test_bdf_failure_keeps_state(oracle_mod):
     assert np.all(stats["nsteps"] < 0)
     np.testing.assert_array_equal(T, st.T)
     np.testing.assert_array_equal(Y, st.Y)
+
+
+def test_constant_volume_rhs_invariants(oracle_mod):
+    """Constant-volume RHS (reactor_constant_volume_cpu.cpp:66-130): mass
+    conservation, an independent numpy restatement of
+    dT/dt = -sum(u_j wdot_j)/(rho cv) with u_j/RT = h_j/RT - 1 and
+    cv_j = cp_j - R/MW_j, and zero rate of change of the mixture internal
+    energy u = sum(u_j(T) Y_j)."""
+    o = oracle_mod.Oracle(*mech_paths("synth_53sp"))
+    g = load_golden("synth_53sp")
+    t = o.tables
+    opts = oracle_mod.default_opts(constant_volume=1)
+    for i in range(len(g["T"])):
+        T, P, Y = g["T"][i], g["P"][i], g["Y"][i]
+        yd, flag = o.rhs(P, np.append(Y, T), opts)
+        assert flag == 0
+        assert abs(np.sum(yd[:-1])) <= 1e-13 * np.sum(np.abs(yd[:-1]))
+        net = g["net"][i]
+        rho = P / (RU * T) / np.sum(Y / t.mw)
+        cv = np.sum((g["cp_R"][i] - 1.0) * RU / t.mw * Y)
+        u = (g["h_RT"][i] - 1.0) * RU * T          # J/kmol
+        dTdt = -np.sum(u * net) / (rho * cv)
+        escale = np.sum(np.abs(u * net)) / (rho * cv)
+        assert abs(yd[-1] - dTdt) <= 1e-12 * escale
+        # du/dt = cv dT/dt + sum(u_j/MW_j dY_j/dt) = 0
+        dudt = cv * yd[-1] + np.sum(u / t.mw * yd[:-1])
+        assert abs(dudt) <= 1e-12 * np.sum(np.abs(u / t.mw * yd[:-1]))
+
+
+def test_constant_volume_integration_conserves_energy_and_density(oracle_mod):
+    """Integrated constant-volume reactors keep u and the density (so the end
+    pressure is rho Ru T/W) and agree with the LSODA referee on the
+    constant-volume RHS."""
+    from scipy.integrate import solve_ivp
+    from zerork_hip import mechgen
+    o = oracle_mod.Oracle(*mech_paths("synth_10sp"))
+    t = o.tables
+    m = mechgen.workload_mechanism("10sp")
+    st = mechgen.workload_states("10sp", m, 4, seed=21)
+    opts = oracle_mod.default_opts(rtol=1e-9, atol=1e-15, constant_volume=1)
+    T, Y, stats = o.integrate(st.T, st.P, st.Y, 1e-4, opts)
+    assert np.all(stats["nsteps"] > 0)
+    for i in range(len(T)):
+        cp0, h0, _ = o.thermo(st.T[i])
+        cp1, h1, _ = o.thermo(T[i])
+        u0 = np.sum((h0 - 1.0) * RU * st.T[i] / t.mw * st.Y[i])
+        u1 = np.sum((h1 - 1.0) * RU * T[i] / t.mw * Y[i])
+        assert abs(u1 - u0) <= 1e-7 * np.sum(np.abs((h0 - 1.0) * RU * st.T[i] / t.mw * st.Y[i]))
+        rho0 = st.P[i] / (RU * st.T[i]) / np.sum(st.Y[i] / t.mw)
+        p_end = rho0 * RU * T[i] * np.sum(Y[i] / t.mw)
+        assert abs(stats["P"][i] - p_end) <= 1e-12 * p_end
+        sol = solve_ivp(lambda tt, y: _cv_rhs(o, y, rho0),
+                        (0.0, 1e-4), np.append(st.Y[i], st.T[i]), method="LSODA", rtol=1e-11,
+                        atol=1e-22)
+        assert sol.success
+        ref = sol.y[:, -1]
+        assert abs(T[i] - ref[-1]) <= 1e-6 * ref[-1]
+        assert np.all(np.abs(Y[i] - ref[:-1]) <= 1e-6 * np.abs(ref[:-1]) + 1e-12)
+
+
+def _cv_rhs(o, y, rho0):
+    """Constant-volume RHS restated in numpy from the oracle's rates and thermo."""
+    t = o.tables
+    Y, T = y[:-1], y[-1]
+    C = rho0 * Y / t.mw
+    _, _, _, net = o.rates(T, C, 1.0e22)
+    cp_R, h_RT, _ = o.thermo(T)
+    cv = np.sum((cp_R - 1.0) * RU / t.mw * Y)
+    yd = np.empty_like(y)
+    yd[:-1] = net * t.mw / rho0
+    yd[-1] = -np.sum((h_RT - 1.0) * net / rho0) * RU * T / cv
+    return yd

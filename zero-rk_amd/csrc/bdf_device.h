@@ -45,7 +45,7 @@ constexpr int kQmax = 5;
 // the temperature row and the MW_i/MW_j scaling follow column by column.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s, double P,
-                                               const BdfOpts& o, double y, double fy,
+                                               double v0, const BdfOpts& o, double y, double fy,
                                                double inv_rho_prev, double* __restrict__ Jl,
                                                double* __restrict__ saved,
                                                double* __restrict__ t2l) {
@@ -68,12 +68,12 @@ __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s
   const double t4in = (lane == nsp) ? Tp : (sp ? y : 0.0);
   double t4;
   RhsAux aux2;
-  rhs64(m, s, P, o.Tref, o.step_limiter, t4in, t4, aux2);
+  rhs64(m, s, P, v0, o.Tref, o.step_limiter, t4in, t4, aux2);
   const double colT = act ? (t4 - fy) * (1.0 / dT) : 0.0;
   // floored evaluation: rates of progress for every species term
   double t3;
   RhsAux aux;
-  rhs64(m, s, P, o.Tref, o.step_limiter, t1, t3, aux);
+  rhs64(m, s, P, v0, o.Tref, o.step_limiter, t1, t3, aux);
   const double Tdot = bcast(t3, nsp);
   const double RuTemp = kRu * yT;
   const double mult = -1.0 / aux.cp_mix;

@@ -209,10 +209,6 @@ zerork_status_t solve(zerork_handle h, double dt, int n, double* T, double* P, d
   const int nsp = m->host.nsp;
   const int stride = h->iopt.count("num_species_stride") ? h->iopt["num_species_stride"] : nsp;
   if (stride < nsp) return ZERORK_STATUS_INVALID_POINTER;
-  if (h->iopt["constant_volume"] != 0) {
-    // the constant-volume reactor is not part of this build's HIP path
-    return ZERORK_STATUS_UNKNOWN_ERROR;
-  }
   if (h->y_src || h->e_src || h->dpdt) {
     bool any = false;
     for (int k = 0; k < n && !any; ++k) any = (h->dpdt && h->dpdt[k] != 0.0) || (h->e_src && h->e_src[k] != 0.0);
@@ -259,6 +255,7 @@ zerork_status_t solve(zerork_handle h, double dt, int n, double* T, double* P, d
   o.cvode_retry_relative_tolerance_adjustment = h->dopt["cvode_retry_relative_tolerance_adjustment"];
   o.max_steps = h->iopt["max_steps"];
   o.cvode_num_retries = h->iopt["cvode_num_retries"];
+  o.constant_volume = h->iopt["constant_volume"] != 0;
   const auto t0 = std::chrono::steady_clock::now();
   int rc = zrk::integrate_device(m, o, n, dt, h->dT, h->dP, h->dY, stride, h->dsteps, nullptr,
                                  sorted ? h->dorder : nullptr, st);
@@ -266,6 +263,8 @@ zerork_status_t solve(zerork_handle h, double dt, int n, double* T, double* P, d
   std::vector<int> steps(n);
   ok = hipMemcpyAsync(T, h->dT, sizeof(double) * n, hipMemcpyDeviceToHost, st) == hipSuccess;
   ok = ok && hipMemcpyAsync(mf, h->dY, sizeof(double) * ny, hipMemcpyDeviceToHost, st) == hipSuccess;
+  if (o.constant_volume)
+    ok = ok && hipMemcpyAsync(P, h->dP, sizeof(double) * n, hipMemcpyDeviceToHost, st) == hipSuccess;
   ok = ok && hipMemcpyAsync(steps.data(), h->dsteps, sizeof(int) * n, hipMemcpyDeviceToHost,
                             st) == hipSuccess;
   if (hipStreamSynchronize(st) != hipSuccess || !ok) return ZERORK_STATUS_UNKNOWN_ERROR;

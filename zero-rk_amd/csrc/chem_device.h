@@ -257,10 +257,15 @@ __device__ __forceinline__ void species_cd(const DevMech& m, const double* __res
 }
 
 // ---------------------------------------------------------------------------
-// Constant-pressure RHS for nsp <= 63: lane j < nsp holds species j, lane nsp
-// holds the temperature variable y_T = T/Tref (padding lanes hold 0).
+// Reactor RHS for nsp <= 63: lane j < nsp holds species j, lane nsp holds the
+// temperature variable y_T = T/Tref (padding lanes hold 0).
+// v0 == 0: constant pressure (reactor_constant_pressure_cpu.cpp:68-133), the
+// density follows from P; h/RT and cp.  v0 > 0: constant volume at specific
+// volume v0 (reactor_constant_volume_cpu.cpp:66-130); u/RT = h/RT - 1 and
+// cv_j = (cp_j/R - 1) R/MW_j (mechanism.cpp:381-386, 428-441).
 // Returns 1 when T <= 0 (recoverable for the integrator), else 0.
-// Outputs per lane: ydot, and for species lanes h/RT and mass cp (J/kg/K).
+// Outputs per lane: ydot, and for species lanes the energy (h/RT or u/RT) and
+// mass heat capacity (cp or cv, J/kg/K) the Jacobian's temperature row uses.
 // ---------------------------------------------------------------------------
 struct RhsLds {
   double* C;    // [>= nsp]
@@ -270,13 +275,14 @@ struct RhsLds {
 
 struct RhsAux {
   double inv_rho;  // 1/density of this evaluation
-  double cp_mix;   // mixture cp (J/kg/K)
-  double h;        // lane: h/RT of species lane
-  double cpm;      // lane: cp (J/kg/K) of species lane
+  double cp_mix;   // mixture cp or cv (J/kg/K)
+  double h;        // lane: h/RT or u/RT of species lane
+  double cpm;      // lane: cp or cv (J/kg/K) of species lane
 };
 
-__device__ __forceinline__ int rhs64(const DevMech& m, const RhsLds& s, double P, double Tref,
-                                     double L, double y, double& ydot, RhsAux& aux) {
+__device__ __forceinline__ int rhs64(const DevMech& m, const RhsLds& s, double P, double v0,
+                                     double Tref, double L, double y, double& ydot,
+                                     RhsAux& aux) {
   const int lane = lane_id();
   const int nsp = m.nsp;
   const bool sp = lane < nsp;
@@ -290,13 +296,21 @@ __device__ __forceinline__ int rhs64(const DevMech& m, const RhsLds& s, double P
   const TPow p = make_tpow(T);
   const double Yj = sp ? y : 0.0;
   const double imw = sp ? m.inv_mw[lane] : 0.0;
-  const double mws = wave_sum(Yj * imw);
-  const double wmix = 1.0 / mws;
-  const double inv_rho = 1.0 / (P * wmix / (kRu * T));
+  const bool cvol = v0 > 0.0;
+  double inv_rho = v0;
+  if (!cvol) {
+    const double mws = wave_sum(Yj * imw);
+    const double wmix = 1.0 / mws;
+    inv_rho = 1.0 / (P * wmix / (kRu * T));
+  }
   const double rho_c = 1.0 / inv_rho;
   double cp = 0.0, h = 0.0, g = 0.0, Cj = 0.0, cpm = 0.0;
   if (sp) {
     thermo_one(m.thermo + 16 * lane, p, cp, h, g);
+    if (cvol) {
+      h -= 1.0;
+      cp -= 1.0;
+    }
     cpm = cp * m.ru_mw[lane];
     Cj = rho_c * imw * Yj;
     s.C[lane] = Cj;

@@ -52,6 +52,7 @@ struct CvWave {
   double* prow;    // LDS [NP]: published pivot row
   double* rvec;    // LDS [NP]: right-hand side of a solve
   double P;
+  double v0;  // constant volume: fixed specific volume; 0 at constant pressure
   // lane vectors
   double zn[kQmax + 1];
   double ewt, acor, tempv, ftemp, y, atolv, delta;
@@ -75,7 +76,7 @@ struct CvWave {
     nfe++;
     RhsAux aux;
     ZRK_TIC(t0);
-    const int fl = rhs64(mech, s, P, opt.Tref, opt.step_limiter, yy, yd, aux);
+    const int fl = rhs64(mech, s, P, v0, opt.Tref, opt.step_limiter, yy, yd, aux);
     ZRK_TOC(t0, 0);
     if (!fl) inv_rho_last = aux.inv_rho;
     if (!active) yd = 0.0;
@@ -374,7 +375,7 @@ struct CvWave {
       nje++;
       nstlj = nst;
       ZRK_TIC(t0);
-      inv_rho_last = jacobian_lds(mech, s, P, opt, ypred, fpred, inv_rho_last, Mlds, savedJ,
+      inv_rho_last = jacobian_lds(mech, s, P, v0, opt, ypred, fpred, inv_rho_last, Mlds, savedJ,
                                   prow);
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();

@@ -20,6 +20,7 @@ struct BdfOpts {
   double rtol, atol, hmax_inv, nlscoef, step_limiter, Tref, min_mass_fraction;
   double retry_atol_adj, retry_rtol_adj;
   int max_steps, num_retries;
+  int constant_volume;
 };
 
 enum RecFlags : int {

@@ -106,8 +106,8 @@ jacobian_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ Tin,
     const double P = Pin[r];
     double fy;
     RhsAux aux;
-    rhs64(m, s, P, o.Tref, o.step_limiter, y, fy, aux);
-    jacobian_lds(m, s, P, o, y, fy, aux.inv_rho, Jl, saved, t2l);
+    rhs64(m, s, P, 0.0, o.Tref, o.step_limiter, y, fy, aux);
+    jacobian_lds(m, s, P, 0.0, o, y, fy, aux.inv_rho, Jl, saved, t2l);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -119,13 +119,13 @@ jacobian_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ Tin,
 
 // ---------------------------------------------------------------------------
 // bdf_kernel (nsp <= 63): persistent; wavefronts pull reactors from *counter.
-// T[n], Y[n*stride] updated in place; nsteps[n] = plugin cost (CVode one-step
-// calls, negative on failure); stats[n*8] (optional): nst nfe nje nsetups nni
-// ncfn netf status.
+// T[n], Y[n*stride] updated in place (and P[n] at constant volume);
+// nsteps[n] = plugin cost (CVode one-step calls, negative on failure);
+// stats[n*8] (optional): nst nfe nje nsetups nni ncfn netf status.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kWave)
 bdf_kernel(DevMech m, BdfOpts o, int n, double dt, double* __restrict__ Tio,
-           const double* __restrict__ Pin, double* __restrict__ Yio, int stride,
+           double* __restrict__ Pio, double* __restrict__ Yio, int stride,
            int* __restrict__ nsteps_out, int* __restrict__ stats_out,
            const int* __restrict__ order, unsigned int* __restrict__ counter,
            double* __restrict__ jscratch) {
@@ -154,7 +154,14 @@ bdf_kernel(DevMech m, BdfOpts o, int n, double dt, double* __restrict__ Tio,
     double y0 = 0.0;
     if (lane < nsp) y0 = Yio[(size_t)r * stride + lane];
     if (lane == nsp) y0 = Tio[r] / o.Tref;
-    cv.P = Pin[r];
+    cv.P = Pio[r];
+    cv.v0 = 0.0;
+    if (o.constant_volume) {
+      // ReactorConstantVolumeCPU::InitializeState: the volume of the initial
+      // state (reactor_constant_volume_cpu.cpp:28-29)
+      const double mws = wave_sum(lane < nsp ? y0 * m.inv_mw[lane] : 0.0);
+      cv.v0 = 1.0 / (cv.P * (1.0 / mws) / (kRu * (bcast(y0, nsp) * o.Tref)));
+    }
     cv.atolv = cv.active ? o.atol : 0.0;
     cv.rtol = o.rtol;
     cv.inv_rho_last = 0.0;
@@ -186,6 +193,12 @@ bdf_kernel(DevMech m, BdfOpts o, int n, double dt, double* __restrict__ Tio,
       tot[4] += cv.nni; tot[5] += cv.ncfn; tot[6] += cv.netf;
       if (lane < nsp) Yio[(size_t)r * stride + lane] = yout;
       if (lane == nsp) Tio[r] = yout * o.Tref;
+      if (o.constant_volume) {
+        // GetState: P = Ru T / (v Wmix) (reactor_constant_volume_cpu.cpp:44-63)
+        const double mws = wave_sum(lane < nsp ? yout * m.inv_mw[lane] : 0.0);
+        const double Tend = bcast(yout, nsp) * o.Tref;
+        if (lane == 0) Pio[r] = kRu * Tend / (cv.v0 * (1.0 / mws));
+      }
     } else {
       nsteps = (nsteps <= 0) ? -1 : -nsteps;
     }
@@ -259,7 +272,7 @@ int bdf_max_resident_waves(const DevMech& m) {
 }
 
 hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, double* T,
-                      const double* P, double* Y, int stride, int* nsteps, int* stats,
+                      double* P, double* Y, int stride, int* nsteps, int* stats,
                       const int* order, unsigned int* counter, double* jscratch, int nwaves,
                       hipStream_t st) {
   if (n <= 0) return hipSuccess;

@@ -16,7 +16,7 @@ hipError_t launch_jacobian(const DevMech& m, const BdfOpts& o, int n, const doub
 int bdf_max_resident_waves(const DevMech& m);
 size_t bdf_lds_bytes(const DevMech& m);
 hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, double* T,
-                      const double* P, double* Y, int stride, int* nsteps, int* stats,
+                      double* P, double* Y, int stride, int* nsteps, int* stats,
                       const int* order, unsigned int* counter, double* jscratch, int nwaves,
                       hipStream_t st);
 

@@ -240,6 +240,7 @@ BdfOpts to_bdf_opts(const zrk_bdf_options& o) {
   b.retry_rtol_adj = o.cvode_retry_relative_tolerance_adjustment;
   b.max_steps = o.max_steps;
   b.num_retries = o.cvode_num_retries;
+  b.constant_volume = o.constant_volume != 0;
   return b;
 }
 
@@ -257,7 +258,7 @@ int ensure_device(MechHandle* h) {
 }
 
 int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt, double* T,
-                     const double* P, double* Y, int stride, int* nsteps, int* stats,
+                     double* P, double* Y, int stride, int* nsteps, int* stats,
                      const int* order, hipStream_t st) {
   if (n < 0 || !T || !P || !Y || stride < h->host.nsp || !(dt > 0.0)) return -1;
   if (h->host.nsp > kNP - 1) return -2;
@@ -291,6 +292,9 @@ void zrk_bdf_default_options(zrk_bdf_options* o) {
   o->cvode_retry_relative_tolerance_adjustment = 1.0;
   o->max_steps = 5000;
   o->cvode_num_retries = 5;
+  // the plugin defaults to constant volume (constant_volume=1); this
+  // device-level default is the constant-pressure reactor
+  o->constant_volume = 0;
 }
 
 int zrk_mech_load(const char* mech_file, const char* therm_file, zrk_mech* out, char* err,
@@ -392,7 +396,7 @@ int zrk_jacobian_device(zrk_mech m, const zrk_bdf_options* o, int n, const doubl
 }
 
 int zrk_integrate_device(zrk_mech m, const zrk_bdf_options* o, int n, double dt, double* T,
-                         const double* P, double* Y, int stride, int* nsteps, int* stats,
+                         double* P, double* Y, int stride, int* nsteps, int* stats,
                          void* stream) {
   if (!m || !o) return -1;
   return zrk::integrate_device(reinterpret_cast<MechHandle*>(m), *o, n, dt, T, P, Y, stride,

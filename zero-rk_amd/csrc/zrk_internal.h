@@ -29,7 +29,7 @@ int build_device_mech(MechHandle* h, std::string* err);
 int ensure_device(MechHandle* h);
 int ensure_workspace(MechHandle* h, int nslots);
 int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt, double* T,
-                     const double* P, double* Y, int stride, int* nsteps, int* stats,
+                     double* P, double* Y, int stride, int* nsteps, int* stats,
                      const int* order, hipStream_t st);
 
 }  // namespace zrk

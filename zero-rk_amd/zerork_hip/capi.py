@@ -50,7 +50,7 @@ class BdfOptions(C.Structure):
         ("reference_temperature", C.c_double), ("min_mass_fraction", C.c_double),
         ("cvode_retry_absolute_tolerance_adjustment", C.c_double),
         ("cvode_retry_relative_tolerance_adjustment", C.c_double),
-        ("max_steps", C.c_int), ("cvode_num_retries", C.c_int),
+        ("max_steps", C.c_int), ("cvode_num_retries", C.c_int), ("constant_volume", C.c_int),
     ]
 
 
