@@ -55,6 +55,11 @@ typedef struct {
    * 1: constant-volume reactor (reactor_constant_volume_cpu.cpp), the
    * plugin's constant_volume option */
   int constant_volume;
+  /* 0: Newton matrix, its inverse and the saved Jacobian in FP64 (the
+   * reference's arithmetic); 1: in FP32, with rates, residual, error test and
+   * state update still in FP64.  No reference counterpart (the reference
+   * preconditions in FP64 only). */
+  int fp32_preconditioner;
 } zrk_bdf_options;
 
 ZRK_EXPORT void zrk_bdf_default_options(zrk_bdf_options* o);
@@ -101,9 +106,9 @@ ZRK_EXPORT int zrk_integrate_device(zrk_mech m, const zrk_bdf_options* o, int n,
                                     double* T, double* P, double* Y, int stride,
                                     int* nsteps, int* stats, void* stream);
 
-/* Number of wavefronts the persistent integrator keeps resident (for
- * roofline bookkeeping). */
-ZRK_EXPORT int zrk_integrator_resident_waves(zrk_mech m);
+/* Number of wavefronts the persistent integrator keeps resident with these
+ * options (NULL: defaults), for roofline bookkeeping. */
+ZRK_EXPORT int zrk_integrator_resident_waves(zrk_mech m, const zrk_bdf_options* o);
 
 #ifdef __cplusplus
 }

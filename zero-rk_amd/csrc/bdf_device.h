@@ -43,11 +43,14 @@ constexpr int kQmax = 5;
 // Species block: lane i walks row i's term list (sorted by column), so each
 // entry has one writer and the loads of the list are coalesced across lanes;
 // the temperature row and the MW_i/MW_j scaling follow column by column.
+// Every entry is accumulated in FP64; MT = float stores the finished entries
+// in FP32 (the FP32 preconditioner keeps its Jacobian in FP32).
 // ---------------------------------------------------------------------------
+template <typename MT>
 __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s, double P,
                                                double v0, const BdfOpts& o, double y, double fy,
-                                               double inv_rho_prev, double* __restrict__ Jl,
-                                               double* __restrict__ saved,
+                                               double inv_rho_prev, MT* __restrict__ Jl,
+                                               MT* __restrict__ saved,
                                                double* __restrict__ t2l) {
   const int lane = lane_id();
   const int nsp = m.nsp;
@@ -78,7 +81,7 @@ __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s
   const double RuTemp = kRu * yT;
   const double mult = -1.0 / aux.cp_mix;
   const double mwi = sp ? m.mw[lane] : 0.0;
-  for (int c = 0; c < nsp; ++c) Jl[c * kNP + lane] = 0.0;
+  for (int c = 0; c < nsp; ++c) Jl[c * kNP + lane] = MT(0);
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -98,7 +101,7 @@ __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s
         }
         const double p = __dmul_rn(s.rop[t & 0xFFFFu], t2l[col]);
         v = (t & 0x80000000u) ? __dsub_rn(v, p) : __dadd_rn(v, p);
-        Jl[col * kNP + lane] = v;
+        Jl[col * kNP + lane] = (MT)v;
       }
     };
     uint4 w = nb > 0 ? jr[0] : make_uint4(0, 0, 0, 0);
@@ -115,33 +118,35 @@ __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   for (int c = 0; c < nsp; ++c) {
-    const double v = sp ? Jl[c * kNP + lane] : 0.0;
+    const double v = sp ? (double)Jl[c * kNP + lane] : 0.0;
     // d(Tdot)/dy_c from the concentration-space column
     const double sc = wave_sum(v * aux.h);
     double val = sc * RuTemp;
     val += Tdot * bcast(aux.cpm, c) * m.mw[c];
     val *= m.inv_mw[c] * mult;
     const double out = sp ? v * (mwi * m.inv_mw[c]) : (lane == nsp ? val : 0.0);
-    Jl[c * kNP + lane] = out;
-    saved[c * kNP + lane] = out;
+    Jl[c * kNP + lane] = (MT)out;
+    saved[c * kNP + lane] = (MT)out;
   }
-  Jl[nsp * kNP + lane] = colT;
-  saved[nsp * kNP + lane] = colT;
+  Jl[nsp * kNP + lane] = (MT)colT;
+  saved[nsp * kNP + lane] = (MT)colT;
   return aux2.inv_rho;
 }
 
 // Gauss-Jordan inverse (no pivoting) of the N x N leading block of a matrix
 // held one row per lane in R[0..63]; padding rows/columns must be identity.
 // After N steps register c holds logical column (c + N) mod 64 of the
-// inverse.  prow: kNP doubles of LDS for the published pivot row.
+// inverse.  prow: kNP entries of LDS for the published pivot row.  MT is the
+// working precision (double, or float for the FP32 preconditioner).
 // Returns false on a zero pivot.
-__device__ __forceinline__ bool gj_invert(double (&R)[kNP], int N, double* __restrict__ prow) {
+template <typename MT>
+__device__ __forceinline__ bool gj_invert(MT (&R)[kNP], int N, MT* __restrict__ prow) {
   const int lane = lane_id();
   bool ok = true;
   for (int k = 0; k < N; ++k) {
-    const double piv = bcast(R[0], k);
-    if (piv == 0.0) ok = false;
-    const double rp = 1.0 / piv;
+    const MT piv = bcast(R[0], k);
+    if (piv == MT(0)) ok = false;
+    const MT rp = MT(1) / piv;
     if (lane == k) {
 #pragma unroll
       for (int c = 1; c < kNP; ++c) R[c] *= rp;
@@ -151,8 +156,8 @@ __device__ __forceinline__ bool gj_invert(double (&R)[kNP], int N, double* __res
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const double l = (lane == k) ? 0.0 : R[0];
-    const double last = (lane == k) ? rp : -l * rp;
+    const MT l = (lane == k) ? MT(0) : R[0];
+    const MT last = (lane == k) ? rp : -l * rp;
 #pragma unroll
     for (int c = 1; c < kNP; ++c) R[c - 1] = fma(-l, prow[c], R[c]);
     R[kNP - 1] = last;

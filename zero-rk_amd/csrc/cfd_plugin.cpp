@@ -89,6 +89,8 @@ struct zerork_handle_impl {
     dopt["solve_temperature_threshold"] = 2.0;
     dopt["step_limiter"] = 1.0e22;
     iopt["gpu"] = 1;
+    // not a reference option: FP32 Newton matrix/inverse (FP64 everywhere else)
+    iopt["fp32_preconditioner"] = 0;
     iopt["initial_gpu_multiplier"] = 8;
     iopt["n_reactors_min"] = 128;
     iopt["n_reactors_max"] = 1024;
@@ -256,6 +258,7 @@ zerork_status_t solve(zerork_handle h, double dt, int n, double* T, double* P, d
   o.max_steps = h->iopt["max_steps"];
   o.cvode_num_retries = h->iopt["cvode_num_retries"];
   o.constant_volume = h->iopt["constant_volume"] != 0;
+  o.fp32_preconditioner = h->iopt["fp32_preconditioner"] != 0;
   const auto t0 = std::chrono::steady_clock::now();
   int rc = zrk::integrate_device(m, o, n, dt, h->dT, h->dP, h->dY, stride, h->dsteps, nullptr,
                                  sorted ? h->dorder : nullptr, st);

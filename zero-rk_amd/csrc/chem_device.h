@@ -54,6 +54,9 @@ __device__ __forceinline__ double bcast(double v, int src) {
   hi = __builtin_amdgcn_readlane(hi, src);
   return __hiloint2double(hi, lo);
 }
+__device__ __forceinline__ float bcast(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
 
 struct TPow {
   double T, invT, logT;

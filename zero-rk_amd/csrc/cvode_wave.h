@@ -39,6 +39,10 @@ enum {
 #define ZRK_TOC(t, i)
 #endif
 
+// MT: precision of the Newton matrix, its inverse and the saved Jacobian
+// (double: the plain FP64 method; float: the FP32 preconditioner, with the
+// residual, error test and state update still in FP64).
+template <typename MT>
 struct CvWave {
 #ifdef ZRK_PHASE_CYCLES
   long long cyc[4];
@@ -46,11 +50,12 @@ struct CvWave {
   DevMech mech;
   RhsLds s;
   BdfOpts opt;
-  double* savedJ;  // this wave's NP x NP column-major scratch in HBM
-  double* Mlds;    // LDS, NP x NP column-major: Jacobian workspace, then the
+  MT* savedJ;      // this wave's NP x NP column-major scratch in HBM
+  MT* Mlds;        // LDS, NP x NP column-major: Jacobian workspace, then the
                    // rotated inverse of the Newton matrix
-  double* prow;    // LDS [NP]: published pivot row
-  double* rvec;    // LDS [NP]: right-hand side of a solve
+  MT* prow;        // LDS [NP]: published pivot row
+  MT* rvec;        // LDS [NP]: right-hand side of a solve
+  double* t2l;     // LDS [NP]: Jacobian column factors
   double P;
   double v0;  // constant volume: fixed specific volume; 0 at constant pressure
   // lane vectors
@@ -365,7 +370,7 @@ struct CvWave {
     const double dgamma = fabs(gamma / gammap - 1.0);
     const bool jbad = (nst == 0) || (nst >= nstlj + kCvlsMsbj) ||
                       ((cf == kFailBadJ) && (dgamma < kCvlsDgmax)) || (cf == kFailOther);
-    double R[kNP];
+    MT R[kNP];
     if (!jbad) {
       jc = 0;
 #pragma unroll
@@ -375,8 +380,8 @@ struct CvWave {
       nje++;
       nstlj = nst;
       ZRK_TIC(t0);
-      inv_rho_last = jacobian_lds(mech, s, P, v0, opt, ypred, fpred, inv_rho_last, Mlds, savedJ,
-                                  prow);
+      inv_rho_last = jacobian_lds<MT>(mech, s, P, v0, opt, ypred, fpred, inv_rho_last, Mlds,
+                                      savedJ, t2l);
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -388,11 +393,11 @@ struct CvWave {
     const double ng = -gamma;
 #pragma unroll
     for (int c = 0; c < kNP; ++c) {
-      double v = (c <= nsp && lane <= nsp) ? ng * R[c] : 0.0;
+      double v = (c <= nsp && lane <= nsp) ? ng * (double)R[c] : 0.0;
       if (c == lane) v += 1.0;
-      R[c] = v;
+      R[c] = (MT)v;
     }
-    const bool ok = gj_invert(R, nsp + 1, prow);
+    const bool ok = gj_invert<MT>(R, nsp + 1, prow);
 #pragma unroll
     for (int c = 0; c < kNP; ++c) Mlds[c * kNP + lane] = R[c];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -406,16 +411,16 @@ struct CvWave {
     ZRK_TIC(t0);
     const int lane = lane_id();
     const int N = mech.nsp + 1;
-    rvec[lane] = b;
+    rvec[lane] = (MT)b;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    double acc = 0.0;
+    MT acc = MT(0);
 #pragma unroll 8
     for (int c = 0; c < kNP; ++c) acc = fma(Mlds[c * kNP + lane], rvec[(c + N) & (kNP - 1)], acc);
     __builtin_amdgcn_wave_barrier();
     ZRK_TOC(t0, 3);
-    return acc;
+    return (double)acc;
   }
 
   // cvNlsLSetup

@@ -21,6 +21,7 @@ struct BdfOpts {
   double retry_atol_adj, retry_rtol_adj;
   int max_steps, num_retries;
   int constant_volume;
+  int fp32_preconditioner;
 };
 
 enum RecFlags : int {

@@ -107,7 +107,7 @@ jacobian_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ Tin,
     double fy;
     RhsAux aux;
     rhs64(m, s, P, 0.0, o.Tref, o.step_limiter, y, fy, aux);
-    jacobian_lds(m, s, P, 0.0, o, y, fy, aux.inv_rho, Jl, saved, t2l);
+    jacobian_lds<double>(m, s, P, 0.0, o, y, fy, aux.inv_rho, Jl, saved, t2l);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -123,6 +123,10 @@ jacobian_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ Tin,
 // nsteps[n] = plugin cost (CVode one-step calls, negative on failure);
 // stats[n*8] (optional): nst nfe nje nsetups nni ncfn netf status.
 // ---------------------------------------------------------------------------
+// MT = double: plain FP64 method; MT = float: FP32 preconditioner.
+// LDS: Mlds[kNP*kNP] MT, prow[kNP] MT, rvec[kNP] MT, t2l[kNP] double, then
+// C[nsp], G[nsp], rop[nstep+1] doubles.
+template <typename MT>
 __global__ void __launch_bounds__(kWave)
 bdf_kernel(DevMech m, BdfOpts o, int n, double dt, double* __restrict__ Tio,
            double* __restrict__ Pio, double* __restrict__ Yio, int stride,
@@ -132,15 +136,17 @@ bdf_kernel(DevMech m, BdfOpts o, int n, double dt, double* __restrict__ Tio,
   extern __shared__ double lds[];
   const int lane = lane_id();
   const int nsp = m.nsp;
-  CvWave cv;
+  CvWave<MT> cv;
   cv.mech = m;
   cv.opt = o;
-  cv.Mlds = lds;
-  cv.prow = lds + kNP * kNP;
+  MT* lm = reinterpret_cast<MT*>(lds);
+  cv.Mlds = lm;
+  cv.prow = lm + kNP * kNP;
   cv.rvec = cv.prow + kNP;
-  double* rl = cv.rvec + kNP;
+  cv.t2l = reinterpret_cast<double*>(cv.rvec + kNP);
+  double* rl = cv.t2l + kNP;
   cv.s = RhsLds{rl, rl + nsp, rl + 2 * nsp};
-  cv.savedJ = jscratch + (size_t)blockIdx.x * kNP * kNP;
+  cv.savedJ = reinterpret_cast<MT*>(jscratch + (size_t)blockIdx.x * kNP * kNP);
   cv.active = lane <= nsp;
   cv.inv_nvar = 1.0 / (double)(nsp + 1);
   cv.nlscoef = o.nlscoef;
@@ -244,8 +250,9 @@ hipError_t launch_rates(const DevMech& m, int n, const double* T, const double* 
 size_t rhs_lds_bytes(const DevMech& m) {
   return sizeof(double) * (size_t)(2 * m.nsp + m.nstep + 1);
 }
-size_t bdf_lds_bytes(const DevMech& m) {
-  return sizeof(double) * (size_t)(kNP * kNP + 2 * kNP) + rhs_lds_bytes(m);
+size_t bdf_lds_bytes(const DevMech& m, bool f32) {
+  const size_t mt = f32 ? sizeof(float) : sizeof(double);
+  return mt * (size_t)(kNP * kNP + 2 * kNP) + sizeof(double) * kNP + rhs_lds_bytes(m);
 }
 
 hipError_t launch_jacobian(const DevMech& m, const BdfOpts& o, int n, const double* T,
@@ -260,14 +267,17 @@ hipError_t launch_jacobian(const DevMech& m, const BdfOpts& o, int n, const doub
   return hipGetLastError();
 }
 
-int bdf_max_resident_waves(const DevMech& m) {
+int bdf_max_resident_waves(const DevMech& m, bool f32) {
   int dev = 0, ncu = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bdf_kernel, kWave,
-                                                   bdf_lds_bytes(m)) != hipSuccess)
-    return 0;
+  const hipError_t e =
+      f32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bdf_kernel<float>, kWave,
+                                                         bdf_lds_bytes(m, true))
+          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bdf_kernel<double>, kWave,
+                                                         bdf_lds_bytes(m, false));
+  if (e != hipSuccess) return 0;
   return ncu * (per_cu > 0 ? per_cu : 1);
 }
 
@@ -279,8 +289,12 @@ hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, doub
   if (m.nsp > kNP - 1) return hipErrorInvalidValue;
   hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, counter, 0u);
   const int grid = nwaves < n ? nwaves : n;
-  hipLaunchKernelGGL(bdf_kernel, dim3(grid), dim3(kWave), bdf_lds_bytes(m), st, m, o, n, dt, T,
-                     P, Y, stride, nsteps, stats, order, counter, jscratch);
+  if (o.fp32_preconditioner)
+    hipLaunchKernelGGL(bdf_kernel<float>, dim3(grid), dim3(kWave), bdf_lds_bytes(m, true), st, m,
+                       o, n, dt, T, P, Y, stride, nsteps, stats, order, counter, jscratch);
+  else
+    hipLaunchKernelGGL(bdf_kernel<double>, dim3(grid), dim3(kWave), bdf_lds_bytes(m, false), st,
+                       m, o, n, dt, T, P, Y, stride, nsteps, stats, order, counter, jscratch);
   return hipGetLastError();
 }
 

@@ -241,6 +241,7 @@ BdfOpts to_bdf_opts(const zrk_bdf_options& o) {
   b.max_steps = o.max_steps;
   b.num_retries = o.cvode_num_retries;
   b.constant_volume = o.constant_volume != 0;
+  b.fp32_preconditioner = o.fp32_preconditioner != 0;
   return b;
 }
 
@@ -264,8 +265,9 @@ int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt, 
   if (h->host.nsp > kNP - 1) return -2;
   if (n == 0) return 0;
   if (int rc = ensure_device(h)) return rc;
-  if (h->resident_waves <= 0) h->resident_waves = bdf_max_resident_waves(h->dev);
-  const int nw = std::max(1, std::min(h->resident_waves, n));
+  const int f32 = o.fp32_preconditioner != 0;
+  if (h->resident_waves[f32] <= 0) h->resident_waves[f32] = bdf_max_resident_waves(h->dev, f32);
+  const int nw = std::max(1, std::min(h->resident_waves[f32], n));
   int rc = ensure_workspace(h, nw);
   if (rc) return rc;
   hipError_t e = launch_bdf(h->dev, to_bdf_opts(o), n, dt, T, P, Y, stride, nsteps, stats,
@@ -295,6 +297,7 @@ void zrk_bdf_default_options(zrk_bdf_options* o) {
   // the plugin defaults to constant volume (constant_volume=1); this
   // device-level default is the constant-pressure reactor
   o->constant_volume = 0;
+  o->fp32_preconditioner = 0;
 }
 
 int zrk_mech_load(const char* mech_file, const char* therm_file, zrk_mech* out, char* err,
@@ -403,12 +406,14 @@ int zrk_integrate_device(zrk_mech m, const zrk_bdf_options* o, int n, double dt,
                                nsteps, stats, nullptr, (hipStream_t)stream);
 }
 
-int zrk_integrator_resident_waves(zrk_mech m) {
+int zrk_integrator_resident_waves(zrk_mech m, const zrk_bdf_options* o) {
   if (!m) return -1;
   auto* h = reinterpret_cast<MechHandle*>(m);
   if (int rc = zrk::ensure_device(h)) return -rc;
-  if (h->resident_waves <= 0) h->resident_waves = zrk::bdf_max_resident_waves(h->dev);
-  return h->resident_waves;
+  const int f32 = (o && o->fp32_preconditioner) ? 1 : 0;
+  if (h->resident_waves[f32] <= 0)
+    h->resident_waves[f32] = zrk::bdf_max_resident_waves(h->dev, f32);
+  return h->resident_waves[f32];
 }
 
 }  // extern "C"

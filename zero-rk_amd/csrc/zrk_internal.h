@@ -17,7 +17,7 @@ struct MechHandle {
   void* dev_blob = nullptr;
   size_t dev_bytes = 0;
   int device = 0;
-  int resident_waves = 0;
+  int resident_waves[2] = {0, 0};  // persistent grid size: FP64, FP32 preconditioner
   // integrator workspace
   double* ws_jac = nullptr;
   unsigned int* ws_counter = nullptr;

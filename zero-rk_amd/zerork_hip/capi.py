@@ -51,6 +51,7 @@ class BdfOptions(C.Structure):
         ("cvode_retry_absolute_tolerance_adjustment", C.c_double),
         ("cvode_retry_relative_tolerance_adjustment", C.c_double),
         ("max_steps", C.c_int), ("cvode_num_retries", C.c_int), ("constant_volume", C.c_int),
+        ("fp32_preconditioner", C.c_int),
     ]
 
 
@@ -112,7 +113,7 @@ def _declare(L):
     L.zrk_jacobian_device.restype = I
     L.zrk_integrate_device.argtypes = [P, C.POINTER(BdfOptions), I, D, P, P, P, I, P, P, P]
     L.zrk_integrate_device.restype = I
-    L.zrk_integrator_resident_waves.argtypes = [P]
+    L.zrk_integrator_resident_waves.argtypes = [P, P]
     L.zrk_integrator_resident_waves.restype = I
 
 
@@ -216,8 +217,8 @@ class Mech:
         if rc != 0:
             raise RuntimeError(f"zrk_integrate_device failed ({rc})")
 
-    def resident_waves(self):
-        return lib().zrk_integrator_resident_waves(self.h)
+    def resident_waves(self, opts: BdfOptions | None = None):
+        return lib().zrk_integrator_resident_waves(self.h, C.byref(opts) if opts else None)
 
 
 class Plugin:
