@@ -133,6 +133,101 @@ __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s
   return aux2.inv_rho;
 }
 
+// ---------------------------------------------------------------------------
+// The same Jacobian for a team of several wavefronts (nsp < 64W): thread i
+// owns row i; J and saved are column-major with column stride js; t2l and hl
+// are kThreads doubles of LDS scratch.  The temperature row is a
+// matrix-vector product read from LDS (column c by thread c, conflict-free for
+// an odd js) instead of one reduction per column.
+// ---------------------------------------------------------------------------
+template <typename MT, class Team>
+__device__ __forceinline__ double jacobian_team(const Team& team, const DevMech& m,
+                                                const RhsLds& s, double P, double v0,
+                                                const BdfOpts& o, double y, double fy,
+                                                double inv_rho_prev, MT* __restrict__ J,
+                                                MT* __restrict__ saved, int js,
+                                                double* __restrict__ t2l,
+                                                double* __restrict__ hl) {
+  const int tid = team.tid();
+  const int nsp = m.nsp;
+  const bool sp = tid < nsp;
+  const bool act = tid <= nsp;
+  const double mmf = o.min_mass_fraction;
+  double t1 = y;
+  if (sp && fabs(t1) < mmf) t1 = (t1 < 0.0) ? -mmf : mmf;
+  t2l[tid] = sp ? m.mw[tid] / t1 * inv_rho_prev : 0.0;
+  const double yT = team.from(y, nsp);
+  double dT = yT * sqrt(DBL_EPSILON);
+  const double Tp = yT + dT;
+  dT = Tp - yT;
+  double t4;
+  RhsAux aux2;
+  rhs_team(team, m, s, P, v0, o.Tref, o.step_limiter, (tid == nsp) ? Tp : (sp ? y : 0.0), t4,
+           aux2);
+  const double colT = act ? (t4 - fy) * (1.0 / dT) : 0.0;
+  double t3;
+  RhsAux aux;
+  rhs_team(team, m, s, P, v0, o.Tref, o.step_limiter, t1, t3, aux);
+  const double Tdot = team.from(t3, nsp);
+  hl[tid] = sp ? aux.h : 0.0;
+  for (int c = 0; c < nsp; ++c) J[c * js + tid] = MT(0);
+  team.sync();
+  {
+    const uint4* __restrict__ jr = m.jr + tid;
+    const int nb = m.jr_blocks;
+    int cur = -1;
+    double v = 0.0;
+    auto term = [&](unsigned t) {
+      const int col = (int)((t >> 16) & 0xFFu);
+      if (col != 0xFF) {
+        if (col != cur) {
+          cur = col;
+          v = 0.0;
+        }
+        const double p = __dmul_rn(s.rop[t & 0xFFFFu], t2l[col]);
+        v = (t & 0x80000000u) ? __dsub_rn(v, p) : __dadd_rn(v, p);
+        J[col * js + tid] = (MT)v;
+      }
+    };
+    uint4 w = nb > 0 ? jr[0] : make_uint4(0, 0, 0, 0);
+    for (int b = 0; b < nb; ++b) {
+      const uint4 x = w;
+      if (b + 1 < nb) w = jr[(b + 1) * Team::kThreads];
+      term(x.x);
+      term(x.y);
+      term(x.z);
+      term(x.w);
+    }
+  }
+  team.sync();
+  // temperature row: thread c forms d(Tdot)/dy_c from column c
+  double trow = 0.0;
+  if (sp) {
+    double acc = 0.0;
+    for (int i = 0; i < nsp; ++i) acc += (double)J[tid * js + i] * hl[i];
+    trow = acc * (kRu * yT);
+    trow += Tdot * aux.cpm * m.mw[tid];
+    trow *= m.inv_mw[tid] * (-1.0 / aux.cp_mix);
+  }
+  team.sync();
+  // row i of the species block: J_ic *= MW_i / MW_c; then the temperature row
+  // (column c's entry at row nsp) and the finite-difference column
+  if (sp) {
+    const double mwi = m.mw[tid];
+    for (int c = 0; c < nsp; ++c) {
+      const MT out = (MT)((double)J[c * js + tid] * (mwi * m.inv_mw[c]));
+      J[c * js + tid] = out;
+      saved[c * js + tid] = out;
+    }
+    J[tid * js + nsp] = (MT)trow;
+    saved[tid * js + nsp] = (MT)trow;
+  }
+  J[nsp * js + tid] = (MT)colT;
+  saved[nsp * js + tid] = (MT)colT;
+  team.sync();
+  return aux2.inv_rho;
+}
+
 // Gauss-Jordan inverse (no pivoting) of the N x N leading block of a matrix
 // held one row per lane in R[0..63]; padding rows/columns must be identity.
 // After N steps register c holds logical column (c + N) mod 64 of the
@@ -162,6 +257,42 @@ __device__ __forceinline__ bool gj_invert(MT (&R)[kNP], int N, MT* __restrict__ 
     for (int c = 1; c < kNP; ++c) R[c - 1] = fma(-l, prow[c], R[c]);
     R[kNP - 1] = last;
     __builtin_amdgcn_wave_barrier();
+  }
+  return ok;
+}
+
+// In-place Gauss-Jordan inverse (no pivoting) of the N x N matrix A held in
+// LDS column-major with column stride js (thread i owns row i), for a team of
+// several wavefronts.  Step k: every thread c scales entry (k, c) of the
+// pivot row into prow[c]; then every row i != k subtracts f_i * prow from its
+// row (f_i its column-k entry, read before the step) and sets its column-k
+// entry to -f_i / pivot, while row k takes prow.  Returns false on a zero pivot.
+template <typename MT, class Team>
+__device__ __forceinline__ bool gj_invert_lds(const Team& team, MT* __restrict__ A, int js, int N,
+                                              MT* __restrict__ prow) {
+  const int tid = team.tid();
+  const bool row = tid < N;
+  bool ok = true;
+  team.sync();
+  for (int k = 0; k < N; ++k) {
+    const MT piv = A[k * js + k];
+    if (piv == MT(0)) ok = false;
+    const MT rp = MT(1) / piv;
+    const MT f = row ? A[k * js + tid] : MT(0);
+    if (row) prow[tid] = (tid == k) ? rp : A[tid * js + k] * rp;
+    team.sync();
+    if (row) {
+      if (tid == k) {
+        for (int c = 0; c < N; ++c) A[c * js + tid] = prow[c];
+      } else {
+#pragma unroll 4
+        for (int c = 0; c < N; ++c) {
+          const MT a = A[c * js + tid];
+          A[c * js + tid] = (c == k) ? -f * rp : fma(-f, prow[c], a);
+        }
+      }
+    }
+    team.sync();
   }
   return ok;
 }

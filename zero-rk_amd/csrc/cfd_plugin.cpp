@@ -258,7 +258,9 @@ zerork_status_t solve(zerork_handle h, double dt, int n, double* T, double* P, d
   o.max_steps = h->iopt["max_steps"];
   o.cvode_num_retries = h->iopt["cvode_num_retries"];
   o.constant_volume = h->iopt["constant_volume"] != 0;
-  o.fp32_preconditioner = h->iopt["fp32_preconditioner"] != 0;
+  // mechanisms above 63 species run on multi-wavefront teams, which keep the
+  // Newton matrix in FP32 only
+  o.fp32_preconditioner = h->iopt["fp32_preconditioner"] != 0 || nsp > 63;
   const auto t0 = std::chrono::steady_clock::now();
   int rc = zrk::integrate_device(m, o, n, dt, h->dT, h->dP, h->dY, stride, h->dsteps, nullptr,
                                  sorted ? h->dorder : nullptr, st);

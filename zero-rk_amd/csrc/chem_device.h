@@ -58,6 +58,62 @@ __device__ __forceinline__ float bcast(float v, int src) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
 }
 
+// ---------------------------------------------------------------------------
+// Teams: the threads that share one reactor.  A TeamWave is one wavefront
+// (state components on lanes 0..63); a TeamBlock<W> is W wavefronts of one
+// workgroup (components on threads 0..64W-1), whose reductions and
+// broadcasts go through a few LDS slots (red[0..W]) and workgroup barriers.
+// Reductions sum the per-wavefront butterfly results in wavefront order, so
+// every thread gets the same bits.
+// ---------------------------------------------------------------------------
+struct TeamWave {
+  static constexpr int kThreads = kWave;
+  __device__ __forceinline__ int tid() const { return lane_id(); }
+  __device__ __forceinline__ double sum(double v) const { return wave_sum(v); }
+  __device__ __forceinline__ double max(double v) const { return wave_max(v); }
+  __device__ __forceinline__ double from(double v, int src) const { return bcast(v, src); }
+  __device__ __forceinline__ void sync() const {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+};
+
+template <int W>
+struct TeamBlock {
+  static constexpr int kThreads = kWave * W;
+  double* red;  // LDS [W + 1]
+  __device__ __forceinline__ int tid() const { return (int)threadIdx.x; }
+  __device__ __forceinline__ void sync() const { __syncthreads(); }
+  __device__ __forceinline__ double sum(double v) const {
+    const double s = wave_sum(v);
+    if (lane_id() == 0) red[threadIdx.x / kWave] = s;
+    __syncthreads();
+    double r = red[0];
+#pragma unroll
+    for (int w = 1; w < W; ++w) r += red[w];
+    __syncthreads();
+    return uni(r);
+  }
+  __device__ __forceinline__ double max(double v) const {
+    const double s = wave_max(v);
+    if (lane_id() == 0) red[threadIdx.x / kWave] = s;
+    __syncthreads();
+    double r = red[0];
+#pragma unroll
+    for (int w = 1; w < W; ++w) r = fmax(r, red[w]);
+    __syncthreads();
+    return uni(r);
+  }
+  __device__ __forceinline__ double from(double v, int src) const {
+    if ((int)threadIdx.x == src) red[W] = v;
+    __syncthreads();
+    const double r = red[W];
+    __syncthreads();
+    return uni(r);
+  }
+};
+
 struct TPow {
   double T, invT, logT;
   double hm0, hm1, hm2, hm3;
@@ -131,20 +187,21 @@ __device__ __forceinline__ RecLoad load_rec(const DevMech& m, int r) {
 // Rate constants, limiter and rates of progress of every step of one reactor.
 // C, G: species concentrations and g/RT in the wave's LDS; rop: [nstep+1] LDS
 // (slot nstep is the zero rate the padded gathers point at).
-// Lane l handles records l, l+64, ...; the next record is loaded while the
-// current one is evaluated.
-__device__ __forceinline__ void wave_rates(const DevMech& m, const double* __restrict__ C,
+// Thread t of a team of S threads handles records t, t+S, ...; the next record
+// is loaded while the current one is evaluated.
+template <int S>
+__device__ __forceinline__ void team_rates(int tid, const DevMech& m, const double* __restrict__ C,
                                            const double* __restrict__ G, double* __restrict__ rop,
                                            const TPow& p, double Csum, double L) {
   const bool limit = (0.0 < L && L < 1.0e300);
   const int nrec = m.nrec;
-  if (lane_id() == 0) rop[m.nstep] = 0.0;
-  int r = lane_id();
+  if (tid == 0) rop[m.nstep] = 0.0;
+  int r = tid;
   RecLoad nx;
   if (r < nrec) nx = load_rec(m, r);
-  for (; r < nrec; r += kWave) {
+  for (; r < nrec; r += S) {
     const RecLoad x = nx;
-    if (r + kWave < nrec) nx = load_rec(m, r + kWave);
+    if (r + S < nrec) nx = load_rec(m, r + S);
     const int fl = x.meta.z;
     const int re[kMaxPart] = {lo16(x.sp.x), hi16(x.sp.x), lo16(x.sp.y), hi16(x.sp.y)};
     const int pr[kMaxPart] = {lo16(x.sp.z), hi16(x.sp.z), lo16(x.sp.w), hi16(x.sp.w)};
@@ -283,13 +340,14 @@ struct RhsAux {
   double cpm;      // lane: cp or cv (J/kg/K) of species lane
 };
 
-__device__ __forceinline__ int rhs64(const DevMech& m, const RhsLds& s, double P, double v0,
-                                     double Tref, double L, double y, double& ydot,
-                                     RhsAux& aux) {
-  const int lane = lane_id();
+template <class Team>
+__device__ __forceinline__ int rhs_team(const Team& team, const DevMech& m, const RhsLds& s,
+                                        double P, double v0, double Tref, double L, double y,
+                                        double& ydot, RhsAux& aux) {
+  const int lane = team.tid();
   const int nsp = m.nsp;
   const bool sp = lane < nsp;
-  const double yT = bcast(y, nsp);
+  const double yT = team.from(y, nsp);
   double T = yT * Tref;
   if (T <= 0.0) {
     ydot = 0.0;
@@ -302,7 +360,7 @@ __device__ __forceinline__ int rhs64(const DevMech& m, const RhsLds& s, double P
   const bool cvol = v0 > 0.0;
   double inv_rho = v0;
   if (!cvol) {
-    const double mws = wave_sum(Yj * imw);
+    const double mws = team.sum(Yj * imw);
     const double wmix = 1.0 / mws;
     inv_rho = 1.0 / (P * wmix / (kRu * T));
   }
@@ -319,13 +377,11 @@ __device__ __forceinline__ int rhs64(const DevMech& m, const RhsLds& s, double P
     s.C[lane] = Cj;
     s.G[lane] = g;
   }
-  const double cp_mix = wave_sum(cpm * Yj);
-  const double Csum = wave_sum(Cj);
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  wave_rates(m, s.C, s.G, s.rop, p, Csum, L);
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
+  const double cp_mix = team.sum(cpm * Yj);
+  const double Csum = team.sum(Cj);
+  team.sync();
+  team_rates<Team::kThreads>(lane, m, s.C, s.G, s.rop, p, Csum, L);
+  team.sync();
   double e = 0.0;
   double yd = 0.0;
   if (sp) {
@@ -335,15 +391,22 @@ __device__ __forceinline__ int rhs64(const DevMech& m, const RhsLds& s, double P
     yd = (net * m.mw[lane]) * inv_rho;
     e = h * net * inv_rho;
   }
-  const double esum = wave_sum(e);
+  const double esum = team.sum(e);
   if (lane == nsp) yd = -esum * kRu * yT / cp_mix + 0.0 / (cp_mix * Tref);
   ydot = yd;
   aux.inv_rho = inv_rho;
   aux.cp_mix = cp_mix;
   aux.h = h;
   aux.cpm = cpm;
-  __builtin_amdgcn_wave_barrier();
+  team.sync();
   return 0;
+}
+
+// One reactor per wavefront (nsp <= 63).
+__device__ __forceinline__ int rhs64(const DevMech& m, const RhsLds& s, double P, double v0,
+                                     double Tref, double L, double y, double& ydot,
+                                     RhsAux& aux) {
+  return rhs_team(TeamWave{}, m, s, P, v0, Tref, L, y, ydot, aux);
 }
 
 }  // namespace zrk

@@ -42,20 +42,28 @@ enum {
 // MT: precision of the Newton matrix, its inverse and the saved Jacobian
 // (double: the plain FP64 method; float: the FP32 preconditioner, with the
 // residual, error test and state update still in FP64).
-template <typename MT>
-struct CvWave {
+// Team: TeamWave (one wavefront per reactor, nsp <= 63) or TeamBlock<W> (W
+// wavefronts per reactor, nsp < 64W, FP32 matrices only).
+template <typename MT, class Team = TeamWave>
+struct Cvode {
+  static constexpr int kL = Team::kThreads;        // state slots = matrix row registers
+  static constexpr bool kWaveTeam = (kL == kWave);
+  static constexpr int kJS = kWaveTeam ? kNP : kL + 1;  // LDS column stride of matrices
+  Team team;
 #ifdef ZRK_PHASE_CYCLES
   long long cyc[4];
 #endif
   DevMech mech;
   RhsLds s;
   BdfOpts opt;
-  MT* savedJ;      // this wave's NP x NP column-major scratch in HBM
-  MT* Mlds;        // LDS, NP x NP column-major: Jacobian workspace, then the
-                   // rotated inverse of the Newton matrix
-  MT* prow;        // LDS [NP]: published pivot row
-  MT* rvec;        // LDS [NP]: right-hand side of a solve
-  double* t2l;     // LDS [NP]: Jacobian column factors
+  MT* savedJ;      // this team's saved Jacobian in HBM (column stride kJS)
+  MT* Mlds;        // LDS, column-major with stride kJS: Jacobian workspace,
+                   // then the inverse of the Newton matrix
+  MT* prow;        // LDS [kL]: published pivot row
+  MT* rvec;        // LDS [kL]: right-hand side of a solve
+  double* t2l;     // LDS [kL]: Jacobian column factors
+  double* hl;      // LDS [kL]: energies for the Jacobian's temperature row (teams of
+                   // several wavefronts)
   double P;
   double v0;  // constant volume: fixed specific volume; 0 at constant pressure
   // lane vectors
@@ -74,14 +82,14 @@ struct CvWave {
 
   __device__ __forceinline__ double wrms(double x) const {
     const double p = active ? x * ewt : 0.0;
-    return sqrt(wave_sum(p * p) * inv_nvar);
+    return sqrt(team.sum(p * p) * inv_nvar);
   }
 
   __device__ __forceinline__ int f(double yy, double& yd) {
     nfe++;
     RhsAux aux;
     ZRK_TIC(t0);
-    const int fl = rhs64(mech, s, P, v0, opt.Tref, opt.step_limiter, yy, yd, aux);
+    const int fl = rhs_team(team, mech, s, P, v0, opt.Tref, opt.step_limiter, yy, yd, aux);
     ZRK_TOC(t0, 0);
     if (!fl) inv_rho_last = aux.inv_rho;
     if (!active) yd = 0.0;
@@ -90,7 +98,7 @@ struct CvWave {
 
   __device__ __forceinline__ int set_ewt(double yy) {
     const double v = rtol * fabs(yy) + atolv;
-    const double bad = wave_max((active && v <= 0.0) ? 1.0 : 0.0);
+    const double bad = team.max((active && v <= 0.0) ? 1.0 : 0.0);
     ewt = active ? 1.0 / v : 0.0;
     return bad > 0.0 ? -1 : 0;
   }
@@ -126,7 +134,7 @@ struct CvWave {
   __device__ __forceinline__ double upper_bound_h0(double tdist) {
     const double t1 = kHubFactor * fabs(zn[0]) + (active ? 1.0 / ewt : 1.0);
     const double t2 = active ? fabs(zn[1]) / t1 : 0.0;
-    const double hub_inv = wave_max(t2);
+    const double hub_inv = team.max(t2);
     double hub = kHubFactor * tdist;
     if (hub * hub_inv > 1.0) hub = 1.0 / hub_inv;
     return hub;
@@ -365,13 +373,24 @@ struct CvWave {
 
   // cvLsSetup/cvLsLinSys + inversion of M = I - gamma*J; >0 recoverable failure
   __device__ __forceinline__ int lsetup(int cf, double ypred, double fpred, int& jc) {
-    const int nsp = mech.nsp;
-    const int lane = lane_id();
+    if constexpr (kWaveTeam)
+      return lsetup_wave(cf, ypred, fpred, jc);
+    else
+      return lsetup_team(cf, ypred, fpred, jc);
+  }
+
+  __device__ __forceinline__ bool jacobian_is_bad(int cf) const {
     const double dgamma = fabs(gamma / gammap - 1.0);
-    const bool jbad = (nst == 0) || (nst >= nstlj + kCvlsMsbj) ||
-                      ((cf == kFailBadJ) && (dgamma < kCvlsDgmax)) || (cf == kFailOther);
+    return (nst == 0) || (nst >= nstlj + kCvlsMsbj) ||
+           ((cf == kFailBadJ) && (dgamma < kCvlsDgmax)) || (cf == kFailOther);
+  }
+
+  // one wavefront: the matrix is inverted in registers (row per lane)
+  __device__ __forceinline__ int lsetup_wave(int cf, double ypred, double fpred, int& jc) {
+    const int nsp = mech.nsp;
+    const int lane = team.tid();
     MT R[kNP];
-    if (!jbad) {
+    if (!jacobian_is_bad(cf)) {
       jc = 0;
 #pragma unroll
       for (int c = 0; c < kNP; ++c) R[c] = savedJ[c * kNP + lane];
@@ -382,9 +401,7 @@ struct CvWave {
       ZRK_TIC(t0);
       inv_rho_last = jacobian_lds<MT>(mech, s, P, v0, opt, ypred, fpred, inv_rho_last, Mlds,
                                       savedJ, t2l);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      team.sync();
 #pragma unroll
       for (int c = 0; c < kNP; ++c) R[c] = Mlds[c * kNP + lane];
       ZRK_TOC(t0, 1);
@@ -400,25 +417,62 @@ struct CvWave {
     const bool ok = gj_invert<MT>(R, nsp + 1, prow);
 #pragma unroll
     for (int c = 0; c < kNP; ++c) Mlds[c * kNP + lane] = R[c];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    team.sync();
     ZRK_TOC(t1, 2);
     return ok ? 0 : 1;
   }
 
-  // x = M^{-1} b with the rotated inverse in LDS
+  // several wavefronts: the matrix is inverted in place in LDS (thread i owns
+  // row i; column-major with odd stride kJS, so every access is conflict-free)
+  __device__ __forceinline__ int lsetup_team(int cf, double ypred, double fpred, int& jc) {
+    const int nsp = mech.nsp;
+    const int N = nsp + 1;
+    const int tid = team.tid();
+    if (!jacobian_is_bad(cf)) {
+      jc = 0;
+      if (tid < N)
+        for (int c = 0; c < N; ++c) Mlds[c * kJS + tid] = savedJ[c * kJS + tid];
+    } else {
+      jc = 1;
+      nje++;
+      nstlj = nst;
+      ZRK_TIC(t0);
+      inv_rho_last = jacobian_team<MT>(team, mech, s, P, v0, opt, ypred, fpred, inv_rho_last,
+                                       Mlds, savedJ, kJS, t2l, hl);
+      ZRK_TOC(t0, 1);
+    }
+    ZRK_TIC(t1);
+    const double ng = -gamma;
+    if (tid < N)
+      for (int c = 0; c < N; ++c) {
+        double v = ng * (double)Mlds[c * kJS + tid];
+        if (c == tid) v += 1.0;
+        Mlds[c * kJS + tid] = (MT)v;
+      }
+    const bool ok = gj_invert_lds<MT>(team, Mlds, kJS, N, prow);
+    ZRK_TOC(t1, 2);
+    return ok ? 0 : 1;
+  }
+
+  // x = M^{-1} b with the inverse in LDS (rotated column order for one
+  // wavefront, logical order for larger teams)
   __device__ __forceinline__ double lsolve(double b) {
     ZRK_TIC(t0);
-    const int lane = lane_id();
+    const int lane = team.tid();
     const int N = mech.nsp + 1;
     rvec[lane] = (MT)b;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    team.sync();
     MT acc = MT(0);
+    if constexpr (kWaveTeam) {
 #pragma unroll 8
-    for (int c = 0; c < kNP; ++c) acc = fma(Mlds[c * kNP + lane], rvec[(c + N) & (kNP - 1)], acc);
-    __builtin_amdgcn_wave_barrier();
+      for (int c = 0; c < kNP; ++c)
+        acc = fma(Mlds[c * kNP + lane], rvec[(c + N) & (kNP - 1)], acc);
+    } else {
+#pragma unroll 8
+      for (int j = 0; j < N; ++j) acc = fma(Mlds[j * kJS + lane], rvec[j], acc);
+      if (lane >= N) acc = MT(0);
+    }
+    team.sync();
     ZRK_TOC(t0, 3);
     return (double)acc;
   }
@@ -733,5 +787,8 @@ struct CvWave {
     return out;
   }
 };
+
+template <typename MT>
+using CvWave = Cvode<MT, TeamWave>;
 
 }  // namespace zrk

@@ -57,7 +57,8 @@ struct DevMech {
   // species j = 64*jb + (j&63), in step order; padding index nstep (rate 0)
   const uint4* g_d;
   const uint4* g_c;
-  // Jacobian species-block terms by row (nsp <= 63): jr[b*64 + row] holds 4
+  // Jacobian species-block terms by row (nsp <= 191): jr[b*S + row] (S = 64,
+  // 128 or 192: the integrator's team size) holds 4
   // terms, each bits 0-15 step, 16-23 column (0xFF = padding), bit 31 set for
   // a destruction term; sorted by column, then destruction before creation,
   // each in step order (the reference's accumulation order)

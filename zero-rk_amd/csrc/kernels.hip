@@ -60,7 +60,7 @@ rates_kernel(DevMech m, int n, const double* __restrict__ Tin, const double* __r
     const double Csum = wave_sum(cs);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    wave_rates(m, C, G, rop, p, Csum, L);
+    team_rates<kWave>(lane, m, C, G, rop, p, Csum, L);
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     double e = 0.0;
@@ -118,35 +118,25 @@ jacobian_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ Tin,
 }
 
 // ---------------------------------------------------------------------------
-// bdf_kernel (nsp <= 63): persistent; wavefronts pull reactors from *counter.
+// Persistent per-reactor BDF: each team (one wavefront, or W wavefronts of a
+// workgroup) takes reactors from *counter until the batch is exhausted.
 // T[n], Y[n*stride] updated in place (and P[n] at constant volume);
 // nsteps[n] = plugin cost (CVode one-step calls, negative on failure);
 // stats[n*8] (optional): nst nfe nje nsetups nni ncfn netf status.
 // ---------------------------------------------------------------------------
-// MT = double: plain FP64 method; MT = float: FP32 preconditioner.
-// LDS: Mlds[kNP*kNP] MT, prow[kNP] MT, rvec[kNP] MT, t2l[kNP] double, then
-// C[nsp], G[nsp], rop[nstep+1] doubles.
-template <typename MT>
-__global__ void __launch_bounds__(kWave)
-bdf_kernel(DevMech m, BdfOpts o, int n, double dt, double* __restrict__ Tio,
-           double* __restrict__ Pio, double* __restrict__ Yio, int stride,
-           int* __restrict__ nsteps_out, int* __restrict__ stats_out,
-           const int* __restrict__ order, unsigned int* __restrict__ counter,
-           double* __restrict__ jscratch) {
-  extern __shared__ double lds[];
-  const int lane = lane_id();
+template <typename MT, class Team>
+__device__ __forceinline__ void bdf_driver(Cvode<MT, Team>& cv, const DevMech& m,
+                                           const BdfOpts& o, int n, double dt,
+                                           double* __restrict__ Tio, double* __restrict__ Pio,
+                                           double* __restrict__ Yio, int stride,
+                                           int* __restrict__ nsteps_out,
+                                           int* __restrict__ stats_out,
+                                           const int* __restrict__ order,
+                                           unsigned int* __restrict__ counter) {
+  const int lane = cv.team.tid();
   const int nsp = m.nsp;
-  CvWave<MT> cv;
   cv.mech = m;
   cv.opt = o;
-  MT* lm = reinterpret_cast<MT*>(lds);
-  cv.Mlds = lm;
-  cv.prow = lm + kNP * kNP;
-  cv.rvec = cv.prow + kNP;
-  cv.t2l = reinterpret_cast<double*>(cv.rvec + kNP);
-  double* rl = cv.t2l + kNP;
-  cv.s = RhsLds{rl, rl + nsp, rl + 2 * nsp};
-  cv.savedJ = reinterpret_cast<MT*>(jscratch + (size_t)blockIdx.x * kNP * kNP);
   cv.active = lane <= nsp;
   cv.inv_nvar = 1.0 / (double)(nsp + 1);
   cv.nlscoef = o.nlscoef;
@@ -154,7 +144,7 @@ bdf_kernel(DevMech m, BdfOpts o, int n, double dt, double* __restrict__ Tio,
   for (;;) {
     unsigned int r = 0;
     if (lane == 0) r = atomicAdd(counter, 1u);
-    r = (unsigned int)__builtin_amdgcn_readfirstlane((int)r);
+    r = (unsigned int)cv.team.from((double)r, 0);
     if (r >= (unsigned int)n) break;
     if (order) r = (unsigned int)order[r];
     double y0 = 0.0;
@@ -165,8 +155,8 @@ bdf_kernel(DevMech m, BdfOpts o, int n, double dt, double* __restrict__ Tio,
     if (o.constant_volume) {
       // ReactorConstantVolumeCPU::InitializeState: the volume of the initial
       // state (reactor_constant_volume_cpu.cpp:28-29)
-      const double mws = wave_sum(lane < nsp ? y0 * m.inv_mw[lane] : 0.0);
-      cv.v0 = 1.0 / (cv.P * (1.0 / mws) / (kRu * (bcast(y0, nsp) * o.Tref)));
+      const double mws = cv.team.sum(lane < nsp ? y0 * m.inv_mw[lane] : 0.0);
+      cv.v0 = 1.0 / (cv.P * (1.0 / mws) / (kRu * (cv.team.from(y0, nsp) * o.Tref)));
     }
     cv.atolv = cv.active ? o.atol : 0.0;
     cv.rtol = o.rtol;
@@ -201,8 +191,8 @@ bdf_kernel(DevMech m, BdfOpts o, int n, double dt, double* __restrict__ Tio,
       if (lane == nsp) Tio[r] = yout * o.Tref;
       if (o.constant_volume) {
         // GetState: P = Ru T / (v Wmix) (reactor_constant_volume_cpu.cpp:44-63)
-        const double mws = wave_sum(lane < nsp ? yout * m.inv_mw[lane] : 0.0);
-        const double Tend = bcast(yout, nsp) * o.Tref;
+        const double mws = cv.team.sum(lane < nsp ? yout * m.inv_mw[lane] : 0.0);
+        const double Tend = cv.team.from(yout, nsp) * o.Tref;
         if (lane == 0) Pio[r] = kRu * Tend / (cv.v0 * (1.0 / mws));
       }
     } else {
@@ -225,7 +215,64 @@ bdf_kernel(DevMech m, BdfOpts o, int n, double dt, double* __restrict__ Tio,
 #endif
       }
     }
+    cv.team.sync();
   }
+}
+
+// One wavefront per reactor (nsp <= 63).  MT = double: plain FP64 method;
+// MT = float: FP32 preconditioner.  LDS: Mlds[kNP*kNP] MT, prow[kNP] MT,
+// rvec[kNP] MT, t2l[kNP] double, then C[nsp], G[nsp], rop[nstep+1] doubles.
+template <typename MT>
+__global__ void __launch_bounds__(kWave)
+bdf_kernel(DevMech m, BdfOpts o, int n, double dt, double* __restrict__ Tio,
+           double* __restrict__ Pio, double* __restrict__ Yio, int stride,
+           int* __restrict__ nsteps_out, int* __restrict__ stats_out,
+           const int* __restrict__ order, unsigned int* __restrict__ counter,
+           double* __restrict__ jscratch) {
+  extern __shared__ double lds[];
+  const int nsp = m.nsp;
+  CvWave<MT> cv;
+  MT* lm = reinterpret_cast<MT*>(lds);
+  cv.Mlds = lm;
+  cv.prow = lm + kNP * kNP;
+  cv.rvec = cv.prow + kNP;
+  cv.t2l = reinterpret_cast<double*>(cv.rvec + kNP);
+  cv.hl = nullptr;
+  double* rl = cv.t2l + kNP;
+  cv.s = RhsLds{rl, rl + nsp, rl + 2 * nsp};
+  cv.savedJ = reinterpret_cast<MT*>(jscratch + (size_t)blockIdx.x * kNP * kNP);
+  bdf_driver(cv, m, o, n, dt, Tio, Pio, Yio, stride, nsteps_out, stats_out, order, counter);
+}
+
+// W wavefronts per reactor (64 <= nsp < 64W), FP32 preconditioner.  LDS
+// (bdf_team_lds_bytes): Mlds[(nsp+1)*(64W+1)] float, prow[64W], rvec[64W]
+// float, t2l[64W], hl[64W], red[W+1] double, then C[nsp], G[nsp],
+// rop[nstep+1] doubles.  savedJ slot: slot_doubles doubles per workgroup.
+template <int W>
+__global__ void __launch_bounds__(kWave * W)
+bdf_team_kernel(DevMech m, BdfOpts o, int n, double dt, double* __restrict__ Tio,
+                double* __restrict__ Pio, double* __restrict__ Yio, int stride,
+                int* __restrict__ nsteps_out, int* __restrict__ stats_out,
+                const int* __restrict__ order, unsigned int* __restrict__ counter,
+                double* __restrict__ jscratch, size_t slot_doubles) {
+  extern __shared__ double lds[];
+  constexpr int L = kWave * W;
+  using CV = Cvode<float, TeamBlock<W>>;
+  const int nsp = m.nsp;
+  CV cv;
+  char* p = reinterpret_cast<char*>(lds);
+  cv.Mlds = reinterpret_cast<float*>(p);
+  p += ((size_t)(nsp + 1) * CV::kJS * sizeof(float) + 15) & ~(size_t)15;
+  cv.prow = reinterpret_cast<float*>(p);
+  cv.rvec = cv.prow + L;
+  cv.t2l = reinterpret_cast<double*>(cv.rvec + L);
+  cv.hl = cv.t2l + L;
+  double* red = cv.hl + L;
+  cv.team.red = red;
+  double* rl = red + ((W + 2) & ~1);
+  cv.s = RhsLds{rl, rl + nsp, rl + 2 * nsp};
+  cv.savedJ = reinterpret_cast<float*>(jscratch + (size_t)blockIdx.x * slot_doubles);
+  bdf_driver(cv, m, o, n, dt, Tio, Pio, Yio, stride, nsteps_out, stats_out, order, counter);
 }
 
 __global__ void fill_u32_kernel(unsigned int* p, unsigned int v) {
@@ -250,11 +297,6 @@ hipError_t launch_rates(const DevMech& m, int n, const double* T, const double* 
 size_t rhs_lds_bytes(const DevMech& m) {
   return sizeof(double) * (size_t)(2 * m.nsp + m.nstep + 1);
 }
-size_t bdf_lds_bytes(const DevMech& m, bool f32) {
-  const size_t mt = f32 ? sizeof(float) : sizeof(double);
-  return mt * (size_t)(kNP * kNP + 2 * kNP) + sizeof(double) * kNP + rhs_lds_bytes(m);
-}
-
 hipError_t launch_jacobian(const DevMech& m, const BdfOpts& o, int n, const double* T,
                            const double* P, const double* Y, int stride, double* J,
                            double* jscratch, int nblocks, hipStream_t st) {
@@ -267,18 +309,72 @@ hipError_t launch_jacobian(const DevMech& m, const BdfOpts& o, int n, const doub
   return hipGetLastError();
 }
 
-int bdf_max_resident_waves(const DevMech& m, bool f32) {
+int bdf_team_width(int nsp) {
+  if (nsp <= kNP - 1) return 1;
+  if (nsp + 1 <= 2 * kWave) return 2;
+  if (nsp + 1 <= 3 * kWave) return 3;
+  return 0;
+}
+
+size_t bdf_slot_doubles(const DevMech& m) {
+  const int W = bdf_team_width(m.nsp);
+  if (W <= 1) return (size_t)kNP * kNP;
+  const size_t floats = (size_t)(m.nsp + 1) * (kWave * W + 1);
+  return ((floats + 1) / 2 + 31) & ~(size_t)31;  // 256-byte aligned slots
+}
+
+size_t bdf_lds_bytes(const DevMech& m, bool f32) {
+  const int W = bdf_team_width(m.nsp);
+  if (W <= 1) {
+    const size_t mt = f32 ? sizeof(float) : sizeof(double);
+    return mt * (size_t)(kNP * kNP + 2 * kNP) + sizeof(double) * kNP + rhs_lds_bytes(m);
+  }
+  const size_t L = (size_t)kWave * W;
+  const size_t mat = ((size_t)(m.nsp + 1) * (L + 1) * sizeof(float) + 15) & ~(size_t)15;
+  return mat + 2 * L * sizeof(float) + 2 * L * sizeof(double) +
+         (size_t)((W + 2) & ~1) * sizeof(double) + rhs_lds_bytes(m);
+}
+
+namespace {
+
+size_t max_lds_per_block() {
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    return 0;
+  return (size_t)v;
+}
+
+template <class K>
+int resident_blocks(K kernel, int threads, size_t lds) {
   int dev = 0, ncu = 0, per_cu = 0;
+  if (lds > max_lds_per_block()) return 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return 0;
-  const hipError_t e =
-      f32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bdf_kernel<float>, kWave,
-                                                         bdf_lds_bytes(m, true))
-          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bdf_kernel<double>, kWave,
-                                                         bdf_lds_bytes(m, false));
-  if (e != hipSuccess) return 0;
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds) != hipSuccess)
+    return 0;
   return ncu * (per_cu > 0 ? per_cu : 1);
+}
+
+}  // namespace
+
+int bdf_max_resident_teams(const DevMech& m, bool f32) {
+  const int W = bdf_team_width(m.nsp);
+  const size_t lds = bdf_lds_bytes(m, f32);
+  switch (W) {
+    case 1:
+      return f32 ? resident_blocks(bdf_kernel<float>, kWave, lds)
+                 : resident_blocks(bdf_kernel<double>, kWave, lds);
+    case 2:
+      return f32 ? resident_blocks(bdf_team_kernel<2>, 2 * kWave, lds) : 0;
+    case 3:
+      return f32 ? resident_blocks(bdf_team_kernel<3>, 3 * kWave, lds) : 0;
+    default:
+      return 0;
+  }
 }
 
 hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, double* T,
@@ -286,15 +382,27 @@ hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, doub
                       const int* order, unsigned int* counter, double* jscratch, int nwaves,
                       hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  if (m.nsp > kNP - 1) return hipErrorInvalidValue;
+  const int W = bdf_team_width(m.nsp);
+  const bool f32 = o.fp32_preconditioner != 0;
+  if (W == 0 || (W > 1 && !f32)) return hipErrorInvalidValue;
+  const size_t lds = bdf_lds_bytes(m, f32);
+  if (lds > max_lds_per_block()) return hipErrorInvalidValue;
   hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, counter, 0u);
   const int grid = nwaves < n ? nwaves : n;
-  if (o.fp32_preconditioner)
-    hipLaunchKernelGGL(bdf_kernel<float>, dim3(grid), dim3(kWave), bdf_lds_bytes(m, true), st, m,
-                       o, n, dt, T, P, Y, stride, nsteps, stats, order, counter, jscratch);
+  if (W == 1 && f32)
+    hipLaunchKernelGGL(bdf_kernel<float>, dim3(grid), dim3(kWave), lds, st, m, o, n, dt, T, P,
+                       Y, stride, nsteps, stats, order, counter, jscratch);
+  else if (W == 1)
+    hipLaunchKernelGGL(bdf_kernel<double>, dim3(grid), dim3(kWave), lds, st, m, o, n, dt, T, P,
+                       Y, stride, nsteps, stats, order, counter, jscratch);
+  else if (W == 2)
+    hipLaunchKernelGGL(bdf_team_kernel<2>, dim3(grid), dim3(2 * kWave), lds, st, m, o, n, dt, T,
+                       P, Y, stride, nsteps, stats, order, counter, jscratch,
+                       bdf_slot_doubles(m));
   else
-    hipLaunchKernelGGL(bdf_kernel<double>, dim3(grid), dim3(kWave), bdf_lds_bytes(m, false), st,
-                       m, o, n, dt, T, P, Y, stride, nsteps, stats, order, counter, jscratch);
+    hipLaunchKernelGGL(bdf_team_kernel<3>, dim3(grid), dim3(3 * kWave), lds, st, m, o, n, dt, T,
+                       P, Y, stride, nsteps, stats, order, counter, jscratch,
+                       bdf_slot_doubles(m));
   return hipGetLastError();
 }
 

@@ -130,13 +130,15 @@ int build_device_mech(MechHandle* h, std::string* err) {
     return g;
   };
   const std::vector<uint16_t> gd = pad_gather(dl, gdb), gc = pad_gather(cl, gcb);
-  // Jacobian species-block terms by row (only the nsp <= 63 kernels use them):
+  // Jacobian species-block terms by row (integrator teams of up to 3
+  // wavefronts, nsp <= 191; row stride = team size):
   // entry (row i, col c) = destruction terms then creation terms, each in step
   // order (the reference's order of accumulation, reactor_nvector_serial.cpp:149-176
   // and 655-669); a row's entries are sorted by column
   std::vector<uint32_t> jrt;
   int jrb = 0;
-  if (nsp <= 63) {
+  const int team = 64 * ((nsp + 1 + 63) / 64);
+  if (nsp <= 191) {
     std::vector<std::vector<int>> jd((size_t)nsp * nsp), jc((size_t)nsp * nsp);
     for (int s = 0; s < nstep; ++s)
       for (int k = M.reac_ptr[s]; k < M.reac_ptr[s + 1]; ++k) {
@@ -158,9 +160,10 @@ int build_device_mech(MechHandle* h, std::string* err) {
     }
     jrb = (int)((maxr + 3) / 4);
     const uint32_t pad = (uint32_t)nstep | (0xFFu << 16);
-    jrt.assign((size_t)jrb * 64 * 4, pad);
+    jrt.assign((size_t)jrb * team * 4, pad);
     for (int i = 0; i < nsp; ++i)
-      for (size_t k = 0; k < rows[i].size(); ++k) jrt[((k / 4) * 64 + i) * 4 + k % 4] = rows[i][k];
+      for (size_t k = 0; k < rows[i].size(); ++k)
+        jrt[((k / 4) * team + i) * 4 + k % 4] = rows[i][k];
   }
   if (jrt.empty()) jrt.assign(4, 0u);
 
@@ -212,18 +215,18 @@ int build_device_mech(MechHandle* h, std::string* err) {
 
 // Integrator workspace (per handle): saved Jacobians per resident wave slot
 // and the work counter.  Grown on demand, released with the handle.
-int ensure_workspace(MechHandle* h, int nslots) {
-  if (h->ws_slots >= nslots && h->ws_jac) return 0;
+int ensure_workspace(MechHandle* h, size_t doubles) {
+  if (h->ws_doubles >= doubles && h->ws_jac) return 0;
   if (h->ws_jac) (void)hipFree(h->ws_jac);
   if (h->ws_counter) (void)hipFree(h->ws_counter);
   h->ws_jac = nullptr;
   h->ws_counter = nullptr;
-  h->ws_slots = 0;
-  hipError_t e = hipMalloc((void**)&h->ws_jac, sizeof(double) * (size_t)nslots * kNP * kNP);
+  h->ws_doubles = 0;
+  hipError_t e = hipMalloc((void**)&h->ws_jac, sizeof(double) * doubles);
   if (e != hipSuccess) return (int)e;
   e = hipMalloc((void**)&h->ws_counter, 256);
   if (e != hipSuccess) return (int)e;
-  h->ws_slots = nslots;
+  h->ws_doubles = doubles;
   return 0;
 }
 
@@ -262,13 +265,16 @@ int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt, 
                      double* P, double* Y, int stride, int* nsteps, int* stats,
                      const int* order, hipStream_t st) {
   if (n < 0 || !T || !P || !Y || stride < h->host.nsp || !(dt > 0.0)) return -1;
-  if (h->host.nsp > kNP - 1) return -2;
+  const int W = bdf_team_width(h->host.nsp);
+  if (W == 0) return -2;                               // mechanism too large
+  if (W > 1 && !o.fp32_preconditioner) return -3;     // nsp > 63: FP32 preconditioner only
   if (n == 0) return 0;
   if (int rc = ensure_device(h)) return rc;
   const int f32 = o.fp32_preconditioner != 0;
-  if (h->resident_waves[f32] <= 0) h->resident_waves[f32] = bdf_max_resident_waves(h->dev, f32);
+  if (h->resident_waves[f32] <= 0) h->resident_waves[f32] = bdf_max_resident_teams(h->dev, f32);
+  if (h->resident_waves[f32] <= 0) return -2;         // does not fit in LDS
   const int nw = std::max(1, std::min(h->resident_waves[f32], n));
-  int rc = ensure_workspace(h, nw);
+  int rc = ensure_workspace(h, (size_t)nw * bdf_slot_doubles(h->dev));
   if (rc) return rc;
   hipError_t e = launch_bdf(h->dev, to_bdf_opts(o), n, dt, T, P, Y, stride, nsteps, stats,
                             order, h->ws_counter, h->ws_jac, nw, st);
@@ -391,7 +397,7 @@ int zrk_jacobian_device(zrk_mech m, const zrk_bdf_options* o, int n, const doubl
   if (h->host.nsp > zrk::kNP - 1) return -2;
   if (int rc = zrk::ensure_device(h)) return rc;
   const int nb = std::max(1, std::min(n, 4096));
-  int rc = zrk::ensure_workspace(h, nb);
+  int rc = zrk::ensure_workspace(h, (size_t)nb * zrk::kNP * zrk::kNP);
   if (rc) return rc;
   hipError_t e = zrk::launch_jacobian(h->dev, zrk::to_bdf_opts(*o), n, T, P, Y, stride, J,
                                       h->ws_jac, nb, (hipStream_t)stream);
@@ -412,8 +418,8 @@ int zrk_integrator_resident_waves(zrk_mech m, const zrk_bdf_options* o) {
   if (int rc = zrk::ensure_device(h)) return -rc;
   const int f32 = (o && o->fp32_preconditioner) ? 1 : 0;
   if (h->resident_waves[f32] <= 0)
-    h->resident_waves[f32] = zrk::bdf_max_resident_waves(h->dev, f32);
-  return h->resident_waves[f32];
+    h->resident_waves[f32] = zrk::bdf_max_resident_teams(h->dev, f32);
+  return h->resident_waves[f32] * zrk::bdf_team_width(h->host.nsp);
 }
 
 }  // extern "C"

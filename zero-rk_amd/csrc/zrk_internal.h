@@ -19,15 +19,15 @@ struct MechHandle {
   int device = 0;
   int resident_waves[2] = {0, 0};  // persistent grid size: FP64, FP32 preconditioner
   // integrator workspace
-  double* ws_jac = nullptr;
+  double* ws_jac = nullptr;            // saved-Jacobian slots, one per resident team
   unsigned int* ws_counter = nullptr;
-  int ws_slots = 0;
+  size_t ws_doubles = 0;
   ~MechHandle();
 };
 
 int build_device_mech(MechHandle* h, std::string* err);
 int ensure_device(MechHandle* h);
-int ensure_workspace(MechHandle* h, int nslots);
+int ensure_workspace(MechHandle* h, size_t doubles);
 int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt, double* T,
                      double* P, double* Y, int stride, int* nsteps, int* stats,
                      const int* order, hipStream_t st);
