@@ -5,13 +5,16 @@ Workload (one "step" = one CFD step of the plugin path): 65,536
 constant-pressure reactors of the seeded synthetic 53-species / 325-reaction
 mechanism (C, H, O, N, Ar; 25 % third-body, 15 % Troe fall-off), T bimodal
 (50 % U[700,950] K, 50 % U[1050,1400] K), P logU[1,50] atm, Y Dirichlet(0.5)
-over 8 majors, integrated over dt = 1e-4 s with the FP64 BDF at rtol 1e-6,
+over 8 majors, integrated over dt = 1e-4 s with the BDF at rtol 1e-6,
 atol 1e-12.  Inputs are resident in HBM; each step restores the initial
 state from a device copy and integrates the whole batch.
 
-Prints one JSON line (metric, value = reactors integrated per second, roofline
-of the dominant kernel, the rate kernel's HBM bandwidth and a CPU baseline of
-the oracle's CVODE restatement on a bounded sample of the same workload).
+Both methods are timed: the plain FP64 integrator and the chosen one (FP32
+Newton matrix and inverse; rates, residual, error test and state in FP64).
+Prints one JSON line: value = the chosen method's reactors per second, the
+plain method beside it, their end-state agreement, the roofline of the
+integrator kernel, the rate kernel's HBM bandwidth and a CPU baseline of the
+oracle's CVODE restatement on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -39,7 +42,22 @@ def parse():
     p.add_argument("--reactors", type=int, default=65536)
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--copy-bytes", type=int, default=0,
+                   help="only run a device-to-device copy of this many bytes (the "
+                        "calibration kernel for the FETCH_SIZE/WRITE_SIZE counters)")
     return p.parse_args()
+
+
+def copy_calibration(nbytes):
+    """One plain device copy of a known size, for scaling the HBM counters."""
+    import torch
+    n = nbytes // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 8}), flush=True)
 
 
 def pmc_traffic(path):
@@ -83,6 +101,9 @@ def log(msg):
 
 def main():
     a = parse()
+    if a.copy_bytes:
+        copy_calibration(a.copy_bytes)
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is described in DESIGN.md)"}))
@@ -101,65 +122,78 @@ def main():
     dt, rtol, atol = 1.0e-4, 1.0e-6, 1.0e-12
     nsp = m.nsp
     t = lambda x: torch.as_tensor(np.ascontiguousarray(x), dtype=torch.float64, device=dev)
-    T0, P, Y0 = t(st.T), t(st.P), t(st.Y)
-    T, Y = T0.clone(), Y0.clone()
+    T0, P0, Y0 = t(st.T), t(st.P), t(st.Y)
+    T, P, Y = T0.clone(), P0.clone(), Y0.clone()
     steps = torch.zeros(n, dtype=torch.int32, device=dev)
     stats = torch.zeros(n * 8, dtype=torch.int32, device=dev)
-    opts = capi.default_options(rel_tol=rtol, abs_tol=atol)
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
-
-    def one_step(with_stats=False):
-        T.copy_(T0)
-        Y.copy_(Y0)
-        m.integrate(n, dt, T, P, Y, nsp, opts, steps, stats if with_stats else None, stream=sp)
-
-    log(f"workload ready: {n} reactors, {nsp} species; warmup {a.warmup}")
-    for _ in range(a.warmup):
-        one_step()
-    torch.cuda.synchronize()
-    log("warmup done; timing")
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * a.steps + 2)]
-    ev[0].record(stream)
-    for k in range(a.steps):
-        T.copy_(T0)
-        Y.copy_(Y0)
-        ev[2 * k + 1].record(stream)
-        m.integrate(n, dt, T, P, Y, nsp, opts, steps, None, stream=sp)
-        ev[2 * k + 2].record(stream)
-    torch.cuda.synchronize()
-    total_ms = ev[0].elapsed_time(ev[2 * a.steps])
-    kern_ms = sum(ev[2 * k + 1].elapsed_time(ev[2 * k + 2]) for k in range(a.steps)) / a.steps
-    ms_per_step = total_ms / a.steps
-    value = n / (ms_per_step / 1e3)
-    nsteps = steps.cpu().numpy()
-    failed = int(np.sum(nsteps < 0))
-
-    log(f"timed {a.steps} steps: {ms_per_step:.2f} ms/step")
-    # counters of the same integration (deterministic per reactor) for the
-    # algorithmic FP64 operation count of the dominant kernel
-    one_step(with_stats=True)
-    torch.cuda.synchronize()
-    s = stats.cpu().numpy().reshape(n, 8).astype(np.int64)
     W = workcount.Work(m)
-    ops = W.bdf_ops(s[:, 1].sum(), s[:, 2].sum(), s[:, 3].sum(), s[:, 4].sum())
-    achieved = ops / (kern_ms / 1e3) / 1e12
+
+    def run(fp32):
+        """warmup + K timed steps of one method; returns its measurements."""
+        opts = capi.default_options(rel_tol=rtol, abs_tol=atol, fp32_preconditioner=fp32)
+
+        def restore():
+            T.copy_(T0)
+            P.copy_(P0)
+            Y.copy_(Y0)
+
+        for _ in range(a.warmup):
+            restore()
+            m.integrate(n, dt, T, P, Y, nsp, opts, steps, None, stream=sp)
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * a.steps + 2)]
+        ev[0].record(stream)
+        for k in range(a.steps):
+            restore()
+            ev[2 * k + 1].record(stream)
+            m.integrate(n, dt, T, P, Y, nsp, opts, steps, None, stream=sp)
+            ev[2 * k + 2].record(stream)
+        torch.cuda.synchronize()
+        total_ms = ev[0].elapsed_time(ev[2 * a.steps])
+        kern_ms = sum(ev[2 * k + 1].elapsed_time(ev[2 * k + 2]) for k in range(a.steps)) / a.steps
+        nst = steps.cpu().numpy()
+        Tend, Yend = T.cpu().numpy(), Y.cpu().numpy()
+        # counters of the same integration (deterministic per reactor) for the
+        # algorithmic FP64 operation count of the integrator kernel
+        restore()
+        m.integrate(n, dt, T, P, Y, nsp, opts, steps, stats, stream=sp)
+        torch.cuda.synchronize()
+        s = stats.cpu().numpy().reshape(n, 8).astype(np.int64)
+        ops = W.bdf_ops(s[:, 1].sum(), s[:, 2].sum(), s[:, 3].sum(), s[:, 4].sum())
+        return {"ms_per_step": total_ms / a.steps, "kern_ms": kern_ms, "ops": ops,
+                "failed": int(np.sum(nst < 0)), "mean_steps": float(nst[nst > 0].mean()),
+                "T": Tend, "Y": Yend, "waves": m.resident_waves(opts)}
+
+    log(f"workload ready: {n} reactors, {nsp} species; warmup {a.warmup}, steps {a.steps}")
+    plain = run(0)
+    log(f"plain FP64: {plain['ms_per_step']:.2f} ms/step")
+    chosen = run(1)
+    log(f"FP32 preconditioner: {chosen['ms_per_step']:.2f} ms/step")
+    # end-state agreement of the two methods, in tolerance units of the plain one
+    du = np.maximum(np.max(np.abs(chosen["Y"] - plain["Y"]) / (rtol * np.abs(plain["Y"]) + atol),
+                           axis=1), np.abs(chosen["T"] - plain["T"]) / (rtol * plain["T"]))
+    value = n / (chosen["ms_per_step"] / 1e3)
+    achieved = chosen["ops"] / (chosen["kern_ms"] / 1e3) / 1e12
     traffic = pmc_traffic(os.path.join(ROOT, "profiles", "bdf_kernel_53sp_65536_pmc.md"))
     roofline = {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
-                "traffic": traffic, "kernel": "bdf_kernel",
-                "kernel_ms": round(kern_ms, 3), "fp64_ops_per_launch": int(ops)}
+                "traffic": traffic, "kernel": "bdf_kernel<float>",
+                "kernel_ms": round(chosen["kern_ms"], 3),
+                "fp64_ops_per_launch": int(chosen["ops"]),
+                "resident_waves": chosen["waves"]}
 
     # standalone rate kernel on the same batch: HBM bytes of one RHS evaluation
     ydot = torch.empty(n * (nsp + 1), dtype=torch.float64, device=dev)
     rop = torch.empty(n * m.nstep, dtype=torch.float64, device=dev)
     for _ in range(2):
-        m.rates(n, T0, P, Y0, nsp, 1.0e22, ydot, rop, stream=sp)
+        m.rates(n, T0, P0, Y0, nsp, 1.0e22, ydot, rop, stream=sp)
     reps = 20
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(reps):
-        m.rates(n, T0, P, Y0, nsp, 1.0e22, ydot, rop, stream=sp)
+        m.rates(n, T0, P0, Y0, nsp, 1.0e22, ydot, rop, stream=sp)
     e1.record(stream)
     torch.cuda.synchronize()
     rate_ms = e0.elapsed_time(e1) / reps
@@ -169,24 +203,34 @@ def main():
             "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(rate_gbps / HBM_PEAK_GBPS, 4),
             "bytes_per_launch": int(rbytes), "kernel_ms": round(rate_ms, 4),
             "fp64_ops_per_launch": int(W.rhs_ops() * n)}
-
     log(f"rate kernel {rate_ms:.3f} ms, {rate_gbps:.0f} GB/s")
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(mech, therm, st, dt, rtol, atol, a.cpu_seconds)
-
-    if cpu:
         log(f"cpu baseline {cpu['value']:.1f} reactors/s on {cpu['cores']} threads")
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "reactors/s", "n_gpus": 1,
-        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 3),
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(chosen["ms_per_step"], 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded mechanism and states)",
-        "config": {"workload": "plain FP64 BDF, 53 species / 325 reactions x 65,536 "
-                               "constant-pressure reactors, dt 1e-4 s, rtol 1e-6, atol 1e-12",
+        "config": {"workload": "53 species / 325 reactions x 65,536 constant-pressure reactors, "
+                               "dt 1e-4 s, rtol 1e-6, atol 1e-12; chosen method = FP32 "
+                               "preconditioner (FP64 rates, residual, error test, state)",
                    "reactors": n, "species": nsp, "steps_per_reaction_set": m.nstep,
-                   "method": "plain_fp64", "failed_reactors": failed,
-                   "mean_bdf_steps": round(float(nsteps[nsteps > 0].mean()), 1)},
+                   "method": "fp32_preconditioner", "failed_reactors": chosen["failed"],
+                   "mean_bdf_steps": round(chosen["mean_steps"], 1)},
+        "plain_fp64": {"value": round(n / (plain["ms_per_step"] / 1e3), 1),
+                       "ms_per_step": round(plain["ms_per_step"], 3),
+                       "kernel_ms": round(plain["kern_ms"], 3),
+                       "fp64_ops_per_launch": int(plain["ops"]),
+                       "tflops": round(plain["ops"] / (plain["kern_ms"] / 1e3) / 1e12, 4),
+                       "failed_reactors": plain["failed"],
+                       "mean_bdf_steps": round(plain["mean_steps"], 1)},
+        "chosen_vs_plain": round(plain["ms_per_step"] / chosen["ms_per_step"], 3),
+        "chosen_vs_plain_end_state_tol_units": {"p50": round(float(np.percentile(du, 50)), 3),
+                                                "p99": round(float(np.percentile(du, 99)), 3),
+                                                "max": round(float(du.max()), 3)},
         "roofline": roofline,
         "rate_kernel": rate,
         "cpu_baseline": cpu,

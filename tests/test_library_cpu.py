@@ -20,7 +20,8 @@ def _declared(header):
     return sorted(set(re.findall(r"\b(zerork_reactor_\w+|zrk_\w+)\s*\(", txt)))
 
 
-@pytest.mark.parametrize("header", ["zerork_cfd_plugin.h", "zrk_device.h"])
+@pytest.mark.parametrize("header", sorted(h for h in os.listdir(os.path.join(ROOT, "include"))
+                                    if h.endswith(".h")))
 def test_exports(product_lib, header):
     names = _declared(header)
     assert names

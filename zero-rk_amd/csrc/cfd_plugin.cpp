@@ -262,7 +262,8 @@ zerork_status_t solve(zerork_handle h, double dt, int n, double* T, double* P, d
   // Newton matrix in FP32 only
   o.fp32_preconditioner = h->iopt["fp32_preconditioner"] != 0 || nsp > 63;
   const auto t0 = std::chrono::steady_clock::now();
-  int rc = zrk::integrate_device(m, o, n, dt, h->dT, h->dP, h->dY, stride, h->dsteps, nullptr,
+  int rc = zrk::integrate_device(m, o, n, dt, nullptr, h->dT, h->dP, h->dY, stride, h->dsteps,
+                                 nullptr,
                                  sorted ? h->dorder : nullptr, st);
   if (rc != 0) return ZERORK_STATUS_UNKNOWN_ERROR;
   std::vector<int> steps(n);

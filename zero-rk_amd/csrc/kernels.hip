@@ -126,7 +126,8 @@ jacobian_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ Tin,
 // ---------------------------------------------------------------------------
 template <typename MT, class Team>
 __device__ __forceinline__ void bdf_driver(Cvode<MT, Team>& cv, const DevMech& m,
-                                           const BdfOpts& o, int n, double dt,
+                                           const BdfOpts& o, int n, double dt_all,
+                                           const double* __restrict__ dts,
                                            double* __restrict__ Tio, double* __restrict__ Pio,
                                            double* __restrict__ Yio, int stride,
                                            int* __restrict__ nsteps_out,
@@ -147,6 +148,7 @@ __device__ __forceinline__ void bdf_driver(Cvode<MT, Team>& cv, const DevMech& m
     r = (unsigned int)cv.team.from((double)r, 0);
     if (r >= (unsigned int)n) break;
     if (order) r = (unsigned int)order[r];
+    const double dt = dts ? dts[r] : dt_all;
     double y0 = 0.0;
     if (lane < nsp) y0 = Yio[(size_t)r * stride + lane];
     if (lane == nsp) y0 = Tio[r] / o.Tref;
@@ -224,7 +226,8 @@ __device__ __forceinline__ void bdf_driver(Cvode<MT, Team>& cv, const DevMech& m
 // rvec[kNP] MT, t2l[kNP] double, then C[nsp], G[nsp], rop[nstep+1] doubles.
 template <typename MT>
 __global__ void __launch_bounds__(kWave)
-bdf_kernel(DevMech m, BdfOpts o, int n, double dt, double* __restrict__ Tio,
+bdf_kernel(DevMech m, BdfOpts o, int n, double dt, const double* __restrict__ dts,
+           double* __restrict__ Tio,
            double* __restrict__ Pio, double* __restrict__ Yio, int stride,
            int* __restrict__ nsteps_out, int* __restrict__ stats_out,
            const int* __restrict__ order, unsigned int* __restrict__ counter,
@@ -241,7 +244,7 @@ bdf_kernel(DevMech m, BdfOpts o, int n, double dt, double* __restrict__ Tio,
   double* rl = cv.t2l + kNP;
   cv.s = RhsLds{rl, rl + nsp, rl + 2 * nsp};
   cv.savedJ = reinterpret_cast<MT*>(jscratch + (size_t)blockIdx.x * kNP * kNP);
-  bdf_driver(cv, m, o, n, dt, Tio, Pio, Yio, stride, nsteps_out, stats_out, order, counter);
+  bdf_driver(cv, m, o, n, dt, dts, Tio, Pio, Yio, stride, nsteps_out, stats_out, order, counter);
 }
 
 // W wavefronts per reactor (64 <= nsp < 64W), FP32 preconditioner.  LDS
@@ -250,7 +253,8 @@ bdf_kernel(DevMech m, BdfOpts o, int n, double dt, double* __restrict__ Tio,
 // rop[nstep+1] doubles.  savedJ slot: slot_doubles doubles per workgroup.
 template <int W>
 __global__ void __launch_bounds__(kWave * W)
-bdf_team_kernel(DevMech m, BdfOpts o, int n, double dt, double* __restrict__ Tio,
+bdf_team_kernel(DevMech m, BdfOpts o, int n, double dt, const double* __restrict__ dts,
+                double* __restrict__ Tio,
                 double* __restrict__ Pio, double* __restrict__ Yio, int stride,
                 int* __restrict__ nsteps_out, int* __restrict__ stats_out,
                 const int* __restrict__ order, unsigned int* __restrict__ counter,
@@ -272,7 +276,7 @@ bdf_team_kernel(DevMech m, BdfOpts o, int n, double dt, double* __restrict__ Tio
   double* rl = red + ((W + 2) & ~1);
   cv.s = RhsLds{rl, rl + nsp, rl + 2 * nsp};
   cv.savedJ = reinterpret_cast<float*>(jscratch + (size_t)blockIdx.x * slot_doubles);
-  bdf_driver(cv, m, o, n, dt, Tio, Pio, Yio, stride, nsteps_out, stats_out, order, counter);
+  bdf_driver(cv, m, o, n, dt, dts, Tio, Pio, Yio, stride, nsteps_out, stats_out, order, counter);
 }
 
 __global__ void fill_u32_kernel(unsigned int* p, unsigned int v) {
@@ -377,8 +381,8 @@ int bdf_max_resident_teams(const DevMech& m, bool f32) {
   }
 }
 
-hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, double* T,
-                      double* P, double* Y, int stride, int* nsteps, int* stats,
+hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, const double* dts,
+                      double* T, double* P, double* Y, int stride, int* nsteps, int* stats,
                       const int* order, unsigned int* counter, double* jscratch, int nwaves,
                       hipStream_t st) {
   if (n <= 0) return hipSuccess;
@@ -390,17 +394,17 @@ hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, doub
   hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, counter, 0u);
   const int grid = nwaves < n ? nwaves : n;
   if (W == 1 && f32)
-    hipLaunchKernelGGL(bdf_kernel<float>, dim3(grid), dim3(kWave), lds, st, m, o, n, dt, T, P,
+    hipLaunchKernelGGL(bdf_kernel<float>, dim3(grid), dim3(kWave), lds, st, m, o, n, dt, dts, T, P,
                        Y, stride, nsteps, stats, order, counter, jscratch);
   else if (W == 1)
-    hipLaunchKernelGGL(bdf_kernel<double>, dim3(grid), dim3(kWave), lds, st, m, o, n, dt, T, P,
+    hipLaunchKernelGGL(bdf_kernel<double>, dim3(grid), dim3(kWave), lds, st, m, o, n, dt, dts, T, P,
                        Y, stride, nsteps, stats, order, counter, jscratch);
   else if (W == 2)
-    hipLaunchKernelGGL(bdf_team_kernel<2>, dim3(grid), dim3(2 * kWave), lds, st, m, o, n, dt, T,
+    hipLaunchKernelGGL(bdf_team_kernel<2>, dim3(grid), dim3(2 * kWave), lds, st, m, o, n, dt, dts, T,
                        P, Y, stride, nsteps, stats, order, counter, jscratch,
                        bdf_slot_doubles(m));
   else
-    hipLaunchKernelGGL(bdf_team_kernel<3>, dim3(grid), dim3(3 * kWave), lds, st, m, o, n, dt, T,
+    hipLaunchKernelGGL(bdf_team_kernel<3>, dim3(grid), dim3(3 * kWave), lds, st, m, o, n, dt, dts, T,
                        P, Y, stride, nsteps, stats, order, counter, jscratch,
                        bdf_slot_doubles(m));
   return hipGetLastError();

@@ -21,9 +21,34 @@ size_t bdf_slot_doubles(const DevMech& m);
 size_t bdf_lds_bytes(const DevMech& m, bool f32);
 // teams the persistent integrator keeps resident (0: does not fit)
 int bdf_max_resident_teams(const DevMech& m, bool f32);
-hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, double* T,
-                      double* P, double* Y, int stride, int* nsteps, int* stats,
+hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, const double* dts,
+                      double* T, double* P, double* Y, int stride, int* nsteps, int* stats,
                       const int* order, unsigned int* counter, double* jscratch, int nwaves,
                       hipStream_t st);
+
+// doubles of per-team LDS used by the RHS (C, G, rop with its zero slot)
+size_t rhs_lds_bytes(const DevMech& m);
+
+// learned surrogate (mlp.hip)
+hipError_t launch_mlp_features(int n, int nsp, const double* T, const double* P, const double* Y,
+                               int stride, const double* dt, double dt_all, const float* mu,
+                               const float* sig, unsigned short* X, int ldx, int nfeat,
+                               hipStream_t st);
+hipError_t launch_gemm_bf16(int M, int N, int K, const unsigned short* A, long sAm, long sAk,
+                            const unsigned short* B, long sBk, long sBn, float* C, long sCm,
+                            long sCn, int accumulate, hipStream_t st);
+hipError_t launch_bias_act(int M, int N, const float* C, const float* b, float* Z,
+                           unsigned short* Yb, float* Yf, int act, hipStream_t st);
+hipError_t launch_act_backward(int M, int N, const float* dY, const float* Z, float* dZ,
+                               unsigned short* dZb, int act, hipStream_t st);
+hipError_t launch_col_sum(int M, int N, const float* dZ, float* db, hipStream_t st);
+hipError_t launch_adam(long n, float* w, const float* g, float* m, float* v, unsigned short* wb,
+                       float lr, float b1, float b2, float eps, float c1, float c2,
+                       hipStream_t st);
+hipError_t launch_to_bf16(long n, const float* x, unsigned short* y, hipStream_t st);
+hipError_t launch_accept(const DevMech& m, const BdfOpts& o, int n, const double* dts,
+                         double dt_all, const double* T0, const double* P, const double* Y0,
+                         const double* T1, const double* Y1, int stride, int* accept,
+                         double* res, hipStream_t st);
 
 }  // namespace zrk

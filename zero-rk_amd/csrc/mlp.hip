@@ -1,0 +1,287 @@
+// Learned surrogate of one reactor step: a small MLP trained and evaluated
+// with BF16-input / FP32-accumulate MFMA kernels (v_mfma_f32_16x16x16_bf16),
+// and the FP64 acceptance test of its proposals.
+//
+//   features     [log T, log P, log max(Y_j, 1e-20) (nsp), log dt] -> normalised BF16
+//   gemm         C(f32) [+]= A(bf16) . B(bf16), any strides; one wavefront per
+//                16x16 tile of C, K walked 16 at a time through the MFMA
+//   bias_act     Z = C + b (f32), Y = gelu(Z) or Z (bf16 and f32)
+//   act_backward dZ = dY * gelu'(Z) (f32 and bf16)
+//   col_sum      db = sum over rows of dZ
+//   adam         FP32 master weights, BF16 copies
+//   accept       FP64 acceptance test of a proposed end state (one wavefront
+//                per reactor, the integrator's RHS)
+#include <hip/hip_runtime.h>
+
+#include "chem_device.h"
+#include "launch.h"
+
+namespace zrk {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short f32_to_bf16(float f) {
+  // round to nearest even (finite inputs; NaN stays NaN)
+  unsigned int u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)0x7fc0;  // NaN
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(unsigned short h) {
+  return __uint_as_float(((unsigned int)h) << 16);
+}
+
+// ---------------------------------------------------------------------------
+__global__ void mlp_features_kernel(int n, int nsp, const double* __restrict__ T,
+                                    const double* __restrict__ P, const double* __restrict__ Y,
+                                    int stride, const double* __restrict__ dt, double dt_all,
+                                    const float* __restrict__ mu, const float* __restrict__ sig,
+                                    unsigned short* __restrict__ X, int ldx, int nfeat) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)n * ldx) return;
+  const int r = (int)(i / ldx), f = (int)(i % ldx);
+  float v = 0.0f;
+  if (f < nfeat) {
+    double x;
+    if (f == 0) x = log(T[r]);
+    else if (f == 1) x = log(P[r]);
+    else if (f < 2 + nsp) x = log(fmax(Y[(size_t)r * stride + (f - 2)], 1.0e-20));
+    else x = log(dt ? dt[r] : dt_all);
+    v = (float)((x - (double)mu[f]) / (double)sig[f]);
+  }
+  X[i] = f32_to_bf16(v);
+}
+
+// C[M x N] (f32) = or += A[M x K] . B[K x N]; A, B bf16 with element strides.
+// MFMA 16x16x16 bf16 operand layout: lane l supplies A[row l%16][k 4(l/16)+j]
+// and B[k 4(l/16)+j][col l%16], j = 0..3, and receives D[row 4(l/16)+j][col l%16].
+__global__ void __launch_bounds__(256)
+gemm_bf16_kernel(int M, int N, int K, const unsigned short* __restrict__ A, long sAm, long sAk,
+                 const unsigned short* __restrict__ B, long sBk, long sBn, float* __restrict__ C,
+                 long sCm, long sCn, int accumulate) {
+  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = lane_id();
+  const int tilesN = (N + 15) / 16;
+  const long tm = wave / tilesN;
+  const int tn = (int)(wave % tilesN);
+  if (tm * 16 >= M) return;
+  const long ra = tm * 16 + (lane & 15);
+  const int cb = tn * 16 + (lane & 15);
+  const int kq = (lane >> 4) * 4;
+  f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    bf16x4 a, b;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = k0 + kq + j;
+      a[j] = (ra < M && k < K) ? (short)A[ra * sAm + (long)k * sAk] : (short)0;
+      b[j] = (cb < N && k < K) ? (short)B[(long)k * sBk + (long)cb * sBn] : (short)0;
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  }
+  const int col = tn * 16 + (lane & 15);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const long row = tm * 16 + (lane >> 4) * 4 + j;
+    if (row < M && col < N) {
+      float* c = C + row * sCm + (long)col * sCn;
+      *c = accumulate ? *c + acc[j] : acc[j];
+    }
+  }
+}
+
+__device__ __forceinline__ float gelu(float z) {
+  return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
+}
+__device__ __forceinline__ float gelu_grad(float z) {
+  const float cdf = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
+  const float pdf = 0.39894228040143268f * expf(-0.5f * z * z);
+  return cdf + z * pdf;
+}
+
+// Z = C + b; Y = act(Z) as bf16 (and f32 if Yf).  act: 1 gelu, 0 identity.
+__global__ void bias_act_kernel(int M, int N, const float* __restrict__ C, const float* __restrict__ b,
+                                float* __restrict__ Z, unsigned short* __restrict__ Yb,
+                                float* __restrict__ Yf, int act) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)M * N) return;
+  const float z = C[i] + b[i % N];
+  if (Z) Z[i] = z;
+  const float y = act ? gelu(z) : z;
+  if (Yb) Yb[i] = f32_to_bf16(y);
+  if (Yf) Yf[i] = y;
+}
+
+// dZ = dY * act'(Z) (f32 and bf16 copies)
+__global__ void act_backward_kernel(int M, int N, const float* __restrict__ dY,
+                                    const float* __restrict__ Z, float* __restrict__ dZ,
+                                    unsigned short* __restrict__ dZb, int act) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)M * N) return;
+  const float g = act ? dY[i] * gelu_grad(Z[i]) : dY[i];
+  dZ[i] = g;
+  dZb[i] = f32_to_bf16(g);
+}
+
+// db[j] = sum_i dZ[i*N + j]: one thread per column, rows in order
+__global__ void col_sum_kernel(int M, int N, const float* __restrict__ dZ, float* __restrict__ db) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  float s = 0.0f;
+  for (int i = 0; i < M; ++i) s += dZ[(long)i * N + j];
+  db[j] = s;
+}
+
+// Adam (bias-corrected) on FP32 master weights; refreshes the BF16 copy
+__global__ void adam_kernel(long n, float* __restrict__ w, const float* __restrict__ g,
+                            float* __restrict__ m, float* __restrict__ v,
+                            unsigned short* __restrict__ wb, float lr, float b1, float b2,
+                            float eps, float c1, float c2) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+  if (wb) wb[i] = f32_to_bf16(wi);
+}
+
+__global__ void to_bf16_kernel(long n, const float* __restrict__ x, unsigned short* __restrict__ y) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = f32_to_bf16(x[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Acceptance of a proposed end state y1 = (T1, Y1) for the step [0, dt] from
+// y0 (one wavefront per reactor, constant pressure).  FP64, in the
+// integrator's error weights w_j = 1 / (rtol |y1_j| + atol):
+//   implicit residual   r = y1 - y0 - dt/2 (f(y0) + f(y1))   (trapezoidal rule)
+//   error estimate      e = dt/2 (f(y1) - f(y0))             (trapezoid - backward Euler)
+// accept[r] = 1 when ||r||_wrms <= 1 and ||e||_wrms <= 1, else 0.
+// res[r*2] = ||r||, res[r*2+1] = ||e|| (optional).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kWave)
+accept_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ dts, double dt_all,
+              const double* __restrict__ T0, const double* __restrict__ P,
+              const double* __restrict__ Y0, const double* __restrict__ T1,
+              const double* __restrict__ Y1, int stride, int* __restrict__ accept,
+              double* __restrict__ res) {
+  extern __shared__ double lds[];
+  const int lane = lane_id();
+  const int nsp = m.nsp;
+  RhsLds s{lds, lds + nsp, lds + 2 * nsp};
+  const bool act = lane <= nsp;
+  for (int r = blockIdx.x; r < n; r += gridDim.x) {
+    const double dt = dts ? dts[r] : dt_all;
+    double y0 = 0.0, y1 = 0.0;
+    if (lane < nsp) {
+      y0 = Y0[(size_t)r * stride + lane];
+      y1 = Y1[(size_t)r * stride + lane];
+    }
+    if (lane == nsp) {
+      y0 = T0[r] / o.Tref;
+      y1 = T1[r] / o.Tref;
+    }
+    double f0, f1;
+    RhsAux aux;
+    const int bad0 = rhs64(m, s, P[r], 0.0, o.Tref, o.step_limiter, y0, f0, aux);
+    const int bad1 = rhs64(m, s, P[r], 0.0, o.Tref, o.step_limiter, y1, f1, aux);
+    const double w = act ? 1.0 / (o.rtol * fabs(y1) + o.atol) : 0.0;
+    const double rr = (y1 - y0 - 0.5 * dt * (f0 + f1)) * w;
+    const double ee = 0.5 * dt * (f1 - f0) * w;
+    const double inv = 1.0 / (double)(nsp + 1);
+    const double rn = sqrt(wave_sum(act ? rr * rr : 0.0) * inv);
+    const double en = sqrt(wave_sum(act ? ee * ee : 0.0) * inv);
+    if (lane == 0) {
+      const bool ok = !bad0 && !bad1 && rn <= 1.0 && en <= 1.0;
+      accept[r] = ok ? 1 : 0;
+      if (res) {
+        res[2 * (size_t)r] = rn;
+        res[2 * (size_t)r + 1] = en;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+namespace {
+inline unsigned blocks_for(long n, int b) { return (unsigned)((n + b - 1) / b); }
+}  // namespace
+
+hipError_t launch_mlp_features(int n, int nsp, const double* T, const double* P, const double* Y,
+                               int stride, const double* dt, double dt_all, const float* mu,
+                               const float* sig, unsigned short* X, int ldx, int nfeat,
+                               hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(mlp_features_kernel, dim3(blocks_for((long)n * ldx, 256)), dim3(256), 0, st,
+                     n, nsp, T, P, Y, stride, dt, dt_all, mu, sig, X, ldx, nfeat);
+  return hipGetLastError();
+}
+
+hipError_t launch_gemm_bf16(int M, int N, int K, const unsigned short* A, long sAm, long sAk,
+                            const unsigned short* B, long sBk, long sBn, float* C, long sCm,
+                            long sCn, int accumulate, hipStream_t st) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  const long tiles = (long)((M + 15) / 16) * ((N + 15) / 16);
+  hipLaunchKernelGGL(gemm_bf16_kernel, dim3(blocks_for(tiles * kWave, 256)), dim3(256), 0, st, M,
+                     N, K, A, sAm, sAk, B, sBk, sBn, C, sCm, sCn, accumulate);
+  return hipGetLastError();
+}
+
+hipError_t launch_bias_act(int M, int N, const float* C, const float* b, float* Z,
+                           unsigned short* Yb, float* Yf, int act, hipStream_t st) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(bias_act_kernel, dim3(blocks_for((long)M * N, 256)), dim3(256), 0, st, M, N,
+                     C, b, Z, Yb, Yf, act);
+  return hipGetLastError();
+}
+
+hipError_t launch_act_backward(int M, int N, const float* dY, const float* Z, float* dZ,
+                               unsigned short* dZb, int act, hipStream_t st) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(act_backward_kernel, dim3(blocks_for((long)M * N, 256)), dim3(256), 0, st, M,
+                     N, dY, Z, dZ, dZb, act);
+  return hipGetLastError();
+}
+
+hipError_t launch_col_sum(int M, int N, const float* dZ, float* db, hipStream_t st) {
+  if (N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(col_sum_kernel, dim3(blocks_for(N, 256)), dim3(256), 0, st, M, N, dZ, db);
+  return hipGetLastError();
+}
+
+hipError_t launch_adam(long n, float* w, const float* g, float* m, float* v, unsigned short* wb,
+                       float lr, float b1, float b2, float eps, float c1, float c2,
+                       hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, w, g, m, v, wb,
+                     lr, b1, b2, eps, c1, c2);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_bf16(long n, const float* x, unsigned short* y, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, x, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_accept(const DevMech& m, const BdfOpts& o, int n, const double* dts,
+                         double dt_all, const double* T0, const double* P, const double* Y0,
+                         const double* T1, const double* Y1, int stride, int* accept,
+                         double* res, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  if (m.nsp > kNP - 1) return hipErrorInvalidValue;
+  const int grid = n < 65536 ? n : 65536;
+  hipLaunchKernelGGL(accept_kernel, dim3(grid), dim3(kWave), rhs_lds_bytes(m), st, m, o, n, dts,
+                     dt_all, T0, P, Y0, T1, Y1, stride, accept, res);
+  return hipGetLastError();
+}
+
+}  // namespace zrk

@@ -261,10 +261,10 @@ int ensure_device(MechHandle* h) {
   return rc;
 }
 
-int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt, double* T,
-                     double* P, double* Y, int stride, int* nsteps, int* stats,
+int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
+                     const double* dts, double* T, double* P, double* Y, int stride, int* nsteps, int* stats,
                      const int* order, hipStream_t st) {
-  if (n < 0 || !T || !P || !Y || stride < h->host.nsp || !(dt > 0.0)) return -1;
+  if (n < 0 || !T || !P || !Y || stride < h->host.nsp || (!dts && !(dt > 0.0))) return -1;
   const int W = bdf_team_width(h->host.nsp);
   if (W == 0) return -2;                               // mechanism too large
   if (W > 1 && !o.fp32_preconditioner) return -3;     // nsp > 63: FP32 preconditioner only
@@ -276,7 +276,7 @@ int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt, 
   const int nw = std::max(1, std::min(h->resident_waves[f32], n));
   int rc = ensure_workspace(h, (size_t)nw * bdf_slot_doubles(h->dev));
   if (rc) return rc;
-  hipError_t e = launch_bdf(h->dev, to_bdf_opts(o), n, dt, T, P, Y, stride, nsteps, stats,
+  hipError_t e = launch_bdf(h->dev, to_bdf_opts(o), n, dt, dts, T, P, Y, stride, nsteps, stats,
                             order, h->ws_counter, h->ws_jac, nw, st);
   return e == hipSuccess ? 0 : (int)e;
 }
@@ -408,8 +408,8 @@ int zrk_integrate_device(zrk_mech m, const zrk_bdf_options* o, int n, double dt,
                          double* P, double* Y, int stride, int* nsteps, int* stats,
                          void* stream) {
   if (!m || !o) return -1;
-  return zrk::integrate_device(reinterpret_cast<MechHandle*>(m), *o, n, dt, T, P, Y, stride,
-                               nsteps, stats, nullptr, (hipStream_t)stream);
+  return zrk::integrate_device(reinterpret_cast<MechHandle*>(m), *o, n, dt, nullptr, T, P, Y,
+                               stride, nsteps, stats, nullptr, (hipStream_t)stream);
 }
 
 int zrk_integrator_resident_waves(zrk_mech m, const zrk_bdf_options* o) {

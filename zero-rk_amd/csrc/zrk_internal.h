@@ -28,8 +28,9 @@ struct MechHandle {
 int build_device_mech(MechHandle* h, std::string* err);
 int ensure_device(MechHandle* h);
 int ensure_workspace(MechHandle* h, size_t doubles);
-int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt, double* T,
-                     double* P, double* Y, int stride, int* nsteps, int* stats,
+BdfOpts to_bdf_opts(const zrk_bdf_options& o);
+int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
+                     const double* dts, double* T, double* P, double* Y, int stride, int* nsteps, int* stats,
                      const int* order, hipStream_t st);
 
 }  // namespace zrk
