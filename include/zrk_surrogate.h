@@ -61,15 +61,19 @@ ZRK_EXPORT int zrk_mlp_adam(long n, float* w, const float* g, float* m, float* v
 ZRK_EXPORT int zrk_to_bf16(long n, const float* x, void* y, void* stream);
 
 /* FP64 acceptance test of proposed end states (T1, Y1) of the constant-
- * pressure step [0, dt] from (T0, Y0) at pressure P: accept[r] = 1 when the
- * trapezoidal implicit residual y1 - y0 - dt/2 (f(y0) + f(y1)) and the error
- * estimate dt/2 (f(y1) - f(y0)) both have weighted RMS norm <= 1 in the
- * integrator's weights 1/(rel_tol |y1| + abs_tol).  res[2r], res[2r+1]
- * (optional) receive the two norms.  Requires nsp <= 63. */
+ * pressure step [0, dt] from (T0, Y0) at pressure P.  With newton > 0 the
+ * proposal is first refined in place by that many Newton iterations on the
+ * trapezoidal equation y - y0 - dt/2 (f(y0) + f(y)) = 0 (iteration matrix
+ * I - dt/2 J at the proposal).  accept[r] = 1 when the trapezoidal implicit
+ * residual y1 - y0 - dt/2 (f(y0) + f(y1)) and the error estimate
+ * dt/2 (f(y1) - f(y0)) both have weighted RMS norm <= 1 in the integrator's
+ * weights 1/(rel_tol |y1| + abs_tol).  res[2r], res[2r+1] (optional) receive
+ * the two norms.  Uses the handle's workspace: calls on one handle must not
+ * overlap.  Requires nsp <= 63. */
 ZRK_EXPORT int zrk_surrogate_accept(zrk_mech m, const zrk_bdf_options* o, int n,
                                     const double* dt, double dt_all, const double* T0,
-                                    const double* P, const double* Y0, const double* T1,
-                                    const double* Y1, int stride, int* accept, double* res,
+                                    const double* P, const double* Y0, double* T1, double* Y1,
+                                    int stride, int newton, int* accept, double* res,
                                     void* stream);
 
 #ifdef __cplusplus

@@ -48,7 +48,7 @@ hipError_t launch_adam(long n, float* w, const float* g, float* m, float* v, uns
 hipError_t launch_to_bf16(long n, const float* x, unsigned short* y, hipStream_t st);
 hipError_t launch_accept(const DevMech& m, const BdfOpts& o, int n, const double* dts,
                          double dt_all, const double* T0, const double* P, const double* Y0,
-                         const double* T1, const double* Y1, int stride, int* accept,
-                         double* res, hipStream_t st);
+                         double* T1, double* Y1, int stride, int newton, int* accept,
+                         double* res, double* jscratch, int nblocks, hipStream_t st);
 
 }  // namespace zrk

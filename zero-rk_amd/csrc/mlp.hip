@@ -13,7 +13,7 @@
 //                per reactor, the integrator's RHS)
 #include <hip/hip_runtime.h>
 
-#include "chem_device.h"
+#include "bdf_device.h"
 #include "launch.h"
 
 namespace zrk {
@@ -157,26 +157,39 @@ __global__ void to_bf16_kernel(long n, const float* __restrict__ x, unsigned sho
 
 // ---------------------------------------------------------------------------
 // Acceptance of a proposed end state y1 = (T1, Y1) for the step [0, dt] from
-// y0 (one wavefront per reactor, constant pressure).  FP64, in the
-// integrator's error weights w_j = 1 / (rtol |y1_j| + atol):
-//   implicit residual   r = y1 - y0 - dt/2 (f(y0) + f(y1))   (trapezoidal rule)
-//   error estimate      e = dt/2 (f(y1) - f(y0))             (trapezoid - backward Euler)
-// accept[r] = 1 when ||r||_wrms <= 1 and ||e||_wrms <= 1, else 0.
-// res[r*2] = ||r||, res[r*2+1] = ||e|| (optional).
+// y0 (one wavefront per reactor, constant pressure), all in FP64.
+// With newton > 0 the proposal is first refined by that many Newton
+// iterations on the trapezoidal equation G(y) = y - y0 - dt/2 (f(y0) + f(y))
+// = 0, with the iteration matrix I - dt/2 J built once at the proposal (the
+// reference Jacobian, Gauss-Jordan inverse), and the refined state is written
+// back to T1/Y1.  Then, in the integrator's weights w = 1/(rtol |y1| + atol):
+//   implicit residual   r = y1 - y0 - dt/2 (f(y0) + f(y1))
+//   error estimate      e = dt/2 (f(y1) - f(y0))       (trapezoid - backward Euler)
+// accept[r] = 1 when ||r||_wrms <= 1 and ||e||_wrms <= 1, else 0;
+// res[2r], res[2r+1] (optional) = the two norms.
+// LDS: Ml[kNP*kNP], prow[kNP], rvec[kNP], t2l[kNP], then C, G, rop.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kWave)
 accept_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ dts, double dt_all,
               const double* __restrict__ T0, const double* __restrict__ P,
-              const double* __restrict__ Y0, const double* __restrict__ T1,
-              const double* __restrict__ Y1, int stride, int* __restrict__ accept,
-              double* __restrict__ res) {
+              const double* __restrict__ Y0, double* __restrict__ T1, double* __restrict__ Y1,
+              int stride, int newton, int* __restrict__ accept, double* __restrict__ res,
+              double* __restrict__ jscratch) {
   extern __shared__ double lds[];
   const int lane = lane_id();
   const int nsp = m.nsp;
-  RhsLds s{lds, lds + nsp, lds + 2 * nsp};
+  const int N = nsp + 1;
+  double* Ml = lds;
+  double* prow = Ml + kNP * kNP;
+  double* rvec = prow + kNP;
+  double* t2l = rvec + kNP;
+  double* rl = t2l + kNP;
+  RhsLds s{rl, rl + nsp, rl + 2 * nsp};
+  double* saved = jscratch + (size_t)blockIdx.x * kNP * kNP;
   const bool act = lane <= nsp;
   for (int r = blockIdx.x; r < n; r += gridDim.x) {
     const double dt = dts ? dts[r] : dt_all;
+    const double Pr = P[r];
     double y0 = 0.0, y1 = 0.0;
     if (lane < nsp) {
       y0 = Y0[(size_t)r * stride + lane];
@@ -188,20 +201,59 @@ accept_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ dts, doubl
     }
     double f0, f1;
     RhsAux aux;
-    const int bad0 = rhs64(m, s, P[r], 0.0, o.Tref, o.step_limiter, y0, f0, aux);
-    const int bad1 = rhs64(m, s, P[r], 0.0, o.Tref, o.step_limiter, y1, f1, aux);
+    int bad = rhs64(m, s, Pr, 0.0, o.Tref, o.step_limiter, y0, f0, aux);
+    if (newton > 0 && !bad) {
+      double fy;
+      bad = rhs64(m, s, Pr, 0.0, o.Tref, o.step_limiter, y1, fy, aux);
+      if (!bad) {
+        jacobian_lds<double>(m, s, Pr, 0.0, o, y1, fy, aux.inv_rho, Ml, saved, t2l);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double R[kNP];
+        const double hg = -0.5 * dt;
+#pragma unroll
+        for (int c = 0; c < kNP; ++c) {
+          double v = (c <= nsp && lane <= nsp) ? hg * Ml[c * kNP + lane] : 0.0;
+          if (c == lane) v += 1.0;
+          R[c] = v;
+        }
+        const bool ok = gj_invert<double>(R, N, prow);
+#pragma unroll
+        for (int c = 0; c < kNP; ++c) Ml[c * kNP + lane] = R[c];
+        if (!ok) bad = 1;
+      }
+      for (int it = 0; it < newton && !bad; ++it) {
+        if (it > 0) bad = rhs64(m, s, Pr, 0.0, o.Tref, o.step_limiter, y1, fy, aux);
+        if (bad) break;
+        const double G = act ? y1 - y0 - 0.5 * dt * (f0 + fy) : 0.0;
+        rvec[lane] = G;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double d = 0.0;
+#pragma unroll 8
+        for (int c = 0; c < kNP; ++c) d = fma(Ml[c * kNP + lane], rvec[(c + N) & (kNP - 1)], d);
+        __builtin_amdgcn_wave_barrier();
+        if (act) y1 -= d;
+      }
+    }
+    if (!bad) bad = rhs64(m, s, Pr, 0.0, o.Tref, o.step_limiter, y1, f1, aux);
     const double w = act ? 1.0 / (o.rtol * fabs(y1) + o.atol) : 0.0;
-    const double rr = (y1 - y0 - 0.5 * dt * (f0 + f1)) * w;
-    const double ee = 0.5 * dt * (f1 - f0) * w;
-    const double inv = 1.0 / (double)(nsp + 1);
+    const double rr = bad ? 0.0 : (y1 - y0 - 0.5 * dt * (f0 + f1)) * w;
+    const double ee = bad ? 0.0 : 0.5 * dt * (f1 - f0) * w;
+    const double inv = 1.0 / (double)N;
     const double rn = sqrt(wave_sum(act ? rr * rr : 0.0) * inv);
     const double en = sqrt(wave_sum(act ? ee * ee : 0.0) * inv);
+    if (newton > 0) {
+      if (lane < nsp) Y1[(size_t)r * stride + lane] = y1;
+      if (lane == nsp) T1[r] = y1 * o.Tref;
+    }
     if (lane == 0) {
-      const bool ok = !bad0 && !bad1 && rn <= 1.0 && en <= 1.0;
-      accept[r] = ok ? 1 : 0;
+      accept[r] = (!bad && rn <= 1.0 && en <= 1.0) ? 1 : 0;
       if (res) {
-        res[2 * (size_t)r] = rn;
-        res[2 * (size_t)r + 1] = en;
+        res[2 * (size_t)r] = bad ? INFINITY : rn;
+        res[2 * (size_t)r + 1] = bad ? INFINITY : en;
       }
     }
     __builtin_amdgcn_wave_barrier();
@@ -274,13 +326,14 @@ hipError_t launch_to_bf16(long n, const float* x, unsigned short* y, hipStream_t
 
 hipError_t launch_accept(const DevMech& m, const BdfOpts& o, int n, const double* dts,
                          double dt_all, const double* T0, const double* P, const double* Y0,
-                         const double* T1, const double* Y1, int stride, int* accept,
-                         double* res, hipStream_t st) {
+                         double* T1, double* Y1, int stride, int newton, int* accept,
+                         double* res, double* jscratch, int nblocks, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   if (m.nsp > kNP - 1) return hipErrorInvalidValue;
-  const int grid = n < 65536 ? n : 65536;
-  hipLaunchKernelGGL(accept_kernel, dim3(grid), dim3(kWave), rhs_lds_bytes(m), st, m, o, n, dts,
-                     dt_all, T0, P, Y0, T1, Y1, stride, accept, res);
+  const int grid = n < nblocks ? n : nblocks;
+  const size_t lds = sizeof(double) * (size_t)(kNP * kNP + 3 * kNP) + rhs_lds_bytes(m);
+  hipLaunchKernelGGL(accept_kernel, dim3(grid), dim3(kWave), lds, st, m, o, n, dts, dt_all, T0,
+                     P, Y0, T1, Y1, stride, newton, accept, res, jscratch);
   return hipGetLastError();
 }
 

@@ -369,7 +369,10 @@ def workload_mechanism(name: str, seed: int | None = None) -> Mechanism:
                         ("C", "H", "O", "N", "AR"), 0.25, 0.15, inert=("N2", "AR"),
                         majors=_C2_BASE, base=_C2_BASE)
     if name == "160sp":
-        return generate(160, 900, seed if seed is not None else 1003,
+        # seed 1089: chosen by a scan of seeds 1000-1120 (64-256 reactors each) as
+        # a mechanism with no failed or endothermically collapsing reactor over
+        # dt and a few percent of reactors igniting
+        return generate(160, 900, seed if seed is not None else 1089,
                         ("C", "H", "O", "N", "AR"), 0.30, 0.10, inert=("N2", "AR"),
                         majors=_C2_BASE + ["C2H6", "CH3OH"], base=_C2_BASE)
     raise ValueError(f"unknown workload {name}")

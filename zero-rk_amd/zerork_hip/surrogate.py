@@ -35,7 +35,7 @@ def _lib():
         L.zrk_mlp_adam.argtypes = [Lg, P, P, P, P, P, F, F, F, F, I, P]
         L.zrk_to_bf16.argtypes = [Lg, P, P, P]
         L.zrk_surrogate_accept.argtypes = [P, C.POINTER(capi.BdfOptions), I, P, D, P, P, P, P, P,
-                                           I, P, P, P]
+                                           I, I, P, P, P]
         for f in ("zrk_integrate_device_select", "zrk_mlp_features", "zrk_gemm_bf16",
                   "zrk_mlp_bias_act", "zrk_mlp_act_backward", "zrk_mlp_col_sum", "zrk_mlp_adam",
                   "zrk_to_bf16", "zrk_surrogate_accept"):
@@ -232,18 +232,19 @@ class Surrogate:
         T1 = T + o[:, self.nsp]
         return T1.contiguous(), Y1.contiguous()
 
-    def accept(self, T0, P, Y0, T1, Y1, dt, opts):
+    def accept(self, T0, P, Y0, T1, Y1, dt, opts, newton=0):
+        """FP64 acceptance (T1/Y1 refined in place when newton > 0)."""
         t = self.torch
         n = T0.numel()
         acc = t.empty(n, dtype=t.int32, device=self.device)
         res = t.empty(2 * n, dtype=t.float64, device=self.device)
         import ctypes as C
         _chk(_lib().zrk_surrogate_accept(self.mech.h, C.byref(opts), n, _p(dt), 0.0, _p(T0), _p(P), _p(Y0),
-                                         _p(T1), _p(Y1), self.nsp, _p(acc), _p(res), 0),
+                                         _p(T1), _p(Y1), self.nsp, newton, _p(acc), _p(res), 0),
              "zrk_surrogate_accept")
         return acc, res.view(n, 2)
 
-    def solve(self, T, P, Y, dt, opts, proposal="mlp"):
+    def solve(self, T, P, Y, dt, opts, proposal="mlp", newton=2):
         """Surrogate-accelerated step, in place on T and Y.  proposal: "mlp" or
         "hold" (y1 = y0, the same test without the network).  Returns the
         accepted mask (device int32) and the BDF step counts of the rest."""
@@ -252,7 +253,7 @@ class Surrogate:
             T1, Y1 = self.propose(T, P, Y, dt)
         else:
             T1, Y1 = T.clone(), Y.clone()
-        acc, _ = self.accept(T, P, Y, T1, Y1, dt, opts)
+        acc, _ = self.accept(T, P, Y, T1, Y1, dt, opts, newton)
         rej = t.nonzero(acc == 0).flatten().to(t.int32).contiguous()
         keep = acc.bool()
         T[keep] = T1[keep]
