@@ -92,7 +92,8 @@ ZRK_EXPORT int zrk_rates_device(zrk_mech m, int n, const double* T, const double
                                 void* stream);
 
 /* Reference analytic Jacobian J[r][row][col] (row-major, N = nsp+1, temperature
- * last) at the states.  Requires nsp <= 63. */
+ * last) at the states, in FP64.  nsp <= 191 (above 63 species computed by
+ * teams of 2-3 wavefronts, as in the integrator). */
 ZRK_EXPORT int zrk_jacobian_device(zrk_mech m, const zrk_bdf_options* o, int n, const double* T,
                                    const double* P, const double* Y, int stride, double* J,
                                    void* stream);
@@ -100,8 +101,9 @@ ZRK_EXPORT int zrk_jacobian_device(zrk_mech m, const zrk_bdf_options* o, int n, 
 /* Integrate n reactors over [0, dt]; T and Y updated in place, and P at
  * constant volume (P = Ru T / (v Wmix) of the end state).  nsteps[r] (optional) = CVode one-step calls, negative on failure
  * (the plugin's COST output); stats[r*8] (optional) = nst nfe nje nsetups nni
- * ncfn netf status.  Requires nsp <= 63.  Returns 0 or an error code; the
- * number of failed reactors is available from nsteps. */
+ * ncfn netf status.  nsp <= 63, or nsp <= 191 with fp32_preconditioner = 1
+ * (-3 otherwise); -2 if the mechanism does not fit.  Returns 0 or an error
+ * code; the number of failed reactors is available from nsteps. */
 ZRK_EXPORT int zrk_integrate_device(zrk_mech m, const zrk_bdf_options* o, int n, double dt,
                                     double* T, double* P, double* Y, int stride,
                                     int* nsteps, int* stats, void* stream);

@@ -15,7 +15,8 @@ from conftest import load_golden, mech_paths
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", ["synth_10sp", "synth_53sp", "ref_h2", "ref_h2_no_falloff"])
+@pytest.mark.parametrize("name", ["synth_10sp", "synth_53sp", "ref_h2", "ref_h2_no_falloff",
+                                  "ref_dme", "synth_160sp"])
 def test_jacobian_matches_oracle(torch_gpu, oracle_mod, name):
     from zerork_hip import capi
     torch = torch_gpu

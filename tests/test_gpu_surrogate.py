@@ -151,3 +151,75 @@ def test_accept_and_subset_integration(torch_gpu, oracle_mod):
         assert abs(res[i, 0] - r) <= 1e-6 * r + 1e-9
         assert abs(res[i, 1] - e) <= 1e-6 * e + 1e-9
         assert acc[i].item() == int(r <= 1.0 and e <= 1.0)
+
+
+def _train_surrogate(torch, capi, surrogate, mechgen, m, n_train, seed, epochs, log=print):
+    """Training set from the GPU integrator on the 53-species workload states
+    with dt ~ logU[1e-6, 1e-4]; returns a trained Surrogate."""
+    mg = mechgen.workload_mechanism("53sp")
+    st = mechgen.workload_states("53sp", mg, n_train, seed=seed)
+    rng = np.random.default_rng(seed)
+    dt = np.exp(rng.uniform(np.log(1e-6), np.log(1e-4), n_train))
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda:0")
+    T, P, Y, D = t(st.T), t(st.P), t(st.Y), t(dt)
+    opts = capi.default_options(rel_tol=1e-6, abs_tol=1e-12)
+    sel = torch.arange(n_train, dtype=torch.int32, device="cuda:0")
+    surrogate._chk(surrogate._lib().zrk_integrate_device_select(
+        m.h, __import__("ctypes").byref(opts), n_train, sel.data_ptr(), D.data_ptr(),
+        T.data_ptr(), P.data_ptr(), Y.data_ptr(), m.nsp, None, None, 0), "select")
+    torch.cuda.synchronize()
+    dY = Y.cpu().numpy() - st.Y
+    dT = T.cpu().numpy() - st.T
+    s = surrogate.Surrogate(torch, m, hidden=256, layers=3, seed=seed)
+    s.fit_normalisation(st.T, st.P, st.Y, dt, dY, dT)
+    X = s.features(t(st.T), t(st.P), t(st.Y), D)
+    tgt = np.zeros((n_train, s.kout), np.float32)
+    tgt[:, :s.nout] = (np.column_stack([dY, dT]) - s.tmu) / s.tsig
+    losses = s.train(X, torch.as_tensor(tgt).cuda(), epochs=epochs, batch=2048, lr=1e-3, log=log)
+    return s, losses
+
+
+def test_surrogate_step_53sp(torch_gpu):
+    """Surrogate-accelerated step on the 53-species workload: trained on the
+    GPU integrator's own results, proposals refined by two FP64 Newton
+    iterations and accepted only through the FP64 residual/error test, the
+    rest integrated by the plain BDF.  Accepted end states must agree with
+    the plain BDF's to the integrator's own scale of differences; the
+    acceptance rate with the network and with the network-free proposal
+    (y1 = y0, same test) are reported."""
+    from zerork_hip import capi, mechgen, surrogate
+    torch = torch_gpu
+    m = capi.Mech(*mech_paths("synth_53sp"))
+    print("case surrogate 53sp", flush=True)
+    s, losses = _train_surrogate(torch, capi, surrogate, mechgen, m, 32768, 11, epochs=6)
+    assert losses[-1] < losses[0]
+    n = 8192
+    st = mechgen.workload_states("53sp", mechgen.workload_mechanism("53sp"), n, seed=12)
+    rng = np.random.default_rng(12)
+    dt = np.exp(rng.uniform(np.log(1e-6), np.log(1e-4), n))
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda:0")
+    opts = capi.default_options(rel_tol=1e-6, abs_tol=1e-12)
+    # plain BDF on every reactor
+    Tp, Pp, Yp = t(st.T), t(st.P), t(st.Y)
+    sel = torch.arange(n, dtype=torch.int32, device="cuda:0")
+    surrogate._chk(surrogate._lib().zrk_integrate_device_select(
+        m.h, __import__("ctypes").byref(opts), n, sel.data_ptr(), t(dt).data_ptr(), Tp.data_ptr(),
+        Pp.data_ptr(), Yp.data_ptr(), m.nsp, None, None, 0), "select")
+    torch.cuda.synchronize()
+    out = {}
+    for prop in ("mlp", "hold"):
+        T, P, Y = t(st.T), t(st.P), t(st.Y)
+        acc, steps = s.solve(T, P, Y, t(dt), opts, proposal=prop, newton=2)
+        torch.cuda.synchronize()
+        a = acc.cpu().numpy().astype(bool)
+        Tg, Yg, Tr, Yr = T.cpu().numpy(), Y.cpu().numpy(), Tp.cpu().numpy(), Yp.cpu().numpy()
+        units = np.maximum(np.max(np.abs(Yg - Yr) / (1e-6 * np.abs(Yr) + 1e-12), axis=1),
+                           np.abs(Tg - Tr) / (1e-6 * Tr))
+        out[prop] = (a.mean(), units[a])
+        print(f"{prop}: accepted {100 * a.mean():.1f} %; accepted vs plain BDF tol units "
+              f"p50/p99/max {np.percentile(units[a], [50, 99, 100]).round(2) if a.any() else '-'}",
+              flush=True)
+        # rejected reactors went through the plain BDF: identical results
+        np.testing.assert_array_equal(Yg[~a], Yr[~a])
+        if a.any():
+            assert np.percentile(units[a], 99) <= 100.0

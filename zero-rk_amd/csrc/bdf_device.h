@@ -136,7 +136,8 @@ __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s
 // ---------------------------------------------------------------------------
 // The same Jacobian for a team of several wavefronts (nsp < 64W): thread i
 // owns row i; J and saved are column-major with column stride js; t2l and hl
-// are kThreads doubles of LDS scratch.  The temperature row is a
+// are kThreads doubles of LDS scratch.  Only rows <= nsp are written (rows
+// beyond are left as they are), so js >= nsp + 1 suffices.  The temperature row is a
 // matrix-vector product read from LDS (column c by thread c, conflict-free for
 // an odd js) instead of one reduction per column.
 // ---------------------------------------------------------------------------
@@ -170,7 +171,8 @@ __device__ __forceinline__ double jacobian_team(const Team& team, const DevMech&
   rhs_team(team, m, s, P, v0, o.Tref, o.step_limiter, t1, t3, aux);
   const double Tdot = team.from(t3, nsp);
   hl[tid] = sp ? aux.h : 0.0;
-  for (int c = 0; c < nsp; ++c) J[c * js + tid] = MT(0);
+  if (tid < js)
+    for (int c = 0; c < nsp; ++c) J[c * js + tid] = MT(0);
   team.sync();
   {
     const uint4* __restrict__ jr = m.jr + tid;
@@ -222,8 +224,10 @@ __device__ __forceinline__ double jacobian_team(const Team& team, const DevMech&
     J[tid * js + nsp] = (MT)trow;
     saved[tid * js + nsp] = (MT)trow;
   }
-  J[nsp * js + tid] = (MT)colT;
-  saved[nsp * js + tid] = (MT)colT;
+  if (act) {
+    J[nsp * js + tid] = (MT)colT;
+    saved[nsp * js + tid] = (MT)colT;
+  }
   team.sync();
   return aux2.inv_rho;
 }

@@ -117,6 +117,42 @@ jacobian_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ Tin,
   }
 }
 
+// jacobian_team_kernel (64 <= nsp < 64W): the same Jacobian computed by a team
+// of W wavefronts (the integrator's team code, FP64), through a global
+// scratch J (column stride N) per workgroup.  LDS: t2l[64W], hl[64W],
+// red[W+1] doubles, then C, G, rop.
+template <int W>
+__global__ void __launch_bounds__(kWave * W)
+jacobian_team_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ Tin,
+                     const double* __restrict__ Pin, const double* __restrict__ Yin, int stride,
+                     double* __restrict__ Jout, double* __restrict__ jscratch) {
+  extern __shared__ double lds[];
+  constexpr int L = kWave * W;
+  const int nsp = m.nsp, N = nsp + 1;
+  TeamBlock<W> team;
+  double* t2l = lds;
+  double* hl = t2l + L;
+  team.red = hl + L;
+  double* rl = team.red + ((W + 2) & ~1);
+  RhsLds s{rl, rl + nsp, rl + 2 * nsp};
+  double* J = jscratch + (size_t)blockIdx.x * 2 * N * N;
+  double* saved = J + (size_t)N * N;
+  const int tid = team.tid();
+  for (int r = blockIdx.x; r < n; r += gridDim.x) {
+    double y = 0.0;
+    if (tid < nsp) y = Yin[(size_t)r * stride + tid];
+    if (tid == nsp) y = Tin[r] / o.Tref;
+    const double P = Pin[r];
+    double fy;
+    RhsAux aux;
+    rhs_team(team, m, s, P, 0.0, o.Tref, o.step_limiter, y, fy, aux);
+    jacobian_team<double>(team, m, s, P, 0.0, o, y, fy, aux.inv_rho, J, saved, N, t2l, hl);
+    if (tid < N)
+      for (int c = 0; c < N; ++c) Jout[(size_t)r * N * N + (size_t)tid * N + c] = J[c * N + tid];
+    team.sync();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Persistent per-reactor BDF: each team (one wavefront, or W wavefronts of a
 // workgroup) takes reactors from *counter until the batch is exhausted.
@@ -305,8 +341,20 @@ hipError_t launch_jacobian(const DevMech& m, const BdfOpts& o, int n, const doub
                            const double* P, const double* Y, int stride, double* J,
                            double* jscratch, int nblocks, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  if (m.nsp > kNP - 1) return hipErrorInvalidValue;
   const int grid = nblocks < n ? nblocks : n;
+  const int W = bdf_team_width(m.nsp);
+  if (W == 0) return hipErrorInvalidValue;
+  if (W > 1) {
+    const size_t L = (size_t)kWave * W;
+    const size_t lds = sizeof(double) * (2 * L + ((W + 2) & ~1)) + rhs_lds_bytes(m);
+    if (W == 2)
+      hipLaunchKernelGGL(jacobian_team_kernel<2>, dim3(grid), dim3(2 * kWave), lds, st, m, o, n,
+                         T, P, Y, stride, J, jscratch);
+    else
+      hipLaunchKernelGGL(jacobian_team_kernel<3>, dim3(grid), dim3(3 * kWave), lds, st, m, o, n,
+                         T, P, Y, stride, J, jscratch);
+    return hipGetLastError();
+  }
   const size_t lds = sizeof(double) * (size_t)(kNP * kNP + kNP) + rhs_lds_bytes(m);
   hipLaunchKernelGGL(jacobian_kernel, dim3(grid), dim3(kWave), lds, st, m, o, n, T, P, Y,
                      stride, J, jscratch);

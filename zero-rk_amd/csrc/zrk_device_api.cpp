@@ -394,10 +394,12 @@ int zrk_jacobian_device(zrk_mech m, const zrk_bdf_options* o, int n, const doubl
   if (!m || !o || n < 0 || !T || !P || !Y || !J) return -1;
   auto* h = reinterpret_cast<MechHandle*>(m);
   if (stride < h->host.nsp) return -1;
-  if (h->host.nsp > zrk::kNP - 1) return -2;
+  if (zrk::bdf_team_width(h->host.nsp) == 0) return -2;
   if (int rc = zrk::ensure_device(h)) return rc;
-  const int nb = std::max(1, std::min(n, 4096));
-  int rc = zrk::ensure_workspace(h, (size_t)nb * zrk::kNP * zrk::kNP);
+  const size_t N = (size_t)h->host.nsp + 1;
+  const int nb = std::max(1, std::min(n, N > (size_t)zrk::kNP ? 512 : 4096));
+  const size_t per = std::max((size_t)zrk::kNP * zrk::kNP, 2 * N * N);
+  int rc = zrk::ensure_workspace(h, (size_t)nb * per);
   if (rc) return rc;
   hipError_t e = zrk::launch_jacobian(h->dev, zrk::to_bdf_opts(*o), n, T, P, Y, stride, J,
                                       h->ws_jac, nb, (hipStream_t)stream);
