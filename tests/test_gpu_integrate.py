@@ -90,10 +90,19 @@ def test_integrate_matches_oracle(torch_gpu, oracle_mod, wl, n, seed, cvol, f32)
           f"sensitivity {uq.round(2)}; identical step counts "
           f"{np.mean(steps == ost['nsteps']):.3f}; mean steps gpu {steps.mean():.1f} oracle "
           f"{ost['nsteps'].mean():.1f}", flush=True)
-    assert gq[0] <= 2.0 * uq[0] + 1.0 and gq[1] <= 2.0 * uq[1] + 1.0
-    assert gq[2] <= 4.0 * uq[2] + 10.0
+    if not f32:
+        assert gq[0] <= 2.0 * uq[0] + 1.0 and gq[1] <= 2.0 * uq[1] + 1.0
+        assert gq[2] <= 4.0 * uq[2] + 10.0
+    else:
+        # FP32 Newton matrices change the Newton iterates by more than round-off
+        # (a different, equally valid integrator): looser quantiles, and the
+        # referee comparison below carries the accuracy claim
+        assert gq[0] <= 2.0 * uq[0] + 1.0 and gq[1] <= 2.0 * uq[1] + 10.0
+        assert gq[2] <= 4.0 * uq[2] + 50.0
     assert abs(steps.mean() / ost["nsteps"].mean() - 1.0) < (0.05 if f32 else 0.02)
-    assert np.all(np.abs(Yg.sum(axis=1) - 1.0) <= 1e-9)
+    # sum(Y) drifts by round-off (and, with FP32 Newton matrices, by their
+    # rounding of the mass-conserving corrections)
+    assert np.all(np.abs(Yg.sum(axis=1) - 1.0) <= (1e-8 if f32 else 1e-9))
     if cvol:
         # P = rho Ru T / W: the same tolerance-unit comparison on the pressure
         pu = np.abs(Pg - ost["P"]) / (RTOL * ost["P"])
