@@ -168,13 +168,11 @@ def _train_surrogate(torch, capi, surrogate, mechgen, m, n_train, seed, epochs, 
         m.h, __import__("ctypes").byref(opts), n_train, sel.data_ptr(), D.data_ptr(),
         T.data_ptr(), P.data_ptr(), Y.data_ptr(), m.nsp, None, None, 0), "select")
     torch.cuda.synchronize()
-    dY = Y.cpu().numpy() - st.Y
-    dT = T.cpu().numpy() - st.T
+    T1, Y1 = T.cpu().numpy(), Y.cpu().numpy()
     s = surrogate.Surrogate(torch, m, hidden=256, layers=3, seed=seed)
-    s.fit_normalisation(st.T, st.P, st.Y, dt, dY, dT)
+    s.fit_normalisation(st.T, st.P, st.Y, dt, T1, Y1)
     X = s.features(t(st.T), t(st.P), t(st.Y), D)
-    tgt = np.zeros((n_train, s.kout), np.float32)
-    tgt[:, :s.nout] = (np.column_stack([dY, dT]) - s.tmu) / s.tsig
+    tgt = s.target_matrix(st.T, st.Y, T1, Y1)
     losses = s.train(X, torch.as_tensor(tgt).cuda(), epochs=epochs, batch=2048, lr=1e-3, log=log)
     return s, losses
 
@@ -191,7 +189,7 @@ def test_surrogate_step_53sp(torch_gpu):
     torch = torch_gpu
     m = capi.Mech(*mech_paths("synth_53sp"))
     print("case surrogate 53sp", flush=True)
-    s, losses = _train_surrogate(torch, capi, surrogate, mechgen, m, 32768, 11, epochs=6)
+    s, losses = _train_surrogate(torch, capi, surrogate, mechgen, m, 65536, 11, epochs=12)
     assert losses[-1] < losses[0]
     n = 8192
     st = mechgen.workload_states("53sp", mechgen.workload_mechanism("53sp"), n, seed=12)

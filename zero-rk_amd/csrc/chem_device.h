@@ -402,11 +402,45 @@ __device__ __forceinline__ int rhs_team(const Team& team, const DevMech& m, cons
   return 0;
 }
 
-// One reactor per wavefront (nsp <= 63).
+// One reactor per wavefront (nsp <= 63).  With ZRK_RHS_CALL the body is one
+// out-of-line function shared by every call site (one copy of the rate code
+// in the integrator instead of one per site); its LDS arrays are addressed by
+// offset from the dynamic LDS base and the mechanism through its device copy.
+struct RhsRes {
+  double ydot, inv_rho, cp_mix, h, cpm;
+  int flag;
+};
+
+__device__ __noinline__ RhsRes rhs_wave_call(const DevMech* __restrict__ mp, int off, double P,
+                                             double v0, double Tref, double L, double y) {
+  extern __shared__ double dyn_lds[];
+  const DevMech m = *mp;
+  const RhsLds s{dyn_lds + off, dyn_lds + off + m.nsp, dyn_lds + off + 2 * m.nsp};
+  RhsRes r;
+  RhsAux aux;
+  r.flag = rhs_team(TeamWave{}, m, s, P, v0, Tref, L, y, r.ydot, aux);
+  r.inv_rho = aux.inv_rho;
+  r.cp_mix = aux.cp_mix;
+  r.h = aux.h;
+  r.cpm = aux.cpm;
+  return r;
+}
+
 __device__ __forceinline__ int rhs64(const DevMech& m, const RhsLds& s, double P, double v0,
                                      double Tref, double L, double y, double& ydot,
                                      RhsAux& aux) {
+#ifdef ZRK_RHS_CALL
+  extern __shared__ double dyn_lds[];
+  const RhsRes r = rhs_wave_call(m.self, (int)(s.C - dyn_lds), P, v0, Tref, L, y);
+  ydot = r.ydot;
+  aux.inv_rho = r.inv_rho;
+  aux.cp_mix = r.cp_mix;
+  aux.h = r.h;
+  aux.cpm = r.cpm;
+  return r.flag;
+#else
   return rhs_team(TeamWave{}, m, s, P, v0, Tref, L, y, ydot, aux);
+#endif
 }
 
 }  // namespace zrk

@@ -51,7 +51,7 @@ struct Cvode {
   static constexpr int kJS = kWaveTeam ? kNP : kL + 1;  // LDS column stride of matrices
   Team team;
 #ifdef ZRK_PHASE_CYCLES
-  long long cyc[4];
+  long long cyc[5];  // rhs, jacobian, inverse, solve, whole reactor
 #endif
   DevMech mech;
   RhsLds s;
@@ -89,7 +89,11 @@ struct Cvode {
     nfe++;
     RhsAux aux;
     ZRK_TIC(t0);
-    const int fl = rhs_team(team, mech, s, P, v0, opt.Tref, opt.step_limiter, yy, yd, aux);
+    int fl;
+    if constexpr (kWaveTeam)
+      fl = rhs64(mech, s, P, v0, opt.Tref, opt.step_limiter, yy, yd, aux);
+    else
+      fl = rhs_team(team, mech, s, P, v0, opt.Tref, opt.step_limiter, yy, yd, aux);
     ZRK_TOC(t0, 0);
     if (!fl) inv_rho_last = aux.inv_rho;
     if (!active) yd = 0.0;

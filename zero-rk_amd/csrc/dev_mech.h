@@ -34,6 +34,7 @@ enum RecFlags : int {
 };
 
 struct DevMech {
+  const DevMech* self;  // this struct's own copy in device memory
   int nsp, nstep, nrxn, nrec;
   int gd_blocks, gc_blocks;  // 8-entry blocks of the padded destroy/create gathers
   int jr_blocks;             // 4-entry blocks of the per-row Jacobian term lists

@@ -201,6 +201,7 @@ __device__ __forceinline__ void bdf_driver(Cvode<MT, Team>& cv, const DevMech& m
     cv.inv_rho_last = 0.0;
 #ifdef ZRK_PHASE_CYCLES
     cv.cyc[0] = cv.cyc[1] = cv.cyc[2] = cv.cyc[3] = 0;
+    const long long tr0 = (long long)__builtin_amdgcn_s_memtime();
 #endif
     cv.reinit(0.0, y0);
     int nsteps = 0, num_tries = 0, flag = kCvSuccess;
@@ -236,6 +237,9 @@ __device__ __forceinline__ void bdf_driver(Cvode<MT, Team>& cv, const DevMech& m
     } else {
       nsteps = (nsteps <= 0) ? -1 : -nsteps;
     }
+#ifdef ZRK_PHASE_CYCLES
+    cv.cyc[4] = (long long)__builtin_amdgcn_s_memtime() - tr0;
+#endif
     if (lane == 0) {
       if (nsteps_out) nsteps_out[r] = nsteps;
       if (stats_out) {
@@ -245,7 +249,7 @@ __device__ __forceinline__ void bdf_driver(Cvode<MT, Team>& cv, const DevMech& m
         for (int k = 0; k < 7; ++k) stats_out[(size_t)r * 16 + k] = tot[k];
         stats_out[(size_t)r * 16 + 7] = flag;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) stats_out[(size_t)r * 16 + 8 + k] = (int)(cv.cyc[k] >> 4);
+        for (int k = 0; k < 5; ++k) stats_out[(size_t)r * 16 + 8 + k] = (int)(cv.cyc[k] >> 4);
 #else
 #pragma unroll
         for (int k = 0; k < 7; ++k) stats_out[(size_t)r * 8 + k] = tot[k];

@@ -207,6 +207,18 @@ int build_device_mech(MechHandle* h, std::string* err) {
   d.g_d = (const uint4*)(p + o_gd);
   d.g_c = (const uint4*)(p + o_gc);
   d.jr = (const uint4*)(p + o_jr);
+  e = hipMalloc((void**)&h->dev_self, sizeof(DevMech));
+  if (e == hipSuccess) {
+    d.self = h->dev_self;
+    e = hipMemcpy(h->dev_self, &d, sizeof(DevMech), hipMemcpyHostToDevice);
+  }
+  if (e != hipSuccess) {
+    (void)hipFree(base);
+    if (h->dev_self) (void)hipFree(h->dev_self);
+    h->dev_self = nullptr;
+    *err = std::string("device mechanism copy: ") + hipGetErrorString(e);
+    return (int)e;
+  }
   h->dev_blob = base;
   h->dev_bytes = b.bytes.size();
   if (hipGetDevice(&h->device) != hipSuccess) h->device = 0;
@@ -252,6 +264,7 @@ MechHandle::~MechHandle() {
   if (ws_jac) (void)hipFree(ws_jac);
   if (ws_counter) (void)hipFree(ws_counter);
   if (dev_blob) (void)hipFree(dev_blob);
+  if (dev_self) (void)hipFree(dev_self);
 }
 
 int ensure_device(MechHandle* h) {

@@ -14,6 +14,7 @@ namespace zrk {
 struct MechHandle {
   Mechanism host;
   DevMech dev{};
+  DevMech* dev_self = nullptr;  // device copy of dev (dev.self points here)
   void* dev_blob = nullptr;
   size_t dev_bytes = 0;
   int device = 0;

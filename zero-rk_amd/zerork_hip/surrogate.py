@@ -1,7 +1,9 @@
 """Learned surrogate of one reactor step (no counterpart in the reference).
 
 An MLP maps the features [ln T, ln P, ln max(Y_j, 1e-20), ln dt] of a reactor
-to its end-of-step increments (dY_j, dT), normalised per output.  Forward,
+to its end-of-step changes in log space, ln(max(Y1_j, 1e-20)/max(Y0_j, 1e-20))
+and ln(T1/T0), normalised per output (relative changes keep minor species,
+which span many decades, as well conditioned as major ones).  Forward,
 backward and Adam run in the library's own BF16-input / FP32-accumulate
 MFMA kernels (include/zrk_surrogate.h); PyTorch only holds the device arrays.
 
@@ -110,18 +112,34 @@ class Surrogate:
         self.step = 0
 
     # -- features and normalisation -------------------------------------------
-    def fit_normalisation(self, T, P, Y, dt, dY, dT):
-        """Feature and target statistics from a training set (host arrays)."""
-        x = np.column_stack([np.log(T), np.log(P), np.log(np.maximum(Y, 1e-20)), np.log(dt)])
+    FLOOR = 1e-20
+
+    @classmethod
+    def raw_targets(cls, T0, Y0, T1, Y1):
+        """Log-space changes of a step (host arrays): [n, nsp + 1]."""
+        f = cls.FLOOR
+        return np.column_stack([np.log(np.maximum(Y1, f)) - np.log(np.maximum(Y0, f)),
+                                np.log(T1 / T0)])
+
+    def fit_normalisation(self, T, P, Y, dt, T1, Y1):
+        """Feature and target statistics from a training set (host arrays):
+        initial states (T, P, Y), step sizes dt and end states (T1, Y1)."""
+        x = np.column_stack([np.log(T), np.log(P), np.log(np.maximum(Y, self.FLOOR)), np.log(dt)])
         mu = np.zeros(self.kin, np.float32)
         sig = np.ones(self.kin, np.float32)
         mu[:self.nfeat] = x.mean(axis=0)
         sig[:self.nfeat] = np.maximum(x.std(axis=0), 1e-6)
         self.mu.copy_(self.torch.as_tensor(mu))
         self.sig.copy_(self.torch.as_tensor(sig))
-        tgt = np.column_stack([dY, dT])
+        tgt = self.raw_targets(T, Y, T1, Y1)
         self.tmu = tgt.mean(axis=0)
-        self.tsig = np.maximum(tgt.std(axis=0), 1e-30)
+        self.tsig = np.maximum(tgt.std(axis=0), 1e-12)
+
+    def target_matrix(self, T0, Y0, T1, Y1):
+        """Normalised, zero-padded f32 training targets [n, kout] (host)."""
+        tgt = np.zeros((len(T0), self.kout), np.float32)
+        tgt[:, :self.nout] = (self.raw_targets(T0, Y0, T1, Y1) - self.tmu) / self.tsig
+        return tgt
 
     def features(self, T, P, Y, dt, stream=0):
         n = T.numel()
@@ -225,11 +243,11 @@ class Surrogate:
         """Proposed end states (T1, Y1) (device f64) for per-reactor dt."""
         t = self.torch
         out = self.forward(self.features(T, P, Y, dt))
-        o = out[:, :self.nout].double() * t.as_tensor(self.tsig, device=self.device) + \
+        z = out[:, :self.nout].double() * t.as_tensor(self.tsig, device=self.device) + \
             t.as_tensor(self.tmu, device=self.device)
-        Y1 = (Y + o[:, :self.nsp]).clamp_min(0.0)
+        Y1 = t.exp(t.log(Y.clamp_min(self.FLOOR)) + z[:, :self.nsp])
         Y1 = Y1 / Y1.sum(dim=1, keepdim=True)
-        T1 = T + o[:, self.nsp]
+        T1 = T * t.exp(z[:, self.nsp])
         return T1.contiguous(), Y1.contiguous()
 
     def accept(self, T0, P, Y0, T1, Y1, dt, opts, newton=0):
