@@ -108,6 +108,21 @@ ZRK_EXPORT int zrk_integrate_device(zrk_mech m, const zrk_bdf_options* o, int n,
                                     double* T, double* P, double* Y, int stride,
                                     int* nsteps, int* stats, void* stream);
 
+/* Autotuner (no counterpart in the reference): times the integrator on the
+ * given device batch (copies of T, P, Y; inputs unchanged) with each team
+ * size that fits the mechanism and these options (1 wavefront per reactor,
+ * or 2 / 3 with fp32_preconditioner) and keeps the fastest for later calls
+ * with the same precision on this handle.  *choice = wavefronts per reactor
+ * chosen; ms[1..3] (optional, 4 floats) = measured times, -1 if not tried. */
+ZRK_EXPORT int zrk_autotune_integrator(zrk_mech m, const zrk_bdf_options* o, int n, double dt,
+                                       const double* T, const double* P, const double* Y,
+                                       int stride, int* choice, float* ms, void* stream);
+
+/* Sets the team size (wavefronts per reactor) of later integrations with the
+ * given precision on this handle, as the autotuner does; 0 restores the
+ * smallest size that fits.  -1 if the size cannot integrate this mechanism. */
+ZRK_EXPORT int zrk_integrator_set_team_width(zrk_mech m, int fp32_preconditioner, int W);
+
 /* Number of wavefronts the persistent integrator keeps resident with these
  * options (NULL: defaults), for roofline bookkeeping. */
 ZRK_EXPORT int zrk_integrator_resident_waves(zrk_mech m, const zrk_bdf_options* o);

@@ -91,3 +91,42 @@ def test_team_sizes_need_fp32(torch_gpu):
     Y[:, 0] = 1.0
     with pytest.raises(RuntimeError, match=r"\(-3\)"):
         m.integrate(n, 1e-6, T, P, Y, m.nsp, capi.default_options(fp32_preconditioner=0))
+
+
+def test_two_wave_team_on_53_species_and_autotune(torch_gpu, oracle_mod):
+    """A 53-species mechanism also runs on a team of two wavefronts (FP32): same
+    parity bar as the one-wavefront FP32 integrator; the autotuner times both
+    sizes on a batch and keeps one of them."""
+    from zerork_hip import capi, mechgen
+    torch = torch_gpu
+    m = capi.Mech(*mech_paths("synth_53sp"))
+    o = oracle_mod.Oracle(*mech_paths("synth_53sp"))
+    st = mechgen.workload_states("53sp", mechgen.workload_mechanism("53sp"), 256, seed=102)
+    n, nsp = st.Y.shape
+    print("case 53sp two-wave team", flush=True)
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda:0")
+    opts = capi.default_options(rel_tol=RTOL, abs_tol=ATOL, fp32_preconditioner=1)
+    m.set_team_width(1, 2)
+    dT, dP, dY = t(st.T), t(st.P), t(st.Y)
+    steps = torch.zeros(n, dtype=torch.int32, device="cuda:0")
+    m.integrate(n, 1e-4, dT, dP, dY, nsp, opts, steps)
+    torch.cuda.synchronize()
+    Tg, Yg, sg = dT.cpu().numpy(), dY.cpu().numpy(), steps.cpu().numpy()
+    oo = oracle_mod.default_opts(rtol=RTOL, atol=ATOL)
+    To, Yo, ost = o.integrate(st.T, st.P, st.Y, 1e-4, oo)
+    Tp, Yp, _ = o.integrate(np.nextafter(st.T, np.inf), st.P, st.Y, 1e-4, oo)
+    gpu, ulp = _units(Yg, Tg, Yo, To), _units(Yp, Tp, Yo, To)
+    gq, uq = np.percentile(gpu, [50, 90, 100]), np.percentile(ulp, [50, 90, 100])
+    print(f"53sp team of 2: GPU-oracle p50/p90/max {gq.round(2)}; 1-ulp {uq.round(2)}; "
+          f"mean steps {sg.mean():.1f} vs {ost['nsteps'].mean():.1f}", flush=True)
+    assert np.all(sg > 0)
+    assert gq[0] <= 2.0 * uq[0] + 1.0 and gq[1] <= 2.0 * uq[1] + 10.0
+    assert gq[2] <= 4.0 * uq[2] + 50.0
+    assert abs(sg.mean() / ost["nsteps"].mean() - 1.0) < 0.05
+    # autotune on a larger batch
+    m.set_team_width(1, 0)
+    big = mechgen.workload_states("53sp", mechgen.workload_mechanism("53sp"), 8192, seed=5)
+    choice, ms = m.autotune(8192, 1e-4, t(big.T), t(big.P), t(big.Y), nsp, opts)
+    print(f"autotune: chose {choice} wavefronts per reactor; times {ms}", flush=True)
+    assert choice in (1, 2) and set(ms) == {1, 2}
+    assert ms[choice] == min(ms.values())

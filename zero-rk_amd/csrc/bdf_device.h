@@ -174,9 +174,11 @@ __device__ __forceinline__ double jacobian_team(const Team& team, const DevMech&
   if (tid < js)
     for (int c = 0; c < nsp; ++c) J[c * js + tid] = MT(0);
   team.sync();
-  {
+  if (tid < m.jr_stride) {
+    // rows beyond the table's stride have no terms (team larger than needed)
     const uint4* __restrict__ jr = m.jr + tid;
     const int nb = m.jr_blocks;
+    const int js_tab = m.jr_stride;
     int cur = -1;
     double v = 0.0;
     auto term = [&](unsigned t) {
@@ -194,7 +196,7 @@ __device__ __forceinline__ double jacobian_team(const Team& team, const DevMech&
     uint4 w = nb > 0 ? jr[0] : make_uint4(0, 0, 0, 0);
     for (int b = 0; b < nb; ++b) {
       const uint4 x = w;
-      if (b + 1 < nb) w = jr[(b + 1) * Team::kThreads];
+      if (b + 1 < nb) w = jr[(b + 1) * js_tab];
       term(x.x);
       term(x.y);
       term(x.z);

@@ -38,6 +38,7 @@ struct DevMech {
   int nsp, nstep, nrxn, nrec;
   int gd_blocks, gc_blocks;  // 8-entry blocks of the padded destroy/create gathers
   int jr_blocks;             // 4-entry blocks of the per-row Jacobian term lists
+  int jr_stride;             // rows per block of jr (64 * ceil((nsp + 1) / 64))
   // species [nsp]
   const double* mw;
   const double* inv_mw;   // 1/mw

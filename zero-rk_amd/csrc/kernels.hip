@@ -372,15 +372,18 @@ int bdf_team_width(int nsp) {
   return 0;
 }
 
-size_t bdf_slot_doubles(const DevMech& m) {
-  const int W = bdf_team_width(m.nsp);
+bool bdf_team_ok(int nsp, bool f32, int W) {
+  if (W == 1) return nsp <= kNP - 1;
+  return f32 && (W == 2 || W == 3) && nsp + 1 <= kWave * W;
+}
+
+size_t bdf_slot_doubles(const DevMech& m, int W) {
   if (W <= 1) return (size_t)kNP * kNP;
   const size_t floats = (size_t)(m.nsp + 1) * (kWave * W + 1);
   return ((floats + 1) / 2 + 31) & ~(size_t)31;  // 256-byte aligned slots
 }
 
-size_t bdf_lds_bytes(const DevMech& m, bool f32) {
-  const int W = bdf_team_width(m.nsp);
+size_t bdf_lds_bytes(const DevMech& m, bool f32, int W) {
   if (W <= 1) {
     const size_t mt = f32 ? sizeof(float) : sizeof(double);
     return mt * (size_t)(kNP * kNP + 2 * kNP) + sizeof(double) * kNP + rhs_lds_bytes(m);
@@ -417,9 +420,9 @@ int resident_blocks(K kernel, int threads, size_t lds) {
 
 }  // namespace
 
-int bdf_max_resident_teams(const DevMech& m, bool f32) {
-  const int W = bdf_team_width(m.nsp);
-  const size_t lds = bdf_lds_bytes(m, f32);
+int bdf_max_resident_teams(const DevMech& m, bool f32, int W) {
+  if (!bdf_team_ok(m.nsp, f32, W)) return 0;
+  const size_t lds = bdf_lds_bytes(m, f32, W);
   switch (W) {
     case 1:
       return f32 ? resident_blocks(bdf_kernel<float>, kWave, lds)
@@ -436,12 +439,11 @@ int bdf_max_resident_teams(const DevMech& m, bool f32) {
 hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, const double* dts,
                       double* T, double* P, double* Y, int stride, int* nsteps, int* stats,
                       const int* order, unsigned int* counter, double* jscratch, int nwaves,
-                      hipStream_t st) {
+                      int W, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  const int W = bdf_team_width(m.nsp);
   const bool f32 = o.fp32_preconditioner != 0;
-  if (W == 0 || (W > 1 && !f32)) return hipErrorInvalidValue;
-  const size_t lds = bdf_lds_bytes(m, f32);
+  if (!bdf_team_ok(m.nsp, f32, W)) return hipErrorInvalidValue;
+  const size_t lds = bdf_lds_bytes(m, f32, W);
   if (lds > max_lds_per_block()) return hipErrorInvalidValue;
   hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, counter, 0u);
   const int grid = nwaves < n ? nwaves : n;
@@ -454,11 +456,11 @@ hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, cons
   else if (W == 2)
     hipLaunchKernelGGL(bdf_team_kernel<2>, dim3(grid), dim3(2 * kWave), lds, st, m, o, n, dt, dts, T,
                        P, Y, stride, nsteps, stats, order, counter, jscratch,
-                       bdf_slot_doubles(m));
+                       bdf_slot_doubles(m, W));
   else
     hipLaunchKernelGGL(bdf_team_kernel<3>, dim3(grid), dim3(3 * kWave), lds, st, m, o, n, dt, dts, T,
                        P, Y, stride, nsteps, stats, order, counter, jscratch,
-                       bdf_slot_doubles(m));
+                       bdf_slot_doubles(m, W));
   return hipGetLastError();
 }
 

@@ -13,18 +13,20 @@ hipError_t launch_rates(const DevMech& m, int n, const double* T, const double* 
 hipError_t launch_jacobian(const DevMech& m, const BdfOpts& o, int n, const double* T,
                            const double* P, const double* Y, int stride, double* J,
                            double* jscratch, int nblocks, hipStream_t st);
-// Wavefronts per reactor of the integrator: 1 (nsp <= 63), 2 or 3 (FP32
-// preconditioner only), 0 if the mechanism is too large.
+// Smallest number of wavefronts per reactor of the integrator: 1 (nsp <= 63),
+// 2 or 3 (FP32 preconditioner only), 0 if the mechanism is too large.
 int bdf_team_width(int nsp);
-// doubles of saved-Jacobian workspace per resident team
-size_t bdf_slot_doubles(const DevMech& m);
-size_t bdf_lds_bytes(const DevMech& m, bool f32);
+// whether a team of W wavefronts can integrate this mechanism
+bool bdf_team_ok(int nsp, bool f32, int W);
+// doubles of saved-Jacobian workspace per resident team of W wavefronts
+size_t bdf_slot_doubles(const DevMech& m, int W);
+size_t bdf_lds_bytes(const DevMech& m, bool f32, int W);
 // teams the persistent integrator keeps resident (0: does not fit)
-int bdf_max_resident_teams(const DevMech& m, bool f32);
+int bdf_max_resident_teams(const DevMech& m, bool f32, int W);
 hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, const double* dts,
                       double* T, double* P, double* Y, int stride, int* nsteps, int* stats,
                       const int* order, unsigned int* counter, double* jscratch, int nwaves,
-                      hipStream_t st);
+                      int W, hipStream_t st);
 
 // doubles of per-team LDS used by the RHS (C, G, rop with its zero slot)
 size_t rhs_lds_bytes(const DevMech& m);

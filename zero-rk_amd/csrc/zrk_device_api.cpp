@@ -193,6 +193,7 @@ int build_device_mech(MechHandle* h, std::string* err) {
   d.gd_blocks = gdb;
   d.gc_blocks = gcb;
   d.jr_blocks = jrb;
+  d.jr_stride = team;
   d.mw = (const double*)(p + o_mw);
   d.inv_mw = (const double*)(p + o_imw);
   d.ru_mw = (const double*)(p + o_rmw);
@@ -274,23 +275,30 @@ int ensure_device(MechHandle* h) {
   return rc;
 }
 
+int team_width_for(MechHandle* h, bool f32) {
+  int W = h->team_pref[f32] ? h->team_pref[f32] : bdf_team_width(h->host.nsp);
+  if (!bdf_team_ok(h->host.nsp, f32, W)) return 0;
+  if (h->resident[f32][W] <= 0) h->resident[f32][W] = bdf_max_resident_teams(h->dev, f32, W);
+  return h->resident[f32][W] > 0 ? W : 0;
+}
+
 int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
                      const double* dts, double* T, double* P, double* Y, int stride, int* nsteps, int* stats,
                      const int* order, hipStream_t st) {
   if (n < 0 || !T || !P || !Y || stride < h->host.nsp || (!dts && !(dt > 0.0))) return -1;
-  const int W = bdf_team_width(h->host.nsp);
-  if (W == 0) return -2;                               // mechanism too large
-  if (W > 1 && !o.fp32_preconditioner) return -3;     // nsp > 63: FP32 preconditioner only
+  const int W0 = bdf_team_width(h->host.nsp);
+  if (W0 == 0) return -2;                               // mechanism too large
+  if (W0 > 1 && !o.fp32_preconditioner) return -3;     // nsp > 63: FP32 preconditioner only
   if (n == 0) return 0;
   if (int rc = ensure_device(h)) return rc;
   const int f32 = o.fp32_preconditioner != 0;
-  if (h->resident_waves[f32] <= 0) h->resident_waves[f32] = bdf_max_resident_teams(h->dev, f32);
-  if (h->resident_waves[f32] <= 0) return -2;         // does not fit in LDS
-  const int nw = std::max(1, std::min(h->resident_waves[f32], n));
-  int rc = ensure_workspace(h, (size_t)nw * bdf_slot_doubles(h->dev));
+  const int W = team_width_for(h, f32);
+  if (W == 0) return -2;                                // does not fit in LDS
+  const int nw = std::max(1, std::min(h->resident[f32][W], n));
+  int rc = ensure_workspace(h, (size_t)nw * bdf_slot_doubles(h->dev, W));
   if (rc) return rc;
   hipError_t e = launch_bdf(h->dev, to_bdf_opts(o), n, dt, dts, T, P, Y, stride, nsteps, stats,
-                            order, h->ws_counter, h->ws_jac, nw, st);
+                            order, h->ws_counter, h->ws_jac, nw, W, st);
   return e == hipSuccess ? 0 : (int)e;
 }
 
@@ -427,14 +435,78 @@ int zrk_integrate_device(zrk_mech m, const zrk_bdf_options* o, int n, double dt,
                                stride, nsteps, stats, nullptr, (hipStream_t)stream);
 }
 
+int zrk_autotune_integrator(zrk_mech m, const zrk_bdf_options* o, int n, double dt,
+                            const double* T, const double* P, const double* Y, int stride,
+                            int* choice, float* ms, void* stream) {
+  if (!m || !o || n <= 0 || !T || !P || !Y || !(dt > 0.0)) return -1;
+  auto* h = reinterpret_cast<MechHandle*>(m);
+  const int nsp = h->host.nsp;
+  if (stride < nsp) return -1;
+  if (int rc = zrk::ensure_device(h)) return rc;
+  const bool f32 = o->fp32_preconditioner != 0;
+  hipStream_t st = (hipStream_t)stream;
+  double *dT = nullptr, *dP = nullptr, *dY = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const size_t ny = (size_t)n * stride;
+  int rc = 0, best = 0;
+  float best_ms = 0.0f;
+  auto hrc = [&](hipError_t e) { if (e != hipSuccess && rc == 0) rc = (int)e; return e == hipSuccess; };
+  if (hrc(hipMalloc((void**)&dT, sizeof(double) * n)) && hrc(hipMalloc((void**)&dP, sizeof(double) * n)) &&
+      hrc(hipMalloc((void**)&dY, sizeof(double) * ny)) && hrc(hipEventCreate(&e0)) &&
+      hrc(hipEventCreate(&e1))) {
+    const int saved = h->team_pref[f32];
+    for (int W = 1; W <= 3 && rc == 0; ++W) {
+      if (ms) ms[W] = -1.0f;
+      if (!zrk::bdf_team_ok(nsp, f32, W)) continue;
+      h->team_pref[f32] = W;
+      if (zrk::team_width_for(h, f32) != W) continue;  // does not fit
+      float t = 0.0f;
+      for (int rep = 0; rep < 2 && rc == 0; ++rep) {  // first run warms up
+        hrc(hipMemcpyAsync(dT, T, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        hrc(hipMemcpyAsync(dP, P, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        hrc(hipMemcpyAsync(dY, Y, sizeof(double) * ny, hipMemcpyDeviceToDevice, st));
+        hrc(hipEventRecord(e0, st));
+        if (int r = zrk::integrate_device(h, *o, n, dt, nullptr, dT, dP, dY, stride, nullptr,
+                                          nullptr, nullptr, st))
+          if (rc == 0) rc = r;
+        hrc(hipEventRecord(e1, st));
+        hrc(hipEventSynchronize(e1));
+        hrc(hipEventElapsedTime(&t, e0, e1));
+      }
+      if (rc) break;
+      if (ms) ms[W] = t;
+      if (best == 0 || t < best_ms) {
+        best = W;
+        best_ms = t;
+      }
+    }
+    h->team_pref[f32] = rc ? saved : best;
+  }
+  if (dT) (void)hipFree(dT);
+  if (dP) (void)hipFree(dP);
+  if (dY) (void)hipFree(dY);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (choice) *choice = best;
+  return rc;
+}
+
+int zrk_integrator_set_team_width(zrk_mech m, int fp32_preconditioner, int W) {
+  if (!m) return -1;
+  auto* h = reinterpret_cast<MechHandle*>(m);
+  const bool f32 = fp32_preconditioner != 0;
+  if (W != 0 && !zrk::bdf_team_ok(h->host.nsp, f32, W)) return -1;
+  h->team_pref[f32] = W;
+  return 0;
+}
+
 int zrk_integrator_resident_waves(zrk_mech m, const zrk_bdf_options* o) {
   if (!m) return -1;
   auto* h = reinterpret_cast<MechHandle*>(m);
   if (int rc = zrk::ensure_device(h)) return -rc;
   const int f32 = (o && o->fp32_preconditioner) ? 1 : 0;
-  if (h->resident_waves[f32] <= 0)
-    h->resident_waves[f32] = zrk::bdf_max_resident_teams(h->dev, f32);
-  return h->resident_waves[f32] * zrk::bdf_team_width(h->host.nsp);
+  const int W = zrk::team_width_for(h, f32);
+  return W ? h->resident[f32][W] * W : 0;
 }
 
 }  // extern "C"

@@ -18,7 +18,10 @@ struct MechHandle {
   void* dev_blob = nullptr;
   size_t dev_bytes = 0;
   int device = 0;
-  int resident_waves[2] = {0, 0};  // persistent grid size: FP64, FP32 preconditioner
+  // persistent grid size in teams, by [FP32 preconditioner][wavefronts per team]
+  int resident[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+  // wavefronts per team chosen by the autotuner for [FP32 preconditioner] (0: smallest)
+  int team_pref[2] = {0, 0};
   // integrator workspace
   double* ws_jac = nullptr;            // saved-Jacobian slots, one per resident team
   unsigned int* ws_counter = nullptr;
@@ -27,6 +30,8 @@ struct MechHandle {
 };
 
 int build_device_mech(MechHandle* h, std::string* err);
+// team width the integrator uses for these options (0 if none fits)
+int team_width_for(MechHandle* h, bool f32);
 int ensure_device(MechHandle* h);
 int ensure_workspace(MechHandle* h, size_t doubles);
 BdfOpts to_bdf_opts(const zrk_bdf_options& o);

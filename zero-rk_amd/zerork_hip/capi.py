@@ -40,6 +40,7 @@ DEVICE_SYMBOLS = [
     "zrk_mech_species", "zrk_mech_molwt", "zrk_mech_steps", "zrk_mech_reactions",
     "zrk_rates_device",
     "zrk_jacobian_device", "zrk_integrate_device", "zrk_integrator_resident_waves",
+    "zrk_autotune_integrator", "zrk_integrator_set_team_width",
 ]
 
 
@@ -114,6 +115,10 @@ def _declare(L):
     L.zrk_integrate_device.argtypes = [P, C.POINTER(BdfOptions), I, D, P, P, P, I, P, P, P]
     L.zrk_integrate_device.restype = I
     L.zrk_integrator_resident_waves.argtypes = [P, P]
+    L.zrk_autotune_integrator.argtypes = [P, C.POINTER(BdfOptions), I, D, P, P, P, I, P, P, P]
+    L.zrk_autotune_integrator.restype = I
+    L.zrk_integrator_set_team_width.argtypes = [P, I, I]
+    L.zrk_integrator_set_team_width.restype = I
     L.zrk_integrator_resident_waves.restype = I
 
 
@@ -216,6 +221,22 @@ class Mech:
                                         stride, _ptr(nsteps), _ptr(stats), stream)
         if rc != 0:
             raise RuntimeError(f"zrk_integrate_device failed ({rc})")
+
+    def autotune(self, n, dt, T, P, Y, stride, opts: BdfOptions, stream=0):
+        """Time each team size on this batch, keep the fastest; returns
+        (wavefronts per reactor, {size: ms})."""
+        choice = C.c_int(0)
+        ms = (C.c_float * 4)()
+        rc = lib().zrk_autotune_integrator(self.h, C.byref(opts), n, dt, _ptr(T), _ptr(P),
+                                           _ptr(Y), stride, C.byref(choice), ms, stream)
+        if rc != 0:
+            raise RuntimeError(f"zrk_autotune_integrator failed ({rc})")
+        return choice.value, {w: ms[w] for w in (1, 2, 3) if ms[w] >= 0}
+
+    def set_team_width(self, fp32, W):
+        rc = lib().zrk_integrator_set_team_width(self.h, int(fp32), int(W))
+        if rc != 0:
+            raise RuntimeError(f"zrk_integrator_set_team_width failed ({rc})")
 
     def resident_waves(self, opts: BdfOptions | None = None):
         return lib().zrk_integrator_resident_waves(self.h, C.byref(opts) if opts else None)
