@@ -79,7 +79,7 @@ def test_integrate_matches_oracle(torch_gpu, oracle_mod, wl, n, seed, cvol, f32)
     Tg, Yg, steps, stats, Pg = _gpu_integrate(torch_gpu, m, st.T, st.P, st.Y, dt, opts, True)
     oopts = oracle_mod.default_opts(rtol=RTOL, atol=ATOL, constant_volume=cvol)
     To, Yo, ost = o.integrate(st.T, st.P, st.Y, dt, oopts)
-    Tp, Yp, _ = o.integrate(np.nextafter(st.T, np.inf), st.P, st.Y, dt, oopts)
+    Tp, Yp, pst = o.integrate(np.nextafter(st.T, np.inf), st.P, st.Y, dt, oopts)
     assert np.all(ost["nsteps"] > 0)
     assert np.all(steps > 0), np.sum(steps <= 0)
     gpu = _units(Yg, Tg, Yo, To, RTOL, ATOL)
@@ -99,7 +99,10 @@ def test_integrate_matches_oracle(torch_gpu, oracle_mod, wl, n, seed, cvol, f32)
         # referee comparison below carries the accuracy claim
         assert gq[0] <= 2.0 * uq[0] + 1.0 and gq[1] <= 2.0 * uq[1] + 10.0
         assert gq[2] <= 4.0 * uq[2] + 50.0
-    assert abs(steps.mean() / ost["nsteps"].mean() - 1.0) < (0.05 if f32 else 0.02)
+    # mean step count: within a fixed margin plus twice the oracle's own 1-ulp
+    # sensitivity of it (stiff reactors' step counts are chaotic in round-off)
+    dsteps = abs(pst["nsteps"].mean() / ost["nsteps"].mean() - 1.0)
+    assert abs(steps.mean() / ost["nsteps"].mean() - 1.0) < (0.05 if f32 else 0.02) + 2.0 * dsteps
     # sum(Y) drifts by round-off (and, with FP32 Newton matrices, by their
     # rounding of the mass-conserving corrections)
     assert np.all(np.abs(Yg.sum(axis=1) - 1.0) <= (1e-8 if f32 else 1e-9))

@@ -4,10 +4,10 @@ in place in LDS) against the FP64 oracle: the seeded 160-species / 900-reaction
 workload mechanism and the reference's 79-species DME mechanism
 (data/mechanisms/dme/dme_24_mech.txt).
 
-As for the one-wavefront integrator, the bar is self-calibrated on the same
-inputs: GPU-vs-oracle differences in tolerance units (rtol |y| + atol) within
-the oracle's own one-ulp sensitivity, mean step counts close, no failures,
-sum(Y) = 1 to 1e-9.
+As for the one-wavefront FP32 integrator, the bar is self-calibrated on the
+same inputs: GPU-vs-oracle differences in tolerance units (rtol |y| + atol)
+within the oracle's own one-ulp sensitivity (with the FP32 preconditioner's
+margins), mean step counts close, no failures, sum(Y) = 1 to 1e-8.
 """
 import numpy as np
 import pytest
@@ -71,10 +71,12 @@ def test_team_integrator_matches_oracle(torch_gpu, oracle_mod, name, n, dt):
     print(f"{name}: GPU-oracle tolerance units p50/p90/max {gq.round(2)}; oracle 1-ulp "
           f"{uq.round(2)}; mean steps gpu {sg.mean():.1f} oracle {ost['nsteps'].mean():.1f}; "
           f"T rise max {np.max(To - T0):.1f} K", flush=True)
-    assert gq[0] <= 2.0 * uq[0] + 1.0 and gq[1] <= 2.0 * uq[1] + 1.0
-    assert gq[2] <= 4.0 * uq[2] + 10.0
+    # the FP32 preconditioner's bar (as for one wavefront): FP32 Newton matrices
+    # change the iterates by more than round-off
+    assert gq[0] <= 2.0 * uq[0] + 1.0 and gq[1] <= 2.0 * uq[1] + 10.0
+    assert gq[2] <= 4.0 * uq[2] + 50.0
     assert abs(sg.mean() / ost["nsteps"].mean() - 1.0) < 0.05
-    assert np.all(np.abs(Yg.sum(axis=1) - 1.0) <= 1e-9)
+    assert np.all(np.abs(Yg.sum(axis=1) - 1.0) <= 1e-8)
 
 
 def test_team_sizes_need_fp32(torch_gpu):
@@ -95,8 +97,8 @@ def test_team_sizes_need_fp32(torch_gpu):
 
 def test_two_wave_team_on_53_species_and_autotune(torch_gpu, oracle_mod):
     """A 53-species mechanism also runs on a team of two wavefronts (FP32): same
-    parity bar as the one-wavefront FP32 integrator; the autotuner times both
-    sizes on a batch and keeps one of them."""
+    parity bar as the one-wavefront FP32 integrator; the autotuner times every
+    team size that fits (1, 2 and 3 wavefronts) on a batch and keeps the fastest."""
     from zerork_hip import capi, mechgen
     torch = torch_gpu
     m = capi.Mech(*mech_paths("synth_53sp"))
@@ -128,5 +130,5 @@ def test_two_wave_team_on_53_species_and_autotune(torch_gpu, oracle_mod):
     big = mechgen.workload_states("53sp", mechgen.workload_mechanism("53sp"), 8192, seed=5)
     choice, ms = m.autotune(8192, 1e-4, t(big.T), t(big.P), t(big.Y), nsp, opts)
     print(f"autotune: chose {choice} wavefronts per reactor; times {ms}", flush=True)
-    assert choice in (1, 2) and set(ms) == {1, 2}
-    assert ms[choice] == min(ms.values())
+    assert 1 in ms and 2 in ms and set(ms) <= {1, 2, 3}
+    assert choice in ms and ms[choice] == min(ms.values())

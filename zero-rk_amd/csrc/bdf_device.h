@@ -38,7 +38,9 @@ constexpr int kQmax = 5;
 // call (at y), which the reference reuses for the 1/C factors
 // (reactor_nvector_serial.cpp:625-637).  Returns the inverse density of the
 // last RHS evaluated here (the reference's carried value).  t2l: kNP doubles
-// of LDS scratch.
+// of LDS scratch.  Jl is column-major with column stride ld >= nsp + 1 and
+// only rows <= nsp are written (the term table holds no terms for lanes at
+// or beyond nsp); saved is column-major with stride kNP.
 //
 // Species block: lane i walks row i's term list (sorted by column), so each
 // entry has one writer and the loads of the list are coalesced across lanes;
@@ -49,7 +51,7 @@ constexpr int kQmax = 5;
 template <typename MT>
 __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s, double P,
                                                double v0, const BdfOpts& o, double y, double fy,
-                                               double inv_rho_prev, MT* __restrict__ Jl,
+                                               double inv_rho_prev, MT* __restrict__ Jl, int ld,
                                                MT* __restrict__ saved,
                                                double* __restrict__ t2l) {
   const int lane = lane_id();
@@ -81,7 +83,8 @@ __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s
   const double RuTemp = kRu * yT;
   const double mult = -1.0 / aux.cp_mix;
   const double mwi = sp ? m.mw[lane] : 0.0;
-  for (int c = 0; c < nsp; ++c) Jl[c * kNP + lane] = MT(0);
+  if (act)
+    for (int c = 0; c < nsp; ++c) Jl[c * ld + lane] = MT(0);
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -101,7 +104,7 @@ __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s
         }
         const double p = __dmul_rn(s.rop[t & 0xFFFFu], t2l[col]);
         v = (t & 0x80000000u) ? __dsub_rn(v, p) : __dadd_rn(v, p);
-        Jl[col * kNP + lane] = (MT)v;
+        Jl[col * ld + lane] = (MT)v;
       }
     };
     uint4 w = nb > 0 ? jr[0] : make_uint4(0, 0, 0, 0);
@@ -118,17 +121,17 @@ __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   for (int c = 0; c < nsp; ++c) {
-    const double v = sp ? (double)Jl[c * kNP + lane] : 0.0;
+    const double v = sp ? (double)Jl[c * ld + lane] : 0.0;
     // d(Tdot)/dy_c from the concentration-space column
     const double sc = wave_sum(v * aux.h);
     double val = sc * RuTemp;
     val += Tdot * bcast(aux.cpm, c) * m.mw[c];
     val *= m.inv_mw[c] * mult;
     const double out = sp ? v * (mwi * m.inv_mw[c]) : (lane == nsp ? val : 0.0);
-    Jl[c * kNP + lane] = (MT)out;
+    if (act) Jl[c * ld + lane] = (MT)out;
     saved[c * kNP + lane] = (MT)out;
   }
-  Jl[nsp * kNP + lane] = (MT)colT;
+  if (act) Jl[nsp * ld + lane] = (MT)colT;
   saved[nsp * kNP + lane] = (MT)colT;
   return aux2.inv_rho;
 }
@@ -236,34 +239,58 @@ __device__ __forceinline__ double jacobian_team(const Team& team, const DevMech&
 
 // Gauss-Jordan inverse (no pivoting) of the N x N leading block of a matrix
 // held one row per lane in R[0..63]; padding rows/columns must be identity.
-// After N steps register c holds logical column (c + N) mod 64 of the
-// inverse.  prow: kNP entries of LDS for the published pivot row.  MT is the
-// working precision (double, or float for the FP32 preconditioner).
+// MT is the working precision (double, or float for the FP32 preconditioner).
 // Returns false on a zero pivot.
+//
+// The pivot row is broadcast from its lane with readlane (no LDS round trip)
+// and is not normalised when it is used: row k is kept unscaled with a
+// per-row factor s_k = 1/a_kk set at its pivot step, and every other row i
+// subtracts g_i * (row k) with g_i = a_ik / a_kk, which does not depend on
+// how row k is scaled.  The pivot column becomes -g_i (1 on the pivot row,
+// whose factor carries 1/a_kk), and every row is multiplied by its factor at
+// the end.  Per column and pivot that is one broadcast and one fma.
+// Register indices must be compile-time constants, so pivots are taken in
+// blocks of kGjBlock whose pivot columns sit in registers 0..kGjBlock-1, and
+// the registers are rotated by kGjBlock after each block: on return register
+// c holds logical column (c + gj_rotation(N)) mod 64 of the inverse.
+constexpr int kGjBlock = 8;
+__host__ __device__ constexpr int gj_rotation(int N) {
+  return (((N + kGjBlock - 1) / kGjBlock) * kGjBlock) & (kNP - 1);
+}
+
 template <typename MT>
-__device__ __forceinline__ bool gj_invert(MT (&R)[kNP], int N, MT* __restrict__ prow) {
+__device__ __forceinline__ bool gj_invert(MT (&R)[kNP], int N) {
   const int lane = lane_id();
   bool ok = true;
-  for (int k = 0; k < N; ++k) {
-    const MT piv = bcast(R[0], k);
-    if (piv == MT(0)) ok = false;
-    const MT rp = MT(1) / piv;
-    if (lane == k) {
+  MT s = MT(1);
+  for (int k0 = 0; k0 < N; k0 += kGjBlock) {
 #pragma unroll
-      for (int c = 1; c < kNP; ++c) R[c] *= rp;
+    for (int kk = 0; kk < kGjBlock; ++kk) {
+      const int k = k0 + kk;
+      if (k < N) {
+        const MT f = R[kk];
+        const MT piv = bcast(f, k);
+        if (piv == MT(0)) ok = false;
+        const MT rp = MT(1) / piv;
+        const bool me = (lane == k);
+        const MT g = me ? MT(0) : f * rp;
+        if (me) s = rp;
 #pragma unroll
-      for (int c = 1; c < kNP; ++c) prow[c] = R[c];
+        for (int c = 0; c < kNP; ++c)
+          if (c != kk) R[c] = fma(-g, bcast(R[c], k), R[c]);
+        R[kk] = me ? MT(1) : -g;
+      }
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const MT l = (lane == k) ? MT(0) : R[0];
-    const MT last = (lane == k) ? rp : -l * rp;
+    MT t[kGjBlock];
 #pragma unroll
-    for (int c = 1; c < kNP; ++c) R[c - 1] = fma(-l, prow[c], R[c]);
-    R[kNP - 1] = last;
-    __builtin_amdgcn_wave_barrier();
+    for (int c = 0; c < kGjBlock; ++c) t[c] = R[c];
+#pragma unroll
+    for (int c = 0; c < kNP - kGjBlock; ++c) R[c] = R[c + kGjBlock];
+#pragma unroll
+    for (int c = 0; c < kGjBlock; ++c) R[kNP - kGjBlock + c] = t[c];
   }
+#pragma unroll
+  for (int c = 0; c < kNP; ++c) R[c] *= s;
   return ok;
 }
 

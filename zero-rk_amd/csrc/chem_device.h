@@ -34,17 +34,42 @@ __device__ __forceinline__ double uni(double v) {
 }
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Butterfly sums: every lane ends with the same (bitwise) value.
+// Lane values moved by a DPP pattern within rows of 16 lanes (both halves of
+// the double); CTRL: 0xB1 quad xor 1, 0x4E quad xor 2, 0x141 half-row mirror,
+// 0x140 row mirror.  All lanes must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, true);
+  hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double lane_value(double v, int src) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+  return __hiloint2double(hi, lo);
+}
+
+// Wave sums: symmetric DPP exchanges inside each row of 16 lanes (every lane
+// of a row ends with the same row sum, since a + b == b + a bitwise), then the
+// four row sums added in row order from scalar registers.  Every lane gets
+// the same bits.  All lanes must be active.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
-  return uni(v);
+  v += dpp_move<0xB1>(v);
+  v += dpp_move<0x4E>(v);
+  v += dpp_move<0x141>(v);
+  v += dpp_move<0x140>(v);
+  return uni(((lane_value(v, 0) + lane_value(v, 16)) + lane_value(v, 32)) + lane_value(v, 48));
 }
 
 __device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, kWave));
-  return uni(v);
+  v = fmax(v, dpp_move<0xB1>(v));
+  v = fmax(v, dpp_move<0x4E>(v));
+  v = fmax(v, dpp_move<0x141>(v));
+  v = fmax(v, dpp_move<0x140>(v));
+  return uni(fmax(fmax(lane_value(v, 0), lane_value(v, 16)),
+                  fmax(lane_value(v, 32), lane_value(v, 48))));
 }
 
 // Value of lane `src` (wave-uniform) broadcast to every lane.

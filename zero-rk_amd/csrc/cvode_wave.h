@@ -48,7 +48,8 @@ template <typename MT, class Team = TeamWave>
 struct Cvode {
   static constexpr int kL = Team::kThreads;        // state slots = matrix row registers
   static constexpr bool kWaveTeam = (kL == kWave);
-  static constexpr int kJS = kWaveTeam ? kNP : kL + 1;  // LDS column stride of matrices
+  static constexpr int kJS = kWaveTeam ? kNP : kL + 1;  // column stride of savedJ (and of
+                                                       // Mlds for teams)
   Team team;
 #ifdef ZRK_PHASE_CYCLES
   long long cyc[5];  // rhs, jacobian, inverse, solve, whole reactor
@@ -57,10 +58,10 @@ struct Cvode {
   RhsLds s;
   BdfOpts opt;
   MT* savedJ;      // this team's saved Jacobian in HBM (column stride kJS)
-  MT* Mlds;        // LDS, column-major with stride kJS: Jacobian workspace,
-                   // then the inverse of the Newton matrix
-  MT* prow;        // LDS [kL]: published pivot row
-  MT* rvec;        // LDS [kL]: right-hand side of a solve
+  MT* Mlds;        // LDS, column-major: Jacobian workspace, then the inverse of the
+                   // Newton matrix; stride nsp + 1 for one wavefront, kJS for teams
+  MT* prow;        // LDS [kL]: published pivot row (teams)
+  MT* rvec;        // LDS [kL]: right-hand side of a solve (teams)
   double* t2l;     // LDS [kL]: Jacobian column factors
   double* hl;      // LDS [kL]: energies for the Jacobian's temperature row (teams of
                    // several wavefronts)
@@ -389,27 +390,27 @@ struct Cvode {
            ((cf == kFailBadJ) && (dgamma < kCvlsDgmax)) || (cf == kFailOther);
   }
 
-  // one wavefront: the matrix is inverted in registers (row per lane)
+  // one wavefront: the matrix is inverted in registers (row per lane); the
+  // inverse is kept in LDS as the N x N block, column-major with stride N
   __device__ __forceinline__ int lsetup_wave(int cf, double ypred, double fpred, int& jc) {
     const int nsp = mech.nsp;
+    const int N = nsp + 1;
     const int lane = team.tid();
-    MT R[kNP];
     if (!jacobian_is_bad(cf)) {
       jc = 0;
-#pragma unroll
-      for (int c = 0; c < kNP; ++c) R[c] = savedJ[c * kNP + lane];
     } else {
       jc = 1;
       nje++;
       nstlj = nst;
       ZRK_TIC(t0);
-      inv_rho_last = jacobian_lds<MT>(mech, s, P, v0, opt, ypred, fpred, inv_rho_last, Mlds,
+      inv_rho_last = jacobian_lds<MT>(mech, s, P, v0, opt, ypred, fpred, inv_rho_last, Mlds, N,
                                       savedJ, t2l);
-      team.sync();
-#pragma unroll
-      for (int c = 0; c < kNP; ++c) R[c] = Mlds[c * kNP + lane];
       ZRK_TOC(t0, 1);
     }
+    // the saved Jacobian (rewritten above by this lane when it was re-evaluated)
+    MT R[kNP];
+#pragma unroll
+    for (int c = 0; c < kNP; ++c) R[c] = savedJ[c * kNP + lane];
     ZRK_TIC(t1);
     const double ng = -gamma;
 #pragma unroll
@@ -418,9 +419,17 @@ struct Cvode {
       if (c == lane) v += 1.0;
       R[c] = (MT)v;
     }
-    const bool ok = gj_invert<MT>(R, nsp + 1, prow);
+    const bool ok = gj_invert<MT>(R, N);
+    team.sync();  // every lane is done with the Jacobian workspace
+    // register c holds logical column (c + rot) mod 64: keep logical columns < N
+    const int rot = gj_rotation(N);
+    if (lane < N) {
 #pragma unroll
-    for (int c = 0; c < kNP; ++c) Mlds[c * kNP + lane] = R[c];
+      for (int c = 0; c < kNP; ++c) {
+        const int j = (c + rot) & (kNP - 1);
+        if (j < N) Mlds[j * N + lane] = R[c];
+      }
+    }
     team.sync();
     ZRK_TOC(t1, 2);
     return ok ? 0 : 1;
@@ -458,20 +467,29 @@ struct Cvode {
     return ok ? 0 : 1;
   }
 
-  // x = M^{-1} b with the inverse in LDS (rotated column order for one
-  // wavefront, logical order for larger teams)
+  // x = M^{-1} b with the inverse in LDS (for one wavefront the entries of b
+  // are broadcast from their lanes; teams publish b in LDS)
   __device__ __forceinline__ double lsolve(double b) {
     ZRK_TIC(t0);
     const int lane = team.tid();
     const int N = mech.nsp + 1;
-    rvec[lane] = (MT)b;
-    team.sync();
     MT acc = MT(0);
     if constexpr (kWaveTeam) {
-#pragma unroll 8
-      for (int c = 0; c < kNP; ++c)
-        acc = fma(Mlds[c * kNP + lane], rvec[(c + N) & (kNP - 1)], acc);
+      const MT bm = (MT)b;
+      if (lane < N) {
+        const MT* __restrict__ col = Mlds + lane;
+        int j = 0;
+        for (; j + 4 <= N; j += 4) {  // same order as one j at a time
+          acc = fma(col[j * N], bcast(bm, j), acc);
+          acc = fma(col[(j + 1) * N], bcast(bm, j + 1), acc);
+          acc = fma(col[(j + 2) * N], bcast(bm, j + 2), acc);
+          acc = fma(col[(j + 3) * N], bcast(bm, j + 3), acc);
+        }
+        for (; j < N; ++j) acc = fma(col[j * N], bcast(bm, j), acc);
+      }
     } else {
+      rvec[lane] = (MT)b;
+      team.sync();
 #pragma unroll 8
       for (int j = 0; j < N; ++j) acc = fma(Mlds[j * kJS + lane], rvec[j], acc);
       if (lane >= N) acc = MT(0);

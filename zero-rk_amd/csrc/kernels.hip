@@ -16,6 +16,12 @@ namespace zrk {
 
 constexpr int kRatesWaves = 4;  // wavefronts per workgroup in rates_kernel
 
+// LDS bytes of the one-wavefront integrator's (nsp+1) x (nsp+1) matrix of
+// element size mt, padded to 16 bytes.
+__host__ __device__ constexpr size_t wave_matrix_bytes(int nsp, size_t mt) {
+  return ((size_t)(nsp + 1) * (size_t)(nsp + 1) * mt + 15) & ~(size_t)15;
+}
+
 // ---------------------------------------------------------------------------
 // rates_kernel: inputs T[n], P[n], Y[n*stride] (reactor-major AoS, as the
 // plugin API); outputs (each optional): ydot[n*(nsp+1)] ([Y.., T] order),
@@ -107,7 +113,7 @@ jacobian_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ Tin,
     double fy;
     RhsAux aux;
     rhs64(m, s, P, 0.0, o.Tref, o.step_limiter, y, fy, aux);
-    jacobian_lds<double>(m, s, P, 0.0, o, y, fy, aux.inv_rho, Jl, saved, t2l);
+    jacobian_lds<double>(m, s, P, 0.0, o, y, fy, aux.inv_rho, Jl, kNP, saved, t2l);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -262,30 +268,64 @@ __device__ __forceinline__ void bdf_driver(Cvode<MT, Team>& cv, const DevMech& m
 }
 
 // One wavefront per reactor (nsp <= 63).  MT = double: plain FP64 method;
-// MT = float: FP32 preconditioner.  LDS: Mlds[kNP*kNP] MT, prow[kNP] MT,
-// rvec[kNP] MT, t2l[kNP] double, then C[nsp], G[nsp], rop[nstep+1] doubles.
+// MT = float: FP32 preconditioner.  LDS (bdf_lds_bytes): Mlds[N*N] MT
+// (N = nsp + 1, padded to 16 bytes), t2l[kNP], then C[nsp], G[nsp],
+// rop[nstep+1] doubles.
 template <typename MT>
-__global__ void __launch_bounds__(kWave)
-bdf_kernel(DevMech m, BdfOpts o, int n, double dt, const double* __restrict__ dts,
-           double* __restrict__ Tio,
-           double* __restrict__ Pio, double* __restrict__ Yio, int stride,
-           int* __restrict__ nsteps_out, int* __restrict__ stats_out,
-           const int* __restrict__ order, unsigned int* __restrict__ counter,
-           double* __restrict__ jscratch) {
+__device__ __forceinline__ void bdf_wave(DevMech m, BdfOpts o, int n, double dt,
+                                         const double* __restrict__ dts, double* __restrict__ Tio,
+                                         double* __restrict__ Pio, double* __restrict__ Yio,
+                                         int stride, int* __restrict__ nsteps_out,
+                                         int* __restrict__ stats_out,
+                                         const int* __restrict__ order,
+                                         unsigned int* __restrict__ counter,
+                                         double* __restrict__ jscratch) {
   extern __shared__ double lds[];
   const int nsp = m.nsp;
   CvWave<MT> cv;
   MT* lm = reinterpret_cast<MT*>(lds);
   cv.Mlds = lm;
-  cv.prow = lm + kNP * kNP;
-  cv.rvec = cv.prow + kNP;
-  cv.t2l = reinterpret_cast<double*>(cv.rvec + kNP);
+  cv.prow = nullptr;
+  cv.rvec = nullptr;
+  cv.t2l = reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + wave_matrix_bytes(nsp, sizeof(MT)));
   cv.hl = nullptr;
   double* rl = cv.t2l + kNP;
   cv.s = RhsLds{rl, rl + nsp, rl + 2 * nsp};
   cv.savedJ = reinterpret_cast<MT*>(jscratch + (size_t)blockIdx.x * kNP * kNP);
   bdf_driver(cv, m, o, n, dt, dts, Tio, Pio, Yio, stride, nsteps_out, stats_out, order, counter);
 }
+
+// Minimum waves per SIMD each instantiation's register allocation targets
+// (build-time tuning knobs).  The FP32 kernel's LDS (N*N floats + the rate
+// arrays, about 18 KB at 53 species) allows 2 waves per SIMD, and holding
+// registers to that budget is faster than one unconstrained wave; the FP64
+// kernel's matrix (about 23 KB) does not leave room for a second wave.
+#ifndef ZRK_BDF_F64_WAVES_PER_EU
+#define ZRK_BDF_F64_WAVES_PER_EU 1
+#endif
+#ifndef ZRK_BDF_F32_WAVES_PER_EU
+#define ZRK_BDF_F32_WAVES_PER_EU 2
+#endif
+#define ZRK_BDF_KERNEL_ARGS                                                                     \
+  DevMech m, BdfOpts o, int n, double dt, const double* __restrict__ dts,                      \
+      double* __restrict__ Tio, double* __restrict__ Pio, double* __restrict__ Yio, int stride, \
+      int* __restrict__ nsteps_out, int* __restrict__ stats_out, const int* __restrict__ order, \
+      unsigned int* __restrict__ counter, double* __restrict__ jscratch
+template <typename MT>
+__global__ void bdf_kernel(ZRK_BDF_KERNEL_ARGS);
+template <>
+__global__ void __launch_bounds__(kWave, ZRK_BDF_F64_WAVES_PER_EU)
+bdf_kernel<double>(ZRK_BDF_KERNEL_ARGS) {
+  bdf_wave<double>(m, o, n, dt, dts, Tio, Pio, Yio, stride, nsteps_out, stats_out, order, counter,
+                   jscratch);
+}
+template <>
+__global__ void __launch_bounds__(kWave, ZRK_BDF_F32_WAVES_PER_EU)
+bdf_kernel<float>(ZRK_BDF_KERNEL_ARGS) {
+  bdf_wave<float>(m, o, n, dt, dts, Tio, Pio, Yio, stride, nsteps_out, stats_out, order, counter,
+                  jscratch);
+}
+#undef ZRK_BDF_KERNEL_ARGS
 
 // W wavefronts per reactor (64 <= nsp < 64W), FP32 preconditioner.  LDS
 // (bdf_team_lds_bytes): Mlds[(nsp+1)*(64W+1)] float, prow[64W], rvec[64W]
@@ -384,10 +424,9 @@ size_t bdf_slot_doubles(const DevMech& m, int W) {
 }
 
 size_t bdf_lds_bytes(const DevMech& m, bool f32, int W) {
-  if (W <= 1) {
-    const size_t mt = f32 ? sizeof(float) : sizeof(double);
-    return mt * (size_t)(kNP * kNP + 2 * kNP) + sizeof(double) * kNP + rhs_lds_bytes(m);
-  }
+  if (W <= 1)
+    return wave_matrix_bytes(m.nsp, f32 ? sizeof(float) : sizeof(double)) + sizeof(double) * kNP +
+           rhs_lds_bytes(m);
   const size_t L = (size_t)kWave * W;
   const size_t mat = ((size_t)(m.nsp + 1) * (L + 1) * sizeof(float) + 15) & ~(size_t)15;
   return mat + 2 * L * sizeof(float) + 2 * L * sizeof(double) +

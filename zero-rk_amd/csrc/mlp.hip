@@ -206,7 +206,7 @@ accept_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ dts, doubl
       double fy;
       bad = rhs64(m, s, Pr, 0.0, o.Tref, o.step_limiter, y1, fy, aux);
       if (!bad) {
-        jacobian_lds<double>(m, s, Pr, 0.0, o, y1, fy, aux.inv_rho, Ml, saved, t2l);
+        jacobian_lds<double>(m, s, Pr, 0.0, o, y1, fy, aux.inv_rho, Ml, kNP, saved, t2l);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -218,7 +218,7 @@ accept_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ dts, doubl
           if (c == lane) v += 1.0;
           R[c] = v;
         }
-        const bool ok = gj_invert<double>(R, N, prow);
+        const bool ok = gj_invert<double>(R, N);
 #pragma unroll
         for (int c = 0; c < kNP; ++c) Ml[c * kNP + lane] = R[c];
         if (!ok) bad = 1;
@@ -232,8 +232,9 @@ accept_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ dts, doubl
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         double d = 0.0;
+        const int rot = gj_rotation(N);  // register c held logical column (c + rot) mod 64
 #pragma unroll 8
-        for (int c = 0; c < kNP; ++c) d = fma(Ml[c * kNP + lane], rvec[(c + N) & (kNP - 1)], d);
+        for (int c = 0; c < kNP; ++c) d = fma(Ml[c * kNP + lane], rvec[(c + rot) & (kNP - 1)], d);
         __builtin_amdgcn_wave_barrier();
         if (act) y1 -= d;
       }
