@@ -187,8 +187,8 @@ __device__ __forceinline__ void thermo_one(const double* __restrict__ c, const T
       a[6];
 }
 
-__device__ __forceinline__ int lo16(int w) { return (int)(short)(w & 0xFFFF); }
-__device__ __forceinline__ int hi16(int w) { return w >> 16; }
+__device__ __forceinline__ int lo16(int w) { return w & 0xFFFF; }
+__device__ __forceinline__ int hi16(int w) { return (int)((unsigned)w >> 16); }
 
 // One reaction record as loaded from HBM (all fields issued together).
 struct RecLoad {
@@ -210,10 +210,14 @@ __device__ __forceinline__ RecLoad load_rec(const DevMech& m, int r) {
 }
 
 // Rate constants, limiter and rates of progress of every step of one reactor.
-// C, G: species concentrations and g/RT in the wave's LDS; rop: [nstep+1] LDS
-// (slot nstep is the zero rate the padded gathers point at).
+// C, G: species concentrations and g/RT in the wave's LDS, with the entries
+// C[nsp] = 1 and G[nsp] = 0 that absent participants (index nsp) read;
+// rop: [nstep+1] LDS (slot nstep is the zero rate the padded gathers point at).
 // Thread t of a team of S threads handles records t, t+S, ...; the next record
-// is loaded while the current one is evaluated.
+// is loaded while the current one is evaluated, and a record's third-body
+// and fall-off data are requested before its rate constants are computed.
+// Records come fall-off first, then third-body, then plain, so the costly
+// fall-off branch is taken by as few wavefront iterations as possible.
 template <int S>
 __device__ __forceinline__ void team_rates(int tid, const DevMech& m, const double* __restrict__ C,
                                            const double* __restrict__ G, double* __restrict__ rop,
@@ -224,10 +228,24 @@ __device__ __forceinline__ void team_rates(int tid, const DevMech& m, const doub
   int r = tid;
   RecLoad nx;
   if (r < nrec) nx = load_rec(m, r);
+  const int4* __restrict__ es = reinterpret_cast<const int4*>(m.e_sp);
+  const double2* __restrict__ ee = reinterpret_cast<const double2*>(m.e_eff);
   for (; r < nrec; r += S) {
     const RecLoad x = nx;
-    if (r + S < nrec) nx = load_rec(m, r + S);
     const int fl = x.meta.z;
+    const bool enh = (fl & (kRecThirdBody | kRecFalloff)) && x.meta.w < 0 && x.e1 > x.e0;
+    // The loads below are issued on every path (from valid dummy addresses
+    // where they are not needed) and in this order: the vector-memory counter
+    // drains in order, so with path-independent counts a wait for this
+    // record's data does not also wait for the next record's.
+    const int e0 = enh ? x.e0 : 0;
+    const int4 es0 = es[e0 >> 2];
+    const double2 ea0 = ee[e0 >> 1], eb0 = ee[(e0 >> 1) + 1];
+    const double* __restrict__ qp = m.r_fo + 7 * ((fl & kRecFalloff) ? r : 0);
+    double q[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) q[k] = qp[k];
+    nx = load_rec(m, (r + S < nrec) ? r + S : r);
     const int re[kMaxPart] = {lo16(x.sp.x), hi16(x.sp.x), lo16(x.sp.y), hi16(x.sp.y)};
     const int pr[kMaxPart] = {lo16(x.sp.z), hi16(x.sp.z), lo16(x.sp.w), hi16(x.sp.w)};
     double kf = exp(x.c0.x + x.c0.y * p.logT - x.c1.x * p.invT);
@@ -235,13 +253,12 @@ __device__ __forceinline__ void team_rates(int tid, const DevMech& m, const doub
     if (fl & kRecRevArr) {
       kr = exp(x.c1.y + x.c2.x * p.logT - x.c2.y * p.invT);
     } else if (fl & kRecRevKeq) {
+      // absent participants (listed last) read the 0 in G[nsp]: s +- 0 == s
       double s = 0.0;
 #pragma unroll
-      for (int k = 0; k < kMaxPart; ++k)
-        if (pr[k] >= 0) s += G[pr[k]];
+      for (int k = 0; k < kMaxPart; ++k) s += G[pr[k]];
 #pragma unroll
-      for (int k = 0; k < kMaxPart; ++k)
-        if (re[k] >= 0) s -= G[re[k]];
+      for (int k = 0; k < kMaxPart; ++k) s -= G[re[k]];
       kr = exp(s - x.c1.y * p.log_patm_rut) * kf;
     }
     if (fl & (kRecThirdBody | kRecFalloff)) {
@@ -250,14 +267,27 @@ __device__ __forceinline__ void team_rates(int tid, const DevMech& m, const doub
       if (fs >= 0) {
         cm = C[fs];
       } else {
+        // list order; padding entries (species 0, efficiency 0) add +-0
         cm = Csum;
-        for (int k = x.e0; k < x.e1; ++k) cm += C[m.e_sp[k]] * m.e_eff[k];
+        if (enh) {
+          cm += C[es0.x] * ea0.x;
+          cm += C[es0.y] * ea0.y;
+          cm += C[es0.z] * eb0.x;
+          cm += C[es0.w] * eb0.y;
+          for (int k = (x.e0 >> 2) + 1; k < (x.e1 >> 2); ++k) {
+            const int4 sk = es[k];
+            const double2 a = ee[2 * k], b = ee[2 * k + 1];
+            cm += C[sk.x] * a.x;
+            cm += C[sk.y] * a.y;
+            cm += C[sk.z] * b.x;
+            cm += C[sk.w] * b.y;
+          }
+        }
       }
       if (fl & kRecThirdBody) {
         kf *= cm;
         kr *= cm;
       } else {
-        const double* q = m.r_fo + 7 * r;
         const double klow = exp(q[0] + q[1] * p.logT - q[2] * p.invT);
         double Pr = klow * cm / kf;
         if (Pr < 1.0e-300) Pr = 1.0e-300;
@@ -286,15 +316,14 @@ __device__ __forceinline__ void team_rates(int tid, const DevMech& m, const doub
       kf *= L / (L + kf);
       kr *= L / (L + kr);
     }
+    // absent participants read the 1 in C[nsp]: kf * 1 == kf
 #pragma unroll
-    for (int k = 0; k < kMaxPart; ++k)
-      if (re[k] >= 0) kf *= C[re[k]];
+    for (int k = 0; k < kMaxPart; ++k) kf *= C[re[k]];
     rop[x.meta.x] = kf;
     const int rs = x.meta.y;
     if (rs >= 0) {
 #pragma unroll
-      for (int k = 0; k < kMaxPart; ++k)
-        if (pr[k] >= 0) kr *= C[pr[k]];
+      for (int k = 0; k < kMaxPart; ++k) kr *= C[pr[k]];
       rop[rs] = kr;
     }
   }
@@ -353,16 +382,24 @@ __device__ __forceinline__ void species_cd(const DevMech& m, const double* __res
 // mass heat capacity (cp or cv, J/kg/K) the Jacobian's temperature row uses.
 // ---------------------------------------------------------------------------
 struct RhsLds {
-  double* C;    // [>= nsp]
-  double* G;    // [>= nsp]
-  double* rop;  // [>= nstep]
+  double* C;    // [nsp + 1]: slot nsp is 1 (the factor of an absent participant)
+  double* G;    // [nsp + 1]: slot nsp is 0 (the g/RT of an absent participant)
+  double* rop;  // [nstep + 1]
 };
+
+// The three arrays laid out from one LDS base (2 (nsp + 1) + nstep + 1 doubles).
+__device__ __forceinline__ RhsLds rhs_lds_at(double* base, int nsp) {
+  return RhsLds{base, base + nsp + 1, base + 2 * (nsp + 1)};
+}
 
 struct RhsAux {
   double inv_rho;  // 1/density of this evaluation
   double cp_mix;   // mixture cp or cv (J/kg/K)
   double h;        // lane: h/RT or u/RT of species lane
   double cpm;      // lane: cp or cv (J/kg/K) of species lane
+#ifdef ZRK_PHASE_CYCLES
+  long long cyc[3];  // thermo and sums, rates of progress, gathers and T row
+#endif
 };
 
 template <class Team>
@@ -372,6 +409,9 @@ __device__ __forceinline__ int rhs_team(const Team& team, const DevMech& m, cons
   const int lane = team.tid();
   const int nsp = m.nsp;
   const bool sp = lane < nsp;
+#ifdef ZRK_PHASE_CYCLES
+  const long long tc0 = (long long)__builtin_amdgcn_s_memtime();
+#endif
   const double yT = team.from(y, nsp);
   double T = yT * Tref;
   if (T <= 0.0) {
@@ -402,11 +442,21 @@ __device__ __forceinline__ int rhs_team(const Team& team, const DevMech& m, cons
     s.C[lane] = Cj;
     s.G[lane] = g;
   }
+  if (lane == nsp) {
+    s.C[nsp] = 1.0;
+    s.G[nsp] = 0.0;
+  }
   const double cp_mix = team.sum(cpm * Yj);
   const double Csum = team.sum(Cj);
   team.sync();
+#ifdef ZRK_PHASE_CYCLES
+  const long long tc1 = (long long)__builtin_amdgcn_s_memtime();
+#endif
   team_rates<Team::kThreads>(lane, m, s.C, s.G, s.rop, p, Csum, L);
   team.sync();
+#ifdef ZRK_PHASE_CYCLES
+  const long long tc2 = (long long)__builtin_amdgcn_s_memtime();
+#endif
   double e = 0.0;
   double yd = 0.0;
   if (sp) {
@@ -424,6 +474,12 @@ __device__ __forceinline__ int rhs_team(const Team& team, const DevMech& m, cons
   aux.h = h;
   aux.cpm = cpm;
   team.sync();
+#ifdef ZRK_PHASE_CYCLES
+  const long long tc3 = (long long)__builtin_amdgcn_s_memtime();
+  aux.cyc[0] = tc1 - tc0;
+  aux.cyc[1] = tc2 - tc1;
+  aux.cyc[2] = tc3 - tc2;
+#endif
   return 0;
 }
 
@@ -440,7 +496,7 @@ __device__ __noinline__ RhsRes rhs_wave_call(const DevMech* __restrict__ mp, int
                                              double v0, double Tref, double L, double y) {
   extern __shared__ double dyn_lds[];
   const DevMech m = *mp;
-  const RhsLds s{dyn_lds + off, dyn_lds + off + m.nsp, dyn_lds + off + 2 * m.nsp};
+  const RhsLds s = rhs_lds_at(dyn_lds + off, m.nsp);
   RhsRes r;
   RhsAux aux;
   r.flag = rhs_team(TeamWave{}, m, s, P, v0, Tref, L, y, r.ydot, aux);

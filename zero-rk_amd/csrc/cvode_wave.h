@@ -52,7 +52,7 @@ struct Cvode {
                                                        // Mlds for teams)
   Team team;
 #ifdef ZRK_PHASE_CYCLES
-  long long cyc[5];  // rhs, jacobian, inverse, solve, whole reactor
+  long long cyc[8];  // rhs, jacobian, inverse, solve, whole reactor; rhs parts
 #endif
   DevMech mech;
   RhsLds s;
@@ -96,6 +96,10 @@ struct Cvode {
     else
       fl = rhs_team(team, mech, s, P, v0, opt.Tref, opt.step_limiter, yy, yd, aux);
     ZRK_TOC(t0, 0);
+#ifdef ZRK_PHASE_CYCLES
+    if (!fl)
+      for (int k = 0; k < 3; ++k) cyc[5 + k] += aux.cyc[k];
+#endif
     if (!fl) inv_rho_last = aux.inv_rho;
     if (!active) yd = 0.0;
     return fl;

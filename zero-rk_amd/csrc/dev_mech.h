@@ -46,12 +46,12 @@ struct DevMech {
   const double* thermo;   // [nsp*16]
   // reaction records [nrec]
   const int4* r_meta;     // {forward step, reverse step or -1, flags, falloff species or -1}
-  const int4* r_sp;       // 16-bit species indices (-1 = none): {re0|re1, re2|re3, pr0|pr1, pr2|pr3}
+  const int4* r_sp;       // 16-bit species indices (nsp = none): {re0|re1, re2|re3, pr0|pr1, pr2|pr3}
   const double2* r_coef;  // [nrec*3]: {fwd lnA, beta}, {fwd Ta, rev lnA}, {rev beta, rev Ta};
                           // for a Keq reverse the reverse lnA slot holds ndelta
-  const int* r_eptr;      // [nrec+1] enhanced third-body list
-  const int* e_sp;
-  const double* e_eff;    // efficiency - 1
+  const int* r_eptr;      // [nrec+1] enhanced third-body list (multiples of 4)
+  const int* e_sp;        // in groups of 4 (read as int4); padding: species 0
+  const double* e_eff;    // efficiency - 1, in pairs (read as double2); padding: 0
   const double* r_fo;     // [nrec*7] falloff parameters
   // per-species gathers, padded to a common length so the 64 lanes of a
   // species block read one contiguous line per block of 8 entries:

@@ -36,11 +36,15 @@ rates_kernel(DevMech m, int n, const double* __restrict__ Tin, const double* __r
   const int wave = threadIdx.x / kWave;
   const int lane = lane_id();
   const int nsp = m.nsp, nstep = m.nstep;
-  const int per_wave = 3 * nsp + nstep + 1;
+  const int per_wave = 3 * nsp + 2 + nstep + 1;
   double* C = lds + wave * per_wave;
-  double* G = C + nsp;
-  double* H = G + nsp;
+  double* G = C + nsp + 1;
+  double* H = G + nsp + 1;
   double* rop = H + nsp;
+  if (lane == 0) {  // entries read by absent reaction participants
+    C[nsp] = 1.0;
+    G[nsp] = 0.0;
+  }
   for (int r = blockIdx.x * kRatesWaves + wave; r < n; r += gridDim.x * kRatesWaves) {
     const double* Y = Yin + (size_t)r * stride;
     const double T = fmin(Tin[r], kTlimit);
@@ -103,7 +107,7 @@ jacobian_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ Tin,
   double* Jl = lds;
   double* t2l = lds + kNP * kNP;
   double* rl = t2l + kNP;
-  RhsLds s{rl, rl + nsp, rl + 2 * nsp};
+  RhsLds s = rhs_lds_at(rl, nsp);
   double* saved = jscratch + (size_t)blockIdx.x * kNP * kNP;
   for (int r = blockIdx.x; r < n; r += gridDim.x) {
     double y = 0.0;
@@ -140,7 +144,7 @@ jacobian_team_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ Tin
   double* hl = t2l + L;
   team.red = hl + L;
   double* rl = team.red + ((W + 2) & ~1);
-  RhsLds s{rl, rl + nsp, rl + 2 * nsp};
+  RhsLds s = rhs_lds_at(rl, nsp);
   double* J = jscratch + (size_t)blockIdx.x * 2 * N * N;
   double* saved = J + (size_t)N * N;
   const int tid = team.tid();
@@ -206,7 +210,7 @@ __device__ __forceinline__ void bdf_driver(Cvode<MT, Team>& cv, const DevMech& m
     cv.rtol = o.rtol;
     cv.inv_rho_last = 0.0;
 #ifdef ZRK_PHASE_CYCLES
-    cv.cyc[0] = cv.cyc[1] = cv.cyc[2] = cv.cyc[3] = 0;
+    for (int k = 0; k < 8; ++k) cv.cyc[k] = 0;
     const long long tr0 = (long long)__builtin_amdgcn_s_memtime();
 #endif
     cv.reinit(0.0, y0);
@@ -250,12 +254,12 @@ __device__ __forceinline__ void bdf_driver(Cvode<MT, Team>& cv, const DevMech& m
       if (nsteps_out) nsteps_out[r] = nsteps;
       if (stats_out) {
 #ifdef ZRK_PHASE_CYCLES
-        // profiling layout: 16 ints per reactor, cycles / 16 in slots 8..11
+        // profiling layout: 16 ints per reactor, cycles / 16 in slots 8..15
 #pragma unroll
         for (int k = 0; k < 7; ++k) stats_out[(size_t)r * 16 + k] = tot[k];
         stats_out[(size_t)r * 16 + 7] = flag;
 #pragma unroll
-        for (int k = 0; k < 5; ++k) stats_out[(size_t)r * 16 + 8 + k] = (int)(cv.cyc[k] >> 4);
+        for (int k = 0; k < 8; ++k) stats_out[(size_t)r * 16 + 8 + k] = (int)(cv.cyc[k] >> 4);
 #else
 #pragma unroll
         for (int k = 0; k < 7; ++k) stats_out[(size_t)r * 8 + k] = tot[k];
@@ -290,7 +294,7 @@ __device__ __forceinline__ void bdf_wave(DevMech m, BdfOpts o, int n, double dt,
   cv.t2l = reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + wave_matrix_bytes(nsp, sizeof(MT)));
   cv.hl = nullptr;
   double* rl = cv.t2l + kNP;
-  cv.s = RhsLds{rl, rl + nsp, rl + 2 * nsp};
+  cv.s = rhs_lds_at(rl, nsp);
   cv.savedJ = reinterpret_cast<MT*>(jscratch + (size_t)blockIdx.x * kNP * kNP);
   bdf_driver(cv, m, o, n, dt, dts, Tio, Pio, Yio, stride, nsteps_out, stats_out, order, counter);
 }
@@ -354,7 +358,7 @@ bdf_team_kernel(DevMech m, BdfOpts o, int n, double dt, const double* __restrict
   double* red = cv.hl + L;
   cv.team.red = red;
   double* rl = red + ((W + 2) & ~1);
-  cv.s = RhsLds{rl, rl + nsp, rl + 2 * nsp};
+  cv.s = rhs_lds_at(rl, nsp);
   cv.savedJ = reinterpret_cast<float*>(jscratch + (size_t)blockIdx.x * slot_doubles);
   bdf_driver(cv, m, o, n, dt, dts, Tio, Pio, Yio, stride, nsteps_out, stats_out, order, counter);
 }
@@ -370,7 +374,7 @@ hipError_t launch_rates(const DevMech& m, int n, const double* T, const double* 
                         const double* Y, int stride, double L, double* ydot, double* rop,
                         double* create, double* destroy, double* net, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  const size_t lds = sizeof(double) * (size_t)kRatesWaves * (3 * m.nsp + m.nstep + 1);
+  const size_t lds = sizeof(double) * (size_t)kRatesWaves * (3 * m.nsp + 2 + m.nstep + 1);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   int blocks = (n + kRatesWaves - 1) / kRatesWaves;
   hipLaunchKernelGGL(rates_kernel, dim3(blocks), dim3(kWave * kRatesWaves), lds, st, m, n, T,
@@ -379,7 +383,7 @@ hipError_t launch_rates(const DevMech& m, int n, const double* T, const double* 
 }
 
 size_t rhs_lds_bytes(const DevMech& m) {
-  return sizeof(double) * (size_t)(2 * m.nsp + m.nstep + 1);
+  return sizeof(double) * (size_t)(2 * (m.nsp + 1) + m.nstep + 1);
 }
 hipError_t launch_jacobian(const DevMech& m, const BdfOpts& o, int n, const double* T,
                            const double* P, const double* Y, int stride, double* J,

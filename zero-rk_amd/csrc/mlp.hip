@@ -184,7 +184,7 @@ accept_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ dts, doubl
   double* rvec = prow + kNP;
   double* t2l = rvec + kNP;
   double* rl = t2l + kNP;
-  RhsLds s{rl, rl + nsp, rl + 2 * nsp};
+  RhsLds s = rhs_lds_at(rl, nsp);
   double* saved = jscratch + (size_t)blockIdx.x * kNP * kNP;
   const bool act = lane <= nsp;
   for (int r = blockIdx.x; r < n; r += gridDim.x) {

@@ -40,15 +40,19 @@ int build_device_mech(MechHandle* h, std::string* err) {
     inv_mw[j] = 1.0 / M.mw[j];
     ru_mw[j] = 8.314462618e3 / M.mw[j];
   }
-  // reaction records, grouped by kind (plain, third-body, falloff)
+  // reaction records, grouped by kind (falloff, third-body, plain)
   std::vector<int> order;
-  for (int pass = 0; pass < 3; ++pass)
+  for (int pass = 2; pass >= 0; --pass)
     for (int j = 0; j < nrxn; ++j) {
       const auto& r = M.rxns[j];
       const int kind = r.falloff ? 2 : (r.third_body ? 1 : 0);
       if (kind == pass) order.push_back(j);
     }
   const int nrec = (int)order.size();
+  if (nrec == 0) {
+    *err = "mechanism has no reactions";
+    return -1;
+  }
   if (nstep >= 0xFFFF || nsp >= 0x7FFF) {
     *err = "mechanism too large for 16-bit step/species indices";
     return -1;
@@ -93,7 +97,8 @@ int build_device_mech(MechHandle* h, std::string* err) {
       *err = "reaction " + std::to_string(j + 1) + " has more than 4 expanded participants";
       return -1;
     }
-    int re[kMaxPart] = {-1, -1, -1, -1}, pr[kMaxPart] = {-1, -1, -1, -1};
+    // absent participants: index nsp (the rate kernels read 1 and 0 there)
+    int re[kMaxPart] = {nsp, nsp, nsp, nsp}, pr[kMaxPart] = {nsp, nsp, nsp, nsp};
     for (int k = 0; k < nr; ++k) re[k] = M.reac_sp[M.reac_ptr[s] + k];
     for (int k = 0; k < np; ++k) pr[k] = M.prod_sp[M.prod_ptr[s] + k];
     spk[(size_t)q * 4 + 0] = pack16(re[0], re[1]);
@@ -104,7 +109,15 @@ int build_device_mech(MechHandle* h, std::string* err) {
       esp.push_back(M.tb_sp[k]);
       eeff.push_back(M.tb_eff[k]);
     }
+    while (esp.size() % 4) {  // groups of 4: species 0 with efficiency 0 adds +-0
+      esp.push_back(0);
+      eeff.push_back(0.0);
+    }
     eptr[q + 1] = (int)esp.size();
+  }
+  if (esp.empty()) {  // one padding group: the rate loop always reads group 0
+    esp.assign(4, 0);
+    eeff.assign(4, 0.0);
   }
   // per-species gather lists in step order (reference summation order),
   // padded with the zero-rate slot nstep to a common length per 64-species block
