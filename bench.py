@@ -1,7 +1,7 @@
 #!/usr/bin/env python
 """Benchmark of the batched reactor-chemistry path on one MI355X.
 
-Workload (one "step" = one CFD step of the plugin path): 65,536
+Workload (one "step" = one CFD step of the plugin path), by default: 65,536
 constant-pressure reactors of the seeded synthetic 53-species / 325-reaction
 mechanism (C, H, O, N, Ar; 25 % third-body, 15 % Troe fall-off), T bimodal
 (50 % U[700,950] K, 50 % U[1050,1400] K), P logU[1,50] atm, Y Dirichlet(0.5)
@@ -42,7 +42,11 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--reactors", type=int, default=65536)
+    p.add_argument("--workload", choices=["53sp", "10sp", "160sp"], default="53sp",
+                   help="53sp (default, the headline workload); 10sp and 160sp are the "
+                        "other seeded workloads, reported the same way")
+    p.add_argument("--reactors", type=int, default=0,
+                   help="batch size (default: the workload's own, 65,536 / 1,024 / 16,384)")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--copy-bytes", type=int, default=0,
@@ -93,7 +97,7 @@ def cpu_baseline(mech, therm, st, dt, rtol, atol, seconds):
     _, _, stats = o.integrate(st.T[idx], st.P[idx], st.Y[idx], dt, opts, nthreads=threads)
     el = time.perf_counter() - t0
     return {"value": n / el, "unit": "reactors/s", "cores": threads, "kind": "port",
-            "sample": f"{n} evenly spaced reactors of the same 65,536-reactor batch, "
+            "sample": f"{n} evenly spaced reactors of the same {len(st.T):,}-reactor batch, "
                       f"{el:.1f} s on {threads} threads (OpenMP, one reactor per thread)",
             "failed": int(np.sum(stats["nsteps"] < 0))}
 
@@ -116,14 +120,16 @@ def main():
     from zerork_hip import capi, mechgen, workcount
 
     dev = torch.device("cuda:0")
-    mech = os.path.join(ROOT, "tests", "golden", "mech", "synth_53sp.mech")
-    therm = os.path.join(ROOT, "tests", "golden", "mech", "synth_53sp.therm")
+    wl = a.workload
+    mech = os.path.join(ROOT, "tests", "golden", "mech", f"synth_{wl}.mech")
+    therm = os.path.join(ROOT, "tests", "golden", "mech", f"synth_{wl}.therm")
     m = capi.Mech(mech, therm)
-    mg = mechgen.workload_mechanism("53sp")
-    n = a.reactors
-    st = mechgen.workload_states("53sp", mg, n, seed=2)
-    dt, rtol, atol = 1.0e-4, 1.0e-6, 1.0e-12
+    mg = mechgen.workload_mechanism(wl)
+    n = a.reactors or mechgen.WORKLOAD_NR[wl]
+    st = mechgen.workload_states(wl, mg, n, seed=2)
+    dt, rtol, atol = mechgen.WORKLOAD_DT[wl], 1.0e-6, 1.0e-12
     nsp = m.nsp
+    has_plain = nsp <= 63  # above 63 species the integrator runs the FP32 method only
     t = lambda x: torch.as_tensor(np.ascontiguousarray(x), dtype=torch.float64, device=dev)
     T0, P0, Y0 = t(st.T), t(st.P), t(st.Y)
     T, P, Y = T0.clone(), P0.clone(), Y0.clone()
@@ -133,14 +139,22 @@ def main():
     sp = stream.cuda_stream
     W = workcount.Work(m)
 
-    def run(fp32):
-        """warmup + K timed steps of one method; returns its measurements."""
+    def run(fp32, Tstart=None):
+        """warmup + K timed steps of one method; returns its measurements.
+        Tstart: only integrate once from these temperatures (no timing)."""
         opts = capi.default_options(rel_tol=rtol, abs_tol=atol, fp32_preconditioner=fp32)
 
         def restore():
             T.copy_(T0)
             P.copy_(P0)
             Y.copy_(Y0)
+
+        if Tstart is not None:
+            restore()
+            T.copy_(Tstart)
+            m.integrate(n, dt, T, P, Y, nsp, opts, steps, None, stream=sp)
+            torch.cuda.synchronize()
+            return {"T": T.cpu().numpy(), "Y": Y.cpu().numpy()}
 
         for _ in range(max(1, a.warmup)):
             restore()
@@ -185,19 +199,35 @@ def main():
                 "T": Tend, "Y": Yend, "waves": m.resident_waves(opts)}
 
     log(f"workload ready: {n} reactors, {nsp} species; warmup {a.warmup}, steps {a.steps}")
-    plain = run(0)
-    log(f"plain FP64: {plain['ms_per_step']:.2f} ms/step")
+    plain = run(0) if has_plain else None
+    if plain:
+        log(f"plain FP64: {plain['ms_per_step']:.2f} ms/step")
     chosen = run(1)
     log(f"FP32 preconditioner: {chosen['ms_per_step']:.2f} ms/step")
-    # end-state agreement of the two methods, in tolerance units of the plain one
-    du = np.maximum(np.max(np.abs(chosen["Y"] - plain["Y"]) / (rtol * np.abs(plain["Y"]) + atol),
-                           axis=1), np.abs(chosen["T"] - plain["T"]) / (rtol * plain["T"]))
+
+    def tol_units(A, B):
+        """per-reactor max difference of two end states in tolerance units of B"""
+        return np.maximum(np.max(np.abs(A["Y"] - B["Y"]) / (rtol * np.abs(B["Y"]) + atol), axis=1),
+                          np.abs(A["T"] - B["T"]) / (rtol * B["T"]))
+
+    def quantiles(u):
+        return {"p50": round(float(np.percentile(u, 50)), 3),
+                "p99": round(float(np.percentile(u, 99)), 3), "max": round(float(u.max()), 3)}
+
+    agreement = sensitivity = None
+    if plain:
+        # end-state agreement of the two methods, in tolerance units of the plain
+        # one, beside the plain method's own spread under a one-ulp change of T
+        agreement = quantiles(tol_units(chosen, plain))
+        sensitivity = quantiles(tol_units(run(0, Tstart=torch.nextafter(
+            T0, torch.full_like(T0, float("inf")))), plain))
     value = n / (chosen["ms_per_step"] / 1e3)
     achieved = chosen["ops"] / (chosen["kern_ms"] / 1e3) / 1e12
-    traffic = pmc_traffic(os.path.join(ROOT, "profiles", "bdf_kernel_53sp_65536_pmc.md"))
+    traffic = pmc_traffic(os.path.join(ROOT, "profiles", f"bdf_kernel_{wl}_{n}_pmc.md"))
     roofline = {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
-                "traffic": traffic, "kernel": "bdf_kernel<float>",
+                "traffic": traffic,
+                "kernel": "bdf_kernel<float>" if nsp <= 63 else "bdf_team_kernel",
                 "kernel_ms": round(chosen["kern_ms"], 3),
                 "fp64_ops_per_launch": int(chosen["ops"]),
                 "resident_waves": chosen["waves"]}
@@ -232,23 +262,23 @@ def main():
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(chosen["ms_per_step"], 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded mechanism and states)",
-        "config": {"workload": "53 species / 325 reactions x 65,536 constant-pressure reactors, "
-                               "dt 1e-4 s, rtol 1e-6, atol 1e-12; chosen method = FP32 "
-                               "preconditioner (FP64 rates, residual, error test, state)",
+        "config": {"workload": f"{nsp} species / {m.nrxn} reactions x {n:,} constant-pressure "
+                               f"reactors, dt {dt:g} s, rtol 1e-6, atol 1e-12; chosen method = "
+                               "FP32 preconditioner (FP64 rates, residual, error test, state)",
                    "reactors": n, "species": nsp, "steps_per_reaction_set": m.nstep,
                    "method": "fp32_preconditioner", "failed_reactors": chosen["failed"],
                    "mean_bdf_steps": round(chosen["mean_steps"], 1)},
-        "plain_fp64": {"value": round(n / (plain["ms_per_step"] / 1e3), 1),
-                       "ms_per_step": round(plain["ms_per_step"], 3),
-                       "kernel_ms": round(plain["kern_ms"], 3),
-                       "fp64_ops_per_launch": int(plain["ops"]),
-                       "tflops": round(plain["ops"] / (plain["kern_ms"] / 1e3) / 1e12, 4),
-                       "failed_reactors": plain["failed"],
-                       "mean_bdf_steps": round(plain["mean_steps"], 1)},
-        "chosen_vs_plain": round(plain["ms_per_step"] / chosen["ms_per_step"], 3),
-        "chosen_vs_plain_end_state_tol_units": {"p50": round(float(np.percentile(du, 50)), 3),
-                                                "p99": round(float(np.percentile(du, 99)), 3),
-                                                "max": round(float(du.max()), 3)},
+        "plain_fp64": None if not plain else {
+            "value": round(n / (plain["ms_per_step"] / 1e3), 1),
+            "ms_per_step": round(plain["ms_per_step"], 3),
+            "kernel_ms": round(plain["kern_ms"], 3),
+            "fp64_ops_per_launch": int(plain["ops"]),
+            "tflops": round(plain["ops"] / (plain["kern_ms"] / 1e3) / 1e12, 4),
+            "failed_reactors": plain["failed"],
+            "mean_bdf_steps": round(plain["mean_steps"], 1)},
+        "chosen_vs_plain": round(plain["ms_per_step"] / chosen["ms_per_step"], 3) if plain else None,
+        "chosen_vs_plain_end_state_tol_units": agreement,
+        "plain_fp64_one_ulp_T_sensitivity_tol_units": sensitivity,
         "roofline": roofline,
         "rate_kernel": rate,
         "cpu_baseline": cpu,

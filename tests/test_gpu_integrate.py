@@ -104,8 +104,11 @@ def test_integrate_matches_oracle(torch_gpu, oracle_mod, wl, n, seed, cvol, f32)
     dsteps = abs(pst["nsteps"].mean() / ost["nsteps"].mean() - 1.0)
     assert abs(steps.mean() / ost["nsteps"].mean() - 1.0) < (0.05 if f32 else 0.02) + 2.0 * dsteps
     # sum(Y) drifts by round-off (and, with FP32 Newton matrices, by their
-    # rounding of the mass-conserving corrections)
-    assert np.all(np.abs(Yg.sum(axis=1) - 1.0) <= (1e-8 if f32 else 1e-9))
+    # rounding of the mass-conserving corrections); the restated CVODE does not
+    # renormalise either, so the bar is a fixed margin or twice the oracle's own
+    # drift on these inputs and their one-ulp perturbation, whichever is larger
+    osum = max(np.abs(Yo.sum(axis=1) - 1.0).max(), np.abs(Yp.sum(axis=1) - 1.0).max())
+    assert np.all(np.abs(Yg.sum(axis=1) - 1.0) <= max(1e-8 if f32 else 1e-9, 2.0 * osum))
     if cvol:
         # P = rho Ru T / W: the same tolerance-unit comparison on the pressure
         pu = np.abs(Pg - ost["P"]) / (RTOL * ost["P"])

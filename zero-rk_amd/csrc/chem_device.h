@@ -209,13 +209,20 @@ __device__ __forceinline__ RecLoad load_rec(const DevMech& m, int r) {
   return x;
 }
 
+// Build-time knob: 1 loads the next record while the current one is
+// evaluated (hides the load latency at the cost of about 22 vector registers;
+// measured no faster in the integrator at two waves per SIMD, so off).
+#ifndef ZRK_RATES_PREFETCH
+#define ZRK_RATES_PREFETCH 0
+#endif
+
 // Rate constants, limiter and rates of progress of every step of one reactor.
 // C, G: species concentrations and g/RT in the wave's LDS, with the entries
 // C[nsp] = 1 and G[nsp] = 0 that absent participants (index nsp) read;
 // rop: [nstep+1] LDS (slot nstep is the zero rate the padded gathers point at).
-// Thread t of a team of S threads handles records t, t+S, ...; the next record
-// is loaded while the current one is evaluated, and a record's third-body
-// and fall-off data are requested before its rate constants are computed.
+// Thread t of a team of S threads handles records t, t+S, ...; a record's
+// third-body and fall-off data are requested before its rate constants are
+// computed.
 // Records come fall-off first, then third-body, then plain, so the costly
 // fall-off branch is taken by as few wavefront iterations as possible.
 template <int S>
@@ -226,12 +233,18 @@ __device__ __forceinline__ void team_rates(int tid, const DevMech& m, const doub
   const int nrec = m.nrec;
   if (tid == 0) rop[m.nstep] = 0.0;
   int r = tid;
+#if ZRK_RATES_PREFETCH
   RecLoad nx;
   if (r < nrec) nx = load_rec(m, r);
+#endif
   const int4* __restrict__ es = reinterpret_cast<const int4*>(m.e_sp);
   const double2* __restrict__ ee = reinterpret_cast<const double2*>(m.e_eff);
   for (; r < nrec; r += S) {
+#if ZRK_RATES_PREFETCH
     const RecLoad x = nx;
+#else
+    const RecLoad x = load_rec(m, r);
+#endif
     const int fl = x.meta.z;
     const bool enh = (fl & (kRecThirdBody | kRecFalloff)) && x.meta.w < 0 && x.e1 > x.e0;
     // The loads below are issued on every path (from valid dummy addresses
@@ -245,7 +258,9 @@ __device__ __forceinline__ void team_rates(int tid, const DevMech& m, const doub
     double q[7];
 #pragma unroll
     for (int k = 0; k < 7; ++k) q[k] = qp[k];
+#if ZRK_RATES_PREFETCH
     nx = load_rec(m, (r + S < nrec) ? r + S : r);
+#endif
     const int re[kMaxPart] = {lo16(x.sp.x), hi16(x.sp.x), lo16(x.sp.y), hi16(x.sp.y)};
     const int pr[kMaxPart] = {lo16(x.sp.z), hi16(x.sp.z), lo16(x.sp.w), hi16(x.sp.w)};
     double kf = exp(x.c0.x + x.c0.y * p.logT - x.c1.x * p.invT);

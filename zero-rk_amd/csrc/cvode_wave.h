@@ -39,6 +39,16 @@ enum {
 #define ZRK_TOC(t, i)
 #endif
 
+// The integrator's uniform FP64 state, one block per wavefront in LDS.
+struct CvUni {
+  double P, v0, rtol, tn, h, hscale, hprime, eta, etamax, hmin, hmax_inv, hu;
+  double tau[kQmax + 2], tq[6], l[kQmax + 1];
+  double rl1, gamma, gammap, gamrat, crate, delp, acnrm, nlscoef, saved_tq5;
+  double etaq, etaqm1, etaqp1, inv_rho_last, inv_nvar;
+  double pad_;
+};
+static_assert(sizeof(CvUni) % 16 == 0, "CvUni keeps the LDS layout 16-byte aligned");
+
 // MT: precision of the Newton matrix, its inverse and the saved Jacobian
 // (double: the plain FP64 method; float: the FP32 preconditioner, with the
 // residual, error test and state update still in FP64).
@@ -52,7 +62,8 @@ struct Cvode {
                                                        // Mlds for teams)
   Team team;
 #ifdef ZRK_PHASE_CYCLES
-  long long cyc[8];  // rhs, jacobian, inverse, solve, whole reactor; rhs parts
+  long long cyc[12];  // rhs, jacobian, inverse, solve, whole reactor; rhs parts (3);
+                      // predict+set, nonlinear solve, error test, step completion
 #endif
   DevMech mech;
   RhsLds s;
@@ -65,21 +76,33 @@ struct Cvode {
   double* t2l;     // LDS [kL]: Jacobian column factors
   double* hl;      // LDS [kL]: energies for the Jacobian's temperature row (teams of
                    // several wavefronts)
-  double P;
-  double v0;  // constant volume: fixed specific volume; 0 at constant pressure
   // lane vectors
   double zn[kQmax + 1];
   double ewt, acor, tempv, ftemp, y, atolv, delta;
   bool active;  // species lane or temperature lane
-  // uniform scalars
-  double rtol, tn, h, hscale, hprime, eta, etamax, hmin, hmax_inv, hu;
-  double tau[kQmax + 2], tq[6], l[kQmax + 1];
-  double rl1, gamma, gammap, gamrat, crate, delp, acnrm, nlscoef, saved_tq5;
-  double etaq, etaqm1, etaqp1, inv_rho_last;
+  // uniform FP64 scalars: kept in this wavefront's CvUni block in LDS (every
+  // lane holds the same values), so they occupy no vector registers across
+  // the rate evaluation and the matrix inversion
+  double &P;
+  double &v0;  // constant volume: fixed specific volume; 0 at constant pressure
+  double &rtol, &tn, &h, &hscale, &hprime, &eta, &etamax, &hmin, &hmax_inv, &hu;
+  double (&tau)[kQmax + 2];
+  double (&tq)[6];
+  double (&l)[kQmax + 1];
+  double &rl1, &gamma, &gammap, &gamrat, &crate, &delp, &acnrm, &nlscoef, &saved_tq5;
+  double &etaq, &etaqm1, &etaqp1, &inv_rho_last, &inv_nvar;
   int q, qprime, qwait, L;
   int nst, nstlp, nstlj, nfe, nje, nsetups, nni, ncfn, netf;
   int convfail, jcur, acnrmcur;
-  double inv_nvar;
+
+  __device__ explicit Cvode(CvUni* u)
+      : P(u->P), v0(u->v0), rtol(u->rtol), tn(u->tn), h(u->h), hscale(u->hscale),
+        hprime(u->hprime), eta(u->eta), etamax(u->etamax), hmin(u->hmin),
+        hmax_inv(u->hmax_inv), hu(u->hu), tau(u->tau), tq(u->tq), l(u->l), rl1(u->rl1),
+        gamma(u->gamma), gammap(u->gammap), gamrat(u->gamrat), crate(u->crate),
+        delp(u->delp), acnrm(u->acnrm), nlscoef(u->nlscoef), saved_tq5(u->saved_tq5),
+        etaq(u->etaq), etaqm1(u->etaqm1), etaqp1(u->etaqp1), inv_rho_last(u->inv_rho_last),
+        inv_nvar(u->inv_nvar) {}
 
   __device__ __forceinline__ double wrms(double x) const {
     const double p = active ? x * ewt : 0.0;
@@ -756,19 +779,27 @@ struct Cvode {
     double dsm = 0.0;
     if (nst > 0 && hprime != h) adjust_params();
     for (;;) {
+      ZRK_TIC(p0);
       predict();
       set();
+      ZRK_TOC(p0, 8);
+      ZRK_TIC(p1);
       nflag = nls(nflag);
+      ZRK_TOC(p1, 9);
       const int kflag = handle_nflag(nflag, saved_t, ncf);
       if (kflag == kPredictAgain) continue;
       if (kflag != kDoErrorTest) return kflag;
+      ZRK_TIC(p2);
       const int eflag = do_error_test(nflag, saved_t, nef, dsm);
+      ZRK_TOC(p2, 10);
       if (eflag == kTryAgain) continue;
       if (eflag != kCvSuccess) return eflag;
       break;
     }
+    ZRK_TIC(p3);
     complete_step();
     prepare_next_step(dsm);
+    ZRK_TOC(p3, 11);
     etamax = (nst <= kSmallNst) ? kEtamx2 : kEtamx3;
     acor *= tq[2];
     return kCvSuccess;

@@ -210,7 +210,7 @@ __device__ __forceinline__ void bdf_driver(Cvode<MT, Team>& cv, const DevMech& m
     cv.rtol = o.rtol;
     cv.inv_rho_last = 0.0;
 #ifdef ZRK_PHASE_CYCLES
-    for (int k = 0; k < 8; ++k) cv.cyc[k] = 0;
+    for (int k = 0; k < 12; ++k) cv.cyc[k] = 0;
     const long long tr0 = (long long)__builtin_amdgcn_s_memtime();
 #endif
     cv.reinit(0.0, y0);
@@ -254,12 +254,12 @@ __device__ __forceinline__ void bdf_driver(Cvode<MT, Team>& cv, const DevMech& m
       if (nsteps_out) nsteps_out[r] = nsteps;
       if (stats_out) {
 #ifdef ZRK_PHASE_CYCLES
-        // profiling layout: 16 ints per reactor, cycles / 16 in slots 8..15
+        // profiling layout: 20 ints per reactor, cycles / 16 in slots 8..19
 #pragma unroll
-        for (int k = 0; k < 7; ++k) stats_out[(size_t)r * 16 + k] = tot[k];
-        stats_out[(size_t)r * 16 + 7] = flag;
+        for (int k = 0; k < 7; ++k) stats_out[(size_t)r * 20 + k] = tot[k];
+        stats_out[(size_t)r * 20 + 7] = flag;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) stats_out[(size_t)r * 16 + 8 + k] = (int)(cv.cyc[k] >> 4);
+        for (int k = 0; k < 12; ++k) stats_out[(size_t)r * 20 + 8 + k] = (int)(cv.cyc[k] >> 4);
 #else
 #pragma unroll
         for (int k = 0; k < 7; ++k) stats_out[(size_t)r * 8 + k] = tot[k];
@@ -272,8 +272,8 @@ __device__ __forceinline__ void bdf_driver(Cvode<MT, Team>& cv, const DevMech& m
 }
 
 // One wavefront per reactor (nsp <= 63).  MT = double: plain FP64 method;
-// MT = float: FP32 preconditioner.  LDS (bdf_lds_bytes): Mlds[N*N] MT
-// (N = nsp + 1, padded to 16 bytes), t2l[kNP], then C[nsp], G[nsp],
+// MT = float: FP32 preconditioner.  LDS (bdf_lds_bytes): CvUni, Mlds[N*N] MT
+// (N = nsp + 1, padded to 16 bytes), t2l[kNP], then C[nsp+1], G[nsp+1],
 // rop[nstep+1] doubles.
 template <typename MT>
 __device__ __forceinline__ void bdf_wave(DevMech m, BdfOpts o, int n, double dt,
@@ -286,12 +286,12 @@ __device__ __forceinline__ void bdf_wave(DevMech m, BdfOpts o, int n, double dt,
                                          double* __restrict__ jscratch) {
   extern __shared__ double lds[];
   const int nsp = m.nsp;
-  CvWave<MT> cv;
-  MT* lm = reinterpret_cast<MT*>(lds);
-  cv.Mlds = lm;
+  CvWave<MT> cv(reinterpret_cast<CvUni*>(lds));
+  char* base = reinterpret_cast<char*>(lds) + sizeof(CvUni);
+  cv.Mlds = reinterpret_cast<MT*>(base);
   cv.prow = nullptr;
   cv.rvec = nullptr;
-  cv.t2l = reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + wave_matrix_bytes(nsp, sizeof(MT)));
+  cv.t2l = reinterpret_cast<double*>(base + wave_matrix_bytes(nsp, sizeof(MT)));
   cv.hl = nullptr;
   double* rl = cv.t2l + kNP;
   cv.s = rhs_lds_at(rl, nsp);
@@ -332,9 +332,10 @@ bdf_kernel<float>(ZRK_BDF_KERNEL_ARGS) {
 #undef ZRK_BDF_KERNEL_ARGS
 
 // W wavefronts per reactor (64 <= nsp < 64W), FP32 preconditioner.  LDS
-// (bdf_team_lds_bytes): Mlds[(nsp+1)*(64W+1)] float, prow[64W], rvec[64W]
-// float, t2l[64W], hl[64W], red[W+1] double, then C[nsp], G[nsp],
-// rop[nstep+1] doubles.  savedJ slot: slot_doubles doubles per workgroup.
+// (bdf_team_lds_bytes): CvUni[W] (one per wavefront), Mlds[(nsp+1)*(64W+1)]
+// float, prow[64W], rvec[64W] float, t2l[64W], hl[64W], red[W+1] double,
+// then C[nsp+1], G[nsp+1], rop[nstep+1] doubles.  savedJ slot: slot_doubles
+// doubles per workgroup.
 template <int W>
 __global__ void __launch_bounds__(kWave * W)
 bdf_team_kernel(DevMech m, BdfOpts o, int n, double dt, const double* __restrict__ dts,
@@ -347,8 +348,8 @@ bdf_team_kernel(DevMech m, BdfOpts o, int n, double dt, const double* __restrict
   constexpr int L = kWave * W;
   using CV = Cvode<float, TeamBlock<W>>;
   const int nsp = m.nsp;
-  CV cv;
-  char* p = reinterpret_cast<char*>(lds);
+  CV cv(reinterpret_cast<CvUni*>(lds) + threadIdx.x / kWave);
+  char* p = reinterpret_cast<char*>(lds) + W * sizeof(CvUni);
   cv.Mlds = reinterpret_cast<float*>(p);
   p += ((size_t)(nsp + 1) * CV::kJS * sizeof(float) + 15) & ~(size_t)15;
   cv.prow = reinterpret_cast<float*>(p);
@@ -429,11 +430,11 @@ size_t bdf_slot_doubles(const DevMech& m, int W) {
 
 size_t bdf_lds_bytes(const DevMech& m, bool f32, int W) {
   if (W <= 1)
-    return wave_matrix_bytes(m.nsp, f32 ? sizeof(float) : sizeof(double)) + sizeof(double) * kNP +
-           rhs_lds_bytes(m);
+    return sizeof(CvUni) + wave_matrix_bytes(m.nsp, f32 ? sizeof(float) : sizeof(double)) +
+           sizeof(double) * kNP + rhs_lds_bytes(m);
   const size_t L = (size_t)kWave * W;
   const size_t mat = ((size_t)(m.nsp + 1) * (L + 1) * sizeof(float) + 15) & ~(size_t)15;
-  return mat + 2 * L * sizeof(float) + 2 * L * sizeof(double) +
+  return W * sizeof(CvUni) + mat + 2 * L * sizeof(float) + 2 * L * sizeof(double) +
          (size_t)((W + 2) & ~1) * sizeof(double) + rhs_lds_bytes(m);
 }
 
