@@ -4,6 +4,7 @@
 // indices inside unrolled loops guarded by the wave-uniform order q, so they
 // stay in registers.  Return codes follow CVODE's.
 #pragma once
+#include <type_traits>
 
 #include "bdf_device.h"
 
@@ -58,6 +59,13 @@ template <typename MT, class Team = TeamWave>
 struct Cvode {
   static constexpr int kL = Team::kThreads;        // state slots = matrix row registers
   static constexpr bool kWaveTeam = (kL == kWave);
+  // teams invert their FP32 Newton matrix with the blocked MFMA Gauss-Jordan
+  // (build with -DZRK_TEAM_MFMA_INVERSE=0 for the element-wise LDS version)
+#ifndef ZRK_TEAM_MFMA_INVERSE
+#define ZRK_TEAM_MFMA_INVERSE 1
+#endif
+  static constexpr bool kUseMfmaInverse =
+      !kWaveTeam && ZRK_TEAM_MFMA_INVERSE && std::is_same<MT, float>::value;
   static constexpr int kJS = kWaveTeam ? kNP : kL + 1;  // column stride of savedJ (and of
                                                        // Mlds for teams)
   Team team;
@@ -489,7 +497,11 @@ struct Cvode {
         if (c == tid) v += 1.0;
         Mlds[c * kJS + tid] = (MT)v;
       }
-    const bool ok = gj_invert_lds<MT>(team, Mlds, kJS, N, prow);
+    bool ok;
+    if constexpr (kUseMfmaInverse)
+      ok = gj_invert_mfma(team, Mlds, kJS, N, prow);
+    else
+      ok = gj_invert_lds<MT>(team, Mlds, kJS, N, prow);
     ZRK_TOC(t1, 2);
     return ok ? 0 : 1;
   }
