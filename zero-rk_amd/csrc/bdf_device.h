@@ -367,7 +367,9 @@ __device__ __forceinline__ bool gj_invert16(float (&d)[16]) {
   for (int k = 0; k < 16; ++k) {
     const float piv = bcast(d[k], k);
     if (piv == 0.0f) ok = false;
-    const float rp = 1.0f / piv;
+    // hardware reciprocal (1 ulp): this inverse only preconditions the Newton
+    // iteration, whose residual and convergence test stay in FP64
+    const float rp = __builtin_amdgcn_rcpf(piv);
     const bool me = (lane == k);
     const float g = me ? 0.0f : d[k] * rp;
     if (me) s = rp;

@@ -66,6 +66,13 @@ struct Cvode {
 #endif
   static constexpr bool kUseMfmaInverse =
       !kWaveTeam && ZRK_TEAM_MFMA_INVERSE && std::is_same<MT, float>::value;
+  // one wavefront, FP32 matrix: the same blocked MFMA inverse on the compact
+  // N x N matrix in LDS (-DZRK_WAVE_MFMA_INVERSE=0: the in-register inverse)
+#ifndef ZRK_WAVE_MFMA_INVERSE
+#define ZRK_WAVE_MFMA_INVERSE 1
+#endif
+  static constexpr bool kWaveMfmaInverse =
+      kWaveTeam && ZRK_WAVE_MFMA_INVERSE && std::is_same<MT, float>::value;
   static constexpr int kJS = kWaveTeam ? kNP : kL + 1;  // column stride of savedJ (and of
                                                        // Mlds for teams)
   Team team;
@@ -441,6 +448,25 @@ struct Cvode {
       inv_rho_last = jacobian_lds<MT>(mech, s, P, v0, opt, ypred, fpred, inv_rho_last, Mlds, N,
                                       savedJ, t2l);
       ZRK_TOC(t0, 1);
+    }
+    if constexpr (kWaveMfmaInverse) {
+      ZRK_TIC(t1);
+      // M = I - gamma J into the compact LDS matrix (rows/columns < N), from
+      // the saved Jacobian (rewritten above by this lane when re-evaluated)
+      const double ng = -gamma;
+      team.sync();  // every lane is done with the Jacobian workspace
+      if (lane < N) {
+        for (int c = 0; c < N; ++c) {
+          double v = ng * (double)savedJ[c * kNP + lane];
+          if (c == lane) v += 1.0;
+          Mlds[c * N + lane] = (MT)v;
+        }
+      }
+      team.sync();
+      const bool ok = gj_invert_mfma(team, reinterpret_cast<float*>(Mlds), N, N,
+                                     reinterpret_cast<float*>(t2l));
+      ZRK_TOC(t1, 2);
+      return ok ? 0 : 1;
     }
     // the saved Jacobian (rewritten above by this lane when it was re-evaluated)
     MT R[kNP];
