@@ -108,6 +108,22 @@ ZRK_EXPORT int zrk_integrate_device(zrk_mech m, const zrk_bdf_options* o, int n,
                                     double* T, double* P, double* Y, int stride,
                                     int* nsteps, int* stats, void* stream);
 
+/* zrk_integrate_device with the plugin's source terms (each device array
+ * optional, null = none): dpdt[n] (Pa/s), esrc[n] (J/kg/s), ysrc[n*stride]
+ * (mass-fraction sources, 1/s, same layout as Y).  Replaces the source-term
+ * arguments of ReactorConstantPressureCPU / ReactorConstantVolumeCPU::
+ * InitializeState and GetTimeDerivative (reactor_constant_pressure_cpu.cpp:
+ * 15-133, reactor_constant_volume_cpu.cpp:15-130), which the manager feeds
+ * from the DPDT, E_SRC and Y_SRC aux fields (zerork_reactor_manager.cpp:
+ * 529-559, 909-1053).  P is updated to the end pressure (constant pressure:
+ * P + dpdt dt, or Ru T / (v Wmix) at the initial v with mass sources;
+ * constant volume: Ru T / (v Wmix) + dpdt dt). */
+ZRK_EXPORT int zrk_integrate_device_src(zrk_mech m, const zrk_bdf_options* o, int n, double dt,
+                                        double* T, double* P, double* Y, int stride,
+                                        const double* dpdt, const double* esrc,
+                                        const double* ysrc, int* nsteps, int* stats,
+                                        void* stream);
+
 /* Autotuner (no counterpart in the reference): times the integrator on the
  * given device batch (copies of T, P, Y; inputs unchanged) with each team
  * size that fits the mechanism and these options (1 wavefront per reactor,

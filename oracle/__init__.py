@@ -84,6 +84,11 @@ def _declare(L):
     L.zrko_jacobian.argtypes = [P, P, C.c_double, C.c_double, P, P, P]
     L.zrko_integrate_batch.argtypes = [P, P, C.c_int, C.c_double, P, P, P, C.c_int, P, C.c_int]
     L.zrko_integrate_batch.restype = C.c_int
+    L.zrko_integrate_batch_src.argtypes = [P, P, C.c_int, C.c_double, P, P, P, C.c_int, P,
+                                           C.c_int, P, P, P]
+    L.zrko_integrate_batch_src.restype = C.c_int
+    L.zrko_rhs_src.argtypes = [P, P, C.c_double, P, C.c_double, P, C.c_double, C.c_double, P, P]
+    L.zrko_rhs_src.restype = C.c_int
     L.zrko_dense_getrf.argtypes = [C.c_int, P, P]
     L.zrko_dense_getrf.restype = C.c_int
     L.zrko_dense_getrs.argtypes = [C.c_int, P, P, P]
@@ -181,17 +186,37 @@ class Oracle:
         self.L.zrko_jacobian(self.ptr(), C.byref(o), P, t, _p(y), _p(np.ascontiguousarray(fy)), _p(J))
         return J.reshape(N, N).T.copy()
 
-    def integrate(self, T, P, Y, dt, opts: Opts | None = None, nthreads=0):
+    def rhs_src(self, P0, y0, t, y, dpdt=0.0, esrc=0.0, ysrc=None, opts: Opts | None = None):
+        """RHS at time t of the reactor initialised at (P0, y0), with the
+        plugin's source terms (ysrc: nsp mass-fraction sources or None)."""
+        o = opts or default_opts()
+        y0 = np.ascontiguousarray(y0, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        ys = None if ysrc is None else np.ascontiguousarray(ysrc, dtype=np.float64)
+        yd = np.zeros_like(y)
+        flag = self.L.zrko_rhs_src(self.ptr(), C.byref(o), P0, _p(y0), t, _p(y), dpdt, esrc,
+                                   _p(ys) if ys is not None else None, _p(yd))
+        return yd, flag
+
+    def integrate(self, T, P, Y, dt, opts: Opts | None = None, nthreads=0, dpdt=None,
+                  esrc=None, ysrc=None):
         """Integrate reactors; returns (T_out, Y_out, stats), stats a dict of
-        per-reactor counters plus "P" (end pressure; changes at constant volume)."""
+        per-reactor counters plus "P" (end pressure).  dpdt[n], esrc[n] and
+        ysrc[n, nsp] are the plugin's optional source terms."""
         o = opts or default_opts()
         T = np.array(T, dtype=np.float64, copy=True)
         P = np.array(P, dtype=np.float64, copy=True)
         Y = np.array(Y, dtype=np.float64, copy=True, order="C")
         n = len(T)
         st = (Stats * n)()
-        self.L.zrko_integrate_batch(self.ptr(), C.byref(o), n, dt, _p(T), _p(P), _p(Y), Y.shape[1],
-                                    C.cast(st, C.c_void_p), nthreads)
+        arr = lambda a, shape: None if a is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(a, dtype=np.float64), shape))
+        dp, es, ys = arr(dpdt, (n,)), arr(esrc, (n,)), arr(ysrc, Y.shape)
+        self.L.zrko_integrate_batch_src(self.ptr(), C.byref(o), n, dt, _p(T), _p(P), _p(Y),
+                                        Y.shape[1], C.cast(st, C.c_void_p), nthreads,
+                                        _p(dp) if dp is not None else None,
+                                        _p(es) if es is not None else None,
+                                        _p(ys) if ys is not None else None)
         stats = {f: np.array([getattr(s, f) for s in st]) for f, _ in Stats._fields_}
         stats["P"] = P
         return T, Y, stats

@@ -195,7 +195,13 @@ typedef struct {
   double inv_density; /* carried between calls like the reference member */
   double v0;          /* constant volume: the fixed specific volume (1/density) */
   double cp_mix;      /* mass cp (constant pressure) or cv (constant volume) */
-  double *C, *h, *cpm, *G, *rop, *cr, *de, *net, *tmp1, *tmp2, *tmp3;
+  /* source terms (InitializeState, reactor_constant_pressure_cpu.cpp:15-42 and
+   * reactor_constant_volume_cpu.cpp:15-42): pressure rate, energy source,
+   * mass-fraction sources; e0 the initial mass enthalpy (constant pressure) or
+   * internal energy (constant volume), vinit the initial 1/density */
+  double dpdt, esrc, e0, vinit;
+  const double* ysrc;
+  double *C, *h, *cpm, *G, *rop, *cr, *de, *net, *esc, *tmp1, *tmp2, *tmp3;
 } reactor_t;
 
 static void reactor_init(reactor_t* r, const zrko_mech* m, const zrko_opts* o, double P) {
@@ -206,6 +212,8 @@ static void reactor_init(reactor_t* r, const zrko_mech* m, const zrko_opts* o, d
   r->inv_density = 0.0;
   r->v0 = 0.0;
   r->cp_mix = 0.0;
+  r->dpdt = r->esrc = r->e0 = r->vinit = 0.0;
+  r->ysrc = NULL;
   double* w = (double*)calloc((size_t)(8 * nsp + m->nstep + 3 * N), sizeof(double));
   r->C = w;
   r->h = w + nsp;
@@ -214,7 +222,7 @@ static void reactor_init(reactor_t* r, const zrko_mech* m, const zrko_opts* o, d
   r->cr = w + 4 * nsp;
   r->de = w + 5 * nsp;
   r->net = w + 6 * nsp;
-  /* 7*nsp: spare */
+  r->esc = w + 7 * nsp; /* scratch of the energy -> temperature solve */
   r->rop = w + 8 * nsp;
   r->tmp1 = r->rop + m->nstep;
   r->tmp2 = r->tmp1 + N;
@@ -230,26 +238,110 @@ static double inv_density_tpy(const zrko_mech* m, double T, double P, const doub
   return 1.0 / (P * (1.0 / mws) / (NIST_RU * T));
 }
 
-/* GetTimeDerivative, no sources: constant pressure
+/* Mass enthalpy (mechanism::getMassEnthalpyFromTY, mechanism.cpp:336-347) or,
+ * with cvol, internal energy (getMassIntEnergyFromTY, :355-367) of (T, y);
+ * w: nsp doubles of scratch. */
+static double mass_energy(const zrko_mech* m, int cvol, double T, const double* y, double* w) {
+  thermo_h(m, T, w);
+  double e = 0.0;
+  for (int j = 0; j < m->nsp; ++j) {
+    if (cvol) w[j] -= 1.0;
+    w[j] *= (NIST_RU / m->mw[j]) * T;
+    e += w[j] * y[j];
+  }
+  return e;
+}
+
+/* Mass cp, or with cv the mass cv (getMassCpFromTY / getMassCvFromTY,
+ * mechanism.cpp:408-441). */
+static double mass_heat_capacity(const zrko_mech* m, int cv, double T, const double* y,
+                                 double* w) {
+  thermo_cp(m, T, w);
+  double c = 0.0;
+  for (int j = 0; j < m->nsp; ++j) {
+    if (cv) w[j] -= 1.0;
+    w[j] *= NIST_RU / m->mw[j];
+    c += w[j] * y[j];
+  }
+  return c;
+}
+
+/* Temperature at which the mass enthalpy (cvol = 0) or internal energy
+ * (cvol = 1) of y equals E: mechanism::getTemperatureFromHY / FromEY
+ * (mechanism.cpp:477-547), including their choices of heat capacity (the HY
+ * iteration and its upper extrapolation use cv). */
+static double temperature_from_energy(const zrko_mech* m, int cvol, double E, const double* y,
+                                      double guess, double* w) {
+  const int max_it = 400;
+  const double tol = 1.0e-6, tmin = 100.0, tmax = 5000.0;
+  const double emin = mass_energy(m, cvol, tmin, y, w);
+  const double emax = mass_energy(m, cvol, tmax, y, w);
+  if (E < emin) return tmin - (emin - E) / mass_heat_capacity(m, cvol, tmin, y, w);
+  if (E > emax) return tmax - (emax - E) / mass_heat_capacity(m, 1, tmax, y, w);
+  double T = guess;
+  if (T < tmin || T > tmax) T = tmin + (tmax - tmin) / (emax - emin) * (E - emin);
+  for (int i = 0; i < max_it; ++i) {
+    const double Ei = mass_energy(m, cvol, T, y, w);
+    const double c = mass_heat_capacity(m, 1, T, y, w);
+    double d = (E - Ei) / c;
+    d = d < -100.0 ? -100.0 : (d > 100.0 ? 100.0 : d);
+    if (fabs(d) < tol || T + d == T) break;
+    T += d;
+  }
+  return T;
+}
+
+/* Mixture pressure at (T, v, y) (mechanism::getPressureFromTVY, mechanism.cpp:292-297) */
+static double pressure_tvy(const zrko_mech* m, double T, double v, const double* y) {
+  double mws = 0.0;
+  for (int j = 0; j < m->nsp; ++j) mws += y[j] * (1.0 / m->mw[j]);
+  return NIST_RU * T / (v * (1.0 / mws));
+}
+
+/* InitializeState's source-term part: y0 the initial state, P0 its pressure */
+static void reactor_set_sources(reactor_t* r, const double* y0, double P0, double dpdt,
+                                double esrc, const double* ysrc) {
+  const zrko_mech* m = r->m;
+  const int cvol = r->o->constant_volume;
+  const double T0 = y0[m->nsp] * r->o->Tref;
+  r->dpdt = dpdt;
+  r->esrc = esrc;
+  r->ysrc = ysrc;
+  r->vinit = inv_density_tpy(m, T0, P0, y0);
+  r->e0 = mass_energy(m, cvol, T0, y0, r->esc);
+  if (!cvol) r->inv_density = r->vinit;
+}
+
+/* GetTimeDerivative: constant pressure
  * (reactor_constant_pressure_cpu.cpp:68-133) recomputes the density from P
  * and uses h/RT and cp; constant volume (reactor_constant_volume_cpu.cpp:
  * 66-130) keeps the initial density and uses u/RT = h/RT - 1 and
  * cv_j = (cp_j/R - 1) R/MW_j (mechanism.cpp:381-386, 428-441). */
 static int reactor_rhs(reactor_t* r, double t, const double* y, double* ydot) {
-  (void)t;
   const zrko_mech* m = r->m;
   const int nsp = m->nsp;
   const int cvol = r->o->constant_volume;
   double T = y[nsp] * r->o->Tref;
   if (T <= 0.0) return 1;
   if (T > TLIMIT) T = TLIMIT;
-  if (cvol) {
-    r->inv_density = r->v0;
-  } else {
+  if (!cvol && !r->ysrc) {
+    /* P + dpdt t (reactor_constant_pressure_cpu.cpp:91-94); with mass sources
+     * the density keeps its initial value */
+    const double Pc = r->P + r->dpdt * t;
     double mws = 0.0;
     for (int j = 0; j < nsp; ++j) mws += y[j] * (1.0 / m->mw[j]);
     const double wmix = 1.0 / mws;
-    r->inv_density = 1.0 / (r->P * wmix / (NIST_RU * T));
+    r->inv_density = 1.0 / (Pc * wmix / (NIST_RU * T));
+  }
+  if (r->esrc != 0.0)  /* :96-99 (constant volume :87-90) */
+    T = temperature_from_energy(m, cvol, r->e0 + r->esrc * t, y, T, r->esc);
+  /* vd: this evaluation's 1/density; r->inv_density is the reference's
+   * carried member (constant volume: the fixed v0 even when dpdt moves vd) */
+  double vd = r->inv_density;
+  if (cvol) {
+    r->inv_density = vd = r->v0;
+    if (r->dpdt != 0.0) /* reactor_constant_volume_cpu.cpp:94-98 */
+      vd = inv_density_tpy(m, T, pressure_tvy(m, T, r->v0, y) + r->dpdt * t, y);
   }
   thermo_h(m, T, r->h);
   thermo_cp(m, T, r->cpm);
@@ -263,17 +355,36 @@ static int reactor_rhs(reactor_t* r, double t, const double* y, double* ydot) {
     cp += r->cpm[j] * y[j];
   }
   r->cp_mix = cp;
-  const double dens = 1.0 / r->inv_density;
+  const double dens = 1.0 / vd;
   for (int j = 0; j < nsp; ++j) r->C[j] = dens * (1.0 / m->mw[j]) * y[j];
   rates_ws(m, T, r->C, r->o->step_limiter, r->rop, r->cr, r->de, r->net, r->G);
   double esum = 0.0;
-  for (int j = 0; j < nsp; ++j) {
-    ydot[j] = (r->net[j] * m->mw[j]) * r->inv_density;
-    esum += r->h[j] * r->net[j] * r->inv_density;
+  if (r->ysrc) {
+    for (int j = 0; j < nsp; ++j) {
+      ydot[j] = (r->net[j] * m->mw[j] * vd + r->ysrc[j]);
+      esum += r->h[j] * (r->net[j] * vd + r->ysrc[j] * (1.0 / m->mw[j]));
+    }
+  } else {
+    for (int j = 0; j < nsp; ++j) {
+      ydot[j] = (r->net[j] * m->mw[j]) * vd;
+      esum += r->h[j] * r->net[j] * vd;
+    }
   }
   const double dTdt = -esum * NIST_RU * y[nsp] / cp;
-  ydot[nsp] = dTdt + 0.0 / (cp * r->o->Tref);
+  ydot[nsp] = dTdt + (r->esrc + r->dpdt * vd) / (cp * r->o->Tref);
   return 0;
+}
+
+int zrko_rhs_src(const zrko_mech* m, const zrko_opts* o, double P0, const double* y0,
+                 double t, const double* y, double dpdt, double esrc, const double* ysrc,
+                 double* ydot) {
+  reactor_t r;
+  reactor_init(&r, m, o, P0);
+  if (o->constant_volume) r.v0 = inv_density_tpy(m, y0[m->nsp] * o->Tref, P0, y0);
+  reactor_set_sources(&r, y0, P0, dpdt, esrc, ysrc);
+  int flag = reactor_rhs(&r, t, y, ydot);
+  reactor_free(&r);
+  return flag;
 }
 
 int zrko_rhs(const zrko_mech* m, const zrko_opts* o, double P, double t, const double* y,
@@ -1075,12 +1186,14 @@ static void cv_dky0(cvmem* cv, double t, double* out) {
 /* CvodeSolver::Integrate (solver_cvode.cpp:51-299): one-step loop, retries,
  * Dky interpolation at min(tcurr, end_time). */
 static int integrate_one(const zrko_mech* m, const zrko_opts* o, double* P, double dt,
-                         double* y, zrko_stats* st) {
+                         double* y, zrko_stats* st, double dpdt, double esrc,
+                         const double* ysrc) {
   const int N = m->nsp + 1;
   reactor_t r;
   reactor_init(&r, m, o, *P);
   /* ReactorConstantVolumeCPU::InitializeState (reactor_constant_volume_cpu.cpp:15-42) */
   if (o->constant_volume) r.v0 = inv_density_tpy(m, y[m->nsp] * o->Tref, *P, y);
+  reactor_set_sources(&r, y, *P, dpdt, esrc, ysrc);
   cvmem cv;
   memset(&cv, 0, sizeof(cv));
   cv.r = &r;
@@ -1127,14 +1240,19 @@ static int integrate_one(const zrko_mech* m, const zrko_opts* o, double* P, doub
     }
   }
   if (flag == CV_SUCCESS) {
-    cv_dky0(&cv, tcurr < dt ? tcurr : dt, y);
-    /* GetState: constant volume reports P = Ru T / (v Wmix)
-     * (reactor_constant_volume_cpu.cpp:44-63, mechanism.cpp:292-297) */
-    if (o->constant_volume) {
-      double mws = 0.0;
-      for (int j = 0; j < m->nsp; ++j) mws += y[j] * (1.0 / m->mw[j]);
-      *P = NIST_RU * (y[m->nsp] * o->Tref) / (r.v0 * (1.0 / mws));
-    }
+    const double tend = tcurr < dt ? tcurr : dt;
+    cv_dky0(&cv, tend, y);
+    /* GetState: constant volume reports P = Ru T / (v Wmix) + dpdt t
+     * (reactor_constant_volume_cpu.cpp:44-63, mechanism.cpp:292-297);
+     * constant pressure P + dpdt t, or with mass sources Ru T / (v Wmix) at
+     * the initial v (reactor_constant_pressure_cpu.cpp:44-66) */
+    const double Tend = y[m->nsp] * o->Tref;
+    if (o->constant_volume)
+      *P = pressure_tvy(m, Tend, r.v0, y) + dpdt * tend;
+    else if (ysrc)
+      *P = pressure_tvy(m, Tend, r.vinit, y);
+    else
+      *P = *P + dpdt * tend;
     nst_total += cv.nst; nfe_total += cv.nfe; nje_total += cv.nje; nset_total += cv.nsetups;
     nni_total += cv.nni; ncfn_total += cv.ncfn; netf_total += cv.netf;
   }
@@ -1160,6 +1278,14 @@ static int integrate_one(const zrko_mech* m, const zrko_opts* o, double* P, doub
 int zrko_integrate_batch(const zrko_mech* m, const zrko_opts* o, int n, double dt, double* T,
                          double* P, double* mf, int stride, zrko_stats* stats,
                          int nthreads) {
+  return zrko_integrate_batch_src(m, o, n, dt, T, P, mf, stride, stats, nthreads, NULL, NULL,
+                                  NULL);
+}
+
+int zrko_integrate_batch_src(const zrko_mech* m, const zrko_opts* o, int n, double dt,
+                             double* T, double* P, double* mf, int stride, zrko_stats* stats,
+                             int nthreads, const double* dpdt, const double* esrc,
+                             const double* ysrc) {
   const int nsp = m->nsp;
   int nfail = 0;
 #ifdef _OPENMP
@@ -1172,7 +1298,9 @@ int zrko_integrate_batch(const zrko_mech* m, const zrko_opts* o, int n, double d
     double* y = (double*)malloc(sizeof(double) * (size_t)(nsp + 1));
     for (int j = 0; j < nsp; ++j) y[j] = mf[(size_t)k * stride + j];
     y[nsp] = T[k] / o->Tref;
-    int flag = integrate_one(m, o, &P[k], dt, y, stats ? stats + k : NULL);
+    int flag = integrate_one(m, o, &P[k], dt, y, stats ? stats + k : NULL,
+                             dpdt ? dpdt[k] : 0.0, esrc ? esrc[k] : 0.0,
+                             ysrc ? ysrc + (size_t)k * stride : NULL);
     for (int j = 0; j < nsp; ++j) mf[(size_t)k * stride + j] = y[j];
     T[k] = y[nsp] * o->Tref;
     if (flag < 0) nfail++;

@@ -101,6 +101,17 @@ void zrko_jacobian(const zrko_mech* m, const zrko_opts* o, double P, double t,
 int zrko_integrate_batch(const zrko_mech* m, const zrko_opts* o, int n, double dt,
                          double* T, double* P, double* mf, int stride,
                          zrko_stats* stats, int nthreads);
+/* The same with the plugin's source terms (each array optional): pressure rate
+ * dpdt[n] (Pa/s), energy source esrc[n] (J/kg/s), mass-fraction sources
+ * ysrc[n*stride] (1/s); P is updated to the end pressure. */
+int zrko_integrate_batch_src(const zrko_mech* m, const zrko_opts* o, int n, double dt,
+                             double* T, double* P, double* mf, int stride,
+                             zrko_stats* stats, int nthreads, const double* dpdt,
+                             const double* esrc, const double* ysrc);
+/* RHS at time t of the reactor initialised at (P0, y0) with sources. */
+int zrko_rhs_src(const zrko_mech* m, const zrko_opts* o, double P0, const double* y0,
+                 double t, const double* y, double dpdt, double esrc, const double* ysrc,
+                 double* ydot);
 
 /* Dense LU with partial pivoting (SUNDIALS denseGETRF/denseGETRS semantics),
  * column-major; exposed for tests. Returns 0 or k+1 at a zero pivot. */

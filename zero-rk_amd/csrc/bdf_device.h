@@ -53,7 +53,7 @@ __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s
                                                double v0, const BdfOpts& o, double y, double fy,
                                                double inv_rho_prev, MT* __restrict__ Jl, int ld,
                                                MT* __restrict__ saved,
-                                               double* __restrict__ t2l) {
+                                               double* __restrict__ t2l, RhsSrc src = {}) {
   const int lane = lane_id();
   const int nsp = m.nsp;
   const bool sp = lane < nsp;
@@ -73,12 +73,12 @@ __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s
   const double t4in = (lane == nsp) ? Tp : (sp ? y : 0.0);
   double t4;
   RhsAux aux2;
-  rhs64(m, s, P, v0, o.Tref, o.step_limiter, t4in, t4, aux2);
+  rhs64(m, s, P, v0, o.Tref, o.step_limiter, t4in, t4, aux2, src);
   const double colT = act ? (t4 - fy) * (1.0 / dT) : 0.0;
   // floored evaluation: rates of progress for every species term
   double t3;
   RhsAux aux;
-  rhs64(m, s, P, v0, o.Tref, o.step_limiter, t1, t3, aux);
+  rhs64(m, s, P, v0, o.Tref, o.step_limiter, t1, t3, aux, src);
   const double Tdot = bcast(t3, nsp);
   const double RuTemp = kRu * yT;
   const double mult = -1.0 / aux.cp_mix;
@@ -151,7 +151,7 @@ __device__ __forceinline__ double jacobian_team(const Team& team, const DevMech&
                                                 double inv_rho_prev, MT* __restrict__ J,
                                                 MT* __restrict__ saved, int js,
                                                 double* __restrict__ t2l,
-                                                double* __restrict__ hl) {
+                                                double* __restrict__ hl, RhsSrc src = {}) {
   const int tid = team.tid();
   const int nsp = m.nsp;
   const bool sp = tid < nsp;
@@ -167,11 +167,11 @@ __device__ __forceinline__ double jacobian_team(const Team& team, const DevMech&
   double t4;
   RhsAux aux2;
   rhs_team(team, m, s, P, v0, o.Tref, o.step_limiter, (tid == nsp) ? Tp : (sp ? y : 0.0), t4,
-           aux2);
+           aux2, src);
   const double colT = act ? (t4 - fy) * (1.0 / dT) : 0.0;
   double t3;
   RhsAux aux;
-  rhs_team(team, m, s, P, v0, o.Tref, o.step_limiter, t1, t3, aux);
+  rhs_team(team, m, s, P, v0, o.Tref, o.step_limiter, t1, t3, aux, src);
   const double Tdot = team.from(t3, nsp);
   hl[tid] = sp ? aux.h : 0.0;
   if (tid < js)

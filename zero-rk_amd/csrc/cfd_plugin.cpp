@@ -8,6 +8,8 @@
 // cost sort of :522-524), copies back, and fills the aux outputs
 // (COST = CVode one-step calls, GPU = seconds per reactor of the batched
 // solve, TEMPERATURE_DELTA = T change above solve_temperature_threshold).
+// The DPDT, E_SRC and Y_SRC aux fields are passed to the integrator as
+// per-reactor source terms (P is then returned as the end pressure).
 // Reactors whose integration fails keep their input state and report a
 // negative cost; the call then returns ZERORK_STATUS_FAILED_SOLVE
 // (:1123-1139).  No C++ exception or exit() crosses the ABI.
@@ -52,7 +54,9 @@ struct zerork_handle_impl {
   double* dY = nullptr;
   int* dsteps = nullptr;
   int* dorder = nullptr;
-  size_t cap_n = 0, cap_y = 0;
+  double* dsrc_n = nullptr;  // dpdt then e_src (2n)
+  double* dysrc = nullptr;   // y_src (n*stride)
+  size_t cap_n = 0, cap_y = 0, cap_src_n = 0, cap_src_y = 0;
 
   zerork_handle_impl() {
     sopt["mech_filename"] = "mech.dat";
@@ -103,6 +107,8 @@ struct zerork_handle_impl {
     if (dY) (void)hipFree(dY);
     if (dsteps) (void)hipFree(dsteps);
     if (dorder) (void)hipFree(dorder);
+    if (dsrc_n) (void)hipFree(dsrc_n);
+    if (dysrc) (void)hipFree(dysrc);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -211,12 +217,6 @@ zerork_status_t solve(zerork_handle h, double dt, int n, double* T, double* P, d
   const int nsp = m->host.nsp;
   const int stride = h->iopt.count("num_species_stride") ? h->iopt["num_species_stride"] : nsp;
   if (stride < nsp) return ZERORK_STATUS_INVALID_POINTER;
-  if (h->y_src || h->e_src || h->dpdt) {
-    bool any = false;
-    for (int k = 0; k < n && !any; ++k) any = (h->dpdt && h->dpdt[k] != 0.0) || (h->e_src && h->e_src[k] != 0.0);
-    if (h->y_src) any = true;
-    if (any) return ZERORK_STATUS_UNKNOWN_ERROR;  // source terms not on the HIP path
-  }
   if (!h->stream && hipStreamCreate(&h->stream) != hipSuccess) return ZERORK_STATUS_UNKNOWN_ERROR;
   if ((size_t)n > h->cap_n) {
     if (!grow(&h->dT, n) || !grow(&h->dP, n) || !grow(&h->dsteps, n) || !grow(&h->dorder, n))
@@ -243,6 +243,36 @@ zerork_status_t solve(zerork_handle h, double dt, int n, double* T, double* P, d
   if (sorted)
     ok = ok && hipMemcpyAsync(h->dorder, order.data(), sizeof(int) * n, hipMemcpyHostToDevice,
                               st) == hipSuccess;
+  // source terms from the DPDT, E_SRC and Y_SRC aux fields (per reactor; Y_SRC
+  // in the layout of mf), as the manager passes them to InitializeState
+  // (zerork_reactor_manager.cpp:909-1053)
+  const bool srcs = h->dpdt || h->e_src || h->y_src;
+  const double *ddpdt = nullptr, *desrc = nullptr, *dysrc = nullptr;
+  if (srcs) {
+    if ((size_t)(2 * n) > h->cap_src_n) {
+      if (!grow(&h->dsrc_n, 2 * (size_t)n)) return ZERORK_STATUS_UNKNOWN_ERROR;
+      h->cap_src_n = 2 * (size_t)n;
+    }
+    if (h->dpdt) {
+      ok = ok && hipMemcpyAsync(h->dsrc_n, h->dpdt, sizeof(double) * n, hipMemcpyHostToDevice,
+                                st) == hipSuccess;
+      ddpdt = h->dsrc_n;
+    }
+    if (h->e_src) {
+      ok = ok && hipMemcpyAsync(h->dsrc_n + n, h->e_src, sizeof(double) * n,
+                                hipMemcpyHostToDevice, st) == hipSuccess;
+      desrc = h->dsrc_n + n;
+    }
+    if (h->y_src) {
+      if (ny > h->cap_src_y) {
+        if (!grow(&h->dysrc, ny)) return ZERORK_STATUS_UNKNOWN_ERROR;
+        h->cap_src_y = ny;
+      }
+      ok = ok && hipMemcpyAsync(h->dysrc, h->y_src, sizeof(double) * ny, hipMemcpyHostToDevice,
+                                st) == hipSuccess;
+      dysrc = h->dysrc;
+    }
+  }
   if (!ok) return ZERORK_STATUS_UNKNOWN_ERROR;
   zrk_bdf_options o;
   zrk_bdf_default_options(&o);
@@ -263,13 +293,12 @@ zerork_status_t solve(zerork_handle h, double dt, int n, double* T, double* P, d
   o.fp32_preconditioner = h->iopt["fp32_preconditioner"] != 0 || nsp > 63;
   const auto t0 = std::chrono::steady_clock::now();
   int rc = zrk::integrate_device(m, o, n, dt, nullptr, h->dT, h->dP, h->dY, stride, h->dsteps,
-                                 nullptr,
-                                 sorted ? h->dorder : nullptr, st);
+                                 nullptr, sorted ? h->dorder : nullptr, st, ddpdt, desrc, dysrc);
   if (rc != 0) return ZERORK_STATUS_UNKNOWN_ERROR;
   std::vector<int> steps(n);
   ok = hipMemcpyAsync(T, h->dT, sizeof(double) * n, hipMemcpyDeviceToHost, st) == hipSuccess;
   ok = ok && hipMemcpyAsync(mf, h->dY, sizeof(double) * ny, hipMemcpyDeviceToHost, st) == hipSuccess;
-  if (o.constant_volume)
+  if (o.constant_volume || srcs)  // end pressure (GetState)
     ok = ok && hipMemcpyAsync(P, h->dP, sizeof(double) * n, hipMemcpyDeviceToHost, st) == hipSuccess;
   ok = ok && hipMemcpyAsync(steps.data(), h->dsteps, sizeof(int) * n, hipMemcpyDeviceToHost,
                             st) == hipSuccess;

@@ -417,10 +417,70 @@ struct RhsAux {
 #endif
 };
 
+// The plugin's source terms at time t of a solve (reactor_constant_pressure_cpu
+// .cpp:15-42 and reactor_constant_volume_cpu.cpp:15-42): pressure rate, energy
+// source, this thread's mass-fraction source; e0 the initial mass enthalpy
+// (constant pressure) or internal energy (constant volume), vinit the initial
+// 1/density (kept at constant pressure when mass sources are present).
+struct RhsSrc {
+  bool on = false;        // any source term
+  bool has_ysrc = false;  // mass-fraction sources
+  double t = 0.0, dpdt = 0.0, esrc = 0.0, e0 = 0.0, vinit = 0.0, ysrc = 0.0;
+};
+
+// Mass enthalpy (cvol = false) or internal energy of the mixture at T
+// (mechanism::getMassEnthalpyFromTY / getMassIntEnergyFromTY,
+// mechanism.cpp:336-367), and its cp or cv (getMassCpFromTY /
+// getMassCvFromTY, :408-441); Yj: this thread's mass fraction (0 off species).
+template <class Team>
+__device__ __forceinline__ void team_mass_energy(const Team& team, const DevMech& m, bool cvol,
+                                                 bool cv_capacity, double T, double Yj,
+                                                 double& e, double& cx) {
+  const int tid = team.tid();
+  double ej = 0.0, cj = 0.0;
+  if (tid < m.nsp) {
+    double cp, h, g;
+    thermo_one(m.thermo + 16 * tid, make_tpow(T), cp, h, g);
+    if (cvol) h -= 1.0;
+    if (cv_capacity) cp -= 1.0;
+    ej = h * (m.ru_mw[tid] * T) * Yj;
+    cj = cp * m.ru_mw[tid] * Yj;
+  }
+  e = team.sum(ej);
+  cx = team.sum(cj);
+}
+
+// Temperature at which the mixture's mass enthalpy (or internal energy) is E:
+// mechanism::getTemperatureFromHY / FromEY (mechanism.cpp:477-547), with their
+// heat-capacity choices (the enthalpy iteration and its upper extrapolation
+// use cv).  Every quantity is a team sum, so the loop is uniform.
+template <class Team>
+__device__ __forceinline__ double team_temperature_from_energy(const Team& team,
+                                                                const DevMech& m, bool cvol,
+                                                                double E, double Yj,
+                                                                double guess) {
+  const double tmin = 100.0, tmax = 5000.0;
+  double emin, emax, c;
+  team_mass_energy(team, m, cvol, cvol, tmin, Yj, emin, c);
+  if (E < emin) return tmin - (emin - E) / c;
+  team_mass_energy(team, m, cvol, true, tmax, Yj, emax, c);
+  if (E > emax) return tmax - (emax - E) / c;
+  double T = guess;
+  if (T < tmin || T > tmax) T = tmin + (tmax - tmin) / (emax - emin) * (E - emin);
+  for (int i = 0; i < 400; ++i) {
+    double Ei;
+    team_mass_energy(team, m, cvol, true, T, Yj, Ei, c);
+    const double d = fmin(fmax((E - Ei) / c, -100.0), 100.0);
+    if (fabs(d) < 1.0e-6 || T + d == T) break;
+    T += d;
+  }
+  return T;
+}
+
 template <class Team>
 __device__ __forceinline__ int rhs_team(const Team& team, const DevMech& m, const RhsLds& s,
                                         double P, double v0, double Tref, double L, double y,
-                                        double& ydot, RhsAux& aux) {
+                                        double& ydot, RhsAux& aux, RhsSrc src = {}) {
   const int lane = team.tid();
   const int nsp = m.nsp;
   const bool sp = lane < nsp;
@@ -434,17 +494,31 @@ __device__ __forceinline__ int rhs_team(const Team& team, const DevMech& m, cons
     return 1;
   }
   T = fmin(T, kTlimit);
-  const TPow p = make_tpow(T);
   const double Yj = sp ? y : 0.0;
   const double imw = sp ? m.inv_mw[lane] : 0.0;
   const bool cvol = v0 > 0.0;
+  // inv_rho: the reference's carried 1/density member; vd: this evaluation's
   double inv_rho = v0;
   if (!cvol) {
-    const double mws = team.sum(Yj * imw);
-    const double wmix = 1.0 / mws;
-    inv_rho = 1.0 / (P * wmix / (kRu * T));
+    if (src.has_ysrc) {
+      inv_rho = src.vinit;  // mass sources: the density keeps its initial value
+    } else {
+      const double Pc = src.on ? P + src.dpdt * src.t : P;  // :91-94
+      const double mws = team.sum(Yj * imw);
+      const double wmix = 1.0 / mws;
+      inv_rho = 1.0 / (Pc * wmix / (kRu * T));
+    }
   }
-  const double rho_c = 1.0 / inv_rho;
+  if (src.on && src.esrc != 0.0)  // reactor_constant_pressure_cpu.cpp:96-99 (volume :87-90)
+    T = team_temperature_from_energy(team, m, cvol, src.e0 + src.esrc * src.t, Yj, T);
+  double vd = inv_rho;
+  if (cvol && src.on && src.dpdt != 0.0) {  // reactor_constant_volume_cpu.cpp:94-98
+    const double wmix = 1.0 / team.sum(Yj * imw);
+    const double Pc = kRu * T / (v0 * wmix) + src.dpdt * src.t;
+    vd = 1.0 / (Pc * wmix / (kRu * T));
+  }
+  const TPow p = make_tpow(T);
+  const double rho_c = 1.0 / vd;
   double cp = 0.0, h = 0.0, g = 0.0, Cj = 0.0, cpm = 0.0;
   if (sp) {
     thermo_one(m.thermo + 16 * lane, p, cp, h, g);
@@ -478,11 +552,18 @@ __device__ __forceinline__ int rhs_team(const Team& team, const DevMech& m, cons
     double cr, de;
     species_cd(m, s.rop, lane, cr, de);
     const double net = cr - de;
-    yd = (net * m.mw[lane]) * inv_rho;
-    e = h * net * inv_rho;
+    if (src.has_ysrc) {
+      yd = (net * m.mw[lane] * vd + src.ysrc);
+      e = h * (net * vd + src.ysrc * imw);
+    } else {
+      yd = (net * m.mw[lane]) * vd;
+      e = h * net * vd;
+    }
   }
   const double esum = team.sum(e);
-  if (lane == nsp) yd = -esum * kRu * yT / cp_mix + 0.0 / (cp_mix * Tref);
+  if (lane == nsp)
+    yd = -esum * kRu * yT / cp_mix +
+         (src.on ? src.esrc + src.dpdt * vd : 0.0) / (cp_mix * Tref);
   ydot = yd;
   aux.inv_rho = inv_rho;
   aux.cp_mix = cp_mix;
@@ -498,45 +579,11 @@ __device__ __forceinline__ int rhs_team(const Team& team, const DevMech& m, cons
   return 0;
 }
 
-// One reactor per wavefront (nsp <= 63).  With ZRK_RHS_CALL the body is one
-// out-of-line function shared by every call site (one copy of the rate code
-// in the integrator instead of one per site); its LDS arrays are addressed by
-// offset from the dynamic LDS base and the mechanism through its device copy.
-struct RhsRes {
-  double ydot, inv_rho, cp_mix, h, cpm;
-  int flag;
-};
-
-__device__ __noinline__ RhsRes rhs_wave_call(const DevMech* __restrict__ mp, int off, double P,
-                                             double v0, double Tref, double L, double y) {
-  extern __shared__ double dyn_lds[];
-  const DevMech m = *mp;
-  const RhsLds s = rhs_lds_at(dyn_lds + off, m.nsp);
-  RhsRes r;
-  RhsAux aux;
-  r.flag = rhs_team(TeamWave{}, m, s, P, v0, Tref, L, y, r.ydot, aux);
-  r.inv_rho = aux.inv_rho;
-  r.cp_mix = aux.cp_mix;
-  r.h = aux.h;
-  r.cpm = aux.cpm;
-  return r;
-}
-
+// One reactor per wavefront (nsp <= 63).
 __device__ __forceinline__ int rhs64(const DevMech& m, const RhsLds& s, double P, double v0,
                                      double Tref, double L, double y, double& ydot,
-                                     RhsAux& aux) {
-#ifdef ZRK_RHS_CALL
-  extern __shared__ double dyn_lds[];
-  const RhsRes r = rhs_wave_call(m.self, (int)(s.C - dyn_lds), P, v0, Tref, L, y);
-  ydot = r.ydot;
-  aux.inv_rho = r.inv_rho;
-  aux.cp_mix = r.cp_mix;
-  aux.h = r.h;
-  aux.cpm = r.cpm;
-  return r.flag;
-#else
-  return rhs_team(TeamWave{}, m, s, P, v0, Tref, L, y, ydot, aux);
-#endif
+                                     RhsAux& aux, RhsSrc src = {}) {
+  return rhs_team(TeamWave{}, m, s, P, v0, Tref, L, y, ydot, aux, src);
 }
 
 }  // namespace zrk

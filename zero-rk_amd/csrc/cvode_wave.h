@@ -46,6 +46,7 @@ struct CvUni {
   double tau[kQmax + 2], tq[6], l[kQmax + 1];
   double rl1, gamma, gammap, gamrat, crate, delp, acnrm, nlscoef, saved_tq5;
   double etaq, etaqm1, etaqp1, inv_rho_last, inv_nvar;
+  double dpdt, esrc, e0, vinit;  // source terms of this reactor (RhsSrc)
   double pad_;
 };
 static_assert(sizeof(CvUni) % 16 == 0, "CvUni keeps the LDS layout 16-byte aligned");
@@ -94,7 +95,9 @@ struct Cvode {
   // lane vectors
   double zn[kQmax + 1];
   double ewt, acor, tempv, ftemp, y, atolv, delta;
+  double ysrc;  // this lane's mass-fraction source (when has_ysrc)
   bool active;  // species lane or temperature lane
+  bool has_src, has_ysrc;  // uniform: any source term / mass-fraction sources
   // uniform FP64 scalars: kept in this wavefront's CvUni block in LDS (every
   // lane holds the same values), so they occupy no vector registers across
   // the rate evaluation and the matrix inversion
@@ -106,6 +109,7 @@ struct Cvode {
   double (&l)[kQmax + 1];
   double &rl1, &gamma, &gammap, &gamrat, &crate, &delp, &acnrm, &nlscoef, &saved_tq5;
   double &etaq, &etaqm1, &etaqp1, &inv_rho_last, &inv_nvar;
+  double &dpdt, &esrc, &e0, &vinit;
   int q, qprime, qwait, L;
   int nst, nstlp, nstlj, nfe, nje, nsetups, nni, ncfn, netf;
   int convfail, jcur, acnrmcur;
@@ -117,22 +121,40 @@ struct Cvode {
         gamma(u->gamma), gammap(u->gammap), gamrat(u->gamrat), crate(u->crate),
         delp(u->delp), acnrm(u->acnrm), nlscoef(u->nlscoef), saved_tq5(u->saved_tq5),
         etaq(u->etaq), etaqm1(u->etaqm1), etaqp1(u->etaqp1), inv_rho_last(u->inv_rho_last),
-        inv_nvar(u->inv_nvar) {}
+        inv_nvar(u->inv_nvar), dpdt(u->dpdt), esrc(u->esrc), e0(u->e0), vinit(u->vinit) {}
+
+  // the source terms at time tt (src.on false when the reactor has none)
+  __device__ __forceinline__ RhsSrc sources(double tt) const {
+    RhsSrc rs;
+    if (has_src) {
+      rs.on = true;
+      rs.has_ysrc = has_ysrc;
+      rs.t = tt;
+      rs.dpdt = dpdt;
+      rs.esrc = esrc;
+      rs.e0 = e0;
+      rs.vinit = vinit;
+      rs.ysrc = ysrc;
+    }
+    return rs;
+  }
 
   __device__ __forceinline__ double wrms(double x) const {
     const double p = active ? x * ewt : 0.0;
     return sqrt(team.sum(p * p) * inv_nvar);
   }
 
-  __device__ __forceinline__ int f(double yy, double& yd) {
+  // f(tt, yy): the reactor's RHS at time tt of the solve
+  __device__ __forceinline__ int f(double tt, double yy, double& yd) {
     nfe++;
     RhsAux aux;
+    const RhsSrc src = sources(tt);
     ZRK_TIC(t0);
     int fl;
     if constexpr (kWaveTeam)
-      fl = rhs64(mech, s, P, v0, opt.Tref, opt.step_limiter, yy, yd, aux);
+      fl = rhs64(mech, s, P, v0, opt.Tref, opt.step_limiter, yy, yd, aux, src);
     else
-      fl = rhs_team(team, mech, s, P, v0, opt.Tref, opt.step_limiter, yy, yd, aux);
+      fl = rhs_team(team, mech, s, P, v0, opt.Tref, opt.step_limiter, yy, yd, aux, src);
     ZRK_TOC(t0, 0);
 #ifdef ZRK_PHASE_CYCLES
     if (!fl)
@@ -191,7 +213,7 @@ struct Cvode {
   __device__ __forceinline__ int ydd_norm(double hg, double& yddnrm) {
     y = hg * zn[1] + zn[0];
     double fy;
-    if (f(y, fy) > 0) return 1;
+    if (f(tn + hg, y, fy) > 0) return 1;
     tempv = (fy - zn[1]) * (1.0 / hg);
     yddnrm = wrms(tempv);
     return 0;
@@ -446,7 +468,7 @@ struct Cvode {
       nstlj = nst;
       ZRK_TIC(t0);
       inv_rho_last = jacobian_lds<MT>(mech, s, P, v0, opt, ypred, fpred, inv_rho_last, Mlds, N,
-                                      savedJ, t2l);
+                                      savedJ, t2l, sources(tn));
       ZRK_TOC(t0, 1);
     }
     if constexpr (kWaveMfmaInverse) {
@@ -512,7 +534,7 @@ struct Cvode {
       nstlj = nst;
       ZRK_TIC(t0);
       inv_rho_last = jacobian_team<MT>(team, mech, s, P, v0, opt, ypred, fpred, inv_rho_last,
-                                       Mlds, savedJ, kJS, t2l, hl);
+                                       Mlds, savedJ, kJS, t2l, hl, sources(tn));
       ZRK_TOC(t0, 1);
     }
     ZRK_TIC(t1);
@@ -579,7 +601,7 @@ struct Cvode {
   // cvNlsResidual
   __device__ __forceinline__ int nls_residual(double ycor, double& res) {
     y = zn[0] + ycor;
-    if (f(y, ftemp) > 0) return kRhsRecvr;
+    if (f(tn, y, ftemp) > 0) return kRhsRecvr;
     res = rl1 * zn[1] + ycor;
     res = -gamma * ftemp + res;
     return kNlsSuccess;
@@ -706,7 +728,7 @@ struct Cvode {
     hscale = h;
     qwait = kLongWait;
     double fz;
-    if (f(zn[0], fz) > 0) return kCvReptdRhsErr;
+    if (f(tn, zn[0], fz) > 0) return kCvReptdRhsErr;
     zn[1] = h * fz;
     return kTryAgain;
   }
@@ -848,7 +870,7 @@ struct Cvode {
     if (nst == 0) {
       if (set_ewt(zn[0]) != 0) return kCvIllInput;
       double f0;
-      if (f(zn[0], f0) > 0) return kCvReptdRhsErr;
+      if (f(tn, zn[0], f0) > 0) return kCvReptdRhsErr;
       zn[1] = f0;
       if (hin(tout) != 0) return kCvIllInput;
       const double rh = fabs(h) * hmax_inv;

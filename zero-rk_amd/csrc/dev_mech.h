@@ -22,6 +22,11 @@ struct BdfOpts {
   int max_steps, num_retries;
   int constant_volume;
   int fp32_preconditioner;
+  // optional per-reactor source terms (the plugin's DPDT, E_SRC, Y_SRC fields):
+  // dpdt[n] Pa/s, esrc[n] J/kg/s, ysrc[n*stride] 1/s (null: none)
+  const double* dpdt = nullptr;
+  const double* esrc = nullptr;
+  const double* ysrc = nullptr;
 };
 
 enum RecFlags : int {

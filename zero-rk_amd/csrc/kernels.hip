@@ -185,6 +185,8 @@ __device__ __forceinline__ void bdf_driver(Cvode<MT, Team>& cv, const DevMech& m
   cv.mech = m;
   cv.opt = o;
   cv.active = lane <= nsp;
+  cv.has_src = o.dpdt || o.esrc || o.ysrc;
+  cv.has_ysrc = o.ysrc != nullptr;
   cv.inv_nvar = 1.0 / (double)(nsp + 1);
   cv.nlscoef = o.nlscoef;
   cv.hmax_inv = o.hmax_inv;
@@ -205,6 +207,22 @@ __device__ __forceinline__ void bdf_driver(Cvode<MT, Team>& cv, const DevMech& m
       // state (reactor_constant_volume_cpu.cpp:28-29)
       const double mws = cv.team.sum(lane < nsp ? y0 * m.inv_mw[lane] : 0.0);
       cv.v0 = 1.0 / (cv.P * (1.0 / mws) / (kRu * (cv.team.from(y0, nsp) * o.Tref)));
+    }
+    // source terms (InitializeState, reactor_constant_pressure_cpu.cpp:15-42,
+    // reactor_constant_volume_cpu.cpp:15-42): the initial mass enthalpy or
+    // internal energy and 1/density
+    cv.dpdt = (o.dpdt ? o.dpdt[r] : 0.0);
+    cv.esrc = (o.esrc ? o.esrc[r] : 0.0);
+    cv.ysrc = (o.ysrc && lane < nsp) ? o.ysrc[(size_t)r * stride + lane] : 0.0;
+    cv.e0 = cv.vinit = 0.0;
+    if (cv.has_src) {
+      const double T0 = cv.team.from(y0, nsp) * o.Tref;
+      const double Yj = lane < nsp ? y0 : 0.0;
+      const double mws = cv.team.sum(lane < nsp ? y0 * m.inv_mw[lane] : 0.0);
+      cv.vinit = 1.0 / (cv.P * (1.0 / mws) / (kRu * T0));
+      double e0, c;
+      team_mass_energy(cv.team, m, o.constant_volume != 0, false, T0, Yj, e0, c);
+      cv.e0 = e0;
     }
     cv.atolv = cv.active ? o.atol : 0.0;
     cv.rtol = o.rtol;
@@ -238,11 +256,20 @@ __device__ __forceinline__ void bdf_driver(Cvode<MT, Team>& cv, const DevMech& m
       tot[4] += cv.nni; tot[5] += cv.ncfn; tot[6] += cv.netf;
       if (lane < nsp) Yio[(size_t)r * stride + lane] = yout;
       if (lane == nsp) Tio[r] = yout * o.Tref;
-      if (o.constant_volume) {
-        // GetState: P = Ru T / (v Wmix) (reactor_constant_volume_cpu.cpp:44-63)
+      // GetState (reactor_constant_volume_cpu.cpp:44-63,
+      // reactor_constant_pressure_cpu.cpp:44-66): constant volume P = Ru T /
+      // (v Wmix) + dpdt t; constant pressure P + dpdt t, or with mass sources
+      // Ru T / (v Wmix) at the initial v
+      const double tend = tcurr < dt ? tcurr : dt;
+      if (o.constant_volume || cv.has_ysrc) {
         const double mws = cv.team.sum(lane < nsp ? yout * m.inv_mw[lane] : 0.0);
         const double Tend = cv.team.from(yout, nsp) * o.Tref;
-        if (lane == 0) Pio[r] = kRu * Tend / (cv.v0 * (1.0 / mws));
+        const double v = o.constant_volume ? cv.v0 : cv.vinit;
+        const double Pend = kRu * Tend / (v * (1.0 / mws)) + (o.constant_volume ? cv.dpdt * tend : 0.0);
+        if (lane == 0) Pio[r] = Pend;
+      } else if (cv.has_src) {
+        const double Pend = cv.P + cv.dpdt * tend;
+        if (lane == 0) Pio[r] = Pend;
       }
     } else {
       nsteps = (nsteps <= 0) ? -1 : -nsteps;

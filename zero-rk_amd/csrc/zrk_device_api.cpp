@@ -297,7 +297,8 @@ int team_width_for(MechHandle* h, bool f32) {
 
 int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
                      const double* dts, double* T, double* P, double* Y, int stride, int* nsteps, int* stats,
-                     const int* order, hipStream_t st) {
+                     const int* order, hipStream_t st, const double* dpdt, const double* esrc,
+                     const double* ysrc) {
   if (n < 0 || !T || !P || !Y || stride < h->host.nsp || (!dts && !(dt > 0.0))) return -1;
   const int W0 = bdf_team_width(h->host.nsp);
   if (W0 == 0) return -2;                               // mechanism too large
@@ -310,8 +311,12 @@ int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
   const int nw = std::max(1, std::min(h->resident[f32][W], n));
   int rc = ensure_workspace(h, (size_t)nw * bdf_slot_doubles(h->dev, W));
   if (rc) return rc;
-  hipError_t e = launch_bdf(h->dev, to_bdf_opts(o), n, dt, dts, T, P, Y, stride, nsteps, stats,
-                            order, h->ws_counter, h->ws_jac, nw, W, st);
+  BdfOpts bo = to_bdf_opts(o);
+  bo.dpdt = dpdt;
+  bo.esrc = esrc;
+  bo.ysrc = ysrc;
+  hipError_t e = launch_bdf(h->dev, bo, n, dt, dts, T, P, Y, stride, nsteps, stats, order,
+                            h->ws_counter, h->ws_jac, nw, W, st);
   return e == hipSuccess ? 0 : (int)e;
 }
 
@@ -446,6 +451,16 @@ int zrk_integrate_device(zrk_mech m, const zrk_bdf_options* o, int n, double dt,
   if (!m || !o) return -1;
   return zrk::integrate_device(reinterpret_cast<MechHandle*>(m), *o, n, dt, nullptr, T, P, Y,
                                stride, nsteps, stats, nullptr, (hipStream_t)stream);
+}
+
+int zrk_integrate_device_src(zrk_mech m, const zrk_bdf_options* o, int n, double dt, double* T,
+                             double* P, double* Y, int stride, const double* dpdt,
+                             const double* esrc, const double* ysrc, int* nsteps, int* stats,
+                             void* stream) {
+  if (!m || !o) return -1;
+  return zrk::integrate_device(reinterpret_cast<MechHandle*>(m), *o, n, dt, nullptr, T, P, Y,
+                               stride, nsteps, stats, nullptr, (hipStream_t)stream, dpdt, esrc,
+                               ysrc);
 }
 
 int zrk_autotune_integrator(zrk_mech m, const zrk_bdf_options* o, int n, double dt,
