@@ -56,6 +56,14 @@ class _Mech(C.Structure):
         ("n_fo", C.c_int), ("fo_fwd", C.c_void_p), ("fo_rev", C.c_void_p),
         ("fo_type", C.c_void_p), ("fo_spc", C.c_void_p), ("fo_param", C.c_void_p),
         ("fo_ptr", C.c_void_p), ("fo_sp", C.c_void_p), ("fo_eff", C.c_void_p),
+        ("n_ni", C.c_int), ("step_ni", C.c_void_p), ("ni_step", C.c_void_p),
+        ("ni_rop_ptr", C.c_void_p), ("ni_rop_sp", C.c_void_p), ("ni_rop_pow", C.c_void_p),
+        ("ni_re_ptr", C.c_void_p), ("ni_re_sp", C.c_void_p), ("ni_re_nu", C.c_void_p),
+        ("ni_pr_ptr", C.c_void_p), ("ni_pr_sp", C.c_void_p), ("ni_pr_nu", C.c_void_p),
+        ("n_plog", C.c_int), ("plog_step", C.c_void_p), ("plog_ptr", C.c_void_p),
+        ("plog_p", C.c_void_p), ("plog_lnp", C.c_void_p), ("plog_lptr", C.c_void_p),
+        ("plog_lnA", C.c_void_p), ("plog_sign", C.c_void_p), ("plog_beta", C.c_void_p),
+        ("plog_Ta", C.c_void_p),
     ]
 
 
@@ -122,13 +130,21 @@ class Oracle:
             tb_fwd=i32(t.tb_fwd), tb_rev=i32(t.tb_rev), tb_ptr=i32(t.tb_ptr), tb_sp=i32(t.tb_sp),
             tb_eff=f64(t.tb_eff), fo_fwd=i32(t.fo_fwd), fo_rev=i32(t.fo_rev),
             fo_type=i32(t.fo_type), fo_spc=i32(t.fo_spc), fo_param=f64(t.fo_param),
-            fo_ptr=i32(t.fo_ptr), fo_sp=i32(t.fo_sp), fo_eff=f64(t.fo_eff))
+            fo_ptr=i32(t.fo_ptr), fo_sp=i32(t.fo_sp), fo_eff=f64(t.fo_eff),
+            step_ni=i32(t.step_ni), ni_step=i32(t.ni_step), ni_rop_ptr=i32(t.ni_rop_ptr),
+            ni_rop_sp=i32(t.ni_rop_sp), ni_rop_pow=f64(t.ni_rop_pow), ni_re_ptr=i32(t.ni_re_ptr),
+            ni_re_sp=i32(t.ni_re_sp), ni_re_nu=f64(t.ni_re_nu), ni_pr_ptr=i32(t.ni_pr_ptr),
+            ni_pr_sp=i32(t.ni_pr_sp), ni_pr_nu=f64(t.ni_pr_nu), plog_step=i32(t.plog_step),
+            plog_ptr=i32(t.plog_ptr), plog_p=f64(t.plog_p), plog_lnp=f64(t.plog_lnp),
+            plog_lptr=i32(t.plog_lptr), plog_lnA=f64(t.plog_lnA), plog_sign=f64(t.plog_sign),
+            plog_beta=f64(t.plog_beta), plog_Ta=f64(t.plog_Ta))
         self._keep = keep
         m = _Mech()
         m.nsp, m.nrxn, m.nstep = t.nsp, t.nrxn, t.nstep
         for k, v in keep.items():
             setattr(m, k, _p(v))
         m.n_arr, m.n_keq, m.n_tb, m.n_fo = len(t.arr_step), len(t.keq_step), len(t.tb_fwd), len(t.fo_fwd)
+        m.n_ni, m.n_plog = len(t.ni_step), len(t.plog_step)
         self._m = m
         self.L = lib()
 

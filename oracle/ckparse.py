@@ -6,8 +6,9 @@ reading a mechanism and evaluating rates:
 * CKReader text semantics (src/CKconverter/CKParser.cpp): sections
   ELEMENTS/SPECIES/THERMO/REACTIONS, '!' comments, 80-column NASA thermo cards
   (:861-1046), activation-energy units on the REACTIONS line (:1094-1110),
-  auxiliary keywords LOW/TROE/REV/DUPLICATE and third-body efficiencies
-  (:1477-1668), '+M', '(+M)' and '(+species)' third bodies.
+  auxiliary keywords LOW/TROE/SRI/REV/DUPLICATE/FORD/RORD/PLOG and third-body
+  efficiencies (:1477-1669), '+M', '(+M)' and '(+species)' third bodies, PLOG
+  exclusions (:312-345).
 * Molecular weights: sum of element count x IUPAC-2007 atomic mass, in card
   order (src/zerork/species.cpp:38-45, atomicMassDB.h).
 * Step numbering: forward step of reaction j, then its reverse step right after
@@ -18,7 +19,18 @@ reading a mechanism and evaluating rates:
   '+M' reactions, Ta = E*convertE, merged Arrhenius triples within 1e-6
   relative (:275-332, 1093-1128), Keq list (:334-386), third-body list with
   efficiency-1 in species-name order (:388-420, 1030-1062), falloff list with
-  param[0] = ln(Alow*1000^-order) (:422-537).
+  param[0] = ln(Alow*1000^-order) (:422-537), SRI parameters a, b, 1/c (-1 when
+  c <= 0), d, e (:508-531).
+* Non-integer reactions (info_net.cpp:411-470: FORD/RORD, or a coefficient
+  further than 1e-14 from an integer): no expanded participants; per step
+  rate-of-progress exponents by species name (summed coefficients, overwritten
+  by FORD/RORD), reactant/product coefficient lists in text order, the step's
+  real order = sum of its exponents for the unit conversions, Keq change in
+  moles = reverse minus forward exponent sums
+  (non_integer_reaction_network.cpp:53-262, rate_const.cpp:334-386).
+* PLOG (rate_const.cpp:1146-1186, plog_reaction.cpp:10-135): pressures in Pa,
+  A*1000^(1-order), selection sort by pressure, lines within 1e-8 relative of
+  the previous distinct pressure grouped with it.
 
 Only tests/, __graft_entry__.smoke() and bench.py's CPU baseline use this.
 """
@@ -48,7 +60,9 @@ ATOMIC_MASS = {
     "CA": 40.078,
 }
 
-FALLOFF_LINDEMANN, FALLOFF_TROE3, FALLOFF_TROE4 = 0, 1, 2
+FALLOFF_LINDEMANN, FALLOFF_TROE3, FALLOFF_TROE4, FALLOFF_SRI = 0, 1, 2, 3
+STOICH_TOL = 1.0e-14   # src/zerork/constants.h:32
+P_ATM = 1.01325e5
 
 
 @dataclass
@@ -66,6 +80,11 @@ class RxnText:
     rev: tuple = ()           # explicit reverse (A, n, E)
     eff: dict = field(default_factory=dict)
     dup: bool = False
+    sri: tuple = ()
+    plog: list = field(default_factory=list)   # [(p atm, A, n, E K)]
+    ford: dict = field(default_factory=dict)
+    rord: dict = field(default_factory=dict)
+    real_order: bool = False
 
 
 def _strip(line: str) -> str:
@@ -241,7 +260,18 @@ def _parse_reactions(lines, units_tok, species):
                 rx.troe = tuple(_num(v) for v in vals)
             elif key == "REV":
                 rx.rev = (_num(vals[0]), _num(vals[1]), _num(vals[2]) * conv)
-            elif key in ("SRI", "PLOG", "FORD", "RORD", "HIGH", "LT", "RLT", "CHEB", "TCHEB", "PCHEB"):
+            elif key == "SRI":
+                if rx.troe:
+                    raise ValueError("cannot specify both SRI and TROE")
+                rx.sri = tuple(_num(v) for v in vals)
+            elif key == "PLOG":
+                if len(vals) != 4:
+                    raise ValueError("expected 4 PLOG parameters")
+                rx.plog.append((_num(vals[0]), _num(vals[1]), _num(vals[2]), _num(vals[3]) * conv))
+            elif key in ("FORD", "RORD"):
+                (rx.ford if key == "FORD" else rx.rord)[vals[0]] = _num(vals[1])
+                rx.real_order = True
+            elif key in ("HIGH", "LT", "RLT", "CHEB", "TCHEB", "PCHEB"):
                 raise NotImplementedError(f"oracle parser: keyword {key} not supported")
             else:
                 if val is not None and vals:
@@ -278,10 +308,33 @@ class Tables:
     fo_rev: np.ndarray
     fo_type: np.ndarray
     fo_spc: np.ndarray           # -1 for M
-    fo_param: np.ndarray         # [nfo,7]: lnAlow, beta, Ta, alpha, T***, T*, T**
+    fo_param: np.ndarray         # [nfo,8]: lnAlow, beta, Ta, then Troe alpha, T***, T*, T**
+                                 # or SRI a, b, 1/c (-1 if c <= 0), d, e
     fo_ptr: np.ndarray
     fo_sp: np.ndarray
     fo_eff: np.ndarray
+    # non-integer steps (in step order)
+    step_ni: np.ndarray          # [nstep] index into the ni lists or -1
+    ni_step: np.ndarray
+    ni_rop_ptr: np.ndarray       # CSR: rate-of-progress exponent species / powers
+    ni_rop_sp: np.ndarray
+    ni_rop_pow: np.ndarray
+    ni_re_ptr: np.ndarray        # CSR: destroyed species / coefficients
+    ni_re_sp: np.ndarray
+    ni_re_nu: np.ndarray
+    ni_pr_ptr: np.ndarray        # CSR: created species / coefficients
+    ni_pr_sp: np.ndarray
+    ni_pr_nu: np.ndarray
+    # PLOG forward steps
+    plog_step: np.ndarray
+    plog_ptr: np.ndarray         # CSR over distinct pressures
+    plog_p: np.ndarray           # Pa
+    plog_lnp: np.ndarray
+    plog_lptr: np.ndarray        # CSR (over all pressures) of Arrhenius lines
+    plog_lnA: np.ndarray         # ln|A| with the unit conversion
+    plog_sign: np.ndarray
+    plog_beta: np.ndarray
+    plog_Ta: np.ndarray
 
     @property
     def nsp(self):
@@ -334,29 +387,68 @@ def build_tables(mech_path: str, therm_path: str | None) -> Tables:
     rxns = _parse_reactions(rx_lines, units, species)
     nrxn = len(rxns)
 
+    def is_nonint(r):
+        if r.real_order:
+            return True
+        for _, c in r.reac + r.prod:
+            if abs(c - int(c + 0.5)) > STOICH_TOL:
+                return True
+        return False
+
+    for r in rxns:
+        if r.plog:
+            if r.falloff or r.low or r.troe or r.sri:
+                raise ValueError("PLOG and falloff reactions (LOW/TROE/SRI) are mutually exclusive")
+            if r.third_body == "M" and not r.falloff:
+                raise ValueError("PLOG and 3rd-body reactions M or (M) are mutually exclusive")
+            if r.rev and r.rev[0] > 0.0:
+                raise ValueError("PLOG does not support REV parameters being specified")
+
     step_rxn, step_dir = [], []
     reac_lists, prod_lists = [], []
+    step_order = []
+    ni = []  # (step, rop [(sp, pow)], re [(sp, nu)], pr [(sp, nu)])
 
     def expand(lst):
         out = []
         for name, coef in lst:
-            k = int(coef + 0.5)
-            assert abs(coef - k) < 1e-14, "non-integer stoichiometry unsupported"
-            out += [idx[name]] * k
+            out += [idx[name]] * int(coef + 0.5)
         return out
 
+    def rop_map(side, over):
+        mp = {}
+        for name, c in side:
+            mp[name] = mp.get(name, 0.0) + c
+        mp.update(over)
+        return [(idx[n], mp[n]) for n in sorted(mp)]
+
     for j, r in enumerate(rxns):
-        step_rxn.append(j); step_dir.append(1)
-        reac_lists.append(expand(r.reac)); prod_lists.append(expand(r.prod))
+        nonint = is_nonint(r)
         has_rev = r.reversible and not (r.rev and r.rev[0] == 0.0)
-        if has_rev:
-            step_rxn.append(j); step_dir.append(-1)
-            reac_lists.append(expand(r.prod)); prod_lists.append(expand(r.reac))
+        re_l = [] if nonint else expand(r.reac)
+        pr_l = [] if nonint else expand(r.prod)
+        for d in range(2 if has_rev else 1):
+            s_id = len(step_rxn)
+            step_rxn.append(j); step_dir.append(1 if d == 0 else -1)
+            reac_lists.append(re_l if d == 0 else pr_l)
+            prod_lists.append(pr_l if d == 0 else re_l)
+            if nonint:
+                fwd_map, rev_map = rop_map(r.reac, r.ford), rop_map(r.prod, r.rord)
+                rop = fwd_map if d == 0 else rev_map
+                sre = [(idx[n], c) for n, c in (r.reac if d == 0 else r.prod)]
+                spr = [(idx[n], c) for n, c in (r.prod if d == 0 else r.reac)]
+                ni.append((s_id, rop, sre, spr))
+                step_order.append(sum(p for _, p in rop))
+            else:
+                step_order.append(float(len(reac_lists[-1])))
     nstep = len(step_rxn)
     fwd_of = {}
     rev_of = {}
-    for s in range(nstep):
-        (fwd_of if step_dir[s] == 1 else rev_of)[step_rxn[s]] = s
+    for s_ in range(nstep):
+        (fwd_of if step_dir[s_] == 1 else rev_of)[step_rxn[s_]] = s_
+    step_ni = np.full(nstep, -1, dtype=np.int32)
+    for q, e in enumerate(ni):
+        step_ni[e[0]] = q
 
     def csr(lists):
         ptr = np.zeros(len(lists) + 1, dtype=np.int32)
@@ -366,27 +458,49 @@ def build_tables(mech_path: str, therm_path: str | None) -> Tables:
     reac_ptr, reac_sp = csr(reac_lists)
     prod_ptr, prod_sp = csr(prod_lists)
 
-    # Arrhenius steps with the reference unit conversion, then merging
+    # Arrhenius steps with the reference unit conversion, then merging; PLOG
+    # forward steps have their own tables
     arr = []  # (A, Tpow, Tact, step)
     keq = []
+    plog = []  # (step, [(p, lnA, sign, beta, Ta) sorted], groups)
     for j, r in enumerate(rxns):
-        s = fwd_of[j]
-        order = reac_ptr[s + 1] - reac_ptr[s]
-        A = r.A * 1000.0 ** (1.0 - order)
-        if r.third_body == "M" and not r.falloff:
-            A /= 1000.0
-        arr.append((A, r.n, r.E, s))
+        s_ = fwd_of[j]
+        tb = r.third_body == "M" and not r.falloff
+        if not r.plog:
+            A = r.A * 1000.0 ** (1.0 - step_order[s_])
+            if tb:
+                A /= 1000.0
+            arr.append((A, r.n, r.E, s_))
+        else:
+            ac = 1000.0 ** (1.0 - step_order[s_])
+            ln = [[p * P_ATM, A * ac, n, E] for p, A, n, E in r.plog]
+            for a in range(len(ln) - 1):      # selection sort (plog_reaction.cpp:54-94)
+                mi = a
+                for b in range(a + 1, len(ln)):
+                    if ln[b][0] < ln[mi][0]:
+                        mi = b
+                if mi != a:
+                    ln[a], ln[mi] = ln[mi], ln[a]
+            pts, lptr = [], []
+            for a, l in enumerate(ln):
+                if not pts or abs(l[0] - pts[-1]) > abs(pts[-1] * 1.0e-8):
+                    pts.append(l[0])
+                    lptr.append(a)
+            lptr.append(len(ln))
+            plog.append((s_, pts, lptr, ln))
         if j in rev_of:
             sr = rev_of[j]
             if r.rev and r.rev[0] > 0.0:
-                order_r = reac_ptr[sr + 1] - reac_ptr[sr]
-                Ar = r.rev[0] * 1000.0 ** (1.0 - order_r)
-                if r.third_body == "M" and not r.falloff:
+                Ar = r.rev[0] * 1000.0 ** (1.0 - step_order[sr])
+                if tb:
                     Ar /= 1000.0
                 arr.append((Ar, r.rev[1], r.rev[2], sr))
             else:
-                nd = float(len(prod_lists[s]) - len(reac_lists[s]))
-                keq.append((sr, s, nd))
+                if step_ni[s_] >= 0:
+                    nd = step_order[sr] - step_order[s_]
+                else:
+                    nd = float(len(prod_lists[s_]) - len(reac_lists[s_]))
+                keq.append((sr, s_, nd))
     arr_sorted = sorted(arr, key=lambda e: (e[0], -e[2], e[1]))
 
     def same(x, y):
@@ -423,14 +537,20 @@ def build_tables(mech_path: str, therm_path: str | None) -> Tables:
         if r.third_body == "M" and not r.falloff:
             tb_fwd.append(fwd_of[j]); tb_rev.append(rev_of.get(j, -1)); tb_l.append(eff_list(r))
         if r.falloff:
-            s = fwd_of[j]
-            order = reac_ptr[s + 1] - reac_ptr[s]
-            fo_fwd.append(s); fo_rev.append(rev_of.get(j, -1))
+            s_ = fwd_of[j]
+            fo_fwd.append(s_); fo_rev.append(rev_of.get(j, -1))
             single = r.third_body.upper() != "M"
             fo_spc.append(idx[r.third_body] if single else -1)
             fo_l.append(eff_list(r) if not single else [])
-            p = [math.log(r.low[0] * 1000.0 ** (-order)), r.low[1], r.low[2], 0.0, 0.0, 0.0, 0.0]
-            if r.troe:
+            p = [math.log(r.low[0] * 1000.0 ** (-step_order[s_])), r.low[1], r.low[2],
+                 0.0, 0.0, 0.0, 0.0, 0.0]
+            if r.sri:
+                c = r.sri[2]
+                p[3:8] = [r.sri[0], r.sri[1], 1.0 / c if c > 0 else -1.0,
+                          r.sri[3] if len(r.sri) >= 4 else 1.0,
+                          r.sri[4] if len(r.sri) == 5 else 0.0]
+                fo_type.append(FALLOFF_SRI)
+            elif r.troe:
                 p[3:3 + len(r.troe)] = list(r.troe)
                 fo_type.append(FALLOFF_TROE3 if len(r.troe) == 3 else FALLOFF_TROE4)
             else:
@@ -441,7 +561,27 @@ def build_tables(mech_path: str, therm_path: str | None) -> Tables:
     fo_ptr, fo_sp = csr([[e[0] for e in l] for l in fo_l])
     fo_eff = np.array([e[1] for l in fo_l for e in l])
     I = lambda v: np.array(v, dtype=np.int32)
+    F = lambda v: np.array(v, dtype=np.float64)
+    ni_rop_ptr, ni_rop_sp = csr([[a for a, _ in e[1]] for e in ni])
+    ni_re_ptr, ni_re_sp = csr([[a for a, _ in e[2]] for e in ni])
+    ni_pr_ptr, ni_pr_sp = csr([[a for a, _ in e[3]] for e in ni])
+    plog_ptr = np.zeros(len(plog) + 1, dtype=np.int32)
+    plog_ptr[1:] = np.cumsum([len(e[1]) for e in plog])
+    lptr, lines = [0], []
+    for e in plog:
+        for k in range(len(e[1])):
+            lines += e[3][e[2][k]:e[2][k + 1]]
+            lptr.append(len(lines))
     return Tables(species, mw, thermo, nrxn, I(step_rxn), I(step_dir), reac_ptr, reac_sp,
                   prod_ptr, prod_sp, arr_step, arr_lnA, arr_beta, arr_Ta, keq_step, keq_fwd,
                   keq_nd, I(tb_fwd), I(tb_rev), tb_ptr, tb_sp, tb_eff, I(fo_fwd), I(fo_rev),
-                  I(fo_type), I(fo_spc), np.array(fo_par).reshape(-1, 7), fo_ptr, fo_sp, fo_eff)
+                  I(fo_type), I(fo_spc), F(fo_par).reshape(-1, 8), fo_ptr, fo_sp, fo_eff,
+                  step_ni, I([e[0] for e in ni]), ni_rop_ptr, ni_rop_sp,
+                  F([p for e in ni for _, p in e[1]]), ni_re_ptr, ni_re_sp,
+                  F([c for e in ni for _, c in e[2]]), ni_pr_ptr, ni_pr_sp,
+                  F([c for e in ni for _, c in e[3]]),
+                  I([e[0] for e in plog]), plog_ptr, F([p for e in plog for p in e[1]]),
+                  F([math.log(p) for e in plog for p in e[1]]), I(lptr),
+                  F([math.log(abs(l[1])) if l[1] != 0 else -math.inf for l in lines]),
+                  F([-1.0 if l[1] < 0.0 else 1.0 for l in lines]),
+                  F([l[2] for l in lines]), F([l[3] for l in lines]))

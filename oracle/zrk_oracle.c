@@ -85,6 +85,40 @@ void zrko_thermo(const zrko_mech* m, double T, double* cp_R, double* h_RT, doubl
   if (g_RT) thermo_g(m, T, g_RT);
 }
 
+/* PLOG rate at pressure-table point k of PLOG step i: the lines of that
+ * pressure summed with the sign of their A (plog_reaction.cpp:163-184) */
+static double plog_at(const zrko_mech* m, int i, int k, double logT, double invT) {
+  const int g = m->plog_ptr[i] + k;
+  double r = 0.0;
+  for (int q = m->plog_lptr[g]; q < m->plog_lptr[g + 1]; ++q)
+    r += m->plog_sign[q] * exp(m->plog_lnA[q] + logT * m->plog_beta[q] - invT * m->plog_Ta[q]);
+  return r;
+}
+
+/* PLogReaction::GetRateCoefficientFromTP (plog_reaction.cpp:186-320): no
+ * extrapolation; log-p interpolation between the bracketing pressures, zero
+ * when either end is not positive */
+static double plog_rate(const zrko_mech* m, int i, double T, double logT, double invT,
+                        double P) {
+  const int np = m->plog_ptr[i + 1] - m->plog_ptr[i];
+  const double* pp = m->plog_p + m->plog_ptr[i];
+  const double* lp = m->plog_lnp + m->plog_ptr[i];
+  if (T <= 0.0 || P <= 0.0) return 0.0;
+  if (np == 1 || P <= pp[0]) return plog_at(m, i, 0, logT, invT);
+  if (P >= pp[np - 1]) return plog_at(m, i, np - 1, logT, invT);
+  int r = 0;
+  for (int j = 0; j < np - 1; ++j)
+    if (pp[j] <= P && P < pp[j + 1]) {
+      r = j;
+      break;
+    }
+  const double k1 = plog_at(m, i, r, logT, invT);
+  if (k1 <= 0.0) return 0.0;
+  const double k2 = plog_at(m, i, r + 1, logT, invT);
+  if (k2 <= 0.0) return 0.0;
+  return k1 * pow(P / pp[r], log(k2 / k1) / (lp[r + 1] - lp[r]));
+}
+
 /* ------------------------------------------------------------------------- */
 /* Rate constants: rate_const::updateK (rate_const.cpp:539-582) and its parts  */
 /* ------------------------------------------------------------------------- */
@@ -100,14 +134,31 @@ static void rate_constants_ws(const zrko_mech* m, double T, const double* C, dou
   /* Arrhenius (rate_const.cpp:669-687) */
   for (int i = 0; i < m->n_arr; ++i)
     K[m->arr_step[i]] = exp(m->arr_lnA[i] + m->arr_beta[i] * logT - m->arr_Ta[i] * invT);
-  /* reverse from equilibrium (rate_const.cpp:690-731) */
+  /* PLOG at the pressure Csum Ru T, before Keq (rate_const.cpp:551, 565-568,
+   * 1188-1200; plog_reaction.cpp:163-320) */
+  if (m->n_plog > 0) {
+    const double P = Csum * NIST_RU * T;
+    for (int i = 0; i < m->n_plog; ++i)
+      K[m->plog_step[i]] = plog_rate(m, i, T, logT, invT, P);
+  }
+  /* reverse from equilibrium (rate_const.cpp:690-731); a non-integer step
+   * weighs g/RT by its rate-of-progress exponents, reverse list minus forward
+   * list (non_integer_reaction_network.cpp:335-358) */
   if (m->n_keq > 0) {
     thermo_g(m, T, G);
     for (int i = 0; i < m->n_keq; ++i) {
       const int f = m->keq_fwd[i];
       double sum = 0.0;
-      for (int k = m->prod_ptr[f]; k < m->prod_ptr[f + 1]; ++k) sum += G[m->prod_sp[k]];
-      for (int k = m->reac_ptr[f]; k < m->reac_ptr[f + 1]; ++k) sum -= G[m->reac_sp[k]];
+      if (m->n_ni > 0 && m->step_ni[f] >= 0) {
+        const int a = m->step_ni[f], b = m->step_ni[m->keq_step[i]];
+        for (int k = m->ni_rop_ptr[b]; k < m->ni_rop_ptr[b + 1]; ++k)
+          sum += m->ni_rop_pow[k] * G[m->ni_rop_sp[k]];
+        for (int k = m->ni_rop_ptr[a]; k < m->ni_rop_ptr[a + 1]; ++k)
+          sum -= m->ni_rop_pow[k] * G[m->ni_rop_sp[k]];
+      } else {
+        for (int k = m->prod_ptr[f]; k < m->prod_ptr[f + 1]; ++k) sum += G[m->prod_sp[k]];
+        for (int k = m->reac_ptr[f]; k < m->reac_ptr[f + 1]; ++k) sum -= G[m->reac_sp[k]];
+      }
       const double arg = sum - m->keq_ndelta[i] * log_patm_rut;
       K[m->keq_step[i]] = exp(arg) * K[f];
     }
@@ -119,9 +170,9 @@ static void rate_constants_ws(const zrko_mech* m, double T, const double* C, dou
     K[m->tb_fwd[i]] *= cm;
     if (m->tb_rev[i] >= 0) K[m->tb_rev[i]] *= cm;
   }
-  /* falloff (rate_const.cpp:890-1012; Lindemann and Troe) */
+  /* falloff (rate_const.cpp:890-1012; Lindemann, Troe and SRI) */
   for (int i = 0; i < m->n_fo; ++i) {
-    const double* p = m->fo_param + 7 * i;
+    const double* p = m->fo_param + 8 * i;
     const double klow = exp(p[0] + p[1] * logT - p[2] * invT);
     double cm;
     if (m->fo_spc[i] >= 0) {
@@ -147,6 +198,15 @@ static void rate_constants_ws(const zrko_mech* m, double T, const double* C, dou
       lpr *= lpr;
       f /= (1.0 + lpr);
       f = pow(10.0, f);
+    } else if (m->fo_type[i] == 3) {
+      /* d (a exp(-b/T) + exp(-T/c))^X T^e with X = 1/(1 + log10(Pr)^2);
+       * d = 1 and e = 0 multiply exactly when they were not given */
+      const double xp = 1.0 / (1.0 + lpr * lpr);
+      f = p[3] * exp(-p[4] * invT);
+      if (p[5] > 0) f += exp(-T * p[5]);
+      f = pow(f, xp);
+      f *= p[6];
+      f *= pow(T, p[7]);
     }
     const double pc = f * Pr / (1.0 + Pr);
     K[m->fo_fwd[i]] *= pc;
@@ -170,6 +230,23 @@ static void rates_ws(const zrko_mech* m, double T, const double* C, double L, do
     for (int k = m->reac_ptr[s]; k < m->reac_ptr[s + 1]; ++k) rop[s] *= C[m->reac_sp[k]];
   memset(create, 0, sizeof(double) * (size_t)m->nsp);
   memset(destroy, 0, sizeof(double) * (size_t)m->nsp);
+  /* non-integer steps first (perf_net.cpp:250-253;
+   * non_integer_reaction_network.cpp:270-333): the rate of progress takes
+   * sign(C) |C|^p per exponent, then creation and destruction by coefficient */
+  for (int q = 0; q < m->n_ni; ++q) {
+    const int s = m->ni_step[q];
+    for (int k = m->ni_rop_ptr[q]; k < m->ni_rop_ptr[q + 1]; ++k) {
+      const double c = C[m->ni_rop_sp[k]];
+      const double f = pow(fabs(c), m->ni_rop_pow[k]);
+      rop[s] *= (c >= 0.0) ? f : -f;
+    }
+  }
+  for (int q = 0; q < m->n_ni; ++q)
+    for (int k = m->ni_pr_ptr[q]; k < m->ni_pr_ptr[q + 1]; ++k)
+      create[m->ni_pr_sp[k]] += m->ni_pr_nu[k] * rop[m->ni_step[q]];
+  for (int q = 0; q < m->n_ni; ++q)
+    for (int k = m->ni_re_ptr[q]; k < m->ni_re_ptr[q + 1]; ++k)
+      destroy[m->ni_re_sp[k]] += m->ni_re_nu[k] * rop[m->ni_step[q]];
   for (int s = 0; s < m->nstep; ++s)
     for (int k = m->reac_ptr[s]; k < m->reac_ptr[s + 1]; ++k) destroy[m->reac_sp[k]] += rop[s];
   for (int s = 0; s < m->nstep; ++s)
@@ -431,6 +508,26 @@ static void reactor_jacobian(reactor_t* r, double t, const double* y, const doub
       for (int q = m->prod_ptr[s]; q < m->prod_ptr[s + 1]; ++q)
         J[col * N + m->prod_sp[q]] += r->rop[s] * t2[col];
     }
+  /* non-integer terms (non_integer_reaction_network.cpp:493-506, 774-905):
+   * summed per entry in process order (step, exponent species with a non-zero
+   * exponent, reactants then products), then added to the entry */
+  if (m->n_ni > 0) {
+    double* nj = (double*)calloc((size_t)N * N, sizeof(double));
+    for (int q = 0; q < m->n_ni; ++q) {
+      const double rp = r->rop[m->ni_step[q]];
+      for (int k = m->ni_rop_ptr[q]; k < m->ni_rop_ptr[q + 1]; ++k) {
+        const double pw = m->ni_rop_pow[k];
+        if (pw == 0.0) continue;
+        const int col = m->ni_rop_sp[k];
+        for (int e = m->ni_re_ptr[q]; e < m->ni_re_ptr[q + 1]; ++e)
+          nj[col * N + m->ni_re_sp[e]] += (-m->ni_re_nu[e] * pw) * rp * t2[col];
+        for (int e = m->ni_pr_ptr[q]; e < m->ni_pr_ptr[q + 1]; ++e)
+          nj[col * N + m->ni_pr_sp[e]] += (m->ni_pr_nu[e] * pw) * rp * t2[col];
+      }
+    }
+    for (int i = 0; i < N * N; ++i) J[i] += nj[i];
+    free(nj);
+  }
   /* d(Tdot)/dy_j */
   const double RuTemp = NIST_RU * y[nsp];
   for (int j = 0; j < nsp; ++j) {

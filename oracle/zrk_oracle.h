@@ -7,11 +7,14 @@
  * Each function cites the reference code it restates:
  *   zrko_thermo            src/zerork/nasa_poly.cpp:50-172
  *   zrko_rate_constants    src/zerork/rate_const.cpp:539-582 (updateK), 669-731,
- *                          862-1012
+ *                          862-1012, 1188-1200; plog_reaction.cpp:163-320;
+ *                          non_integer_reaction_network.cpp:335-358 (Keq)
  *   zrko_rates             src/zerork/perf_net.cpp:190-275
- *                          (calcRatesFromTC_StepLimiter)
+ *                          (calcRatesFromTC_StepLimiter),
+ *                          non_integer_reaction_network.cpp:270-333
  *   zrko_rhs               applications/cfd_plugin/reactor_constant_pressure_cpu.cpp:68-133
- *   zrko_jacobian          applications/cfd_plugin/reactor_nvector_serial.cpp:610-771
+ *   zrko_jacobian          applications/cfd_plugin/reactor_nvector_serial.cpp:610-771,
+ *                          non_integer_reaction_network.cpp:493-506, 774-905
  *   zrko_integrate         applications/cfd_plugin/solver_cvode.cpp:51-299 driving a
  *                          restatement of SUNDIALS CVODE v5.8.0 BDF (dense direct
  *                          Newton, the plugin's dense=1/iterative=0 mode), pinned
@@ -50,12 +53,37 @@ typedef struct {
   int n_fo;
   const int* fo_fwd;
   const int* fo_rev;
-  const int* fo_type;      /* 0 Lindemann, 1 Troe-3, 2 Troe-4 */
+  const int* fo_type;      /* 0 Lindemann, 1 Troe-3, 2 Troe-4, 3 SRI */
   const int* fo_spc;       /* -1: mixture M */
-  const double* fo_param;  /* [n_fo*7] lnAlow, beta, Ta, alpha, T***, T*, T** */
+  const double* fo_param;  /* [n_fo*8] lnAlow, beta, Ta, then alpha, T***, T*, T** (Troe)
+                              or a, b, 1/c (-1 if c <= 0), d, e (SRI) */
   const int* fo_ptr;
   const int* fo_sp;
   const double* fo_eff;
+  /* non-integer steps (no expanded participants), in step order */
+  int n_ni;
+  const int* step_ni;      /* [nstep] index into the ni lists or -1 */
+  const int* ni_step;
+  const int* ni_rop_ptr;   /* CSR of rate-of-progress exponents */
+  const int* ni_rop_sp;
+  const double* ni_rop_pow;
+  const int* ni_re_ptr;    /* CSR of destroyed species and coefficients */
+  const int* ni_re_sp;
+  const double* ni_re_nu;
+  const int* ni_pr_ptr;    /* CSR of created species and coefficients */
+  const int* ni_pr_sp;
+  const double* ni_pr_nu;
+  /* PLOG forward steps */
+  int n_plog;
+  const int* plog_step;
+  const int* plog_ptr;     /* CSR over distinct pressures */
+  const double* plog_p;    /* Pa */
+  const double* plog_lnp;
+  const int* plog_lptr;    /* CSR (over all pressures) of Arrhenius lines */
+  const double* plog_lnA;
+  const double* plog_sign;
+  const double* plog_beta;
+  const double* plog_Ta;
 } zrko_mech;
 
 typedef struct {

@@ -72,6 +72,28 @@ def run_probe(mech, therm, T, P, Y):
     return res
 
 
+H2_FEATURES = """! ---- added for the extended-record tests ----
+   ho2+h <=> oh+oh   1.0e13 0.0 0.0
+     plog / 0.1   3.0e13  0.0  300.0 /
+     plog / 1.0   5.0e13  0.0  300.0 /
+     plog / 1.0   2.0e13  0.0  600.0 /
+     plog / 10.0  9.0e13  0.0  250.0 /
+     plog / 100.0 1.2e14 -0.1  200.0 /
+   h+o2(+m) <=> ho2(+m)   4.0e12  0.5  0.0
+     low / 6.0e18 -1.0 0.0 /
+     sri / 0.45 797.0 979.0 /
+     h2/2.0/ h2o/11.0/
+   h+oh(+m) <=> h2o(+m)   2.0e13  0.0  0.0
+     low / 4.0e22 -2.0 0.0 /
+     sri / 0.5 500.0 1200.0 1.1 0.2 /
+   2h2 + o2 => 2h2o   1.0e11 0.0 3.0e4
+     ford / h2 1.1 /
+     ford / o2 0.6 /
+   h2 + 0.5o2 <=> h2o   5.0e9 0.0 2.0e4
+     rord / h2o 1.2 /
+   h2o2 => 0.5o2 + h2o   2.0e12 0.0 4.0e4
+"""
+
 # state seeds of the synthetic cases (kept from when they were first written)
 SEEDS = {"synth_10sp": 5256, "synth_53sp": 5257, "synth_160sp": 5258}
 
@@ -93,6 +115,16 @@ def main():
          "data/mechanisms/hydrogen/h2_v1a_therm.txt"),
         ("ref_troe_test", "data/mechanisms/ideal/troe_test.mech",
          "data/mechanisms/ideal/const_specific_heat.therm"),
+        ("ref_sri", "data/mechanisms/ideal/sri_reaction.mech",
+         "data/mechanisms/ideal/const_specific_heat.therm"),
+        ("ref_plog", "data/mechanisms/ideal/plog_test.mech",
+         "data/mechanisms/ideal/const_specific_heat.therm"),
+        ("ref_non_integer", "data/mechanisms/ideal/non_integer_test.mech",
+         "data/mechanisms/ideal/const_specific_heat.therm"),
+        ("ref_ford_propane", "data/mechanisms/real_order/ford_propane.mech",
+         "data/mechanisms/ideal/const_specific_heat.therm"),
+        ("ref_rord_propane", "data/mechanisms/real_order/rord_propane.mech",
+         "data/mechanisms/ideal/const_specific_heat.therm"),
     ]
     for name, mrel, trel in refdata:
         mp = os.path.join(MECH_DIR, name + ".mech")
@@ -102,6 +134,18 @@ def main():
         os.chmod(mp, 0o644)
         os.chmod(tp, 0o644)
         cases.append((name, mp, tp, None, None))
+    # the H2 mechanism with mass-conserving PLOG, SRI, FORD/RORD and
+    # fractional-coefficient reactions added: one realistic mechanism that
+    # exercises every extended record kind together
+    mp = os.path.join(MECH_DIR, "ref_h2_features.mech")
+    tp = os.path.join(MECH_DIR, "ref_h2_features.therm")
+    text = open(os.path.join(REF, "data/mechanisms/hydrogen/h2_v1b_mech.txt")).read()
+    cut = text.lower().rstrip().rfind("end")
+    with open(mp, "w") as f:
+        f.write(text[:cut] + H2_FEATURES + text[cut:])
+    shutil.copyfile(os.path.join(REF, "data/mechanisms/hydrogen/h2_v1a_therm.txt"), tp)
+    os.chmod(tp, 0o644)
+    cases.append(("ref_h2_features", mp, tp, None, None))
     sys.path.insert(0, ROOT)
     from oracle import ckparse
     for name, mp, tp, m, wl in cases:
@@ -109,9 +153,11 @@ def main():
         nsp = t.nsp
         seed = SEEDS.get(name, 4242 + sum(map(ord, name)))
         rng = np.random.default_rng(seed)
-        n = {"synth_160sp": 6, "ref_dme": 6}.get(name, 16)
+        n = {"synth_160sp": 6, "ref_dme": 6, "ref_plog": 48}.get(name, 16)
         T = rng.uniform(700.0, 2400.0, n)
-        P = np.exp(rng.uniform(np.log(0.3), np.log(60.0), n)) * 101325.0
+        # PLOG tables span 0.01-100 atm: sample beyond both ends
+        plo, phi = (0.003, 300.0) if name == "ref_plog" else (0.3, 60.0)
+        P = np.exp(rng.uniform(np.log(plo), np.log(phi), n)) * 101325.0
         Y = rng.dirichlet(np.ones(nsp), n)
         if wl is not None:  # add states from the workload distribution (zeros present)
             st = mechgen.workload_states(wl, m, 4, seed + 1)

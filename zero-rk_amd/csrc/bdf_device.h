@@ -102,8 +102,14 @@ __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s
           cur = col;
           v = 0.0;
         }
-        const double p = __dmul_rn(s.rop[t & 0xFFFFu], t2l[col]);
-        v = (t & 0x80000000u) ? __dsub_rn(v, p) : __dadd_rn(v, p);
+        if (t & kJtNonInt) {
+          // (multiplier * ROP) / C_col (non_integer_reaction_network.cpp:493-506)
+          const double2 e = m.ni_jt[t & 0xFFFFu];
+          v = __dadd_rn(v, __dmul_rn(__dmul_rn(e.x, s.rop[(int)e.y]), t2l[col]));
+        } else {
+          const double p = __dmul_rn(s.rop[t & 0xFFFFu], t2l[col]);
+          v = (t & 0x80000000u) ? __dsub_rn(v, p) : __dadd_rn(v, p);
+        }
         Jl[col * ld + lane] = (MT)v;
       }
     };
@@ -191,8 +197,13 @@ __device__ __forceinline__ double jacobian_team(const Team& team, const DevMech&
           cur = col;
           v = 0.0;
         }
-        const double p = __dmul_rn(s.rop[t & 0xFFFFu], t2l[col]);
-        v = (t & 0x80000000u) ? __dsub_rn(v, p) : __dadd_rn(v, p);
+        if (t & kJtNonInt) {
+          const double2 e = m.ni_jt[t & 0xFFFFu];
+          v = __dadd_rn(v, __dmul_rn(__dmul_rn(e.x, s.rop[(int)e.y]), t2l[col]));
+        } else {
+          const double p = __dmul_rn(s.rop[t & 0xFFFFu], t2l[col]);
+          v = (t & 0x80000000u) ? __dsub_rn(v, p) : __dadd_rn(v, p);
+        }
         J[col * js + tid] = (MT)v;
       }
     };

@@ -209,12 +209,160 @@ __device__ __forceinline__ RecLoad load_rec(const DevMech& m, int r) {
   return x;
 }
 
+// PLOG forward rate coefficient at temperature p.T and pressure P from the
+// table at t (plog_reaction.cpp:163-320): the lines of one pressure are summed
+// with the sign of their A; between two tabulated pressures the rate is
+// interpolated linearly in log p (zero if either end is not positive);
+// outside the table the end pressure's rate is used.
+__device__ __forceinline__ double plog_at(const double* __restrict__ t, int np, int k,
+                                          const TPow& p) {
+  const double* __restrict__ lptr = t + 1 + 2 * np;
+  const double* __restrict__ ln = lptr + np + 1;
+  double r = 0.0;
+  const int a = (int)lptr[k], b = (int)lptr[k + 1];
+  for (int q = a; q < b; ++q) {
+    const double* __restrict__ l = ln + 4 * q;
+    r += l[1] * exp(l[0] + p.logT * l[2] - p.invT * l[3]);
+  }
+  return r;
+}
+
+__device__ __forceinline__ double plog_rate(const double* __restrict__ t, const TPow& p,
+                                            double P) {
+  const int np = (int)t[0];
+  const double* __restrict__ pp = t + 1;
+  const double* __restrict__ lp = pp + np;
+  if (p.T <= 0.0 || P <= 0.0) return 0.0;
+  if (np == 1 || P <= pp[0]) return plog_at(t, np, 0, p);
+  if (P >= pp[np - 1]) return plog_at(t, np, np - 1, p);
+  int r = 0;
+  while (r + 2 < np && P >= pp[r + 1]) ++r;  // pp[r] <= P < pp[r + 1]
+  const double k1 = plog_at(t, np, r, p);
+  if (k1 <= 0.0) return 0.0;
+  const double k2 = plog_at(t, np, r + 1, p);
+  if (k2 <= 0.0) return 0.0;
+  return k1 * pow(P / pp[r], log(k2 / k1) / (lp[r + 1] - lp[r]));
+}
+
+// k times prod sign(C_s) |C_s|^p_s over a non-integer step's exponent list
+// (non_integer_reaction_network.cpp:270-293).
+__device__ __forceinline__ double ni_rop(double k, const DevMech& m, const double* __restrict__ C,
+                                         int off, int n) {
+  for (int q = off; q < off + n; ++q) {
+    const double c = C[m.ni_sp[q]];
+    const double f = pow(fabs(c), m.ni_pow[q]);
+    k *= (c >= 0.0) ? f : -f;
+  }
+  return k;
+}
+
 // Build-time knob: 1 loads the next record while the current one is
 // evaluated (hides the load latency at the cost of about 22 vector registers;
 // measured no faster in the integrator at two waves per SIMD, so off).
 #ifndef ZRK_RATES_PREFETCH
 #define ZRK_RATES_PREFETCH 0
 #endif
+
+// One record of any kind, plainly evaluated: the records with a PLOG forward
+// rate, non-integer exponents or SRI blending (rare; kept out of the main
+// loop so that its register use does not grow).  The order of the updates is
+// the reference's (rate_const.cpp:539-582): Arrhenius or PLOG (precomputed
+// into rop), reverse from REV or Keq, third body, fall-off; then the limiter
+// (perf_net.cpp:204-216) and the rates of progress.
+__device__ __forceinline__ void rates_general_record(const DevMech& m, int r,
+                                                  const double* __restrict__ C,
+                                                  const double* __restrict__ G,
+                                                  double* __restrict__ rop, const TPow& p,
+                                                  double Csum, double L) {
+  const RecLoad x = load_rec(m, r);
+  const int fl = x.meta.z;
+  const int4 xe = m.r_ext[fl >> kRecExtShift];
+  const int re[kMaxPart] = {lo16(x.sp.x), hi16(x.sp.x), lo16(x.sp.y), hi16(x.sp.y)};
+  const int pr[kMaxPart] = {lo16(x.sp.z), hi16(x.sp.z), lo16(x.sp.w), hi16(x.sp.w)};
+  const int nf = xe.w & 0xFFFF, nrv = xe.w >> 16;
+  // a PLOG forward rate was stored in rop[forward step] by the pass before
+  double kf = (fl & kRecPlog) ? rop[x.meta.x] : exp(x.c0.x + x.c0.y * p.logT - x.c1.x * p.invT);
+  double kr = 0.0;
+  if (fl & kRecRevArr) {
+    kr = exp(x.c1.y + x.c2.x * p.logT - x.c2.y * p.invT);
+  } else if (fl & kRecRevKeq) {
+    double s = 0.0;
+    if (fl & kRecNonInt) {
+      // exponent-weighted change of g/RT: reverse list minus forward list
+      // (non_integer_reaction_network.cpp:335-358)
+      for (int k = xe.z; k < xe.z + nrv; ++k) s += m.ni_pow[k] * G[m.ni_sp[k]];
+      for (int k = xe.y; k < xe.y + nf; ++k) s -= m.ni_pow[k] * G[m.ni_sp[k]];
+    } else {
+#pragma unroll
+      for (int k = 0; k < kMaxPart; ++k) s += G[pr[k]];
+#pragma unroll
+      for (int k = 0; k < kMaxPart; ++k) s -= G[re[k]];
+    }
+    kr = exp(s - x.c1.y * p.log_patm_rut) * kf;
+  }
+  if (fl & (kRecThirdBody | kRecFalloff)) {
+    double cm;
+    if (x.meta.w >= 0) {
+      cm = C[x.meta.w];
+    } else {
+      cm = Csum;
+      for (int k = x.e0; k < x.e1; ++k) cm += C[m.e_sp[k]] * m.e_eff[k];
+    }
+    if (fl & kRecThirdBody) {
+      kf *= cm;
+      kr *= cm;
+    } else {
+      const double* __restrict__ q = m.r_fo + 8 * r;
+      const double klow = exp(q[0] + q[1] * p.logT - q[2] * p.invT);
+      double Pr = klow * cm / kf;
+      if (Pr < 1.0e-300) Pr = 1.0e-300;
+      double lpr = log10(Pr);
+      double f = 1.0;
+      if (fl & kRecTroe) {
+        double fc = 0.0;
+        if (q[4] != 0) fc += (1.0 - q[3]) * exp(-p.T / q[4]);
+        if (q[5] != 0) fc += q[3] * exp(-p.T / q[5]);
+        if (fl & kRecTroe4) fc += exp(-q[6] * p.invT);
+        if (fc < 1.0e-300) fc = 1.0e-300;
+        f = log10(fc);
+        const double n = 0.75 - 1.27 * f;
+        lpr -= (0.4 + 0.67 * f);
+        lpr = lpr / (n - 0.14 * lpr);
+        lpr *= lpr;
+        f /= (1.0 + lpr);
+        f = pow(10.0, f);
+      } else if (fl & kRecSri) {
+        // d (a exp(-b/T) + exp(-T/c))^X T^e, X = 1/(1 + log10(Pr)^2)
+        // (rate_const.cpp:975-1004; d = 1 and e = 0 when not given)
+        const double xp = 1.0 / (1.0 + lpr * lpr);
+        f = q[3] * exp(-q[4] * p.invT);
+        if (q[5] > 0) f += exp(-p.T * q[5]);
+        f = pow(f, xp);
+        f *= q[6];
+        f *= pow(p.T, q[7]);
+      }
+      const double pc = f * Pr / (1.0 + Pr);
+      kf *= pc;
+      kr *= pc;
+    }
+  }
+  if (0.0 < L && L < 1.0e300) {
+    kf *= L / (L + kf);
+    kr *= L / (L + kr);
+  }
+  for (int k = 0; k < kMaxPart; ++k) kf *= C[re[k]];
+  const int rs = x.meta.y;
+  if (fl & kRecNonInt) {
+    // a non-integer reaction has no expanded participants: its exponent
+    // lists give the rates of progress
+    kf = ni_rop(kf, m, C, xe.y, nf);
+    if (rs >= 0) kr = ni_rop(kr, m, C, xe.z, nrv);
+  } else if (rs >= 0) {
+    for (int k = 0; k < kMaxPart; ++k) kr *= C[pr[k]];
+  }
+  rop[x.meta.x] = kf;
+  if (rs >= 0) rop[rs] = kr;
+}
 
 // Rate constants, limiter and rates of progress of every step of one reactor.
 // C, G: species concentrations and g/RT in the wave's LDS, with the entries
@@ -230,7 +378,9 @@ __device__ __forceinline__ void team_rates(int tid, const DevMech& m, const doub
                                            const double* __restrict__ G, double* __restrict__ rop,
                                            const TPow& p, double Csum, double L) {
   const bool limit = (0.0 < L && L < 1.0e300);
-  const int nrec = m.nrec;
+  // PLOG, non-integer and SRI records come last and are evaluated by
+  // rates_general_record
+  const int nrec = m.nrec - m.n_ext;
   if (tid == 0) rop[m.nstep] = 0.0;
   int r = tid;
 #if ZRK_RATES_PREFETCH
@@ -254,7 +404,7 @@ __device__ __forceinline__ void team_rates(int tid, const DevMech& m, const doub
     const int e0 = enh ? x.e0 : 0;
     const int4 es0 = es[e0 >> 2];
     const double2 ea0 = ee[e0 >> 1], eb0 = ee[(e0 >> 1) + 1];
-    const double* __restrict__ qp = m.r_fo + 7 * ((fl & kRecFalloff) ? r : 0);
+    const double* __restrict__ qp = m.r_fo + 8 * ((fl & kRecFalloff) ? r : 0);
     double q[7];
 #pragma unroll
     for (int k = 0; k < 7; ++k) q[k] = qp[k];
@@ -342,6 +492,18 @@ __device__ __forceinline__ void team_rates(int tid, const DevMech& m, const doub
       rop[rs] = kr;
     }
   }
+  if (m.n_ext > 0) {
+    // PLOG forward rates at the pressure of the concentrations, Csum Ru T
+    // (rate_const.cpp:551, 565-568), in a pass of their own (kept apart from
+    // the record evaluation, which then reads them before the reverse rate
+    // from Keq); a record is handled by the same thread in both passes
+    for (int q = nrec + tid; q < m.nrec; q += S) {
+      const int4 meta = m.r_meta[q];
+      if (meta.z & kRecPlog)
+        rop[meta.x] = plog_rate(m.plog + m.r_ext[meta.z >> kRecExtShift].x, p, Csum * kRu * p.T);
+    }
+    for (int q = nrec + tid; q < m.nrec; q += S) rates_general_record(m, q, C, G, rop, p, Csum, L);
+  }
 }
 
 // Creation and destruction rates of species j (gathers in the reference's
@@ -352,6 +514,17 @@ __device__ __forceinline__ void species_cd(const DevMech& m, const double* __res
   const uint4* __restrict__ gd = m.g_d + (size_t)jb * m.gd_blocks * kWave + jl;
   const uint4* __restrict__ gc = m.g_c + (size_t)jb * m.gc_blocks * kWave + jl;
   double d = 0.0, c = 0.0;
+  if (m.has_nig) {
+    // non-integer terms first (perf_net.cpp:250-253)
+    for (int k = m.nig_dptr[j]; k < m.nig_dptr[j + 1]; ++k) {
+      const double2 e = m.nig[k];
+      d += e.x * rop[(int)e.y];
+    }
+    for (int k = m.nig_cptr[j]; k < m.nig_cptr[j + 1]; ++k) {
+      const double2 e = m.nig[k];
+      c += e.x * rop[(int)e.y];
+    }
+  }
   const int nd = m.gd_blocks, nc = m.gc_blocks;
   const int nb = nd > nc ? nd : nc;
   uint4 wd = nd > 0 ? gd[0] : make_uint4(0, 0, 0, 0);

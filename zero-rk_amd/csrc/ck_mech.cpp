@@ -290,6 +290,7 @@ bool parse_reactions(const std::vector<std::string>& lines, const std::vector<st
       size_t k = j;
       while (k < s.size() && std::isspace((unsigned char)s[k])) ++k;
       std::vector<double> vals;
+      std::vector<std::string> raw;
       bool has_vals = false;
       if (k < s.size() && s[k] == '/') {
         size_t e = s.find('/', k + 1);
@@ -298,19 +299,31 @@ bool parse_reactions(const std::vector<std::string>& lines, const std::vector<st
           return false;
         }
         has_vals = true;
-        for (auto& t : tokens(s.substr(k + 1, e - k - 1))) {
-          double v;
-          if (!to_double(t, &v)) {
-            *err = "bad number in: " + s;
-            return false;
-          }
-          vals.push_back(v);
-        }
+        raw = tokens(s.substr(k + 1, e - k - 1));
         i = e + 1;
       } else {
         i = j;
       }
       std::string key = upper(name);
+      if (key == "FORD" || key == "RORD") {
+        // species name and its reaction order (CKParser.cpp:1615-1639)
+        double v;
+        if (raw.size() != 2 || !to_double(raw[1], &v)) {
+          *err = key + " needs a species and an order: " + s;
+          return false;
+        }
+        (key == "FORD" ? r.ford : r.rord)[raw[0]] = v;
+        r.real_order = true;
+        continue;
+      }
+      for (auto& t : raw) {
+        double v;
+        if (!to_double(t, &v)) {
+          *err = "bad number in: " + s;
+          return false;
+        }
+        vals.push_back(v);
+      }
       if (key == "DUP" || key == "DUPLICATE") {
       } else if (key == "LOW") {
         if (vals.size() != 3) { *err = "LOW needs 3 values"; return false; }
@@ -318,13 +331,22 @@ bool parse_reactions(const std::vector<std::string>& lines, const std::vector<st
         r.low[0] = vals[0]; r.low[1] = vals[1]; r.low[2] = vals[2] * conv;
       } else if (key == "TROE") {
         if (vals.size() != 3 && vals.size() != 4) { *err = "TROE needs 3 or 4 values"; return false; }
+        if (!r.sri.empty()) { *err = "cannot specify both SRI and TROE"; return false; }
         r.troe = vals;
+      } else if (key == "SRI") {
+        if (vals.size() < 3 || vals.size() > 5) { *err = "SRI needs 3 to 5 values"; return false; }
+        if (!r.troe.empty()) { *err = "cannot specify both SRI and TROE"; return false; }
+        r.sri = vals;
+      } else if (key == "PLOG") {
+        // one line per pressure: p [atm], A, n, E (CKParser.cpp:1643-1669)
+        if (vals.size() != 4) { *err = "expected 4 PLOG parameters"; return false; }
+        if (vals[0] < 0.0) { *err = "PLOG pressures are required to be >= 0.0"; return false; }
+        r.plog.push_back({vals[0], vals[1], vals[2], vals[3] * conv});
       } else if (key == "REV") {
         if (vals.size() != 3) { *err = "REV needs 3 values"; return false; }
         r.has_rev = true;
         r.rev[0] = vals[0]; r.rev[1] = vals[1]; r.rev[2] = vals[2] * conv;
-      } else if (key == "SRI" || key == "PLOG" || key == "FORD" || key == "RORD" || key == "HIGH" ||
-                 key == "LT" || key == "RLT" || key == "CHEB") {
+      } else if (key == "HIGH" || key == "LT" || key == "RLT" || key == "CHEB") {
         *err = "unsupported auxiliary keyword " + key;
         return false;
       } else if (has_vals && vals.size() == 1) {
@@ -436,7 +458,38 @@ bool load_mechanism(const std::string& mech_file, const std::string& therm_file,
   if (!parse_reactions(rx_lines, units, &M.rxns, err)) return false;
   M.nrxn = (int)M.rxns.size();
 
-  // step numbering (info_net.cpp:46-99) and expanded participants (:179-300)
+  // which reactions are non-integer (info_net.cpp:411-470): FORD/RORD, or a
+  // stoichiometric coefficient further than 1e-14 from an integer
+  constexpr double kStoichTol = 1.0e-14;  // src/zerork/constants.h:32
+  std::vector<char> nonint(M.nrxn, 0);
+  for (int j = 0; j < M.nrxn; ++j) {
+    const auto& r = M.rxns[j];
+    bool ni = r.real_order;
+    for (const auto* side : {&r.reac, &r.prod})
+      for (auto& e : *side) {
+        const int k = (int)(e.second + 0.5);
+        if (std::fabs(e.second - (double)k) > kStoichTol) ni = true;
+      }
+    nonint[j] = ni;
+    // PLOG exclusions (CKParser.cpp:312-345)
+    if (!r.plog.empty()) {
+      if (r.falloff || r.has_low || !r.troe.empty() || !r.sri.empty()) {
+        *err = "PLOG and falloff reactions (LOW/TROE/SRI) are mutually exclusive";
+        return false;
+      }
+      if (r.third_body) {
+        *err = "PLOG and 3rd-body reactions M or (M) are mutually exclusive";
+        return false;
+      }
+      if (r.has_rev && r.rev[0] > 0.0) {
+        *err = "PLOG does not support REV parameters being specified";
+        return false;
+      }
+    }
+  }
+
+  // step numbering (info_net.cpp:46-99) and expanded participants (:179-300);
+  // the steps of non-integer reactions have no expanded lists
   auto expand = [&](const std::vector<std::pair<std::string, double>>& lst,
                     std::vector<int>* outv) -> bool {
     for (auto& e : lst) {
@@ -445,12 +498,41 @@ bool load_mechanism(const std::string& mech_file, const std::string& therm_file,
         *err = "unknown species in reaction: " + e.first;
         return false;
       }
-      int k = (int)(e.second + 0.5);
-      if (std::fabs(e.second - k) >= 1.0e-14) {
-        *err = "non-integer stoichiometry is not supported";
+      const int k = (int)(e.second + 0.5);
+      for (int q = 0; q < k; ++q) outv->push_back(it->second);
+    }
+    return true;
+  };
+  // rate-of-progress exponents of a non-integer reaction, by species name
+  // (non_integer_reaction_network.cpp:53-205): the summed stoichiometric
+  // coefficients of each side, overwritten or extended by FORD (forward) and
+  // RORD (reverse)
+  auto rop_map = [&](const std::vector<std::pair<std::string, double>>& side,
+                     const std::map<std::string, double>& over, NonIntStep* st) -> bool {
+    std::map<std::string, double> mp;
+    for (auto& e : side) mp[e.first] += e.second;
+    for (auto& e : over) mp[e.first] = e.second;
+    for (auto& e : mp) {
+      auto it = idx.find(e.first);
+      if (it == idx.end()) {
+        *err = "unknown species in reaction order: " + e.first;
         return false;
       }
-      for (int q = 0; q < k; ++q) outv->push_back(it->second);
+      st->rop_sp.push_back(it->second);
+      st->rop_pow.push_back(e.second);
+    }
+    return true;
+  };
+  auto nu_list = [&](const std::vector<std::pair<std::string, double>>& side,
+                     std::vector<int>* sp, std::vector<double>* nu) -> bool {
+    for (auto& e : side) {
+      auto it = idx.find(e.first);
+      if (it == idx.end()) {
+        *err = "unknown species in reaction: " + e.first;
+        return false;
+      }
+      sp->push_back(it->second);
+      nu->push_back(e.second);
     }
     return true;
   };
@@ -458,55 +540,105 @@ bool load_mechanism(const std::string& mech_file, const std::string& therm_file,
   M.prod_ptr.assign(1, 0);
   M.fwd_step.assign(M.nrxn, -1);
   M.rev_step.assign(M.nrxn, -1);
+  std::vector<double> step_order;  // real order (sum of rate-of-progress exponents)
   for (int j = 0; j < M.nrxn; ++j) {
     const auto& r = M.rxns[j];
     std::vector<int> re, pr;
-    if (!expand(r.reac, &re) || !expand(r.prod, &pr)) return false;
-    M.fwd_step[j] = (int)M.step_rxn.size();
-    M.step_rxn.push_back(j);
-    M.step_dir.push_back(1);
-    M.reac_sp.insert(M.reac_sp.end(), re.begin(), re.end());
-    M.reac_ptr.push_back((int)M.reac_sp.size());
-    M.prod_sp.insert(M.prod_sp.end(), pr.begin(), pr.end());
-    M.prod_ptr.push_back((int)M.prod_sp.size());
-    bool has_rev = r.reversible && !(r.has_rev && r.rev[0] == 0.0);
-    if (has_rev) {
-      M.rev_step[j] = (int)M.step_rxn.size();
+    if (!nonint[j] && (!expand(r.reac, &re) || !expand(r.prod, &pr))) return false;
+    const bool has_rev = r.reversible && !(r.has_rev && r.rev[0] == 0.0);
+    for (int dir = 0; dir < (has_rev ? 2 : 1); ++dir) {
+      const int s = (int)M.step_rxn.size();
+      (dir == 0 ? M.fwd_step : M.rev_step)[j] = s;
       M.step_rxn.push_back(j);
-      M.step_dir.push_back(-1);
-      M.reac_sp.insert(M.reac_sp.end(), pr.begin(), pr.end());
+      M.step_dir.push_back(dir == 0 ? 1 : -1);
+      const std::vector<int>& a = dir == 0 ? re : pr;
+      const std::vector<int>& b = dir == 0 ? pr : re;
+      M.reac_sp.insert(M.reac_sp.end(), a.begin(), a.end());
       M.reac_ptr.push_back((int)M.reac_sp.size());
-      M.prod_sp.insert(M.prod_sp.end(), re.begin(), re.end());
+      M.prod_sp.insert(M.prod_sp.end(), b.begin(), b.end());
       M.prod_ptr.push_back((int)M.prod_sp.size());
+      if (nonint[j]) {
+        NonIntStep st;
+        st.step = s;
+        NonIntStep other;
+        if (!rop_map(r.reac, r.ford, dir == 0 ? &st : &other) ||
+            !rop_map(r.prod, r.rord, dir == 0 ? &other : &st) ||
+            !nu_list(dir == 0 ? r.reac : r.prod, &st.re_sp, &st.re_nu) ||
+            !nu_list(dir == 0 ? r.prod : r.reac, &st.pr_sp, &st.pr_nu))
+          return false;
+        double o = 0.0;
+        for (double q : st.rop_pow) o += q;
+        step_order.push_back(o);
+        M.ni.push_back(std::move(st));
+      } else {
+        step_order.push_back((double)a.size());
+      }
     }
   }
   M.nstep = (int)M.step_rxn.size();
+  M.step_ni.assign(M.nstep, -1);
+  for (size_t q = 0; q < M.ni.size(); ++q) M.step_ni[M.ni[q].step] = (int)q;
 
   // Arrhenius parameters with unit conversion (rate_const.cpp:212-260) and
-  // merging of triples equal within 1e-6 relative (:275-332, 1093-1128).
+  // merging of triples equal within 1e-6 relative (:275-332, 1093-1128);
+  // PLOG forward steps have no Arrhenius entry (:159-190)
   struct Arr { double A, n, Ta; int step; };
   std::vector<Arr> arr;
   M.rev_type.assign(M.nrxn, kRevNone);
   M.keq_ndelta.assign(M.nrxn, 0.0);
+  M.rxn_plog.assign(M.nrxn, -1);
   for (int j = 0; j < M.nrxn; ++j) {
     const auto& r = M.rxns[j];
-    int s = M.fwd_step[j];
-    int order = M.reac_ptr[s + 1] - M.reac_ptr[s];
-    double A = r.A * std::pow(1000.0, 1.0 - order);
-    if (r.third_body) A /= 1000.0;
-    arr.push_back({A, r.n, r.E, s});
+    const int s = M.fwd_step[j];
+    if (r.plog.empty()) {
+      double A = r.A * std::pow(1000.0, 1.0 - step_order[s]);
+      if (r.third_body) A /= 1000.0;
+      arr.push_back({A, r.n, r.E, s});
+    } else {
+      // PLOG lines (rate_const.cpp:1146-1186; plog_reaction.cpp:10-135):
+      // selection sort by pressure, then lines within 1e-8 relative of the
+      // previous distinct pressure share it
+      std::vector<std::array<double, 4>> ln = r.plog;
+      const double ac = std::pow(1000.0, 1.0 - step_order[s]);
+      for (auto& l : ln) {
+        l[0] *= 1.01325e5;
+        l[1] *= ac;
+      }
+      for (size_t a = 0; a + 1 < ln.size(); ++a) {
+        size_t mi = a;
+        for (size_t b = a + 1; b < ln.size(); ++b)
+          if (ln[b][0] < ln[mi][0]) mi = b;
+        if (mi != a) std::swap(ln[a], ln[mi]);
+      }
+      PlogStep ps;
+      ps.step = s;
+      for (size_t a = 0; a < ln.size(); ++a) {
+        if (a == 0 || std::fabs(ln[a][0] - ps.p.back()) > std::fabs(ps.p.back() * 1.0e-8)) {
+          ps.p.push_back(ln[a][0]);
+          ps.line_ptr.push_back((int)a);
+        }
+        ps.lnA.push_back(std::log(std::fabs(ln[a][1])));
+        ps.sign.push_back(ln[a][1] < 0.0 ? -1.0 : 1.0);
+        ps.beta.push_back(ln[a][2]);
+        ps.Ta.push_back(ln[a][3]);
+      }
+      ps.line_ptr.push_back((int)ln.size());
+      for (double pp : ps.p) ps.lnp.push_back(std::log(pp));
+      M.rxn_plog[j] = (int)M.plog.size();
+      M.plog.push_back(std::move(ps));
+    }
     int sr = M.rev_step[j];
     if (sr >= 0) {
       if (r.has_rev && r.rev[0] > 0.0) {
-        int order_r = M.reac_ptr[sr + 1] - M.reac_ptr[sr];
-        double Ar = r.rev[0] * std::pow(1000.0, 1.0 - order_r);
+        double Ar = r.rev[0] * std::pow(1000.0, 1.0 - step_order[sr]);
         if (r.third_body) Ar /= 1000.0;
         arr.push_back({Ar, r.rev[1], r.rev[2], sr});
         M.rev_type[j] = kRevArrhenius;
       } else {
         M.rev_type[j] = kRevKeq;
-        M.keq_ndelta[j] = (double)((M.prod_ptr[s + 1] - M.prod_ptr[s]) -
-                                   (M.reac_ptr[s + 1] - M.reac_ptr[s]));
+        M.keq_ndelta[j] = nonint[j] ? step_order[sr] - step_order[s]
+                                    : (double)((M.prod_ptr[s + 1] - M.prod_ptr[s]) -
+                                               (M.reac_ptr[s + 1] - M.reac_ptr[s]));
       }
     }
   }
@@ -541,7 +673,7 @@ bool load_mechanism(const std::string& mech_file, const std::string& therm_file,
   M.tb_sp_ptr.assign(1, 0);
   M.falloff_type.assign(M.nrxn, -1);
   M.falloff_spc.assign(M.nrxn, -1);
-  M.falloff_param.assign((size_t)M.nrxn * 7, 0.0);
+  M.falloff_param.assign((size_t)M.nrxn * kFoParams, 0.0);
   for (int j = 0; j < M.nrxn; ++j) {
     const auto& r = M.rxns[j];
     bool single = r.falloff && !r.falloff_species.empty();
@@ -567,13 +699,20 @@ bool load_mechanism(const std::string& mech_file, const std::string& therm_file,
         }
         M.falloff_spc[j] = it->second;
       }
-      int s = M.fwd_step[j];
-      int order = M.reac_ptr[s + 1] - M.reac_ptr[s];
-      double* p = &M.falloff_param[(size_t)j * 7];
-      p[0] = std::log(r.low[0] * std::pow(1000.0, -(double)order));
+      const int s = M.fwd_step[j];
+      double* p = &M.falloff_param[(size_t)j * kFoParams];
+      p[0] = std::log(r.low[0] * std::pow(1000.0, -step_order[s]));
       p[1] = r.low[1];
       p[2] = r.low[2];
-      if (r.troe.empty()) {
+      if (!r.sri.empty()) {
+        // a, b, 1/c (or -1 when c <= 0), d (default 1), e (default 0)
+        M.falloff_type[j] = kSri;
+        p[3] = r.sri[0];
+        p[4] = r.sri[1];
+        p[5] = r.sri[2] > 0.0 ? 1.0 / r.sri[2] : -1.0;
+        p[6] = r.sri.size() >= 4 ? r.sri[3] : 1.0;
+        p[7] = r.sri.size() == 5 ? r.sri[4] : 0.0;
+      } else if (r.troe.empty()) {
         M.falloff_type[j] = kLindemann;
       } else {
         for (size_t k = 0; k < r.troe.size(); ++k) p[3 + k] = r.troe[k];

@@ -9,15 +9,25 @@
 //   * rate-constant conversions, Arrhenius merging, Keq/third-body/falloff
 //     lists: src/zerork/rate_const.cpp:50-134, 212-537, 1030-1128
 //   * Jacobian term lists: applications/cfd_plugin/reactor_nvector_serial.cpp:122-281
+//   * SRI falloff: rate_const.cpp:508-531 (parameters), :975-1004 (evaluation)
+//   * PLOG: CKParser.cpp:1643-1669 and :312-345 (text, exclusions),
+//     rate_const.cpp:1146-1186 (units), plog_reaction.cpp:10-135 (line sort,
+//     duplicate pressures within 1e-8 relative)
+//   * FORD/RORD and non-integer stoichiometry: CKParser.cpp:1615-1639,
+//     info_net.cpp:411-470 (which reactions), non_integer_reaction_network.cpp:
+//     53-262 (rate-of-progress exponents, step direction), rate_const.cpp:
+//     334-386 (Keq change in moles from the exponents)
 #pragma once
 
+#include <array>
 #include <map>
 #include <string>
 #include <vector>
 
 namespace zrk {
 
-enum FalloffType { kLindemann = 0, kTroe3 = 1, kTroe4 = 2 };
+enum FalloffType { kLindemann = 0, kTroe3 = 1, kTroe4 = 2, kSri = 3 };
+constexpr int kFoParams = 8;  // per falloff reaction: lnAlow, beta, Ta, then Troe or SRI
 enum RevType { kRevNone = 0, kRevArrhenius = 1, kRevKeq = 2 };
 
 struct Species {
@@ -40,7 +50,34 @@ struct ReactionText {
   std::vector<double> troe;
   bool has_rev = false;
   double rev[3] = {0, 0, 0};
+  std::vector<double> sri;            // SRI a, b, c [, d [, e]]
+  std::vector<std::array<double, 4>> plog;  // PLOG lines: p [atm], A, n, E [K]
+  std::map<std::string, double> ford, rord;  // real reaction orders by species name
+  bool real_order = false;            // FORD or RORD present
   std::map<std::string, double> eff;  // sorted by species name, like CKReader's map
+};
+
+// A step of a non-integer reaction (fractional stoichiometry or FORD/RORD):
+// it has no expanded participant lists; its rate of progress is
+// K * prod sign(C_s) |C_s|^p_s over rop_sp, and it destroys nu * ROP of each
+// reactant and creates nu * ROP of each product (lists in reaction-text order,
+// repeated names kept separate).  rop lists are in species-name order.
+struct NonIntStep {
+  int step = -1;
+  std::vector<int> rop_sp;
+  std::vector<double> rop_pow;
+  std::vector<int> re_sp, pr_sp;
+  std::vector<double> re_nu, pr_nu;
+};
+
+// PLOG forward rate of one step: distinct pressures (Pa) ascending, each with
+// a range of Arrhenius lines (ln|A| with the unit conversion, sign of A, beta,
+// Ta) that are summed.
+struct PlogStep {
+  int step = -1;
+  std::vector<double> p, lnp;
+  std::vector<int> line_ptr;  // [npts + 1]
+  std::vector<double> lnA, sign, beta, Ta;
 };
 
 // Reference-layout tables (host copy).
@@ -63,7 +100,13 @@ struct Mechanism {
   std::vector<double> tb_eff;          // efficiency - 1
   std::vector<int> falloff_type;       // per reaction, -1 if not falloff
   std::vector<int> falloff_spc;        // per reaction, -1 for (+M)
-  std::vector<double> falloff_param;   // per reaction [7]
+  std::vector<double> falloff_param;   // per reaction [kFoParams]: lnAlow, beta, Ta,
+                                       // Troe alpha, T***, T*, T** or SRI a, b, 1/c
+                                       // (-1 if c <= 0), d, e
+  std::vector<int> step_ni;            // per step: index into ni or -1
+  std::vector<NonIntStep> ni;          // in step order
+  std::vector<int> rxn_plog;           // per reaction: index into plog or -1
+  std::vector<PlogStep> plog;
 };
 
 // Parse CHEMKIN mechanism (+ optional separate thermo file) and build tables.

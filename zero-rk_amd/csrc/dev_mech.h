@@ -36,7 +36,15 @@ enum RecFlags : int {
   kRecFalloff = 8,     // (+M) / (+species)
   kRecTroe = 16,       // Troe broadening (3 or 4 parameters)
   kRecTroe4 = 32,      // 4-parameter Troe
+  kRecSri = 64,        // SRI blending function
+  kRecPlog = 128,      // forward rate from PLOG tables
+  kRecNonInt = 256,    // non-integer reaction (real rate-of-progress exponents)
 };
+// Records with kRecPlog or kRecNonInt carry an index into r_ext in the bits of
+// their flags from kRecExtShift up.
+constexpr int kRecExtShift = 12;
+// A Jacobian term with this bit set is a non-integer term: bits 0-15 index ni_jt.
+constexpr unsigned kJtNonInt = 0x40000000u;
 
 struct DevMech {
   const DevMech* self;  // this struct's own copy in device memory
@@ -57,7 +65,26 @@ struct DevMech {
   const int* r_eptr;      // [nrec+1] enhanced third-body list (multiples of 4)
   const int* e_sp;        // in groups of 4 (read as int4); padding: species 0
   const double* e_eff;    // efficiency - 1, in pairs (read as double2); padding: 0
-  const double* r_fo;     // [nrec*7] falloff parameters
+  const double* r_fo;     // [nrec*8] falloff parameters (lnAlow, beta, Ta, Troe or SRI)
+  // PLOG and non-integer reactions (absent from most mechanisms): r_ext[e] =
+  // {PLOG table offset or -1, forward exponent list offset, reverse exponent
+  // list offset or -1, forward count | reverse count << 16}.  A PLOG table at
+  // offset o in plog: npts, pressures [npts] (Pa), their logs [npts], first
+  // line of each pressure [npts + 1], then lines of 4 doubles {ln|A|, sign of
+  // A, beta, Ta}.  Exponent lists: species ni_sp[k], exponent ni_pow[k].
+  int n_ext;
+  const int4* r_ext;
+  const double* plog;
+  const int* ni_sp;
+  const double* ni_pow;
+  // non-integer destruction and creation terms by species (added before the
+  // gathers below, as the reference does): entries nig[k] = {nu, step} for
+  // k in [nig_dptr[j], nig_dptr[j+1]) and [nig_cptr[j], nig_cptr[j+1])
+  int has_nig;
+  const int* nig_dptr;
+  const int* nig_cptr;
+  const double2* nig;
+  const double2* ni_jt;   // Jacobian non-integer terms {multiplier, step}
   // per-species gathers, padded to a common length so the 64 lanes of a
   // species block read one contiguous line per block of 8 entries:
   // g_d[(jb*gd_blocks + b)*64 + (j&63)] holds 8 x uint16 step indices of
@@ -67,8 +94,9 @@ struct DevMech {
   // Jacobian species-block terms by row (nsp <= 191): jr[b*S + row] (S = 64,
   // 128 or 192: the integrator's team size) holds 4
   // terms, each bits 0-15 step, 16-23 column (0xFF = padding), bit 31 set for
-  // a destruction term; sorted by column, then destruction before creation,
-  // each in step order (the reference's accumulation order)
+  // a destruction term, or bit 30 (kJtNonInt) for a non-integer term; sorted
+  // by column, then destruction before creation, each in step order, then the
+  // non-integer terms (the reference's accumulation order)
   const uint4* jr;
 };
 

@@ -40,14 +40,20 @@ int build_device_mech(MechHandle* h, std::string* err) {
     inv_mw[j] = 1.0 / M.mw[j];
     ru_mw[j] = 8.314462618e3 / M.mw[j];
   }
-  // reaction records, grouped by kind (falloff, third-body, plain)
+  // reaction records, grouped by kind (falloff, third-body, plain), then the
+  // records that need the general evaluation (PLOG, non-integer, SRI)
+  auto is_ext = [&](int j) {
+    return M.rxn_plog[j] >= 0 || M.step_ni[M.fwd_step[j]] >= 0 || M.falloff_type[j] == kSri;
+  };
   std::vector<int> order;
   for (int pass = 2; pass >= 0; --pass)
     for (int j = 0; j < nrxn; ++j) {
       const auto& r = M.rxns[j];
-      const int kind = r.falloff ? 2 : (r.third_body ? 1 : 0);
+      const int kind = is_ext(j) ? -1 : (r.falloff ? 2 : (r.third_body ? 1 : 0));
       if (kind == pass) order.push_back(j);
     }
+  for (int j = 0; j < nrxn; ++j)
+    if (is_ext(j)) order.push_back(j);
   const int nrec = (int)order.size();
   if (nrec == 0) {
     *err = "mechanism has no reactions";
@@ -58,7 +64,17 @@ int build_device_mech(MechHandle* h, std::string* err) {
     return -1;
   }
   std::vector<int> meta((size_t)nrec * 4), spk((size_t)nrec * 4), eptr(nrec + 1, 0), esp;
-  std::vector<double> coef((size_t)nrec * 6, 0.0), fo((size_t)nrec * 7, 0.0), eeff;
+  std::vector<double> coef((size_t)nrec * 6, 0.0), fo((size_t)nrec * kFoParams, 0.0), eeff;
+  // PLOG tables and non-integer exponent lists of the extended records
+  std::vector<int> ext, ni_sp;
+  std::vector<double> plog_t, ni_pow;
+  auto add_rop_list = [&](int step, int* off, int* cnt) {
+    const int q = M.step_ni[step];
+    *off = (int)ni_sp.size();
+    *cnt = (int)M.ni[q].rop_sp.size();
+    ni_sp.insert(ni_sp.end(), M.ni[q].rop_sp.begin(), M.ni[q].rop_sp.end());
+    ni_pow.insert(ni_pow.end(), M.ni[q].rop_pow.begin(), M.ni[q].rop_pow.end());
+  };
   auto pack16 = [](int a, int b) { return (int)((unsigned)(a & 0xFFFF) | ((unsigned)(b & 0xFFFF) << 16)); };
   for (int q = 0; q < nrec; ++q) {
     const int j = order[q];
@@ -84,8 +100,43 @@ int build_device_mech(MechHandle* h, std::string* err) {
       fl |= kRecFalloff;
       if (M.falloff_type[j] == kTroe3 || M.falloff_type[j] == kTroe4) fl |= kRecTroe;
       if (M.falloff_type[j] == kTroe4) fl |= kRecTroe4;
-      for (int k = 0; k < 7; ++k) fo[(size_t)q * 7 + k] = M.falloff_param[(size_t)j * 7 + k];
+      if (M.falloff_type[j] == kSri) fl |= kRecSri;
+      for (int k = 0; k < kFoParams; ++k)
+        fo[(size_t)q * kFoParams + k] = M.falloff_param[(size_t)j * kFoParams + k];
       fsp = M.falloff_spc[j];
+    }
+    const bool is_plog = M.rxn_plog[j] >= 0, is_ni = M.step_ni[s] >= 0;
+    if (is_ext(j)) {
+      int e4[4] = {-1, 0, -1, 0};
+      if (is_plog) {
+        fl |= kRecPlog;
+        const PlogStep& ps = M.plog[M.rxn_plog[j]];
+        const int np = (int)ps.p.size();
+        e4[0] = (int)plog_t.size();
+        plog_t.push_back((double)np);
+        plog_t.insert(plog_t.end(), ps.p.begin(), ps.p.end());
+        plog_t.insert(plog_t.end(), ps.lnp.begin(), ps.lnp.end());
+        for (int k = 0; k <= np; ++k) plog_t.push_back((double)ps.line_ptr[k]);
+        for (size_t k = 0; k < ps.lnA.size(); ++k) {
+          plog_t.push_back(ps.lnA[k]);
+          plog_t.push_back(ps.sign[k]);
+          plog_t.push_back(ps.beta[k]);
+          plog_t.push_back(ps.Ta[k]);
+        }
+      }
+      if (is_ni) {
+        fl |= kRecNonInt;
+        int nf = 0, nr = 0;
+        add_rop_list(s, &e4[1], &nf);
+        if (M.rev_step[j] >= 0) add_rop_list(M.rev_step[j], &e4[2], &nr);
+        if (nf > 0xFFFF || nr > 0x7FFF) {
+          *err = "reaction " + std::to_string(j + 1) + " has too many reaction orders";
+          return -1;
+        }
+        e4[3] = nf | (nr << 16);
+      }
+      fl |= (int)(ext.size() / 4) << kRecExtShift;
+      ext.insert(ext.end(), e4, e4 + 4);
     }
     meta[(size_t)q * 4 + 0] = s;
     meta[(size_t)q * 4 + 1] = M.rev_step[j];
@@ -143,12 +194,38 @@ int build_device_mech(MechHandle* h, std::string* err) {
     return g;
   };
   const std::vector<uint16_t> gd = pad_gather(dl, gdb), gc = pad_gather(cl, gcb);
+  // non-integer destruction/creation terms by species, in step order and
+  // within a step in reaction-text order (non_integer_reaction_network.cpp:
+  // 295-333; perf_net.cpp:250-253 adds them before the integer terms)
+  std::vector<std::vector<std::pair<double, int>>> nd(nsp), nc(nsp);
+  for (const auto& st : M.ni) {
+    for (size_t k = 0; k < st.re_sp.size(); ++k) nd[st.re_sp[k]].push_back({st.re_nu[k], st.step});
+    for (size_t k = 0; k < st.pr_sp.size(); ++k) nc[st.pr_sp[k]].push_back({st.pr_nu[k], st.step});
+  }
+  std::vector<int> nig_dptr(nsp + 1, 0), nig_cptr(nsp + 1, 0);
+  std::vector<double> nig;  // pairs {nu, step}
+  for (int j = 0; j < nsp; ++j) {
+    for (auto& e : nd[j]) {
+      nig.push_back(e.first);
+      nig.push_back((double)e.second);
+    }
+    nig_dptr[j + 1] = (int)(nig.size() / 2);
+  }
+  nig_cptr[0] = nig_dptr[nsp];  // creation entries follow the destruction entries
+  for (int j = 0; j < nsp; ++j) {
+    for (auto& e : nc[j]) {
+      nig.push_back(e.first);
+      nig.push_back((double)e.second);
+    }
+    nig_cptr[j + 1] = (int)(nig.size() / 2);
+  }
   // Jacobian species-block terms by row (integrator teams of up to 3
   // wavefronts, nsp <= 191; row stride = team size):
   // entry (row i, col c) = destruction terms then creation terms, each in step
   // order (the reference's order of accumulation, reactor_nvector_serial.cpp:149-176
   // and 655-669); a row's entries are sorted by column
   std::vector<uint32_t> jrt;
+  std::vector<double> ni_jt;  // pairs {multiplier, step}
   int jrb = 0;
   const int team = 64 * ((nsp + 1 + 63) / 64);
   if (nsp <= 191) {
@@ -161,6 +238,29 @@ int build_device_mech(MechHandle* h, std::string* err) {
         for (int q = M.prod_ptr[s]; q < M.prod_ptr[s + 1]; ++q)
           jc[(size_t)M.prod_sp[q] * nsp + c].push_back(s);
       }
+    // non-integer terms (non_integer_reaction_network.cpp:774-905): for each
+    // step, each exponent species c with a non-zero exponent p, the rows of
+    // its reactants (-nu p) and products (+nu p); value (mult ROP) / C_c
+    std::vector<std::vector<int>> jn((size_t)nsp * nsp);
+    for (const auto& st : M.ni)
+      for (size_t k = 0; k < st.rop_sp.size(); ++k) {
+        if (st.rop_pow[k] == 0.0) continue;
+        const int c = st.rop_sp[k];
+        for (size_t q = 0; q < st.re_sp.size(); ++q) {
+          jn[(size_t)st.re_sp[q] * nsp + c].push_back((int)(ni_jt.size() / 2));
+          ni_jt.push_back(-st.re_nu[q] * st.rop_pow[k]);
+          ni_jt.push_back((double)st.step);
+        }
+        for (size_t q = 0; q < st.pr_sp.size(); ++q) {
+          jn[(size_t)st.pr_sp[q] * nsp + c].push_back((int)(ni_jt.size() / 2));
+          ni_jt.push_back(st.pr_nu[q] * st.rop_pow[k]);
+          ni_jt.push_back((double)st.step);
+        }
+      }
+    if (ni_jt.size() / 2 > 0xFFFF) {
+      *err = "too many non-integer Jacobian terms";
+      return -1;
+    }
     std::vector<std::vector<uint32_t>> rows(nsp);
     size_t maxr = 0;
     for (int i = 0; i < nsp; ++i) {
@@ -168,6 +268,7 @@ int build_device_mech(MechHandle* h, std::string* err) {
         const size_t e = (size_t)i * nsp + c;
         for (int st : jd[e]) rows[i].push_back((uint32_t)st | ((uint32_t)c << 16) | 0x80000000u);
         for (int st : jc[e]) rows[i].push_back((uint32_t)st | ((uint32_t)c << 16));
+        for (int t : jn[e]) rows[i].push_back((uint32_t)t | ((uint32_t)c << 16) | kJtNonInt);
       }
       maxr = std::max(maxr, rows[i].size());
     }
@@ -179,12 +280,26 @@ int build_device_mech(MechHandle* h, std::string* err) {
         jrt[((k / 4) * team + i) * 4 + k % 4] = rows[i][k];
   }
   if (jrt.empty()) jrt.assign(4, 0u);
+  const int n_ext = (int)(ext.size() / 4);
+  const bool has_nig = !nig.empty();
+  // keep every table non-empty so that each device pointer is valid
+  if (ext.empty()) ext.assign(4, -1);
+  if (plog_t.empty()) plog_t.assign(1, 0.0);
+  if (ni_sp.empty()) {
+    ni_sp.assign(1, 0);
+    ni_pow.assign(1, 0.0);
+  }
+  if (nig.empty()) nig.assign(2, 0.0);
+  if (ni_jt.empty()) ni_jt.assign(2, 0.0);
 
   BlobBuilder b;
   const size_t o_mw = b.add(M.mw), o_imw = b.add(inv_mw), o_rmw = b.add(ru_mw),
                o_th = b.add(M.thermo), o_meta = b.add(meta), o_sp = b.add(spk),
                o_coef = b.add(coef), o_ep = b.add(eptr), o_es = b.add(esp), o_ee = b.add(eeff),
-               o_fo = b.add(fo), o_gd = b.add(gd), o_gc = b.add(gc), o_jr = b.add(jrt);
+               o_fo = b.add(fo), o_gd = b.add(gd), o_gc = b.add(gc), o_jr = b.add(jrt),
+               o_ext = b.add(ext), o_pl = b.add(plog_t), o_nsp = b.add(ni_sp),
+               o_npw = b.add(ni_pow), o_ndp = b.add(nig_dptr), o_ncp = b.add(nig_cptr),
+               o_nig = b.add(nig), o_njt = b.add(ni_jt);
   void* base = nullptr;
   hipError_t e = hipMalloc(&base, b.bytes.size());
   if (e != hipSuccess) {
@@ -221,6 +336,16 @@ int build_device_mech(MechHandle* h, std::string* err) {
   d.g_d = (const uint4*)(p + o_gd);
   d.g_c = (const uint4*)(p + o_gc);
   d.jr = (const uint4*)(p + o_jr);
+  d.n_ext = n_ext;
+  d.r_ext = (const int4*)(p + o_ext);
+  d.plog = (const double*)(p + o_pl);
+  d.ni_sp = (const int*)(p + o_nsp);
+  d.ni_pow = (const double*)(p + o_npw);
+  d.has_nig = has_nig ? 1 : 0;
+  d.nig_dptr = (const int*)(p + o_ndp);
+  d.nig_cptr = (const int*)(p + o_ncp);
+  d.nig = (const double2*)(p + o_nig);
+  d.ni_jt = (const double2*)(p + o_njt);
   e = hipMalloc((void**)&h->dev_self, sizeof(DevMech));
   if (e == hipSuccess) {
     d.self = h->dev_self;
