@@ -48,7 +48,7 @@ constexpr int kQmax = 5;
 // Every entry is accumulated in FP64; MT = float stores the finished entries
 // in FP32 (the FP32 preconditioner keeps its Jacobian in FP32).
 // ---------------------------------------------------------------------------
-template <typename MT>
+template <typename MT, bool kExt = false>
 __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s, double P,
                                                double v0, const BdfOpts& o, double y, double fy,
                                                double inv_rho_prev, MT* __restrict__ Jl, int ld,
@@ -73,12 +73,12 @@ __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s
   const double t4in = (lane == nsp) ? Tp : (sp ? y : 0.0);
   double t4;
   RhsAux aux2;
-  rhs64(m, s, P, v0, o.Tref, o.step_limiter, t4in, t4, aux2, src);
+  rhs64<kExt>(m, s, P, v0, o.Tref, o.step_limiter, t4in, t4, aux2, src);
   const double colT = act ? (t4 - fy) * (1.0 / dT) : 0.0;
   // floored evaluation: rates of progress for every species term
   double t3;
   RhsAux aux;
-  rhs64(m, s, P, v0, o.Tref, o.step_limiter, t1, t3, aux, src);
+  rhs64<kExt>(m, s, P, v0, o.Tref, o.step_limiter, t1, t3, aux, src);
   const double Tdot = bcast(t3, nsp);
   const double RuTemp = kRu * yT;
   const double mult = -1.0 / aux.cp_mix;
@@ -102,7 +102,7 @@ __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s
           cur = col;
           v = 0.0;
         }
-        if (t & kJtNonInt) {
+        if (kExt && (t & kJtNonInt)) {
           // (multiplier * ROP) / C_col (non_integer_reaction_network.cpp:493-506)
           const double2 e = m.ni_jt[t & 0xFFFFu];
           v = __dadd_rn(v, __dmul_rn(__dmul_rn(e.x, s.rop[(int)e.y]), t2l[col]));
@@ -197,7 +197,7 @@ __device__ __forceinline__ double jacobian_team(const Team& team, const DevMech&
           cur = col;
           v = 0.0;
         }
-        if (t & kJtNonInt) {
+        if (Team::kExt && (t & kJtNonInt)) {
           const double2 e = m.ni_jt[t & 0xFFFFu];
           v = __dadd_rn(v, __dmul_rn(__dmul_rn(e.x, s.rop[(int)e.y]), t2l[col]));
         } else {

@@ -90,9 +90,14 @@ __device__ __forceinline__ float bcast(float v, int src) {
 // broadcasts go through a few LDS slots (red[0..W]) and workgroup barriers.
 // Reductions sum the per-wavefront butterfly results in wavefront order, so
 // every thread gets the same bits.
+// kExt: the mechanism has records that need the general evaluation (PLOG,
+// non-integer, SRI); the code for them is compiled only into kExt variants,
+// so the common kernels carry none of it.
 // ---------------------------------------------------------------------------
-struct TeamWave {
+template <bool E>
+struct TeamWaveX {
   static constexpr int kThreads = kWave;
+  static constexpr bool kExt = E;
   __device__ __forceinline__ int tid() const { return lane_id(); }
   __device__ __forceinline__ double sum(double v) const { return wave_sum(v); }
   __device__ __forceinline__ double max(double v) const { return wave_max(v); }
@@ -104,9 +109,12 @@ struct TeamWave {
   }
 };
 
-template <int W>
+using TeamWave = TeamWaveX<false>;
+
+template <int W, bool E = false>
 struct TeamBlock {
   static constexpr int kThreads = kWave * W;
+  static constexpr bool kExt = E;
   double* red;  // LDS [W + 1]
   __device__ __forceinline__ int tid() const { return (int)threadIdx.x; }
   __device__ __forceinline__ void sync() const { __syncthreads(); }
@@ -373,14 +381,14 @@ __device__ __forceinline__ void rates_general_record(const DevMech& m, int r,
 // computed.
 // Records come fall-off first, then third-body, then plain, so the costly
 // fall-off branch is taken by as few wavefront iterations as possible.
-template <int S>
+template <int S, bool kExt>
 __device__ __forceinline__ void team_rates(int tid, const DevMech& m, const double* __restrict__ C,
                                            const double* __restrict__ G, double* __restrict__ rop,
                                            const TPow& p, double Csum, double L) {
   const bool limit = (0.0 < L && L < 1.0e300);
   // PLOG, non-integer and SRI records come last and are evaluated by
-  // rates_general_record
-  const int nrec = m.nrec - m.n_ext;
+  // rates_general_record (kExt variants only)
+  const int nrec = kExt ? m.nrec - m.n_ext : m.nrec;
   if (tid == 0) rop[m.nstep] = 0.0;
   int r = tid;
 #if ZRK_RATES_PREFETCH
@@ -492,7 +500,7 @@ __device__ __forceinline__ void team_rates(int tid, const DevMech& m, const doub
       rop[rs] = kr;
     }
   }
-  if (m.n_ext > 0) {
+  if constexpr (kExt) {
     // PLOG forward rates at the pressure of the concentrations, Csum Ru T
     // (rate_const.cpp:551, 565-568), in a pass of their own (kept apart from
     // the record evaluation, which then reads them before the reverse rate
@@ -508,13 +516,14 @@ __device__ __forceinline__ void team_rates(int tid, const DevMech& m, const doub
 
 // Creation and destruction rates of species j (gathers in the reference's
 // summation order; padding entries add the zero rate, which is exact).
+template <bool kExt>
 __device__ __forceinline__ void species_cd(const DevMech& m, const double* __restrict__ rop, int j,
                                            double& create, double& destroy) {
   const int jb = j >> 6, jl = j & 63;
   const uint4* __restrict__ gd = m.g_d + (size_t)jb * m.gd_blocks * kWave + jl;
   const uint4* __restrict__ gc = m.g_c + (size_t)jb * m.gc_blocks * kWave + jl;
   double d = 0.0, c = 0.0;
-  if (m.has_nig) {
+  if (kExt && m.has_nig) {
     // non-integer terms first (perf_net.cpp:250-253)
     for (int k = m.nig_dptr[j]; k < m.nig_dptr[j + 1]; ++k) {
       const double2 e = m.nig[k];
@@ -714,7 +723,7 @@ __device__ __forceinline__ int rhs_team(const Team& team, const DevMech& m, cons
 #ifdef ZRK_PHASE_CYCLES
   const long long tc1 = (long long)__builtin_amdgcn_s_memtime();
 #endif
-  team_rates<Team::kThreads>(lane, m, s.C, s.G, s.rop, p, Csum, L);
+  team_rates<Team::kThreads, Team::kExt>(lane, m, s.C, s.G, s.rop, p, Csum, L);
   team.sync();
 #ifdef ZRK_PHASE_CYCLES
   const long long tc2 = (long long)__builtin_amdgcn_s_memtime();
@@ -723,7 +732,7 @@ __device__ __forceinline__ int rhs_team(const Team& team, const DevMech& m, cons
   double yd = 0.0;
   if (sp) {
     double cr, de;
-    species_cd(m, s.rop, lane, cr, de);
+    species_cd<Team::kExt>(m, s.rop, lane, cr, de);
     const double net = cr - de;
     if (src.has_ysrc) {
       yd = (net * m.mw[lane] * vd + src.ysrc);
@@ -753,10 +762,11 @@ __device__ __forceinline__ int rhs_team(const Team& team, const DevMech& m, cons
 }
 
 // One reactor per wavefront (nsp <= 63).
+template <bool kExt = false>
 __device__ __forceinline__ int rhs64(const DevMech& m, const RhsLds& s, double P, double v0,
                                      double Tref, double L, double y, double& ydot,
                                      RhsAux& aux, RhsSrc src = {}) {
-  return rhs_team(TeamWave{}, m, s, P, v0, Tref, L, y, ydot, aux, src);
+  return rhs_team(TeamWaveX<kExt>{}, m, s, P, v0, Tref, L, y, ydot, aux, src);
 }
 
 }  // namespace zrk

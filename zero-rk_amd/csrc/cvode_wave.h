@@ -152,7 +152,7 @@ struct Cvode {
     ZRK_TIC(t0);
     int fl;
     if constexpr (kWaveTeam)
-      fl = rhs64(mech, s, P, v0, opt.Tref, opt.step_limiter, yy, yd, aux, src);
+      fl = rhs64<Team::kExt>(mech, s, P, v0, opt.Tref, opt.step_limiter, yy, yd, aux, src);
     else
       fl = rhs_team(team, mech, s, P, v0, opt.Tref, opt.step_limiter, yy, yd, aux, src);
     ZRK_TOC(t0, 0);
@@ -467,8 +467,9 @@ struct Cvode {
       nje++;
       nstlj = nst;
       ZRK_TIC(t0);
-      inv_rho_last = jacobian_lds<MT>(mech, s, P, v0, opt, ypred, fpred, inv_rho_last, Mlds, N,
-                                      savedJ, t2l, sources(tn));
+      inv_rho_last = jacobian_lds<MT, Team::kExt>(mech, s, P, v0, opt, ypred, fpred,
+                                                  inv_rho_last, Mlds, N, savedJ, t2l,
+                                                  sources(tn));
       ZRK_TOC(t0, 1);
     }
     if constexpr (kWaveMfmaInverse) {
@@ -905,7 +906,7 @@ struct Cvode {
   }
 };
 
-template <typename MT>
-using CvWave = Cvode<MT, TeamWave>;
+template <typename MT, bool E = false>
+using CvWave = Cvode<MT, TeamWaveX<E>>;
 
 }  // namespace zrk

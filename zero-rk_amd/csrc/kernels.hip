@@ -9,6 +9,8 @@
 //                    the batch is exhausted.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "cvode_wave.h"
 #include "launch.h"
 
@@ -27,6 +29,7 @@ __host__ __device__ constexpr size_t wave_matrix_bytes(int nsp, size_t mt) {
 // plugin API); outputs (each optional): ydot[n*(nsp+1)] ([Y.., T] order),
 // rop[n*nstep], create/destroy/net[n*nsp].
 // ---------------------------------------------------------------------------
+template <bool kExt>
 __global__ void __launch_bounds__(kWave* kRatesWaves)
 rates_kernel(DevMech m, int n, const double* __restrict__ Tin, const double* __restrict__ Pin,
              const double* __restrict__ Yin, int stride, double L, double* __restrict__ ydot,
@@ -70,13 +73,13 @@ rates_kernel(DevMech m, int n, const double* __restrict__ Tin, const double* __r
     const double Csum = wave_sum(cs);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    team_rates<kWave>(lane, m, C, G, rop, p, Csum, L);
+    team_rates<kWave, kExt>(lane, m, C, G, rop, p, Csum, L);
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     double e = 0.0;
     for (int j = lane; j < nsp; j += kWave) {
       double cr, de;
-      species_cd(m, rop, j, cr, de);
+      species_cd<kExt>(m, rop, j, cr, de);
       const double net = cr - de;
       if (create_out) create_out[(size_t)r * nsp + j] = cr;
       if (destroy_out) destroy_out[(size_t)r * nsp + j] = de;
@@ -97,6 +100,7 @@ rates_kernel(DevMech m, int n, const double* __restrict__ Tin, const double* __r
 // ---------------------------------------------------------------------------
 // jacobian_kernel (nsp <= 63): J[n][N][N] row-major, N = nsp+1, T last.
 // ---------------------------------------------------------------------------
+template <bool kExt>
 __global__ void __launch_bounds__(kWave)
 jacobian_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ Tin,
                 const double* __restrict__ Pin, const double* __restrict__ Yin, int stride,
@@ -116,8 +120,8 @@ jacobian_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ Tin,
     const double P = Pin[r];
     double fy;
     RhsAux aux;
-    rhs64(m, s, P, 0.0, o.Tref, o.step_limiter, y, fy, aux);
-    jacobian_lds<double>(m, s, P, 0.0, o, y, fy, aux.inv_rho, Jl, kNP, saved, t2l);
+    rhs64<kExt>(m, s, P, 0.0, o.Tref, o.step_limiter, y, fy, aux);
+    jacobian_lds<double, kExt>(m, s, P, 0.0, o, y, fy, aux.inv_rho, Jl, kNP, saved, t2l);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -131,7 +135,7 @@ jacobian_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ Tin,
 // of W wavefronts (the integrator's team code, FP64), through a global
 // scratch J (column stride N) per workgroup.  LDS: t2l[64W], hl[64W],
 // red[W+1] doubles, then C, G, rop.
-template <int W>
+template <int W, bool kExt>
 __global__ void __launch_bounds__(kWave * W)
 jacobian_team_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ Tin,
                      const double* __restrict__ Pin, const double* __restrict__ Yin, int stride,
@@ -139,7 +143,7 @@ jacobian_team_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ Tin
   extern __shared__ double lds[];
   constexpr int L = kWave * W;
   const int nsp = m.nsp, N = nsp + 1;
-  TeamBlock<W> team;
+  TeamBlock<W, kExt> team;
   double* t2l = lds;
   double* hl = t2l + L;
   team.red = hl + L;
@@ -302,7 +306,7 @@ __device__ __forceinline__ void bdf_driver(Cvode<MT, Team>& cv, const DevMech& m
 // MT = float: FP32 preconditioner.  LDS (bdf_lds_bytes): CvUni, Mlds[N*N] MT
 // (N = nsp + 1, padded to 16 bytes), t2l[kNP], then C[nsp+1], G[nsp+1],
 // rop[nstep+1] doubles.
-template <typename MT>
+template <typename MT, bool kExt>
 __device__ __forceinline__ void bdf_wave(DevMech m, BdfOpts o, int n, double dt,
                                          const double* __restrict__ dts, double* __restrict__ Tio,
                                          double* __restrict__ Pio, double* __restrict__ Yio,
@@ -313,7 +317,7 @@ __device__ __forceinline__ void bdf_wave(DevMech m, BdfOpts o, int n, double dt,
                                          double* __restrict__ jscratch) {
   extern __shared__ double lds[];
   const int nsp = m.nsp;
-  CvWave<MT> cv(reinterpret_cast<CvUni*>(lds));
+  CvWave<MT, kExt> cv(reinterpret_cast<CvUni*>(lds));
   char* base = reinterpret_cast<char*>(lds) + sizeof(CvUni);
   cv.Mlds = reinterpret_cast<MT*>(base);
   cv.prow = nullptr;
@@ -343,18 +347,14 @@ __device__ __forceinline__ void bdf_wave(DevMech m, BdfOpts o, int n, double dt,
       int* __restrict__ nsteps_out, int* __restrict__ stats_out, const int* __restrict__ order, \
       unsigned int* __restrict__ counter, double* __restrict__ jscratch
 template <typename MT>
-__global__ void bdf_kernel(ZRK_BDF_KERNEL_ARGS);
-template <>
-__global__ void __launch_bounds__(kWave, ZRK_BDF_F64_WAVES_PER_EU)
-bdf_kernel<double>(ZRK_BDF_KERNEL_ARGS) {
-  bdf_wave<double>(m, o, n, dt, dts, Tio, Pio, Yio, stride, nsteps_out, stats_out, order, counter,
-                   jscratch);
+constexpr int bdf_waves_per_eu() {
+  return sizeof(MT) == sizeof(float) ? ZRK_BDF_F32_WAVES_PER_EU : ZRK_BDF_F64_WAVES_PER_EU;
 }
-template <>
-__global__ void __launch_bounds__(kWave, ZRK_BDF_F32_WAVES_PER_EU)
-bdf_kernel<float>(ZRK_BDF_KERNEL_ARGS) {
-  bdf_wave<float>(m, o, n, dt, dts, Tio, Pio, Yio, stride, nsteps_out, stats_out, order, counter,
-                  jscratch);
+template <typename MT, bool kExt>
+__global__ void __launch_bounds__(kWave, bdf_waves_per_eu<MT>())
+bdf_kernel(ZRK_BDF_KERNEL_ARGS) {
+  bdf_wave<MT, kExt>(m, o, n, dt, dts, Tio, Pio, Yio, stride, nsteps_out, stats_out, order,
+                     counter, jscratch);
 }
 #undef ZRK_BDF_KERNEL_ARGS
 
@@ -363,7 +363,7 @@ bdf_kernel<float>(ZRK_BDF_KERNEL_ARGS) {
 // float, prow[64W], rvec[64W] float, t2l[64W], hl[64W], red[W+1] double,
 // then C[nsp+1], G[nsp+1], rop[nstep+1] doubles.  savedJ slot: slot_doubles
 // doubles per workgroup.
-template <int W>
+template <int W, bool kExt>
 __global__ void __launch_bounds__(kWave * W)
 bdf_team_kernel(DevMech m, BdfOpts o, int n, double dt, const double* __restrict__ dts,
                 double* __restrict__ Tio,
@@ -373,7 +373,7 @@ bdf_team_kernel(DevMech m, BdfOpts o, int n, double dt, const double* __restrict
                 double* __restrict__ jscratch, size_t slot_doubles) {
   extern __shared__ double lds[];
   constexpr int L = kWave * W;
-  using CV = Cvode<float, TeamBlock<W>>;
+  using CV = Cvode<float, TeamBlock<W, kExt>>;
   const int nsp = m.nsp;
   CV cv(reinterpret_cast<CvUni*>(lds) + threadIdx.x / kWave);
   char* p = reinterpret_cast<char*>(lds) + W * sizeof(CvUni);
@@ -396,8 +396,17 @@ __global__ void fill_u32_kernel(unsigned int* p, unsigned int v) {
 }
 
 // ---------------------------------------------------------------------------
-// launchers
+// launchers.  The kExt variants (with the general record evaluation) are
+// launched only for mechanisms that have PLOG, non-integer or SRI records.
 // ---------------------------------------------------------------------------
+namespace {
+template <class F>
+hipError_t with_ext(const DevMech& m, F&& f) {
+  if (m.n_ext > 0) return f(std::true_type{});
+  return f(std::false_type{});
+}
+}  // namespace
+
 hipError_t launch_rates(const DevMech& m, int n, const double* T, const double* P,
                         const double* Y, int stride, double L, double* ydot, double* rop,
                         double* create, double* destroy, double* net, hipStream_t st) {
@@ -405,9 +414,12 @@ hipError_t launch_rates(const DevMech& m, int n, const double* T, const double* 
   const size_t lds = sizeof(double) * (size_t)kRatesWaves * (3 * m.nsp + 2 + m.nstep + 1);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   int blocks = (n + kRatesWaves - 1) / kRatesWaves;
-  hipLaunchKernelGGL(rates_kernel, dim3(blocks), dim3(kWave * kRatesWaves), lds, st, m, n, T,
-                     P, Y, stride, L, ydot, rop, create, destroy, net);
-  return hipGetLastError();
+  return with_ext(m, [&](auto e) {
+    hipLaunchKernelGGL((rates_kernel<decltype(e)::value>), dim3(blocks),
+                       dim3(kWave * kRatesWaves), lds, st, m, n, T, P, Y, stride, L, ydot, rop,
+                       create, destroy, net);
+    return hipGetLastError();
+  });
 }
 
 size_t rhs_lds_bytes(const DevMech& m) {
@@ -420,21 +432,24 @@ hipError_t launch_jacobian(const DevMech& m, const BdfOpts& o, int n, const doub
   const int grid = nblocks < n ? nblocks : n;
   const int W = bdf_team_width(m.nsp);
   if (W == 0) return hipErrorInvalidValue;
-  if (W > 1) {
-    const size_t L = (size_t)kWave * W;
-    const size_t lds = sizeof(double) * (2 * L + ((W + 2) & ~1)) + rhs_lds_bytes(m);
-    if (W == 2)
-      hipLaunchKernelGGL(jacobian_team_kernel<2>, dim3(grid), dim3(2 * kWave), lds, st, m, o, n,
-                         T, P, Y, stride, J, jscratch);
-    else
-      hipLaunchKernelGGL(jacobian_team_kernel<3>, dim3(grid), dim3(3 * kWave), lds, st, m, o, n,
-                         T, P, Y, stride, J, jscratch);
+  return with_ext(m, [&](auto e) {
+    constexpr bool E = decltype(e)::value;
+    if (W > 1) {
+      const size_t L = (size_t)kWave * W;
+      const size_t lds = sizeof(double) * (2 * L + ((W + 2) & ~1)) + rhs_lds_bytes(m);
+      if (W == 2)
+        hipLaunchKernelGGL((jacobian_team_kernel<2, E>), dim3(grid), dim3(2 * kWave), lds, st, m,
+                           o, n, T, P, Y, stride, J, jscratch);
+      else
+        hipLaunchKernelGGL((jacobian_team_kernel<3, E>), dim3(grid), dim3(3 * kWave), lds, st, m,
+                           o, n, T, P, Y, stride, J, jscratch);
+      return hipGetLastError();
+    }
+    const size_t lds = sizeof(double) * (size_t)(kNP * kNP + kNP) + rhs_lds_bytes(m);
+    hipLaunchKernelGGL((jacobian_kernel<E>), dim3(grid), dim3(kWave), lds, st, m, o, n, T, P, Y,
+                       stride, J, jscratch);
     return hipGetLastError();
-  }
-  const size_t lds = sizeof(double) * (size_t)(kNP * kNP + kNP) + rhs_lds_bytes(m);
-  hipLaunchKernelGGL(jacobian_kernel, dim3(grid), dim3(kWave), lds, st, m, o, n, T, P, Y,
-                     stride, J, jscratch);
-  return hipGetLastError();
+  });
 }
 
 int bdf_team_width(int nsp) {
@@ -494,17 +509,26 @@ int resident_blocks(K kernel, int threads, size_t lds) {
 int bdf_max_resident_teams(const DevMech& m, bool f32, int W) {
   if (!bdf_team_ok(m.nsp, f32, W)) return 0;
   const size_t lds = bdf_lds_bytes(m, f32, W);
-  switch (W) {
-    case 1:
-      return f32 ? resident_blocks(bdf_kernel<float>, kWave, lds)
-                 : resident_blocks(bdf_kernel<double>, kWave, lds);
-    case 2:
-      return f32 ? resident_blocks(bdf_team_kernel<2>, 2 * kWave, lds) : 0;
-    case 3:
-      return f32 ? resident_blocks(bdf_team_kernel<3>, 3 * kWave, lds) : 0;
-    default:
-      return 0;
-  }
+  int r = 0;
+  (void)with_ext(m, [&](auto e) {
+    constexpr bool E = decltype(e)::value;
+    switch (W) {
+      case 1:
+        r = f32 ? resident_blocks(bdf_kernel<float, E>, kWave, lds)
+                : resident_blocks(bdf_kernel<double, E>, kWave, lds);
+        break;
+      case 2:
+        r = f32 ? resident_blocks(bdf_team_kernel<2, E>, 2 * kWave, lds) : 0;
+        break;
+      case 3:
+        r = f32 ? resident_blocks(bdf_team_kernel<3, E>, 3 * kWave, lds) : 0;
+        break;
+      default:
+        r = 0;
+    }
+    return hipSuccess;
+  });
+  return r;
 }
 
 hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, const double* dts,
@@ -518,21 +542,24 @@ hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, cons
   if (lds > max_lds_per_block()) return hipErrorInvalidValue;
   hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, counter, 0u);
   const int grid = nwaves < n ? nwaves : n;
-  if (W == 1 && f32)
-    hipLaunchKernelGGL(bdf_kernel<float>, dim3(grid), dim3(kWave), lds, st, m, o, n, dt, dts, T, P,
-                       Y, stride, nsteps, stats, order, counter, jscratch);
-  else if (W == 1)
-    hipLaunchKernelGGL(bdf_kernel<double>, dim3(grid), dim3(kWave), lds, st, m, o, n, dt, dts, T, P,
-                       Y, stride, nsteps, stats, order, counter, jscratch);
-  else if (W == 2)
-    hipLaunchKernelGGL(bdf_team_kernel<2>, dim3(grid), dim3(2 * kWave), lds, st, m, o, n, dt, dts, T,
-                       P, Y, stride, nsteps, stats, order, counter, jscratch,
-                       bdf_slot_doubles(m, W));
-  else
-    hipLaunchKernelGGL(bdf_team_kernel<3>, dim3(grid), dim3(3 * kWave), lds, st, m, o, n, dt, dts, T,
-                       P, Y, stride, nsteps, stats, order, counter, jscratch,
-                       bdf_slot_doubles(m, W));
-  return hipGetLastError();
+  return with_ext(m, [&](auto e) {
+    constexpr bool E = decltype(e)::value;
+    if (W == 1 && f32)
+      hipLaunchKernelGGL((bdf_kernel<float, E>), dim3(grid), dim3(kWave), lds, st, m, o, n, dt,
+                         dts, T, P, Y, stride, nsteps, stats, order, counter, jscratch);
+    else if (W == 1)
+      hipLaunchKernelGGL((bdf_kernel<double, E>), dim3(grid), dim3(kWave), lds, st, m, o, n, dt,
+                         dts, T, P, Y, stride, nsteps, stats, order, counter, jscratch);
+    else if (W == 2)
+      hipLaunchKernelGGL((bdf_team_kernel<2, E>), dim3(grid), dim3(2 * kWave), lds, st, m, o, n,
+                         dt, dts, T, P, Y, stride, nsteps, stats, order, counter, jscratch,
+                         bdf_slot_doubles(m, W));
+    else
+      hipLaunchKernelGGL((bdf_team_kernel<3, E>), dim3(grid), dim3(3 * kWave), lds, st, m, o, n,
+                         dt, dts, T, P, Y, stride, nsteps, stats, order, counter, jscratch,
+                         bdf_slot_doubles(m, W));
+    return hipGetLastError();
+  });
 }
 
 }  // namespace zrk

@@ -169,6 +169,7 @@ __global__ void to_bf16_kernel(long n, const float* __restrict__ x, unsigned sho
 // res[2r], res[2r+1] (optional) = the two norms.
 // LDS: Ml[kNP*kNP], prow[kNP], rvec[kNP], t2l[kNP], then C, G, rop.
 // ---------------------------------------------------------------------------
+template <bool kExt>
 __global__ void __launch_bounds__(kWave)
 accept_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ dts, double dt_all,
               const double* __restrict__ T0, const double* __restrict__ P,
@@ -201,12 +202,12 @@ accept_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ dts, doubl
     }
     double f0, f1;
     RhsAux aux;
-    int bad = rhs64(m, s, Pr, 0.0, o.Tref, o.step_limiter, y0, f0, aux);
+    int bad = rhs64<kExt>(m, s, Pr, 0.0, o.Tref, o.step_limiter, y0, f0, aux);
     if (newton > 0 && !bad) {
       double fy;
-      bad = rhs64(m, s, Pr, 0.0, o.Tref, o.step_limiter, y1, fy, aux);
+      bad = rhs64<kExt>(m, s, Pr, 0.0, o.Tref, o.step_limiter, y1, fy, aux);
       if (!bad) {
-        jacobian_lds<double>(m, s, Pr, 0.0, o, y1, fy, aux.inv_rho, Ml, kNP, saved, t2l);
+        jacobian_lds<double, kExt>(m, s, Pr, 0.0, o, y1, fy, aux.inv_rho, Ml, kNP, saved, t2l);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -224,7 +225,7 @@ accept_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ dts, doubl
         if (!ok) bad = 1;
       }
       for (int it = 0; it < newton && !bad; ++it) {
-        if (it > 0) bad = rhs64(m, s, Pr, 0.0, o.Tref, o.step_limiter, y1, fy, aux);
+        if (it > 0) bad = rhs64<kExt>(m, s, Pr, 0.0, o.Tref, o.step_limiter, y1, fy, aux);
         if (bad) break;
         const double G = act ? y1 - y0 - 0.5 * dt * (f0 + fy) : 0.0;
         rvec[lane] = G;
@@ -239,7 +240,7 @@ accept_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ dts, doubl
         if (act) y1 -= d;
       }
     }
-    if (!bad) bad = rhs64(m, s, Pr, 0.0, o.Tref, o.step_limiter, y1, f1, aux);
+    if (!bad) bad = rhs64<kExt>(m, s, Pr, 0.0, o.Tref, o.step_limiter, y1, f1, aux);
     const double w = act ? 1.0 / (o.rtol * fabs(y1) + o.atol) : 0.0;
     const double rr = bad ? 0.0 : (y1 - y0 - 0.5 * dt * (f0 + f1)) * w;
     const double ee = bad ? 0.0 : 0.5 * dt * (f1 - f0) * w;
@@ -333,8 +334,12 @@ hipError_t launch_accept(const DevMech& m, const BdfOpts& o, int n, const double
   if (m.nsp > kNP - 1) return hipErrorInvalidValue;
   const int grid = n < nblocks ? n : nblocks;
   const size_t lds = sizeof(double) * (size_t)(kNP * kNP + 3 * kNP) + rhs_lds_bytes(m);
-  hipLaunchKernelGGL(accept_kernel, dim3(grid), dim3(kWave), lds, st, m, o, n, dts, dt_all, T0,
-                     P, Y0, T1, Y1, stride, newton, accept, res, jscratch);
+  if (m.n_ext > 0)
+    hipLaunchKernelGGL((accept_kernel<true>), dim3(grid), dim3(kWave), lds, st, m, o, n, dts,
+                       dt_all, T0, P, Y0, T1, Y1, stride, newton, accept, res, jscratch);
+  else
+    hipLaunchKernelGGL((accept_kernel<false>), dim3(grid), dim3(kWave), lds, st, m, o, n, dts,
+                       dt_all, T0, P, Y0, T1, Y1, stride, newton, accept, res, jscratch);
   return hipGetLastError();
 }
 
