@@ -60,6 +60,14 @@ typedef struct {
    * state update still in FP64.  No reference counterpart (the reference
    * preconditions in FP64 only). */
   int fp32_preconditioner;
+  /* > 0: report the time at which T crosses T0 + delta_temperature_ignition
+   * (the plugin option of that name, zerork_reactor_manager.cpp:87; root
+   * function reactor_nvector_serial.cpp:774-784, located by CVODE's root
+   * finding as solver_cvode.cpp:90-92, 259-264 do); 0: off */
+  double delta_temperature_ignition;
+  /* 1: end a reactor's integration at its ignition time (the plugin option
+   * stop_after_ignition, solver_cvode.cpp:262-264) */
+  int stop_after_ignition;
 } zrk_bdf_options;
 
 ZRK_EXPORT void zrk_bdf_default_options(zrk_bdf_options* o);
@@ -108,21 +116,28 @@ ZRK_EXPORT int zrk_integrate_device(zrk_mech m, const zrk_bdf_options* o, int n,
                                     double* T, double* P, double* Y, int stride,
                                     int* nsteps, int* stats, void* stream);
 
-/* zrk_integrate_device with the plugin's source terms (each device array
- * optional, null = none): dpdt[n] (Pa/s), esrc[n] (J/kg/s), ysrc[n*stride]
- * (mass-fraction sources, 1/s, same layout as Y).  Replaces the source-term
- * arguments of ReactorConstantPressureCPU / ReactorConstantVolumeCPU::
- * InitializeState and GetTimeDerivative (reactor_constant_pressure_cpu.cpp:
- * 15-133, reactor_constant_volume_cpu.cpp:15-130), which the manager feeds
- * from the DPDT, E_SRC and Y_SRC aux fields (zerork_reactor_manager.cpp:
- * 529-559, 909-1053).  P is updated to the end pressure (constant pressure:
+/* zrk_integrate_device with the plugin's aux fields (each device array
+ * optional, null = none): source terms dpdt[n] (Pa/s), esrc[n] (J/kg/s),
+ * ysrc[n*stride] (mass-fraction sources, 1/s, same layout as Y), and the
+ * ignition-time output ignition_time[n].  Replaces the source-term arguments
+ * of ReactorConstantPressureCPU / ReactorConstantVolumeCPU::InitializeState
+ * and GetTimeDerivative (reactor_constant_pressure_cpu.cpp:15-133,
+ * reactor_constant_volume_cpu.cpp:15-130), which the manager feeds from the
+ * DPDT, E_SRC and Y_SRC aux fields, and the IGNITION_TIME field the manager
+ * fills from the reactor's root time (zerork_reactor_manager.cpp:529-559,
+ * 909-1053, 1131).  P is updated to the end pressure (constant pressure:
  * P + dpdt dt, or Ru T / (v Wmix) at the initial v with mass sources;
- * constant volume: Ru T / (v Wmix) + dpdt dt). */
-ZRK_EXPORT int zrk_integrate_device_src(zrk_mech m, const zrk_bdf_options* o, int n, double dt,
+ * constant volume: Ru T / (v Wmix) + dpdt dt).  With
+ * o->delta_temperature_ignition > 0, ignition_time[r] receives the last time
+ * in [0, dt] (or just beyond dt, inside the last step) at which T crossed
+ * T0 + delta_temperature_ignition, or 0 if it did not, and each crossing
+ * counts one more CVode call in nsteps (the reference's one-step loop); with
+ * o->stop_after_ignition = 1 the state returned is the state at the crossing. */
+ZRK_EXPORT int zrk_integrate_device_aux(zrk_mech m, const zrk_bdf_options* o, int n, double dt,
                                         double* T, double* P, double* Y, int stride,
                                         const double* dpdt, const double* esrc,
-                                        const double* ysrc, int* nsteps, int* stats,
-                                        void* stream);
+                                        const double* ysrc, double* ignition_time, int* nsteps,
+                                        int* stats, void* stream);
 
 /* Autotuner (no counterpart in the reference): times the integrator on the
  * given device batch (copies of T, P, Y; inputs unchanged) with each team

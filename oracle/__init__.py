@@ -72,7 +72,8 @@ class Opts(C.Structure):
                 ("nlscoef", C.c_double), ("max_steps", C.c_int), ("num_retries", C.c_int),
                 ("retry_atol_adj", C.c_double), ("retry_rtol_adj", C.c_double),
                 ("min_mass_fraction", C.c_double), ("step_limiter", C.c_double),
-                ("Tref", C.c_double), ("constant_volume", C.c_int)]
+                ("Tref", C.c_double), ("constant_volume", C.c_int),
+                ("delta_temperature_ignition", C.c_double), ("stop_after_ignition", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -92,9 +93,9 @@ def _declare(L):
     L.zrko_jacobian.argtypes = [P, P, C.c_double, C.c_double, P, P, P]
     L.zrko_integrate_batch.argtypes = [P, P, C.c_int, C.c_double, P, P, P, C.c_int, P, C.c_int]
     L.zrko_integrate_batch.restype = C.c_int
-    L.zrko_integrate_batch_src.argtypes = [P, P, C.c_int, C.c_double, P, P, P, C.c_int, P,
-                                           C.c_int, P, P, P]
-    L.zrko_integrate_batch_src.restype = C.c_int
+    L.zrko_integrate_batch_aux.argtypes = [P, P, C.c_int, C.c_double, P, P, P, C.c_int, P,
+                                           C.c_int, P, P, P, P]
+    L.zrko_integrate_batch_aux.restype = C.c_int
     L.zrko_rhs_src.argtypes = [P, P, C.c_double, P, C.c_double, P, C.c_double, C.c_double, P, P]
     L.zrko_rhs_src.restype = C.c_int
     L.zrko_dense_getrf.argtypes = [C.c_int, P, P]
@@ -217,8 +218,9 @@ class Oracle:
     def integrate(self, T, P, Y, dt, opts: Opts | None = None, nthreads=0, dpdt=None,
                   esrc=None, ysrc=None):
         """Integrate reactors; returns (T_out, Y_out, stats), stats a dict of
-        per-reactor counters plus "P" (end pressure).  dpdt[n], esrc[n] and
-        ysrc[n, nsp] are the plugin's optional source terms."""
+        per-reactor counters plus "P" (end pressure) and, with
+        opts.delta_temperature_ignition > 0, "ignition_time".  dpdt[n], esrc[n]
+        and ysrc[n, nsp] are the plugin's optional source terms."""
         o = opts or default_opts()
         T = np.array(T, dtype=np.float64, copy=True)
         P = np.array(P, dtype=np.float64, copy=True)
@@ -228,11 +230,15 @@ class Oracle:
         arr = lambda a, shape: None if a is None else np.ascontiguousarray(
             np.broadcast_to(np.asarray(a, dtype=np.float64), shape))
         dp, es, ys = arr(dpdt, (n,)), arr(esrc, (n,)), arr(ysrc, Y.shape)
-        self.L.zrko_integrate_batch_src(self.ptr(), C.byref(o), n, dt, _p(T), _p(P), _p(Y),
+        ign = np.zeros(n) if o.delta_temperature_ignition > 0.0 else None
+        self.L.zrko_integrate_batch_aux(self.ptr(), C.byref(o), n, dt, _p(T), _p(P), _p(Y),
                                         Y.shape[1], C.cast(st, C.c_void_p), nthreads,
                                         _p(dp) if dp is not None else None,
                                         _p(es) if es is not None else None,
-                                        _p(ys) if ys is not None else None)
+                                        _p(ys) if ys is not None else None,
+                                        _p(ign) if ign is not None else None)
         stats = {f: np.array([getattr(s, f) for s in st]) for f, _ in Stats._fields_}
         stats["P"] = P
+        if ign is not None:
+            stats["ignition_time"] = ign
         return T, Y, stats

@@ -28,6 +28,8 @@ void zrko_default_opts(zrko_opts* o) {
   o->step_limiter = 1.0e22;
   o->Tref = 1.0;
   o->constant_volume = 0;
+  o->delta_temperature_ignition = 0.0;
+  o->stop_after_ignition = 0;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1282,9 +1284,113 @@ static void cv_dky0(cvmem* cv, double t, double* out) {
 
 /* CvodeSolver::Integrate (solver_cvode.cpp:51-299): one-step loop, retries,
  * Dky interpolation at min(tcurr, end_time). */
+/* ------------------------------------------------------------------------- */
+/* Root finding for the ignition time: SUNDIALS CVODE 5.8.0 cvRcheck1-3 and    */
+/* cvRootfind for the one function g = T0 + delta_T - T                        */
+/* (reactor_nvector_serial.cpp:774-784), both directions, in the one-step loop  */
+/* of solver_cvode.cpp:228-266                                                  */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  double tlo, glo, Tign, tret, Tref;
+  int iT; /* index of the temperature component */
+} rootst;
+
+#define ROOT_MAX_ITER 200
+#define ROOT_MAX_RETURNS 64 /* root returns per reactor (the device's bound) */
+
+/* temperature component of CVodeGetDky(t, 0), summed as cv_dky0 does */
+static double dky0_T(const cvmem* cv, int iT, double t) {
+  const double s = (t - cv->tn) / cv->h;
+  double out = 0.0;
+  for (int j = cv->q; j >= 0; --j) {
+    double c = 1.0;
+    for (int i = 0; i < j; ++i) c *= s;
+    out += c * cv->zn[j][iT];
+  }
+  return out;
+}
+static double root_g(const rootst* rs, double yT) { return rs->Tign - yT * rs->Tref; }
+static double root_ttol(const cvmem* cv) {
+  return (fabs(cv->tn) + fabs(cv->h)) * DBL_EPSILON * 100.0;
+}
+/* cvRcheck1: the interval starts at the (re)initialisation time */
+static void root_init(const cvmem* cv, rootst* rs) {
+  rs->tlo = cv->tn;
+  rs->glo = root_g(rs, cv->zn[0][rs->iT]);
+  rs->tret = cv->tn;
+}
+/* cvRootfind: Illinois-weighted secant on g over (tlo, thi]; returns 1 when
+ * the interval holds a root, which is then thi (ghi = g there) */
+static int rootfind(const cvmem* cv, const rootst* rs, double* tlo, double* glo, double* thi,
+                    double* ghi, double ttol) {
+  const int zroot0 = *ghi == 0.0;
+  if (zroot0 || !(*glo * *ghi < 0.0)) return zroot0;
+  double alph = 1.0;
+  int side = 0, sideprev = -1;
+  for (int it = 0; it < ROOT_MAX_ITER; ++it) {
+    alph = (sideprev == side) ? ((side == 2) ? alph * 2.0 : alph * 0.5) : 1.0;
+    double tmid = *thi - (*thi - *tlo) * *ghi / (*ghi - alph * *glo);
+    if (fabs(tmid - *tlo) < 0.5 * ttol) {
+      const double fracint = fabs(*thi - *tlo) / ttol;
+      const double fracsub = (fracint > 5.0) ? 0.1 : 0.5 / fracint;
+      tmid = *tlo + fracsub * (*thi - *tlo);
+    }
+    if (fabs(*thi - tmid) < 0.5 * ttol) {
+      const double fracint = fabs(*thi - *tlo) / ttol;
+      const double fracsub = (fracint > 5.0) ? 0.1 : 0.5 / fracint;
+      tmid = *thi - fracsub * (*thi - *tlo);
+    }
+    const double gmid = root_g(rs, dky0_T(cv, rs->iT, tmid));
+    sideprev = side;
+    if (gmid != 0.0 && *glo * gmid < 0.0) { /* sign change in (tlo, tmid) */
+      *thi = tmid;
+      *ghi = gmid;
+      side = 1;
+      if (fabs(*thi - *tlo) <= ttol) break;
+      continue;
+    }
+    if (gmid == 0.0) { /* a zero at tmid */
+      *thi = tmid;
+      *ghi = gmid;
+      break;
+    }
+    *tlo = tmid; /* the sign change is in (tmid, thi) */
+    *glo = gmid;
+    side = 2;
+    if (fabs(*thi - *tlo) <= ttol) break;
+  }
+  return 1;
+}
+/* cvRcheck3 after a step (thi = tn, y = zn[0]) */
+static int root_check3(const cvmem* cv, rootst* rs, double* troot) {
+  double tlo = rs->tlo, glo = rs->glo, thi = cv->tn;
+  double ghi = root_g(rs, cv->zn[0][rs->iT]);
+  const int found = rootfind(cv, rs, &tlo, &glo, &thi, &ghi, root_ttol(cv));
+  rs->tlo = thi;
+  rs->glo = ghi;
+  *troot = thi;
+  return found;
+}
+/* cvRcheck2 at the start of the call after a root return */
+static void root_check2(const cvmem* cv, rootst* rs) {
+  const double ylo = dky0_T(cv, rs->iT, rs->tlo);
+  double g = root_g(rs, ylo);
+  if (g == 0.0) {
+    const double ttol = root_ttol(cv);
+    const double smallh = cv->h > 0.0 ? ttol : -ttol;
+    const double tplus = rs->tlo + smallh;
+    const double yp = ((tplus - cv->tn) * cv->h >= 0.0)
+                          ? ylo + (smallh / cv->h) * cv->zn[1][rs->iT]
+                          : dky0_T(cv, rs->iT, tplus);
+    const double gp = root_g(rs, yp);
+    if (gp != 0.0) g = gp;
+  }
+  rs->glo = g;
+}
+
 static int integrate_one(const zrko_mech* m, const zrko_opts* o, double* P, double dt,
                          double* y, zrko_stats* st, double dpdt, double esrc,
-                         const double* ysrc) {
+                         const double* ysrc, double T0, double* ign_time) {
   const int N = m->nsp + 1;
   reactor_t r;
   reactor_init(&r, m, o, *P);
@@ -1312,12 +1418,43 @@ static int integrate_one(const zrko_mech* m, const zrko_opts* o, double* P, doub
   cv.nlscoef = o->nlscoef;
   cv.hmax_inv = (o->max_dt > 0.0) ? 1.0 / o->max_dt : 0.0;
   cv_reinit(&cv, 0.0, y);
+  /* ignition root function: the crossing of T0 + delta_T, T0 the input T */
+  const int ign = o->delta_temperature_ignition > 0.0;
+  rootst rs;
+  rs.Tign = T0 + o->delta_temperature_ignition;
+  rs.Tref = o->Tref;
+  rs.iT = m->nsp;
+  if (ign) root_init(&cv, &rs);
+  int irfnd = 0, nroots = 0;
+  double troot_last = 0.0;
 
   long nsteps = 0, nst_total = 0, nfe_total = 0, nje_total = 0, nset_total = 0;
   long nni_total = 0, ncfn_total = 0, netf_total = 0;
   int num_tries = 0, flag = CV_SUCCESS;
   double tcurr = 0.0;
+  /* one CVode call per iteration, CV_ONE_STEP (solver_cvode.cpp:228-266) */
   while (tcurr < dt) {
+    if (ign && cv.nst > 0) {
+      /* a later call: after a root return, roots in the rest of the last
+       * step, else the step's end tn, which was not returned yet (CVode's
+       * stop tests at the start of a call) */
+      if (irfnd) root_check2(&cv, &rs);
+      const double troundoff = 100.0 * DBL_EPSILON * (fabs(cv.tn) + fabs(cv.h));
+      if (fabs(cv.tn - rs.tret) > troundoff) {
+        nsteps += 1;
+        double troot;
+        if (nroots < ROOT_MAX_RETURNS && root_check3(&cv, &rs, &troot)) {
+          ++nroots;
+          irfnd = 1;
+          tcurr = rs.tret = troot_last = troot;
+          if (o->stop_after_ignition) break;
+          continue;
+        }
+        irfnd = 0;
+        tcurr = rs.tret = cv.tn;
+        continue;
+      }
+    }
     flag = cv_one_step(&cv, dt, &tcurr);
     nsteps += 1;
     if (cv.nst >= o->max_steps) flag = CV_TOO_MUCH_WORK;
@@ -1334,11 +1471,27 @@ static int integrate_one(const zrko_mech* m, const zrko_opts* o, double* P, doub
       memcpy(ysave, y0, sizeof(double) * (size_t)N);
       cv_reinit(&cv, tcurr, ysave);
       free(ysave);
+      if (ign) {
+        root_init(&cv, &rs);
+        irfnd = 0;
+      }
+    } else if (ign) {
+      rs.tret = cv.tn;
+      double troot;
+      if (nroots < ROOT_MAX_RETURNS && root_check3(&cv, &rs, &troot)) { /* before tn */
+        ++nroots;
+        irfnd = 1;
+        tcurr = rs.tret = troot_last = troot;
+        if (o->stop_after_ignition) break;
+      }
     }
   }
   if (flag == CV_SUCCESS) {
-    const double tend = tcurr < dt ? tcurr : dt;
-    cv_dky0(&cv, tend, y);
+    if (ign && ign_time) *ign_time = troot_last;
+    /* the state at dt, or at the ignition time when stopped there; GetState's
+     * pressure uses the solve's dt (the manager passes dt_calc_) */
+    cv_dky0(&cv, tcurr < dt ? tcurr : dt, y);
+    const double tend = dt;
     /* GetState: constant volume reports P = Ru T / (v Wmix) + dpdt t
      * (reactor_constant_volume_cpu.cpp:44-63, mechanism.cpp:292-297);
      * constant pressure P + dpdt t, or with mass sources Ru T / (v Wmix) at
@@ -1375,14 +1528,14 @@ static int integrate_one(const zrko_mech* m, const zrko_opts* o, double* P, doub
 int zrko_integrate_batch(const zrko_mech* m, const zrko_opts* o, int n, double dt, double* T,
                          double* P, double* mf, int stride, zrko_stats* stats,
                          int nthreads) {
-  return zrko_integrate_batch_src(m, o, n, dt, T, P, mf, stride, stats, nthreads, NULL, NULL,
-                                  NULL);
+  return zrko_integrate_batch_aux(m, o, n, dt, T, P, mf, stride, stats, nthreads, NULL, NULL,
+                                  NULL, NULL);
 }
 
-int zrko_integrate_batch_src(const zrko_mech* m, const zrko_opts* o, int n, double dt,
+int zrko_integrate_batch_aux(const zrko_mech* m, const zrko_opts* o, int n, double dt,
                              double* T, double* P, double* mf, int stride, zrko_stats* stats,
                              int nthreads, const double* dpdt, const double* esrc,
-                             const double* ysrc) {
+                             const double* ysrc, double* ign_time) {
   const int nsp = m->nsp;
   int nfail = 0;
 #ifdef _OPENMP
@@ -1397,7 +1550,8 @@ int zrko_integrate_batch_src(const zrko_mech* m, const zrko_opts* o, int n, doub
     y[nsp] = T[k] / o->Tref;
     int flag = integrate_one(m, o, &P[k], dt, y, stats ? stats + k : NULL,
                              dpdt ? dpdt[k] : 0.0, esrc ? esrc[k] : 0.0,
-                             ysrc ? ysrc + (size_t)k * stride : NULL);
+                             ysrc ? ysrc + (size_t)k * stride : NULL, T[k],
+                             ign_time ? ign_time + k : NULL);
     for (int j = 0; j < nsp; ++j) mf[(size_t)k * stride + j] = y[j];
     T[k] = y[nsp] * o->Tref;
     if (flag < 0) nfail++;

@@ -98,6 +98,8 @@ typedef struct {
   double step_limiter;
   double Tref;             /* reference_temperature */
   int constant_volume;     /* 0: constant pressure, 1: constant volume */
+  double delta_temperature_ignition; /* > 0: locate T crossing T0 + this */
+  int stop_after_ignition; /* 1: end the integration at the crossing */
 } zrko_opts;
 
 typedef struct {
@@ -129,13 +131,15 @@ void zrko_jacobian(const zrko_mech* m, const zrko_opts* o, double P, double t,
 int zrko_integrate_batch(const zrko_mech* m, const zrko_opts* o, int n, double dt,
                          double* T, double* P, double* mf, int stride,
                          zrko_stats* stats, int nthreads);
-/* The same with the plugin's source terms (each array optional): pressure rate
+/* The same with the plugin's aux fields (each array optional): pressure rate
  * dpdt[n] (Pa/s), energy source esrc[n] (J/kg/s), mass-fraction sources
- * ysrc[n*stride] (1/s); P is updated to the end pressure. */
-int zrko_integrate_batch_src(const zrko_mech* m, const zrko_opts* o, int n, double dt,
+ * ysrc[n*stride] (1/s), ignition-time output ign_time[n] (with
+ * o->delta_temperature_ignition > 0: the last crossing time, 0 if none); P is
+ * updated to the end pressure. */
+int zrko_integrate_batch_aux(const zrko_mech* m, const zrko_opts* o, int n, double dt,
                              double* T, double* P, double* mf, int stride,
                              zrko_stats* stats, int nthreads, const double* dpdt,
-                             const double* esrc, const double* ysrc);
+                             const double* esrc, const double* ysrc, double* ign_time);
 /* RHS at time t of the reactor initialised at (P0, y0) with sources. */
 int zrko_rhs_src(const zrko_mech* m, const zrko_opts* o, double P0, const double* y0,
                  double t, const double* y, double dpdt, double esrc, const double* ysrc,

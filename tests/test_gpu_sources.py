@@ -1,6 +1,6 @@
 """GPU parity of the plugin's source terms (DPDT, E_SRC, Y_SRC aux fields:
 reactor_constant_pressure_cpu.cpp:15-133, reactor_constant_volume_cpu.cpp:
-15-130) through zrk_integrate_device_src, against the oracle's restatement
+15-130) through zrk_integrate_device_aux, against the oracle's restatement
 (pinned by the invariants in test_oracle_sources.py), with the integrator
 tests' self-calibrated bar: GPU-vs-oracle end-state differences in tolerance
 units within the oracle's own one-ulp sensitivity on the same inputs; end

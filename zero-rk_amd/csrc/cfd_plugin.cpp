@@ -9,7 +9,10 @@
 // (COST = CVode one-step calls, GPU = seconds per reactor of the batched
 // solve, TEMPERATURE_DELTA = T change above solve_temperature_threshold).
 // The DPDT, E_SRC and Y_SRC aux fields are passed to the integrator as
-// per-reactor source terms (P is then returned as the end pressure).
+// per-reactor source terms (P is then returned as the end pressure); with
+// delta_temperature_ignition > 0 the IGNITION_TIME field receives each
+// reactor's time of crossing T0 + delta_temperature_ignition (0 if none), and
+// stop_after_ignition ends a reactor's integration there.
 // Reactors whose integration fails keep their input state and report a
 // negative cost; the call then returns ZERORK_STATUS_FAILED_SOLVE
 // (:1123-1139).  No C++ exception or exit() crosses the ABI.
@@ -88,6 +91,7 @@ struct zerork_handle_impl {
     iopt["constant_volume"] = 1;
     dopt["reference_temperature"] = 1.0;
     dopt["delta_temperature_ignition"] = 0.0;
+    iopt["stop_after_ignition"] = 0;
     dopt["min_mass_fraction"] = 1.0e-30;
     iopt["always_solve_temperature"] = 1;
     dopt["solve_temperature_threshold"] = 2.0;
@@ -247,11 +251,21 @@ zerork_status_t solve(zerork_handle h, double dt, int n, double* T, double* P, d
   // in the layout of mf), as the manager passes them to InitializeState
   // (zerork_reactor_manager.cpp:909-1053)
   const bool srcs = h->dpdt || h->e_src || h->y_src;
+  // IGNITION_TIME: the reactor's root time (zerork_reactor_manager.cpp:1131)
+  // when delta_temperature_ignition > 0; copied in first so that a failed
+  // reactor keeps the value it had, as the manager leaves it
+  const bool ign = h->dopt["delta_temperature_ignition"] > 0.0 && h->ign != nullptr;
   const double *ddpdt = nullptr, *desrc = nullptr, *dysrc = nullptr;
-  if (srcs) {
-    if ((size_t)(2 * n) > h->cap_src_n) {
-      if (!grow(&h->dsrc_n, 2 * (size_t)n)) return ZERORK_STATUS_UNKNOWN_ERROR;
-      h->cap_src_n = 2 * (size_t)n;
+  double* dign = nullptr;
+  if (srcs || ign) {
+    if ((size_t)(3 * n) > h->cap_src_n) {
+      if (!grow(&h->dsrc_n, 3 * (size_t)n)) return ZERORK_STATUS_UNKNOWN_ERROR;
+      h->cap_src_n = 3 * (size_t)n;
+    }
+    if (ign) {
+      dign = h->dsrc_n + 2 * (size_t)n;
+      ok = ok && hipMemcpyAsync(dign, h->ign, sizeof(double) * n, hipMemcpyHostToDevice, st) ==
+                     hipSuccess;
     }
     if (h->dpdt) {
       ok = ok && hipMemcpyAsync(h->dsrc_n, h->dpdt, sizeof(double) * n, hipMemcpyHostToDevice,
@@ -291,9 +305,12 @@ zerork_status_t solve(zerork_handle h, double dt, int n, double* T, double* P, d
   // mechanisms above 63 species run on multi-wavefront teams, which keep the
   // Newton matrix in FP32 only
   o.fp32_preconditioner = h->iopt["fp32_preconditioner"] != 0 || nsp > 63;
+  o.delta_temperature_ignition = h->dopt["delta_temperature_ignition"];
+  o.stop_after_ignition = h->iopt["stop_after_ignition"] != 0;
   const auto t0 = std::chrono::steady_clock::now();
   int rc = zrk::integrate_device(m, o, n, dt, nullptr, h->dT, h->dP, h->dY, stride, h->dsteps,
-                                 nullptr, sorted ? h->dorder : nullptr, st, ddpdt, desrc, dysrc);
+                                 nullptr, sorted ? h->dorder : nullptr, st, ddpdt, desrc, dysrc,
+                                 dign);
   if (rc != 0) return ZERORK_STATUS_UNKNOWN_ERROR;
   std::vector<int> steps(n);
   ok = hipMemcpyAsync(T, h->dT, sizeof(double) * n, hipMemcpyDeviceToHost, st) == hipSuccess;
@@ -302,6 +319,9 @@ zerork_status_t solve(zerork_handle h, double dt, int n, double* T, double* P, d
     ok = ok && hipMemcpyAsync(P, h->dP, sizeof(double) * n, hipMemcpyDeviceToHost, st) == hipSuccess;
   ok = ok && hipMemcpyAsync(steps.data(), h->dsteps, sizeof(int) * n, hipMemcpyDeviceToHost,
                             st) == hipSuccess;
+  if (ign)
+    ok = ok && hipMemcpyAsync(h->ign, dign, sizeof(double) * n, hipMemcpyDeviceToHost, st) ==
+                   hipSuccess;
   if (hipStreamSynchronize(st) != hipSuccess || !ok) return ZERORK_STATUS_UNKNOWN_ERROR;
   const double secs =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

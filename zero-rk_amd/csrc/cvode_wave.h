@@ -47,6 +47,9 @@ struct CvUni {
   double rl1, gamma, gammap, gamrat, crate, delp, acnrm, nlscoef, saved_tq5;
   double etaq, etaqm1, etaqp1, inv_rho_last, inv_nvar;
   double dpdt, esrc, e0, vinit;  // source terms of this reactor (RhsSrc)
+  // root finding (ignition): interval start tlo and g(tlo), the ignition
+  // temperature, the last time returned by a one-step call
+  double rt_tlo, rt_glo, rt_Tign, rt_tret;
   double pad_;
 };
 static_assert(sizeof(CvUni) % 16 == 0, "CvUni keeps the LDS layout 16-byte aligned");
@@ -110,6 +113,7 @@ struct Cvode {
   double &rl1, &gamma, &gammap, &gamrat, &crate, &delp, &acnrm, &nlscoef, &saved_tq5;
   double &etaq, &etaqm1, &etaqp1, &inv_rho_last, &inv_nvar;
   double &dpdt, &esrc, &e0, &vinit;
+  double &rt_tlo, &rt_glo, &rt_Tign, &rt_tret;
   int q, qprime, qwait, L;
   int nst, nstlp, nstlj, nfe, nje, nsetups, nni, ncfn, netf;
   int convfail, jcur, acnrmcur;
@@ -121,7 +125,8 @@ struct Cvode {
         gamma(u->gamma), gammap(u->gammap), gamrat(u->gamrat), crate(u->crate),
         delp(u->delp), acnrm(u->acnrm), nlscoef(u->nlscoef), saved_tq5(u->saved_tq5),
         etaq(u->etaq), etaqm1(u->etaqm1), etaqp1(u->etaqp1), inv_rho_last(u->inv_rho_last),
-        inv_nvar(u->inv_nvar), dpdt(u->dpdt), esrc(u->esrc), e0(u->e0), vinit(u->vinit) {}
+        inv_nvar(u->inv_nvar), dpdt(u->dpdt), esrc(u->esrc), e0(u->e0), vinit(u->vinit),
+        rt_tlo(u->rt_tlo), rt_glo(u->rt_glo), rt_Tign(u->rt_Tign), rt_tret(u->rt_tret) {}
 
   // the source terms at time tt (src.on false when the reactor has none)
   __device__ __forceinline__ RhsSrc sources(double tt) const {
@@ -887,6 +892,100 @@ struct Cvode {
     const int kflag = step();
     tret = tn;
     return kflag;
+  }
+
+  // ---- root finding for the ignition time: SUNDIALS CVODE 5.8.0's
+  // cvRcheck1-3 and cvRootfind for the one root function
+  // g = T0 + delta_T - T (reactor_nvector_serial.cpp:774-784), both
+  // directions, in the one-step mode of solver_cvode.cpp:228-266.  All values
+  // are uniform over the team.
+  static constexpr int kRootMaxIter = 200;  // bound on the secant iterations
+
+  __device__ __forceinline__ double root_g_of(double yv) const {
+    return rt_Tign - team.from(yv, mech.nsp) * opt.Tref;
+  }
+  __device__ __forceinline__ double root_ttol() const {
+    return (fabs(tn) + fabs(h)) * DBL_EPSILON * 100.0;
+  }
+
+  // cvRcheck1: the interval starts at the (re)initialisation time
+  __device__ __forceinline__ void root_init() {
+    rt_tlo = tn;
+    rt_glo = root_g_of(zn[0]);
+    rt_tret = tn;
+  }
+
+  // cvRootfind: a sign change of g over (tlo, thi] is narrowed to within
+  // ttol by the Illinois-weighted secant; true when (tlo, thi] holds a root,
+  // which is then thi (ghi = g there)
+  __device__ __forceinline__ bool rootfind(double& tlo, double& glo, double& thi, double& ghi,
+                           double ttol) const {
+    const bool zroot0 = ghi == 0.0;
+    if (zroot0 || !(glo * ghi < 0.0)) return zroot0;
+    double alph = 1.0;
+    int side = 0, sideprev = -1;
+    for (int it = 0; it < kRootMaxIter; ++it) {
+      alph = (sideprev == side) ? ((side == 2) ? alph * 2.0 : alph * 0.5) : 1.0;
+      double tmid = thi - (thi - tlo) * ghi / (ghi - alph * glo);
+      if (fabs(tmid - tlo) < 0.5 * ttol) {
+        const double fracint = fabs(thi - tlo) / ttol;
+        const double fracsub = (fracint > 5.0) ? 0.1 : 0.5 / fracint;
+        tmid = tlo + fracsub * (thi - tlo);
+      }
+      if (fabs(thi - tmid) < 0.5 * ttol) {
+        const double fracint = fabs(thi - tlo) / ttol;
+        const double fracsub = (fracint > 5.0) ? 0.1 : 0.5 / fracint;
+        tmid = thi - fracsub * (thi - tlo);
+      }
+      const double gmid = root_g_of(dky0(tmid));
+      sideprev = side;
+      if (gmid != 0.0 && glo * gmid < 0.0) {  // sign change in (tlo, tmid)
+        thi = tmid;
+        ghi = gmid;
+        side = 1;
+        if (fabs(thi - tlo) <= ttol) break;
+        continue;
+      }
+      if (gmid == 0.0) {  // a zero at tmid
+        thi = tmid;
+        ghi = gmid;
+        break;
+      }
+      tlo = tmid;  // the sign change is in (tmid, thi)
+      glo = gmid;
+      side = 2;
+      if (fabs(thi - tlo) <= ttol) break;
+    }
+    return true;
+  }
+
+  // cvRcheck3 after a step (thi = tn, y = zn[0]): true with the root time in
+  // troot when (tlo, tn] holds a root; the interval start moves to the root or
+  // to tn
+  __device__ __forceinline__ bool root_check3(double& troot) {
+    double tlo = rt_tlo, glo = rt_glo, thi = tn;
+    double ghi = root_g_of(zn[0]);
+    const bool found = rootfind(tlo, glo, thi, ghi, root_ttol());
+    rt_tlo = thi;
+    rt_glo = ghi;
+    troot = thi;
+    return found;
+  }
+
+  // cvRcheck2 at the start of the call after a root return: g at the root
+  // again; if it is exactly zero, its value a roundoff interval later
+  __device__ __forceinline__ void root_check2() {
+    const double ylo = dky0(rt_tlo);
+    double g = root_g_of(ylo);
+    if (g == 0.0) {
+      const double ttol = root_ttol();
+      const double smallh = h > 0.0 ? ttol : -ttol;
+      const double tplus = rt_tlo + smallh;
+      const double yp = ((tplus - tn) * h >= 0.0) ? ylo + (smallh / h) * zn[1] : dky0(tplus);
+      const double gp = root_g_of(yp);
+      if (gp != 0.0) g = gp;
+    }
+    rt_glo = g;
   }
 
   // CVodeGetDky(t, 0)

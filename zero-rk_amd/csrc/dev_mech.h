@@ -27,6 +27,12 @@ struct BdfOpts {
   const double* dpdt = nullptr;
   const double* esrc = nullptr;
   const double* ysrc = nullptr;
+  // ignition: root of T - (T0 + ign_dT) (off when ign_dT <= 0); ign_time[n]
+  // receives the last root time (0 if none); stop_after_ignition ends the
+  // integration at the root
+  double ign_dT = 0.0;
+  int stop_after_ignition = 0;
+  double* ign_time = nullptr;
 };
 
 enum RecFlags : int {

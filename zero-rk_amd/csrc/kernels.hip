@@ -236,10 +236,46 @@ __device__ __forceinline__ void bdf_driver(Cvode<MT, Team>& cv, const DevMech& m
     const long long tr0 = (long long)__builtin_amdgcn_s_memtime();
 #endif
     cv.reinit(0.0, y0);
+    // ignition root function (reactor_nvector_serial.cpp:774-784): the
+    // crossing of T0 + delta_T, T0 the input temperature (compiled into the
+    // kExt variants only)
+    const bool ign = Team::kExt && o.ign_dT > 0.0;
+    double troot_last = 0.0;
+    bool irfnd = false;
+    // root returns per reactor are bounded (an interpolant of degree <= 5 has
+    // at most 5 roots per step; round-off flips of g near a double root must
+    // not keep a wavefront here)
+    constexpr int kMaxRootReturns = 64;
+    int nroots = 0;
+    if (ign) {
+      cv.rt_Tign = Tio[r] + o.ign_dT;
+      cv.root_init();
+    }
     int nsteps = 0, num_tries = 0, flag = kCvSuccess;
     int tot[7] = {0, 0, 0, 0, 0, 0, 0};
     double tcurr = 0.0;
+    // one CVode call per iteration, CV_ONE_STEP (solver_cvode.cpp:228-266)
     while (tcurr < dt) {
+      if (ign && cv.nst > 0) {
+        // a later call: after a root return, roots in the rest of the last
+        // step, else the step's end tn, which was not returned yet
+        if (irfnd) cv.root_check2();
+        const double troundoff = 100.0 * DBL_EPSILON * (fabs(cv.tn) + fabs(cv.h));
+        if (fabs(cv.tn - cv.rt_tret) > troundoff) {
+          nsteps += 1;
+          double troot;
+          if (nroots < kMaxRootReturns && cv.root_check3(troot)) {
+            ++nroots;
+            irfnd = true;
+            tcurr = cv.rt_tret = troot_last = troot;
+            if (o.stop_after_ignition) break;
+            continue;
+          }
+          irfnd = false;
+          tcurr = cv.rt_tret = cv.tn;
+          continue;
+        }
+      }
       flag = cv.one_step(dt, tcurr);
       nsteps += 1;
       if (cv.nst >= o.max_steps) flag = kCvTooMuchWork;
@@ -251,10 +287,25 @@ __device__ __forceinline__ void bdf_driver(Cvode<MT, Team>& cv, const DevMech& m
         tot[4] += cv.nni; tot[5] += cv.ncfn; tot[6] += cv.netf;
         if (num_tries == o.num_retries) break;
         cv.reinit(tcurr, cv.zn[0]);
+        if (ign) {
+          cv.root_init();
+          irfnd = false;
+        }
+      } else if (ign) {
+        cv.rt_tret = cv.tn;
+        double troot;
+        if (nroots < kMaxRootReturns && cv.root_check3(troot)) {  // returned before tn
+          ++nroots;
+          irfnd = true;
+          tcurr = cv.rt_tret = troot_last = troot;
+          if (o.stop_after_ignition) break;
+        }
       }
     }
     // failed reactors keep their input state (zerork_reactor_manager.cpp:1123-1139)
     if (flag == kCvSuccess) {
+      if (ign && o.ign_time && lane == 0) o.ign_time[r] = troot_last;
+      // the state at dt, or at the ignition time when stopped there
       const double yout = cv.dky0(tcurr < dt ? tcurr : dt);
       tot[0] += cv.nst; tot[1] += cv.nfe; tot[2] += cv.nje; tot[3] += cv.nsetups;
       tot[4] += cv.nni; tot[5] += cv.ncfn; tot[6] += cv.netf;
@@ -263,8 +314,9 @@ __device__ __forceinline__ void bdf_driver(Cvode<MT, Team>& cv, const DevMech& m
       // GetState (reactor_constant_volume_cpu.cpp:44-63,
       // reactor_constant_pressure_cpu.cpp:44-66): constant volume P = Ru T /
       // (v Wmix) + dpdt t; constant pressure P + dpdt t, or with mass sources
-      // Ru T / (v Wmix) at the initial v
-      const double tend = tcurr < dt ? tcurr : dt;
+      // Ru T / (v Wmix) at the initial v; t = dt (the manager passes the
+      // solve's dt, also when the integration stopped at ignition)
+      const double tend = dt;
       if (o.constant_volume || cv.has_ysrc) {
         const double mws = cv.team.sum(lane < nsp ? yout * m.inv_mw[lane] : 0.0);
         const double Tend = cv.team.from(yout, nsp) * o.Tref;
@@ -396,13 +448,15 @@ __global__ void fill_u32_kernel(unsigned int* p, unsigned int v) {
 }
 
 // ---------------------------------------------------------------------------
-// launchers.  The kExt variants (with the general record evaluation) are
-// launched only for mechanisms that have PLOG, non-integer or SRI records.
+// launchers.  The kExt variants (with the general record evaluation and, in
+// the integrator, the ignition root finding) are launched only for
+// mechanisms that have PLOG, non-integer or SRI records, or integrations
+// that ask for ignition times.
 // ---------------------------------------------------------------------------
 namespace {
 template <class F>
-hipError_t with_ext(const DevMech& m, F&& f) {
-  if (m.n_ext > 0) return f(std::true_type{});
+hipError_t with_ext(bool ext, F&& f) {
+  if (ext) return f(std::true_type{});
   return f(std::false_type{});
 }
 }  // namespace
@@ -414,7 +468,7 @@ hipError_t launch_rates(const DevMech& m, int n, const double* T, const double* 
   const size_t lds = sizeof(double) * (size_t)kRatesWaves * (3 * m.nsp + 2 + m.nstep + 1);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   int blocks = (n + kRatesWaves - 1) / kRatesWaves;
-  return with_ext(m, [&](auto e) {
+  return with_ext(m.n_ext > 0, [&](auto e) {
     hipLaunchKernelGGL((rates_kernel<decltype(e)::value>), dim3(blocks),
                        dim3(kWave * kRatesWaves), lds, st, m, n, T, P, Y, stride, L, ydot, rop,
                        create, destroy, net);
@@ -432,7 +486,7 @@ hipError_t launch_jacobian(const DevMech& m, const BdfOpts& o, int n, const doub
   const int grid = nblocks < n ? nblocks : n;
   const int W = bdf_team_width(m.nsp);
   if (W == 0) return hipErrorInvalidValue;
-  return with_ext(m, [&](auto e) {
+  return with_ext(m.n_ext > 0, [&](auto e) {
     constexpr bool E = decltype(e)::value;
     if (W > 1) {
       const size_t L = (size_t)kWave * W;
@@ -510,7 +564,7 @@ int bdf_max_resident_teams(const DevMech& m, bool f32, int W) {
   if (!bdf_team_ok(m.nsp, f32, W)) return 0;
   const size_t lds = bdf_lds_bytes(m, f32, W);
   int r = 0;
-  (void)with_ext(m, [&](auto e) {
+  (void)with_ext(m.n_ext > 0, [&](auto e) {
     constexpr bool E = decltype(e)::value;
     switch (W) {
       case 1:
@@ -542,7 +596,7 @@ hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, cons
   if (lds > max_lds_per_block()) return hipErrorInvalidValue;
   hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, counter, 0u);
   const int grid = nwaves < n ? nwaves : n;
-  return with_ext(m, [&](auto e) {
+  return with_ext(m.n_ext > 0 || o.ign_dT > 0.0, [&](auto e) {
     constexpr bool E = decltype(e)::value;
     if (W == 1 && f32)
       hipLaunchKernelGGL((bdf_kernel<float, E>), dim3(grid), dim3(kWave), lds, st, m, o, n, dt,

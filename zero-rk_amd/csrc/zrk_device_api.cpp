@@ -396,6 +396,8 @@ BdfOpts to_bdf_opts(const zrk_bdf_options& o) {
   b.num_retries = o.cvode_num_retries;
   b.constant_volume = o.constant_volume != 0;
   b.fp32_preconditioner = o.fp32_preconditioner != 0;
+  b.ign_dT = o.delta_temperature_ignition > 0.0 ? o.delta_temperature_ignition : 0.0;
+  b.stop_after_ignition = o.stop_after_ignition != 0;
   return b;
 }
 
@@ -423,7 +425,7 @@ int team_width_for(MechHandle* h, bool f32) {
 int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
                      const double* dts, double* T, double* P, double* Y, int stride, int* nsteps, int* stats,
                      const int* order, hipStream_t st, const double* dpdt, const double* esrc,
-                     const double* ysrc) {
+                     const double* ysrc, double* ign_time) {
   if (n < 0 || !T || !P || !Y || stride < h->host.nsp || (!dts && !(dt > 0.0))) return -1;
   const int W0 = bdf_team_width(h->host.nsp);
   if (W0 == 0) return -2;                               // mechanism too large
@@ -440,6 +442,7 @@ int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
   bo.dpdt = dpdt;
   bo.esrc = esrc;
   bo.ysrc = ysrc;
+  bo.ign_time = bo.ign_dT > 0.0 ? ign_time : nullptr;
   hipError_t e = launch_bdf(h->dev, bo, n, dt, dts, T, P, Y, stride, nsteps, stats, order,
                             h->ws_counter, h->ws_jac, nw, W, st);
   return e == hipSuccess ? 0 : (int)e;
@@ -468,6 +471,8 @@ void zrk_bdf_default_options(zrk_bdf_options* o) {
   // device-level default is the constant-pressure reactor
   o->constant_volume = 0;
   o->fp32_preconditioner = 0;
+  o->delta_temperature_ignition = 0.0;
+  o->stop_after_ignition = 0;
 }
 
 int zrk_mech_load(const char* mech_file, const char* therm_file, zrk_mech* out, char* err,
@@ -578,14 +583,14 @@ int zrk_integrate_device(zrk_mech m, const zrk_bdf_options* o, int n, double dt,
                                stride, nsteps, stats, nullptr, (hipStream_t)stream);
 }
 
-int zrk_integrate_device_src(zrk_mech m, const zrk_bdf_options* o, int n, double dt, double* T,
+int zrk_integrate_device_aux(zrk_mech m, const zrk_bdf_options* o, int n, double dt, double* T,
                              double* P, double* Y, int stride, const double* dpdt,
-                             const double* esrc, const double* ysrc, int* nsteps, int* stats,
-                             void* stream) {
+                             const double* esrc, const double* ysrc, double* ignition_time,
+                             int* nsteps, int* stats, void* stream) {
   if (!m || !o) return -1;
   return zrk::integrate_device(reinterpret_cast<MechHandle*>(m), *o, n, dt, nullptr, T, P, Y,
                                stride, nsteps, stats, nullptr, (hipStream_t)stream, dpdt, esrc,
-                               ysrc);
+                               ysrc, ignition_time);
 }
 
 int zrk_autotune_integrator(zrk_mech m, const zrk_bdf_options* o, int n, double dt,

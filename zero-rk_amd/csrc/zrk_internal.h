@@ -35,10 +35,12 @@ int team_width_for(MechHandle* h, bool f32);
 int ensure_device(MechHandle* h);
 int ensure_workspace(MechHandle* h, size_t doubles);
 BdfOpts to_bdf_opts(const zrk_bdf_options& o);
-// dpdt[n], esrc[n], ysrc[n*stride]: optional device arrays of source terms
+// dpdt[n], esrc[n], ysrc[n*stride]: optional device arrays of source terms;
+// ign_time[n]: optional ignition-time output (with o.delta_temperature_ignition > 0)
 int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
                      const double* dts, double* T, double* P, double* Y, int stride, int* nsteps, int* stats,
                      const int* order, hipStream_t st, const double* dpdt = nullptr,
-                     const double* esrc = nullptr, const double* ysrc = nullptr);
+                     const double* esrc = nullptr, const double* ysrc = nullptr,
+                     double* ign_time = nullptr);
 
 }  // namespace zrk

@@ -39,7 +39,7 @@ DEVICE_SYMBOLS = [
     "zrk_bdf_default_options", "zrk_mech_load", "zrk_mech_free", "zrk_mech_info",
     "zrk_mech_species", "zrk_mech_molwt", "zrk_mech_steps", "zrk_mech_reactions",
     "zrk_rates_device",
-    "zrk_jacobian_device", "zrk_integrate_device", "zrk_integrate_device_src",
+    "zrk_jacobian_device", "zrk_integrate_device", "zrk_integrate_device_aux",
     "zrk_integrator_resident_waves",
     "zrk_autotune_integrator", "zrk_integrator_set_team_width",
 ]
@@ -53,7 +53,8 @@ class BdfOptions(C.Structure):
         ("cvode_retry_absolute_tolerance_adjustment", C.c_double),
         ("cvode_retry_relative_tolerance_adjustment", C.c_double),
         ("max_steps", C.c_int), ("cvode_num_retries", C.c_int), ("constant_volume", C.c_int),
-        ("fp32_preconditioner", C.c_int),
+        ("fp32_preconditioner", C.c_int), ("delta_temperature_ignition", C.c_double),
+        ("stop_after_ignition", C.c_int),
     ]
 
 
@@ -115,9 +116,9 @@ def _declare(L):
     L.zrk_jacobian_device.restype = I
     L.zrk_integrate_device.argtypes = [P, C.POINTER(BdfOptions), I, D, P, P, P, I, P, P, P]
     L.zrk_integrate_device.restype = I
-    L.zrk_integrate_device_src.argtypes = [P, C.POINTER(BdfOptions), I, D, P, P, P, I, P, P, P,
-                                           P, P, P]
-    L.zrk_integrate_device_src.restype = I
+    L.zrk_integrate_device_aux.argtypes = [P, C.POINTER(BdfOptions), I, D, P, P, P, I, P, P, P,
+                                           P, P, P, P]
+    L.zrk_integrate_device_aux.restype = I
     L.zrk_integrator_resident_waves.argtypes = [P, P]
     L.zrk_autotune_integrator.argtypes = [P, C.POINTER(BdfOptions), I, D, P, P, P, I, P, P, P]
     L.zrk_autotune_integrator.restype = I
@@ -219,17 +220,20 @@ class Mech:
             raise RuntimeError(f"zrk_jacobian_device failed ({rc})")
 
     def integrate(self, n, dt, T, P, Y, stride, opts: BdfOptions | None = None, nsteps=None,
-                  stats=None, stream=0, dpdt=None, esrc=None, ysrc=None):
+                  stats=None, stream=0, dpdt=None, esrc=None, ysrc=None, ignition_time=None):
         """Integrate n reactors (device tensors); dpdt[n], esrc[n], ysrc[n*stride]
-        are the optional source terms (device tensors)."""
+        are the optional source terms and ignition_time[n] the optional output
+        of ignition times (with opts.delta_temperature_ignition > 0), all
+        device tensors."""
         o = opts or default_options()
-        if dpdt is None and esrc is None and ysrc is None:
+        if dpdt is None and esrc is None and ysrc is None and ignition_time is None:
             rc = lib().zrk_integrate_device(self.h, C.byref(o), n, dt, _ptr(T), _ptr(P), _ptr(Y),
                                             stride, _ptr(nsteps), _ptr(stats), stream)
         else:
-            rc = lib().zrk_integrate_device_src(self.h, C.byref(o), n, dt, _ptr(T), _ptr(P),
+            rc = lib().zrk_integrate_device_aux(self.h, C.byref(o), n, dt, _ptr(T), _ptr(P),
                                                 _ptr(Y), stride, _ptr(dpdt), _ptr(esrc),
-                                                _ptr(ysrc), _ptr(nsteps), _ptr(stats), stream)
+                                                _ptr(ysrc), _ptr(ignition_time), _ptr(nsteps),
+                                                _ptr(stats), stream)
         if rc != 0:
             raise RuntimeError(f"zrk_integrate_device failed ({rc})")
 
