@@ -73,3 +73,35 @@ def test_plugin_failed_solve_keeps_state(torch_gpu):
     np.testing.assert_array_equal(T, st.T)
     np.testing.assert_array_equal(Y, st.Y)
     assert np.all(cost < 0)
+
+
+def test_plugin_reactor_dumps(torch_gpu, tmp_path, monkeypatch):
+    """dump_reactors (zerork_reactor_manager.cpp:904-906, 1042-1045, 1477-1497):
+    per call, dumpfile_pre_<call>_<rank> and dumpfile_postg_<call>_<rank> with
+    one line per reactor "id T P cost gpu mf..." in the solve order (descending
+    previous cost), at 20 significant digits (exact round trip)."""
+    from zerork_hip import capi, mechgen
+    monkeypatch.chdir(tmp_path)
+    st = mechgen.workload_states("10sp", mechgen.workload_mechanism("10sp"), 64, seed=44)
+    n, nsp = st.Y.shape
+    print("case plugin dump_reactors", flush=True)
+    p = _plugin(capi, "synth_10sp", constant_volume=0, rel_tol=1e-6, abs_tol=1e-12,
+                dump_reactors=1)
+    T, P, Y = st.T.copy(), st.P.copy(), st.Y.copy()
+    cost = np.zeros(n)
+    gpu = np.zeros(n)
+    assert p.set_aux("COST", cost) == 0 and p.set_aux("GPU", gpu) == 0
+    for call in (1, 2):
+        T0, P0, Y0, c0 = T.copy(), P.copy(), Y.copy(), cost.copy()
+        assert p.solve(call - 1, 0.0, 1e-5, T, P, Y) == 0
+        order = np.argsort(-c0, kind="stable")
+        for tag, (Tx, Px, Yx, cx) in (("pre", (T0, P0, Y0, c0)), ("postg", (T, P, Y, cost))):
+            rows = [l.rstrip("\n").rstrip("\t").split("\t")
+                    for l in open(tmp_path / f"dumpfile_{tag}_{call:06d}_000000")]
+            assert len(rows) == n
+            a = np.array([[float(v) for v in r] for r in rows])
+            np.testing.assert_array_equal(a[:, 0], np.arange(n))
+            np.testing.assert_array_equal(a[:, 1], Tx[order])
+            np.testing.assert_array_equal(a[:, 2], Px[order])
+            np.testing.assert_array_equal(a[:, 3], cx[order])
+            np.testing.assert_array_equal(a[:, 5:], Yx[order])

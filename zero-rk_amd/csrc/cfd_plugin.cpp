@@ -57,9 +57,10 @@ struct zerork_handle_impl {
   double* dY = nullptr;
   int* dsteps = nullptr;
   int* dorder = nullptr;
-  double* dsrc_n = nullptr;  // dpdt then e_src (2n)
+  double* dsrc_n = nullptr;  // dpdt, e_src, ignition times (3n)
   double* dysrc = nullptr;   // y_src (n*stride)
   size_t cap_n = 0, cap_y = 0, cap_src_n = 0, cap_src_y = 0;
+  int n_calls = 0;           // solves so far (names the reactor dump files)
 
   zerork_handle_impl() {
     sopt["mech_filename"] = "mech.dat";
@@ -214,6 +215,21 @@ zerork_status_t read_options(zerork_handle h, const char* fn) {
   return ZERORK_STATUS_SUCCESS;
 }
 
+// ZeroRKReactorManager::DumpReactor (zerork_reactor_manager.cpp:1477-1497):
+// one tab-separated line "id T P cost gpu mf_0 .. mf_{nsp-1}" (20 significant
+// digits) appended to dumpfile_<tag>_<call, 6 digits>_<rank, 6 digits>.
+void dump_reactor(const zerork_handle_impl* h, const char* tag, int id, double T, double P,
+                  double rc, double rg, const double* mf, int nsp) {
+  char name[96];
+  std::snprintf(name, sizeof name, "dumpfile_%s_%06d_%06d", tag, h->n_calls, 0);
+  std::FILE* f = std::fopen(name, "a");
+  if (!f) return;
+  std::fprintf(f, "%d\t%.20g\t%.20g\t%.20g\t%.20g\t", id, T, P, rc, rg);
+  for (int k = 0; k < nsp; ++k) std::fprintf(f, "%.20g\t", mf[k]);
+  std::fprintf(f, "\n");
+  std::fclose(f);
+}
+
 zerork_status_t solve(zerork_handle h, double dt, int n, double* T, double* P, double* mf) {
   if (n < 0 || !T || !P || (n > 0 && !mf)) return ZERORK_STATUS_INVALID_POINTER;
   if (n == 0) return ZERORK_STATUS_SUCCESS;
@@ -240,6 +256,16 @@ zerork_status_t solve(zerork_handle h, double dt, int n, double* T, double* P, d
   if (sorted)
     std::stable_sort(order.begin(), order.end(),
                      [&](int a, int b) { return h->cost[a] > h->cost[b]; });
+  // reactor dumps (zerork_reactor_manager.cpp:837, 904-906, 1042-1045, 1124-1128):
+  // ids are positions in the solve order
+  h->n_calls++;
+  const bool dump = h->iopt["dump_reactors"] != 0;
+  if (dump)
+    for (int j = 0; j < n; ++j) {
+      const int k = order[j];
+      dump_reactor(h, "pre", j, T[k], P[k], h->cost ? h->cost[k] : 0.0, h->gpu ? h->gpu[k] : 0.0,
+                   mf + (size_t)k * stride, nsp);
+    }
   hipStream_t st = h->stream;
   bool ok = hipMemcpyAsync(h->dT, T, sizeof(double) * n, hipMemcpyHostToDevice, st) == hipSuccess;
   ok = ok && hipMemcpyAsync(h->dP, P, sizeof(double) * n, hipMemcpyHostToDevice, st) == hipSuccess;
@@ -336,6 +362,15 @@ zerork_status_t solve(zerork_handle h, double dt, int n, double* T, double* P, d
       h->tdelta[k] = d < thr ? 0.0 : d;
     }
   }
+  const bool dump_failed = h->iopt["dump_failed_reactors"] != 0;
+  if (dump || dump_failed)
+    for (int j = 0; j < n; ++j) {
+      const int k = order[j];
+      const double* y = mf + (size_t)k * stride;
+      if (dump_failed && steps[k] < 0)  // the kept input state
+        dump_reactor(h, "failed_state", j, T[k], P[k], steps[k], secs / n, y, nsp);
+      if (dump) dump_reactor(h, "postg", j, T[k], P[k], steps[k], secs / n, y, nsp);
+    }
   return flag;
 }
 
