@@ -63,7 +63,7 @@ class _Mech(C.Structure):
         ("n_plog", C.c_int), ("plog_step", C.c_void_p), ("plog_ptr", C.c_void_p),
         ("plog_p", C.c_void_p), ("plog_lnp", C.c_void_p), ("plog_lptr", C.c_void_p),
         ("plog_lnA", C.c_void_p), ("plog_sign", C.c_void_p), ("plog_beta", C.c_void_p),
-        ("plog_Ta", C.c_void_p),
+        ("plog_Ta", C.c_void_p), ("ru", C.c_double),
     ]
 
 
@@ -118,8 +118,10 @@ def default_opts(**kw) -> Opts:
 class Oracle:
     """ctypes view of one mechanism for the C oracle."""
 
-    def __init__(self, mech_path: str, therm_path: str | None = None):
-        t = ckparse.build_tables(mech_path, therm_path)
+    def __init__(self, mech_path: str, therm_path: str | None = None, ru: float | None = None):
+        """ru: gas constant (J/kmol/K) for reproducing outputs of reference
+        releases that used another value; default the current one."""
+        t = ckparse.build_tables(mech_path, therm_path, ru=ru or ckparse.NIST_RU)
         self.tables = t
         i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
         f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
@@ -146,6 +148,7 @@ class Oracle:
             setattr(m, k, _p(v))
         m.n_arr, m.n_keq, m.n_tb, m.n_fo = len(t.arr_step), len(t.keq_step), len(t.tb_fwd), len(t.fo_fwd)
         m.n_ni, m.n_plog = len(t.ni_step), len(t.plog_step)
+        m.ru = float(ru or 0.0)
         self._m = m
         self.L = lib()
 

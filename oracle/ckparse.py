@@ -175,6 +175,14 @@ def _parse_thermo(lines, names_wanted):
 _SPLIT_EQ = re.compile(r"<=>|=>|=")
 
 
+def _ln(a: float) -> float:
+    """C's log: -inf for 0 (an A-factor of 0 gives a zero rate, as in
+    rate_const.cpp's log of the A-factor), NaN for negative values."""
+    if a > 0.0:
+        return math.log(a)
+    return -math.inf if a == 0.0 else math.nan
+
+
 def _split_species(side: str):
     """'2H + O2' -> [('H',2), ('O2',1)]; '+M' handled by caller."""
     out = []
@@ -200,14 +208,15 @@ def _split_species(side: str):
     return out
 
 
-def _parse_reactions(lines, units_tok, species):
-    conv = CAL_PER_MOL_TACT
+def _parse_reactions(lines, units_tok, species, ru=NIST_RU):
+    cal = 4184.0 / ru
+    conv = cal
     for t in units_tok:
         tu = t.upper()
         for k in ("KCAL/MOL", "KJOULES/MOL", "JOULES/MOL", "CAL/MOL", "KELVIN"):
             if tu.startswith(k):
-                conv = {"KCAL/MOL": UNITS_TACT["KCAL/MOLE"], "KJOULES/MOL": UNITS_TACT["KJOULES/MOLE"],
-                        "JOULES/MOL": UNITS_TACT["JOULES/MOLE"], "CAL/MOL": UNITS_TACT["CAL/MOLE"],
+                conv = {"KCAL/MOL": cal * 1000.0, "KJOULES/MOL": 238.846 * cal,
+                        "JOULES/MOL": 238.846 * cal / 1000.0, "CAL/MOL": cal,
                         "KELVIN": 1.0}[k]
                 break
     spset = set(species)
@@ -345,7 +354,7 @@ class Tables:
         return len(self.step_rxn)
 
 
-def build_tables(mech_path: str, therm_path: str | None) -> Tables:
+def build_tables(mech_path: str, therm_path: str | None, ru: float = NIST_RU) -> Tables:
     text = open(mech_path).read()
     secs = _sections(text)
     elements, species, rx_lines, units = [], [], [], []
@@ -384,7 +393,7 @@ def build_tables(mech_path: str, therm_path: str | None) -> Tables:
         thermo[j, 1:8] = low
         thermo[j, 8:15] = high
     idx = {s: j for j, s in enumerate(species)}
-    rxns = _parse_reactions(rx_lines, units, species)
+    rxns = _parse_reactions(rx_lines, units, species, ru)
     nrxn = len(rxns)
 
     def is_nonint(r):
@@ -514,7 +523,7 @@ def build_tables(mech_path: str, therm_path: str | None) -> Tables:
     for e in arr_sorted:
         if last is None or not same(last, e):
             last = e
-        rep.append((e[3], math.log(last[0]), last[1], last[2]))
+        rep.append((e[3], _ln(last[0]), last[1], last[2]))
     rep.sort(key=lambda t: t[0])
     arr_step = np.array([t[0] for t in rep], dtype=np.int32)
     arr_lnA = np.array([t[1] for t in rep])
@@ -542,7 +551,7 @@ def build_tables(mech_path: str, therm_path: str | None) -> Tables:
             single = r.third_body.upper() != "M"
             fo_spc.append(idx[r.third_body] if single else -1)
             fo_l.append(eff_list(r) if not single else [])
-            p = [math.log(r.low[0] * 1000.0 ** (-step_order[s_])), r.low[1], r.low[2],
+            p = [_ln(r.low[0] * 1000.0 ** (-step_order[s_])), r.low[1], r.low[2],
                  0.0, 0.0, 0.0, 0.0, 0.0]
             if r.sri:
                 c = r.sri[2]
@@ -582,6 +591,6 @@ def build_tables(mech_path: str, therm_path: str | None) -> Tables:
                   F([c for e in ni for _, c in e[3]]),
                   I([e[0] for e in plog]), plog_ptr, F([p for e in plog for p in e[1]]),
                   F([math.log(p) for e in plog for p in e[1]]), I(lptr),
-                  F([math.log(abs(l[1])) if l[1] != 0 else -math.inf for l in lines]),
+                  F([_ln(abs(l[1])) for l in lines]),
                   F([-1.0 if l[1] < 0.0 else 1.0 for l in lines]),
                   F([l[2] for l in lines]), F([l[3] for l in lines]))

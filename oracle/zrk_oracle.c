@@ -10,7 +10,9 @@
 #include <omp.h>
 #endif
 
-#define NIST_RU 8.314462618e3 /* src/zerork/constants.h:10 */
+#define NIST_RU_DEFAULT 8.314462618e3 /* src/zerork/constants.h:10 */
+/* the mechanism's gas constant (zrko_mech.ru; 0 selects the default) */
+#define NIST_RU (m->ru > 0.0 ? m->ru : NIST_RU_DEFAULT)
 #define P_ATM 1.01325e5       /* src/zerork/constants.h:12 */
 #define TLIMIT 1.0e4          /* reactor_constant_pressure_cpu.cpp:88 */
 

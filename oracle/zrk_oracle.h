@@ -84,6 +84,9 @@ typedef struct {
   const double* plog_sign;
   const double* plog_beta;
   const double* plog_Ta;
+  /* gas constant J/kmol/K; 0: src/zerork/constants.h:10's 8.314462618e3 (the
+   * reference's earlier releases used CODATA 2014's 8.3144598e3) */
+  double ru;
 } zrko_mech;
 
 typedef struct {

@@ -98,7 +98,40 @@ H2_FEATURES = """! ---- added for the extended-record tests ----
 SEEDS = {"synth_10sp": 5256, "synth_53sp": 5257, "synth_160sp": 5258}
 
 
+# The reference tester's outputs (examples/cfd_plugin_tester/outputs.ref) and
+# inputs: reactor 0's history over 10 constant-volume solves of 32 identical
+# reactors (T 1450 K, p 2.0265e6 Pa, phi 1; rtol 1e-8, atol 1e-18; ignition
+# root at +400 K), per mechanism
+TESTER = {
+    "h2": ("data/mechanisms/hydrogen/h2_v1b_mech.txt", "data/mechanisms/hydrogen/h2_v1a_therm.txt",
+           ["h2_dense"]),
+    "dme": ("data/mechanisms/dme/dme_24_mech.txt", "data/mechanisms/dme/dme_24_therm.txt",
+            ["dme_dense", "dme_sparse"]),
+    "nc7h16-skel": ("data/mechanisms/n-heptane_reduced/heptanesymp159_mec.txt",
+                    "data/mechanisms/n-heptane_reduced/heptanesymp_therm.txt",
+                    ["nc7h16-skel_sparse"]),
+}
+
+
+def copy_tester_outputs():
+    out = os.path.join(HERE, "tester")
+    os.makedirs(out, exist_ok=True)
+    ex = os.path.join(REF, "examples", "cfd_plugin_tester")
+    for name, (mrel, trel, runs) in TESTER.items():
+        files = [(os.path.join(ex, "inputs", f"{name}_fracs.log"), f"{name}_fracs.log")]
+        for run in runs:
+            files.append((os.path.join(ex, "outputs.ref", f"{run}_000.hist"), f"{run}_000.hist"))
+            files.append((os.path.join(ex, "outputs.ref", f"{run}.log"), f"{run}.log"))
+        files.append((os.path.join(REF, mrel), f"{name}.mech"))
+        files.append((os.path.join(REF, trel), f"{name}.therm"))
+        for src, dst in files:
+            shutil.copyfile(src, os.path.join(out, dst))
+            os.chmod(os.path.join(out, dst), 0o644)
+    print("tester outputs:", sorted(os.listdir(out)))
+
+
 def main():
+    copy_tester_outputs()
     if not os.path.exists(PROBE):
         raise SystemExit("build the reference probe first: make -C oracle ref")
     os.makedirs(MECH_DIR, exist_ok=True)
