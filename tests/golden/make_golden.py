@@ -118,10 +118,11 @@ def copy_tester_outputs():
     os.makedirs(out, exist_ok=True)
     ex = os.path.join(REF, "examples", "cfd_plugin_tester")
     for name, (mrel, trel, runs) in TESTER.items():
-        files = [(os.path.join(ex, "inputs", f"{name}_fracs.log"), f"{name}_fracs.log")]
+        # (".txt": logs are not shipped to the GPU machines)
+        files = [(os.path.join(ex, "inputs", f"{name}_fracs.log"), f"{name}_fracs.txt")]
         for run in runs:
             files.append((os.path.join(ex, "outputs.ref", f"{run}_000.hist"), f"{run}_000.hist"))
-            files.append((os.path.join(ex, "outputs.ref", f"{run}.log"), f"{run}.log"))
+            files.append((os.path.join(ex, "outputs.ref", f"{run}.log"), f"{run}_log.txt"))
         files.append((os.path.join(REF, mrel), f"{name}.mech"))
         files.append((os.path.join(REF, trel), f"{name}.therm"))
         for src, dst in files:

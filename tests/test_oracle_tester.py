@@ -52,7 +52,7 @@ def initial_mass_fractions(o, name, mech):
     comp = [dict(th[s][3] if s in th else thl[s.upper()][3]) for s in sp]
     idx = {s: j for j, s in enumerate(sp)}
     fuel, oxid = np.zeros(len(sp)), np.zeros(len(sp))
-    for line in open(os.path.join(TESTER, f"{name}_fracs.log")):
+    for line in open(os.path.join(TESTER, f"{name}_fracs.txt")):
         if line.startswith("#") or not line.strip():
             continue
         k, v = line.split(":")
