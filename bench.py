@@ -227,7 +227,7 @@ def main():
     roofline = {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
                 "traffic": traffic,
-                "kernel": "bdf_kernel<float>" if nsp <= 63 else "bdf_team_kernel",
+                "kernel": "bdf_kernel<float, false>" if nsp <= 63 else "bdf_team_kernel",
                 "kernel_ms": round(chosen["kern_ms"], 3),
                 "fp64_ops_per_launch": int(chosen["ops"]),
                 "resident_waves": chosen["waves"]}
@@ -247,7 +247,7 @@ def main():
     rate_ms = e0.elapsed_time(e1) / reps
     rbytes = workcount.rate_bytes_per_reactor(nsp, m.nstep) * n
     rate_gbps = rbytes / (rate_ms / 1e3) / 1e9
-    rate = {"kernel": "rates_kernel", "bound": "hbm", "achieved": round(rate_gbps, 1),
+    rate = {"kernel": "rates_kernel<false>", "bound": "hbm", "achieved": round(rate_gbps, 1),
             "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(rate_gbps / HBM_PEAK_GBPS, 4),
             "bytes_per_launch": int(rbytes), "kernel_ms": round(rate_ms, 4),
             "fp64_ops_per_launch": int(W.rhs_ops() * n)}
