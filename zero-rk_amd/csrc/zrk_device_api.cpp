@@ -402,6 +402,8 @@ BdfOpts to_bdf_opts(const zrk_bdf_options& o) {
 }
 
 MechHandle::~MechHandle() {
+  delete plan;
+  if (sp_blob) (void)hipFree(sp_blob);
   if (ws_jac) (void)hipFree(ws_jac);
   if (ws_counter) (void)hipFree(ws_counter);
   if (dev_blob) (void)hipFree(dev_blob);
@@ -414,6 +416,99 @@ int ensure_device(MechHandle* h) {
   if (rc) std::fprintf(stderr, "zerork-hip: device mechanism upload failed: %s\n", msg.c_str());
   return rc;
 }
+
+int ensure_sparse(MechHandle* h) {
+  if (h->sp_blob) return 0;
+  if (int rc = ensure_device(h)) return rc;
+  std::string err;
+  SparsePlan* P = new SparsePlan();
+  if (!build_sparse_plan(h->host, P, &err)) {
+    std::fprintf(stderr, "zerork-hip: sparse plan failed: %s\n", err.c_str());
+    delete P;
+    return -5;
+  }
+  // one device allocation: int arrays, then the term list
+  std::vector<const std::vector<int>*> parts = {&P->perm, &P->rp, &P->ci, &P->dpos, &P->lf_ptr,
+                                                &P->lf_rows, &P->lb_ptr, &P->lb_rows,
+                                                &P->jt_ptr, &P->pos_row, &P->pos_col,
+                                                &P->cp_ptr, &P->cp_pos, &P->trow_pos,
+                                                &P->tcol_pos};
+  std::vector<size_t> off;
+  size_t n = 0;
+  for (auto* v : parts) {
+    off.push_back(n);
+    n += (v->size() + 63) & ~(size_t)63;
+  }
+  const size_t jt_off = n;
+  n += P->jt.size();
+  std::vector<int> blob(n, 0);
+  for (size_t k = 0; k < parts.size(); ++k)
+    std::copy(parts[k]->begin(), parts[k]->end(), blob.begin() + off[k]);
+  std::memcpy(blob.data() + jt_off, P->jt.data(), sizeof(uint32_t) * P->jt.size());
+  void* d = nullptr;
+  hipError_t e = hipMalloc(&d, sizeof(int) * n);
+  if (e != hipSuccess) {
+    delete P;
+    return (int)e;
+  }
+  e = hipMemcpy(d, blob.data(), sizeof(int) * n, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(d);
+    delete P;
+    return (int)e;
+  }
+  const int* b = static_cast<const int*>(d);
+  DevSparse& S = h->sp_dev;
+  S.N = P->N;
+  S.nnz = P->nnz;
+  S.nlev_f = P->nlev_f;
+  S.nlev_b = P->nlev_b;
+  S.max_row = P->max_row;
+  S.perm = b + off[0];
+  S.rp = b + off[1];
+  S.ci = b + off[2];
+  S.dpos = b + off[3];
+  S.lf_ptr = b + off[4];
+  S.lf_rows = b + off[5];
+  S.lb_ptr = b + off[6];
+  S.lb_rows = b + off[7];
+  S.jt_ptr = b + off[8];
+  S.pos_row = b + off[9];
+  S.pos_col = b + off[10];
+  S.cp_ptr = b + off[11];
+  S.cp_pos = b + off[12];
+  S.trow_pos = b + off[13];
+  S.tcol_pos = b + off[14];
+  S.jt = reinterpret_cast<const uint32_t*>(b + jt_off);
+  h->sp_blob = d;
+  h->plan = P;
+  h->sp_resident = sparse_max_resident(h->dev);
+  return h->sp_resident > 0 ? 0 : -2;
+}
+
+namespace {
+
+// the sparse path: one reactor per workgroup, FP32 sparse LU (see bdf_sparse.hip)
+int integrate_sparse(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
+                     const double* dts, double* T, double* P, double* Y, int stride,
+                     int* nsteps, int* stats, const int* order, hipStream_t st,
+                     const double* dpdt, const double* esrc, const double* ysrc) {
+  if (!o.fp32_preconditioner) return -3;
+  // source terms, the ignition root and PLOG / non-integer / SRI records are
+  // not on this path
+  if (dpdt || esrc || ysrc || o.delta_temperature_ignition > 0.0 || h->dev.n_ext > 0) return -4;
+  if (n == 0) return 0;
+  if (int rc = ensure_sparse(h)) return rc;
+  const int nb = std::max(1, std::min(h->sp_resident, n));
+  const size_t per = sparse_slot_bytes(h->dev, h->sp_dev) / sizeof(double);
+  if (int rc = ensure_workspace(h, (size_t)nb * per)) return rc;
+  BdfOpts bo = to_bdf_opts(o);
+  const hipError_t e = launch_bdf_sparse(h->dev, h->sp_dev, bo, n, dt, dts, T, P, Y, stride,
+                                         nsteps, stats, order, h->ws_counter, h->ws_jac, nb, st);
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+}  // namespace
 
 int team_width_for(MechHandle* h, bool f32) {
   int W = h->team_pref[f32] ? h->team_pref[f32] : bdf_team_width(h->host.nsp);
@@ -428,7 +523,12 @@ int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
                      const double* ysrc, double* ign_time) {
   if (n < 0 || !T || !P || !Y || stride < h->host.nsp || (!dts && !(dt > 0.0))) return -1;
   const int W0 = bdf_team_width(h->host.nsp);
-  if (W0 == 0) return -2;                               // mechanism too large
+  if (W0 == 0 || h->force_sparse) {  // above 191 species: the sparse path
+    if (n > 0)
+      if (int rc = ensure_device(h)) return rc;
+    return integrate_sparse(h, o, n, dt, dts, T, P, Y, stride, nsteps, stats, order, st, dpdt,
+                            esrc, ysrc);
+  }
   if (W0 > 1 && !o.fp32_preconditioner) return -3;     // nsp > 63: FP32 preconditioner only
   if (n == 0) return 0;
   if (int rc = ensure_device(h)) return rc;
@@ -649,6 +749,22 @@ int zrk_autotune_integrator(zrk_mech m, const zrk_bdf_options* o, int n, double 
   return rc;
 }
 
+int zrk_integrator_set_sparse(zrk_mech m, int on) {
+  if (!m) return -1;
+  reinterpret_cast<MechHandle*>(m)->force_sparse = on != 0;
+  return 0;
+}
+
+int zrk_sparse_plan_info(zrk_mech m, int* nnz, int* levels, long long* factor_flops) {
+  if (!m) return -1;
+  auto* h = reinterpret_cast<MechHandle*>(m);
+  if (int rc = zrk::ensure_sparse(h)) return rc;
+  if (nnz) *nnz = h->plan->nnz;
+  if (levels) *levels = h->plan->nlev_f;
+  if (factor_flops) *factor_flops = h->plan->factor_flops;
+  return 0;
+}
+
 int zrk_integrator_set_team_width(zrk_mech m, int fp32_preconditioner, int W) {
   if (!m) return -1;
   auto* h = reinterpret_cast<MechHandle*>(m);
@@ -663,6 +779,10 @@ int zrk_integrator_resident_waves(zrk_mech m, const zrk_bdf_options* o) {
   auto* h = reinterpret_cast<MechHandle*>(m);
   if (int rc = zrk::ensure_device(h)) return -rc;
   const int f32 = (o && o->fp32_preconditioner) ? 1 : 0;
+  if (zrk::bdf_team_width(h->host.nsp) == 0 || h->force_sparse) {
+    if (!f32 || zrk::ensure_sparse(h)) return 0;
+    return h->sp_resident * zrk::kSparseWaves;
+  }
   const int W = zrk::team_width_for(h, f32);
   return W ? h->resident[f32][W] * W : 0;
 }

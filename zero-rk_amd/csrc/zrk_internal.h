@@ -8,6 +8,7 @@
 #include "../../include/zrk_device.h"
 #include "ck_mech.h"
 #include "dev_mech.h"
+#include "sparse_plan.h"
 
 namespace zrk {
 
@@ -26,6 +27,13 @@ struct MechHandle {
   double* ws_jac = nullptr;            // saved-Jacobian slots, one per resident team
   unsigned int* ws_counter = nullptr;
   size_t ws_doubles = 0;
+  // sparse path (mechanisms above 191 species, or forced): plan, its device
+  // copy and the resident workgroups
+  SparsePlan* plan = nullptr;
+  DevSparse sp_dev{};
+  void* sp_blob = nullptr;
+  int sp_resident = 0;
+  int force_sparse = 0;
   ~MechHandle();
 };
 
@@ -34,6 +42,8 @@ int build_device_mech(MechHandle* h, std::string* err);
 int team_width_for(MechHandle* h, bool f32);
 int ensure_device(MechHandle* h);
 int ensure_workspace(MechHandle* h, size_t doubles);
+// builds and uploads the sparse plan once (0 on success)
+int ensure_sparse(MechHandle* h);
 BdfOpts to_bdf_opts(const zrk_bdf_options& o);
 // dpdt[n], esrc[n], ysrc[n*stride]: optional device arrays of source terms;
 // ign_time[n]: optional ignition-time output (with o.delta_temperature_ignition > 0)

@@ -41,7 +41,8 @@ DEVICE_SYMBOLS = [
     "zrk_rates_device",
     "zrk_jacobian_device", "zrk_integrate_device", "zrk_integrate_device_aux",
     "zrk_integrator_resident_waves",
-    "zrk_autotune_integrator", "zrk_integrator_set_team_width",
+    "zrk_autotune_integrator", "zrk_integrator_set_team_width", "zrk_integrator_set_sparse",
+    "zrk_sparse_plan_info",
 ]
 
 
@@ -124,6 +125,11 @@ def _declare(L):
     L.zrk_autotune_integrator.restype = I
     L.zrk_integrator_set_team_width.argtypes = [P, I, I]
     L.zrk_integrator_set_team_width.restype = I
+    L.zrk_integrator_set_sparse.argtypes = [P, I]
+    L.zrk_integrator_set_sparse.restype = I
+    L.zrk_sparse_plan_info.argtypes = [P, C.POINTER(I), C.POINTER(I),
+                                       C.POINTER(C.c_longlong)]
+    L.zrk_sparse_plan_info.restype = I
     L.zrk_integrator_resident_waves.restype = I
 
 
@@ -247,6 +253,21 @@ class Mech:
         if rc != 0:
             raise RuntimeError(f"zrk_autotune_integrator failed ({rc})")
         return choice.value, {w: ms[w] for w in (1, 2, 3) if ms[w] >= 0}
+
+    def set_sparse(self, on=True):
+        """Route later integrations through the sparse path (any size)."""
+        if lib().zrk_integrator_set_sparse(self.h, int(bool(on))) != 0:
+            raise RuntimeError("zrk_integrator_set_sparse failed")
+
+    def sparse_plan_info(self):
+        """(LU entries, forward levels, multiply-adds per factorisation) of the
+        sparse plan (builds it; needs the device)."""
+        nnz, lev, fl = C.c_int(0), C.c_int(0), C.c_longlong(0)
+        rc = lib().zrk_sparse_plan_info(self.h, C.byref(nnz), C.byref(lev),
+                                        C.byref(fl))
+        if rc != 0:
+            raise RuntimeError(f"zrk_sparse_plan_info failed ({rc})")
+        return nnz.value, lev.value, fl.value
 
     def set_team_width(self, fp32, W):
         rc = lib().zrk_integrator_set_team_width(self.h, int(fp32), int(W))

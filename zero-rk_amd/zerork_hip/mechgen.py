@@ -188,14 +188,14 @@ def _species_pool(elements):
     return names
 
 
-def _extra_species(rng, elements, n, taken):
+def _extra_species(rng, elements, n, taken, cmax=12):
     """Formula species beyond the known table (CxHyOz(Nw)) with additive thermo."""
     out = []
     hasC, hasN = "C" in elements, "N" in elements
     tries = 0
     while len(out) < n and tries < 200000:
         tries += 1
-        c = int(rng.integers(1, 13)) if hasC else 0
+        c = int(rng.integers(1, cmax + 1)) if hasC else 0
         h = int(rng.integers(0, 2 * c + 3)) if hasC else int(rng.integers(1, 5))
         o = int(rng.integers(0, 4))
         w = int(rng.integers(0, 2)) if hasN and rng.random() < 0.15 else 0
@@ -224,33 +224,9 @@ def _thermo_for(rng, name, comp):
     return _nasa_coeffs(rng, cp0, hf, s, atom)
 
 
-def generate(n_species: int, n_reactions: int, seed: int, elements=("H", "O", "N"),
-             frac_third_body: float = 0.25, frac_falloff: float = 0.15,
-             inert=("N2", "AR"), majors=None, max_participants: int = 2,
-             base=None) -> Mechanism:
-    """Seeded synthetic element-balanced CHEMKIN mechanism."""
-    rng = np.random.default_rng(seed)
-    elements = [e.upper() for e in elements]
-    pool = _species_pool(elements)
-    if base is not None:
-        chosen = [s for s in base]
-    else:
-        chosen = []
-    for s in pool:
-        if len(chosen) >= n_species:
-            break
-        if s not in chosen:
-            chosen.append(s)
-    if len(chosen) < n_species:
-        chosen += _extra_species(rng, elements, n_species - len(chosen), set(chosen))
-    species = chosen[:n_species]
-    comp = [parse_formula(s) for s in species]
-    # element order as in the ELEMENTS list
-    comp = [{e: c[e] for e in elements if e in c} for c in comp]
-    thermo = [_thermo_for(rng, s, c) for s, c in zip(species, comp)]
-
-    reactive = [i for i, s in enumerate(species) if s not in inert]
-    evec = np.array([[c.get(e, 0) for e in elements] for c in comp], dtype=np.int64)
+def _enumerate_pairs(rng, evec, reactive, n_el, n_tb, n_fo, major_idx, max_participants):
+    """Element-balanced reaction sides by enumeration of every species set of up
+    to max_participants participants (small mechanisms)."""
     groups: dict = {}
     for size in range(1, max_participants + 1):
         for ms in itertools.combinations_with_replacement(reactive, size):
@@ -268,16 +244,12 @@ def generate(n_species: int, n_reactions: int, seed: int, elements=("H", "O", "N
                 bimol.append((a, b))
             elif (len(a) == 1) != (len(b) == 1):
                 assoc.append((a, b) if len(a) > len(b) else (b, a))
-    n_fo = int(round(frac_falloff * n_reactions))
-    n_tb = int(round(frac_third_body * n_reactions))
-    n_el = n_reactions - n_fo - n_tb
     if len(bimol) < n_el or len(assoc) < n_fo + n_tb:
         raise ValueError(f"species set too small: {len(bimol)} bimolecular, {len(assoc)} "
                          f"association candidates for {n_el} + {n_tb + n_fo} reactions")
     # grow a connected network: a candidate is accepted only when one of its
     # sides consists of species already reachable from the initial majors, so
     # every reaction can fire from a mixture the reactors actually contain
-    major_idx = [species.index(x) for x in (majors or species[:4]) if x in species]
     reach = set(major_idx)
     want = {"elem": n_el, "tb": n_tb, "falloff": n_fo}
     pools = {"elem": bimol, "tb": assoc, "falloff": assoc}
@@ -308,6 +280,126 @@ def generate(n_species: int, n_reactions: int, seed: int, elements=("H", "O", "N
         kinds.append(kind)
         pairs.append((a, b))
     want.clear()
+    return pairs, kinds
+
+
+def _sample_pairs(rng, evec, reactive, n_el, n_tb, n_fo, major_idx):
+    """Element-balanced reaction sides for large species sets (thousands of
+    species), sampled instead of enumerated: one side is drawn from species
+    already reachable from the initial majors (so every reaction can fire from
+    a mixture the reactors contain), the other from the species or species
+    pairs with the same element totals; no reaction appears twice."""
+    R = np.asarray(reactive, dtype=np.int64)
+    base = int(evec.max()) * 2 + 1
+    key_all = evec @ (base ** np.arange(evec.shape[1], dtype=np.int64))
+    ii, jj = np.triu_indices(len(R))
+    pa, pb = R[ii], R[jj]
+    k2 = key_all[pa] + key_all[pb]
+    order = np.argsort(k2, kind="stable")
+    uniq, start, count = np.unique(k2[order], return_index=True, return_counts=True)
+    single: dict = {}
+    for s in R:
+        single.setdefault(int(key_all[s]), []).append(int(s))
+
+    def pair_group(k):
+        pos = int(np.searchsorted(uniq, k))
+        if pos < len(uniq) and uniq[pos] == k:
+            return order[start[pos]:start[pos] + count[pos]]
+        return None
+
+    rset = set(int(x) for x in R)
+    reach = [m for m in major_idx if m in rset]
+    in_reach = set(reach)
+    seen = set()
+    kinds, pairs = [], []
+    schedule = list(rng.permutation(["elem"] * n_el + ["tb"] * n_tb + ["falloff"] * n_fo))
+    for kind in schedule:
+        cand = None
+        for _ in range(2000):
+            if kind == "elem":
+                x, y = sorted(int(v) for v in rng.choice(reach, 2))
+                g = pair_group(int(key_all[x] + key_all[y]))
+                if g is None:
+                    continue
+                q = int(g[rng.integers(len(g))])
+                c, d = int(pa[q]), int(pb[q])
+                if {c, d} & {x, y}:
+                    continue
+                cand = ((x, y), (c, d))
+            elif rng.random() < 0.5:  # a reachable species and a pair of its elements
+                s = int(rng.choice(reach))
+                g = pair_group(int(key_all[s]))
+                if g is None:
+                    continue
+                q = int(g[rng.integers(len(g))])
+                c, d = int(pa[q]), int(pb[q])
+                if s in (c, d):
+                    continue
+                cand = ((c, d), (s,))
+            else:  # a reachable pair and the species of its elements
+                x, y = sorted(int(v) for v in rng.choice(reach, 2))
+                lst = single.get(int(key_all[x] + key_all[y]))
+                if not lst:
+                    continue
+                s = lst[int(rng.integers(len(lst)))]
+                if s in (x, y):
+                    continue
+                cand = ((x, y), (s,))
+            sig = frozenset((cand[0], cand[1]))
+            if sig in seen:
+                cand = None
+                continue
+            seen.add(sig)
+            break
+        if cand is None:
+            raise ValueError(f"no element-balanced candidate left for a {kind} reaction")
+        for v in cand[0] + cand[1]:
+            if v not in in_reach:
+                in_reach.add(v)
+                reach.append(v)
+        kinds.append(kind)
+        pairs.append(cand)
+    return pairs, kinds
+
+
+def generate(n_species: int, n_reactions: int, seed: int, elements=("H", "O", "N"),
+             frac_third_body: float = 0.25, frac_falloff: float = 0.15,
+             inert=("N2", "AR"), majors=None, max_participants: int = 2,
+             base=None) -> Mechanism:
+    """Seeded synthetic element-balanced CHEMKIN mechanism."""
+    rng = np.random.default_rng(seed)
+    elements = [e.upper() for e in elements]
+    pool = _species_pool(elements)
+    if base is not None:
+        chosen = [s for s in base]
+    else:
+        chosen = []
+    for s in pool:
+        if len(chosen) >= n_species:
+            break
+        if s not in chosen:
+            chosen.append(s)
+    large = n_species > 400
+    if len(chosen) < n_species:
+        chosen += _extra_species(rng, elements, n_species - len(chosen), set(chosen),
+                                 cmax=24 if large else 12)
+    species = chosen[:n_species]
+    comp = [parse_formula(s) for s in species]
+    # element order as in the ELEMENTS list
+    comp = [{e: c[e] for e in elements if e in c} for c in comp]
+    thermo = [_thermo_for(rng, s, c) for s, c in zip(species, comp)]
+
+    reactive = [i for i, s in enumerate(species) if s not in inert]
+    evec = np.array([[c.get(e, 0) for e in elements] for c in comp], dtype=np.int64)
+    n_fo = int(round(frac_falloff * n_reactions))
+    n_tb = int(round(frac_third_body * n_reactions))
+    n_el = n_reactions - n_fo - n_tb
+    major_idx = [species.index(x) for x in (majors or species[:4]) if x in species]
+    if large:
+        pairs, kinds = _sample_pairs(rng, evec, reactive, n_el, n_tb, n_fo, major_idx)
+    else:
+        pairs, kinds = _enumerate_pairs(rng, evec, reactive, n_el, n_tb, n_fo, major_idx,
+                                       max_participants)
     order = rng.permutation(n_reactions)
     # standard enthalpy of formation at 298.15 K from the generated NASA-7 data
     h298 = np.array([_h298(lo) for lo, _ in thermo])
@@ -350,12 +442,13 @@ def generate(n_species: int, n_reactions: int, seed: int, elements=("H", "O", "N
 _C2_BASE = ["H2", "O2", "N2", "H2O", "CO", "CO2", "CH4", "AR"]
 
 
-WORKLOADS = ("10sp", "53sp", "160sp")
+WORKLOADS = ("10sp", "53sp", "160sp", "2000sp")
 
 
 def workload_mechanism(name: str, seed: int | None = None) -> Mechanism:
     """Seeded synthetic mechanism of a named workload: "10sp" (H/O/N, 27 reactions),
-    "53sp" (C/H/O/N/Ar, 325 reactions) or "160sp" (C/H/O/N/Ar, 900 reactions)."""
+    "53sp" (C/H/O/N/Ar, 325 reactions), "160sp" (C/H/O/N/Ar, 900 reactions) or
+    "2000sp" (C/H/O/N/Ar, 8,000 reactions, sampled element-balanced sides)."""
     if name == "10sp":
         base = ["H2", "O2", "H2O", "OH", "H", "O", "HO2", "H2O2", "O3", "N2"]
         return generate(10, 27, seed if seed is not None else 1001, ("H", "O", "N"),
@@ -375,6 +468,10 @@ def workload_mechanism(name: str, seed: int | None = None) -> Mechanism:
         return generate(160, 900, seed if seed is not None else 1089,
                         ("C", "H", "O", "N", "AR"), 0.30, 0.10, inert=("N2", "AR"),
                         majors=_C2_BASE + ["C2H6", "CH3OH"], base=_C2_BASE)
+    if name == "2000sp":
+        return generate(2000, 8000, seed if seed is not None else 2001,
+                        ("C", "H", "O", "N", "AR"), 0.25, 0.10, inert=("N2", "AR"),
+                        majors=_C2_BASE + ["C2H6", "CH3OH", "C2H4", "C3H8"], base=_C2_BASE)
     raise ValueError(f"unknown workload {name}")
 
 
@@ -405,6 +502,11 @@ def workload_states(name: str, mech: Mechanism, n: int, seed: int) -> States:
         P = np.exp(rng.uniform(math.log(1.0), math.log(50.0), n)) * 101325.0
         maj = [idx[s] for s in mech.majors]
         Y[:, maj] = rng.dirichlet(0.5 * np.ones(len(maj)), n)
+    elif name == "2000sp":
+        T = rng.uniform(650.0, 1100.0, n)
+        P = np.exp(rng.uniform(math.log(10.0), math.log(60.0), n)) * 101325.0
+        maj = [idx[s] for s in mech.majors]
+        Y[:, maj] = rng.dirichlet(0.3 * np.ones(len(maj)), n)
     elif name == "160sp":
         T = rng.uniform(800.0, 1600.0, n)
         P = np.exp(rng.uniform(math.log(1.0), math.log(40.0), n)) * 101325.0
@@ -428,5 +530,5 @@ def random_dense_states(mech: Mechanism, n: int, seed: int, Tlo=800.0, Thi=2200.
     return States(T, P, Y, seed)
 
 
-WORKLOAD_DT = {"10sp": 1.0e-4, "53sp": 1.0e-4, "160sp": 1.0e-5}
-WORKLOAD_NR = {"10sp": 1024, "53sp": 65536, "160sp": 16384}
+WORKLOAD_DT = {"10sp": 1.0e-4, "53sp": 1.0e-4, "160sp": 1.0e-5, "2000sp": 1.0e-4}
+WORKLOAD_NR = {"10sp": 1024, "53sp": 65536, "160sp": 16384, "2000sp": 4096}
