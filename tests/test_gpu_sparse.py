@@ -12,8 +12,7 @@ through zrk_integrate_device.
   count, sum(Y), the end pressure at constant volume, and accuracy against
   the LSODA referee on a subset (no worse than the oracle's own distance
   from it, plus the allowance the FP32 dense path has).
-* 2,000 species / 8,000 reactions (the large workload; skipped until the
-  factorisation is fast enough for the suite): every reactor
+* 2,000 species / 8,000 reactions (the large workload): every reactor
   succeeds, sum(Y) = 1, the adiabatic constant-pressure invariant (the
   mixture's mass enthalpy keeps its initial value to the integration
   tolerance), and identical reactors give identical results.  The oracle
@@ -87,10 +86,6 @@ def _mass_enthalpy(o, T, Y):
     return np.sum(h * RU * T[:, None] / mw * Y, axis=1)
 
 
-@pytest.mark.skip(reason="too slow for the suite: one factorisation of this mechanism's Newton "
-                         "matrix (1,744,495 LU entries, 730M multiply-adds, 1,300 levels) takes "
-                         "about 3.9 s with the row-by-row kernel (bounded single-reactor run), "
-                         "so 16 reactors over 2 us did not finish in 180 s")
 def test_sparse_path_2000_species(torch_gpu, oracle_mod):
     from zerork_hip import capi, mechgen
     name = "synth_2000sp"
@@ -99,14 +94,14 @@ def test_sparse_path_2000_species(torch_gpu, oracle_mod):
     print(f"case sparse 2000sp: LU entries {nnz}, levels {lev}, multiply-adds {flops}",
           flush=True)
     o = oracle_mod.Oracle(*mech_paths(name))
-    # a short interval: the factorisation of this randomly connected
-    # mechanism's Newton matrix (about 40% dense after fill) dominates
-    n = 16
+    # a short interval: about 100 steps per reactor (a bounded single-reactor
+    # run takes 457 steps, 73 factorisations and 7.8 s over 1e-5 s)
+    n = 8
     st = mechgen.workload_states("2000sp", mechgen.workload_mechanism("2000sp"), n, seed=62)
     st.T[n // 2:] = st.T[:n // 2]
     st.P[n // 2:] = st.P[:n // 2]
     st.Y[n // 2:] = st.Y[:n // 2]
-    dt = 2.0e-6
+    dt = 1.0e-6
     opts = capi.default_options(rel_tol=RTOL, abs_tol=ATOL, constant_volume=0,
                                 fp32_preconditioner=1)
     t0 = time.perf_counter()

@@ -31,10 +31,12 @@ constexpr int kSpQmax = 5;
 using SpTeam = TeamBlock<kSpWaves, false>;
 
 // doubles of one reactor's slot: 16 vectors of N, C, G, h, cp of N, the rates
-// of progress (nstep + 1), the Jacobian (nnz) and the FP32 LU (nnz floats)
-__host__ __device__ inline size_t sparse_slot_doubles(int nsp, int nstep, int nnz) {
+// of progress (nstep + 1), the Jacobian (nnz), the FP32 LU (nnz floats) and
+// the FP32 dense trailing block (nd x nd floats)
+__host__ __device__ inline size_t sparse_slot_doubles(int nsp, int nstep, int nnz, int nd) {
   const size_t N = (size_t)nsp + 1;
-  const size_t d = 20 * N + (size_t)nstep + 1 + (size_t)nnz + ((size_t)nnz + 1) / 2;
+  const size_t d = 20 * N + (size_t)nstep + 1 + (size_t)nnz + ((size_t)nnz + 1) / 2 +
+                   ((size_t)nd * nd + 1) / 2 + 1;
   return (d + 31) & ~(size_t)31;
 }
 
@@ -54,6 +56,7 @@ struct SpCv {
   double *zn[kSpQmax + 1], *ewt, *acor, *tempv, *ftemp, *y, *delta, *t1, *t2, *t3, *x;
   double *C, *G, *H, *CPM, *rop, *J;
   float* lu;
+  float* D;  // dense trailing block, row-major nd x nd
   // reactor
   double P, v0, inv_rho_last, cp_last;
   // CVODE (names as in cvode.c)
@@ -85,6 +88,8 @@ struct SpCv {
     rop = p; p += m.nstep + 1;
     J = p; p += S.nnz;
     lu = reinterpret_cast<float*>(p);
+    p += ((size_t)S.nnz + 1) / 2;
+    D = reinterpret_cast<float*>(p);
   }
 
   __device__ __forceinline__ double wrms(const double* v) {
@@ -221,26 +226,38 @@ struct SpCv {
     team.sync();
   }
 
-  // in-place LU of the FP32 matrix in lu; false on a zero pivot
+  // row i of the LU by one wavefront (up-looking): scatter into w, eliminate
+  // the lower-part columns below kend in ascending order
+  __device__ __forceinline__ void eliminate_row(int i, int kend) {
+    const int b = S.rp[i], e = S.rp[i + 1];
+    for (int p = b + lane; p < e; p += kWave) w[S.ci[p]] = lu[p];
+    wave_sync();
+    for (int p = b; p < e; ++p) {
+      const int k = S.ci[p];
+      if (k >= kend) break;
+      const int dk = S.dpos[k], ek = S.rp[k + 1];
+      const float lk = w[k] / lu[dk];
+      wave_sync();
+      if (lane == 0) w[k] = lk;
+      for (int q2 = dk + 1 + lane; q2 < ek; q2 += kWave) w[S.ci[q2]] -= lk * lu[q2];
+      wave_sync();
+    }
+  }
+
+  // in-place LU of the FP32 matrix in lu; false on a zero pivot.  Leading
+  // rows (< split) row by row over their level sets; trailing rows against
+  // the leading columns (independent of each other) into the dense block;
+  // the block by a right-looking dense LU with the whole workgroup; the
+  // block back into the CSR positions (its other entries stay exactly zero)
   __device__ bool factor() {
+    const int sp = S.split, nd = N - sp;
     bool ok = true;
-    for (int lev = 0; lev < S.nlev_f; ++lev) {
-      for (int idx = S.lf_ptr[lev] + wave; idx < S.lf_ptr[lev + 1]; idx += kSpWaves) {
-        const int i = S.lf_rows[idx];
-        const int b = S.rp[i], e = S.rp[i + 1], d = S.dpos[i];
-        for (int p = b + lane; p < e; p += kWave) w[S.ci[p]] = lu[p];
-        wave_sync();
-        for (int p = b; p < d; ++p) {
-          const int k = S.ci[p];
-          const int dk = S.dpos[k], ek = S.rp[k + 1];
-          const float lk = w[k] / lu[dk];
-          wave_sync();
-          if (lane == 0) w[k] = lk;
-          for (int q2 = dk + 1 + lane; q2 < ek; q2 += kWave) w[S.ci[q2]] -= lk * lu[q2];
-          wave_sync();
-        }
+    for (int lev = 0; lev < S.nlev_l; ++lev) {
+      for (int idx = S.ll_ptr[lev] + wave; idx < S.ll_ptr[lev + 1]; idx += kSpWaves) {
+        const int i = S.ll_rows[idx];
+        eliminate_row(i, i);
         if (w[i] == 0.0f) ok = false;
-        for (int p = b + lane; p < e; p += kWave) {
+        for (int p = S.rp[i] + lane; p < S.rp[i + 1]; p += kWave) {
           const int c = S.ci[p];
           lu[p] = w[c];
           w[c] = 0.0f;
@@ -249,17 +266,56 @@ struct SpCv {
       }
       team.sync();
     }
+    for (int i = sp + wave; i < N; i += kSpWaves) {
+      eliminate_row(i, sp);
+      for (int p = S.rp[i] + lane; p < S.rp[i + 1]; p += kWave) {
+        const int c = S.ci[p];
+        if (c < sp) lu[p] = w[c];
+      }
+      float* Di = D + (size_t)(i - sp) * nd;
+      for (int c = sp + lane; c < N; c += kWave) Di[c - sp] = w[c];
+      wave_sync();
+      for (int p = S.rp[i] + lane; p < S.rp[i + 1]; p += kWave) w[S.ci[p]] = 0.0f;
+      wave_sync();
+    }
+    team.sync();
+    for (int k = 0; k < nd; ++k) {
+      const float* Dk = D + (size_t)k * nd;
+      const float piv = Dk[k];
+      if (piv == 0.0f) ok = false;
+      for (int i = k + 1 + wave; i < nd; i += kSpWaves) {
+        float* Di = D + (size_t)i * nd;
+        const float a = Di[k];
+        if (a == 0.0f) continue;
+        const float li = a / piv;
+        for (int j = k + 1 + lane; j < nd; j += kWave) Di[j] -= li * Dk[j];
+        if (lane == 0) Di[k] = li;
+      }
+      team.sync();
+    }
+    for (int i = sp + wave; i < N; i += kSpWaves) {
+      const float* Di = D + (size_t)(i - sp) * nd;
+      for (int p = S.rp[i] + lane; p < S.rp[i + 1]; p += kWave) {
+        const int c = S.ci[p];
+        if (c >= sp) lu[p] = Di[c - sp];
+      }
+    }
+    team.sync();
     return team.max(ok ? 0.0 : 1.0) == 0.0;
   }
 
-  // b <- (LU)^-1 b in the plan's order
+  // b <- (LU)^-1 b in the plan's order.  Leading rows over their level sets
+  // (one wavefront per row); the trailing block's rows against the leading
+  // columns; the block densely in column panels of 64 (the panel's triangle
+  // by one wavefront with shuffles, the rows beyond it by all wavefronts)
   __device__ void solve(double* b) {
+    const int sp = S.split, nd = N - sp;
     team.sync();
     for (int i = tid; i < N; i += kSpThreads) x[i] = b[S.perm[i]];
     team.sync();
-    for (int lev = 0; lev < S.nlev_f; ++lev) {
-      for (int idx = S.lf_ptr[lev] + wave; idx < S.lf_ptr[lev + 1]; idx += kSpWaves) {
-        const int i = S.lf_rows[idx];
+    for (int lev = 0; lev < S.nlev_l; ++lev) {
+      for (int idx = S.ll_ptr[lev] + wave; idx < S.ll_ptr[lev + 1]; idx += kSpWaves) {
+        const int i = S.ll_rows[idx];
         double s = 0.0;
         for (int p = S.rp[i] + lane; p < S.dpos[i]; p += kWave) s += (double)lu[p] * x[S.ci[p]];
         s = wave_sum(s);
@@ -267,9 +323,57 @@ struct SpCv {
       }
       team.sync();
     }
-    for (int lev = 0; lev < S.nlev_b; ++lev) {
-      for (int idx = S.lb_ptr[lev] + wave; idx < S.lb_ptr[lev + 1]; idx += kSpWaves) {
-        const int i = S.lb_rows[idx];
+    for (int i = sp + wave; i < N; i += kSpWaves) {
+      double s = 0.0;
+      for (int p = S.rp[i] + lane; p < S.rp[i + 1]; p += kWave)
+        if (S.ci[p] < sp) s += (double)lu[p] * x[S.ci[p]];
+      s = wave_sum(s);
+      if (lane == 0) x[i] -= s;
+    }
+    team.sync();
+    double* xb = x + sp;
+    for (int kb = 0; kb < nd; kb += kWave) {  // forward, unit lower
+      const int kn = min(kWave, nd - kb);
+      if (wave == 0) {
+        double xv = lane < kn ? xb[kb + lane] : 0.0;
+        for (int c = 0; c < kn; ++c) {
+          const double xc = __shfl(xv, c);
+          if (lane > c && lane < kn) xv -= (double)D[(size_t)(kb + lane) * nd + kb + c] * xc;
+        }
+        if (lane < kn) xb[kb + lane] = xv;
+      }
+      team.sync();
+      const double xl = lane < kn ? xb[kb + lane] : 0.0;
+      for (int j = kb + kn + wave; j < nd; j += kSpWaves) {
+        double s = lane < kn ? (double)D[(size_t)j * nd + kb + lane] * xl : 0.0;
+        s = wave_sum(s);
+        if (lane == 0) xb[j] -= s;
+      }
+      team.sync();
+    }
+    for (int kb = ((nd - 1) / kWave) * kWave; nd > 0 && kb >= 0; kb -= kWave) {  // backward
+      const int kn = min(kWave, nd - kb);
+      if (wave == 0) {
+        double xv = lane < kn ? xb[kb + lane] : 0.0;
+        for (int c = kn - 1; c >= 0; --c) {
+          if (lane == c) xv /= (double)D[(size_t)(kb + c) * nd + kb + c];
+          const double xc = __shfl(xv, c);
+          if (lane < c) xv -= (double)D[(size_t)(kb + lane) * nd + kb + c] * xc;
+        }
+        if (lane < kn) xb[kb + lane] = xv;
+      }
+      team.sync();
+      const double xl = lane < kn ? xb[kb + lane] : 0.0;
+      for (int j = wave; j < kb; j += kSpWaves) {
+        double s = lane < kn ? (double)D[(size_t)j * nd + kb + lane] * xl : 0.0;
+        s = wave_sum(s);
+        if (lane == 0) xb[j] -= s;
+      }
+      team.sync();
+    }
+    for (int lev = 0; lev < S.nlev_lb; ++lev) {
+      for (int idx = S.lbl_ptr[lev] + wave; idx < S.lbl_ptr[lev + 1]; idx += kSpWaves) {
+        const int i = S.lbl_rows[idx];
         double s = 0.0;
         for (int p = S.dpos[i] + 1 + lane; p < S.rp[i + 1]; p += kWave)
           s += (double)lu[p] * x[S.ci[p]];
@@ -947,7 +1051,7 @@ bdf_sparse_kernel(DevMech m, DevSparse S, BdfOpts o, int n, double dt_all,
 }
 
 size_t sparse_slot_bytes(const DevMech& m, const DevSparse& S) {
-  return sizeof(double) * sparse_slot_doubles(m.nsp, m.nstep, S.nnz);
+  return sizeof(double) * sparse_slot_doubles(m.nsp, m.nstep, S.nnz, S.N - S.split);
 }
 
 size_t sparse_lds_bytes(const DevMech& m) {
@@ -983,7 +1087,7 @@ hipError_t launch_bdf_sparse(const DevMech& m, const DevSparse& S, const BdfOpts
   const int grid = nblocks < n ? nblocks : n;
   hipLaunchKernelGGL(bdf_sparse_kernel, dim3(grid), dim3(kSpThreads), lds, st, m, S, o, n, dt,
                      dts, T, P, Y, stride, nsteps, stats, order, counter, ws,
-                     sparse_slot_doubles(m.nsp, m.nstep, S.nnz));
+                     sparse_slot_doubles(m.nsp, m.nstep, S.nnz, S.N - S.split));
   return hipGetLastError();
 }
 

@@ -174,6 +174,55 @@ bool build_sparse_plan(const Mechanism& M, SparsePlan* P, std::string* err) {
   P->nlev_b = nl;
   bucket(nl, P->lb_ptr, P->lb_rows);
 
+  // the trailing block [split, N) x [split, N): the largest one (at least 64
+  // rows) whose share of LU entries is at least 90%
+  {
+    std::vector<long long> colcnt(N, 0);  // L entries (r > c) per column
+    for (int i = 0; i < N; ++i)
+      for (int p = P->rp[i]; p < P->dpos[i]; ++p) colcnt[P->ci[p]]++;
+    long long cnt = 0;
+    P->split = N;
+    for (int i = N - 1; i >= 0; --i) {
+      cnt += P->rp[i + 1] - P->dpos[i];  // U part of row i with the diagonal
+      cnt += colcnt[i];                  // L part of column i (rows > i, all in the block)
+      const long long nd = N - i;
+      if (nd >= 64 && cnt * 10 >= nd * nd * 9) P->split = i;
+    }
+  }
+  {
+    const int s = P->split;
+    std::vector<int> lv(s, 0);
+    int nl2 = 0;
+    for (int i = 0; i < s; ++i) {
+      int l = 0;
+      for (int p = P->rp[i]; p < P->dpos[i]; ++p) l = std::max(l, lv[P->ci[p]] + 1);
+      lv[i] = l;
+      nl2 = std::max(nl2, l + 1);
+    }
+    P->nlev_l = nl2;
+    P->ll_ptr.assign(nl2 + 1, 0);
+    for (int i = 0; i < s; ++i) P->ll_ptr[lv[i] + 1]++;
+    for (int l = 0; l < nl2; ++l) P->ll_ptr[l + 1] += P->ll_ptr[l];
+    std::vector<int> fill(P->ll_ptr.begin(), P->ll_ptr.end() - 1);
+    P->ll_rows.assign(std::max(s, 1), 0);
+    for (int i = 0; i < s; ++i) P->ll_rows[fill[lv[i]]++] = i;
+    // backward levels of the leading rows (the block's rows are solved first)
+    nl2 = 0;
+    for (int i = s - 1; i >= 0; --i) {
+      int l = 0;
+      for (int p = P->dpos[i] + 1; p < P->rp[i + 1]; ++p)
+        if (P->ci[p] < s) l = std::max(l, lv[P->ci[p]] + 1);
+      lv[i] = l;
+      nl2 = std::max(nl2, l + 1);
+    }
+    P->nlev_lb = nl2;
+    P->lbl_ptr.assign(nl2 + 1, 0);
+    for (int i = 0; i < s; ++i) P->lbl_ptr[lv[i] + 1]++;
+    for (int l = 0; l < nl2; ++l) P->lbl_ptr[l + 1] += P->lbl_ptr[l];
+    std::vector<int> fill2(P->lbl_ptr.begin(), P->lbl_ptr.end() - 1);
+    P->lbl_rows.assign(std::max(s, 1), 0);
+    for (int i = 0; i < s; ++i) P->lbl_rows[fill2[lv[i]]++] = i;
+  }
   // positions in old indices
   auto pos_of = [&](int orow, int ocol) {
     const int i = iperm[orow], c = iperm[ocol];

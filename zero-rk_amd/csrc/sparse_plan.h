@@ -4,7 +4,8 @@
 // column against the rows of the step's reactants and products), a fill-
 // reducing symmetric ordering with the temperature last, the filled LU
 // pattern, level sets for the row-by-row factorisation and the two
-// triangular solves, and the Jacobian term lists by LU position.
+// triangular solves, the trailing block that fill makes nearly dense (it is
+// factorised as a dense matrix), and the Jacobian term lists by LU position.
 //
 // Diagonal pivots: the reference's SuperLU options ask for them
 // (DiagPivotThresh = 0, applications/cfd_plugin/interfaces/superlu_manager/
@@ -29,6 +30,13 @@ struct DevSparse {
   int N, nnz;
   int nlev_f, nlev_b;     // level counts: factorisation / forward solve, backward solve
   int max_row;            // longest LU row
+  // rows and columns >= split form the trailing block, factorised densely
+  // (split = N: no dense block); the leading rows' own level sets
+  int split, nlev_l, nlev_lb;
+  const int* ll_ptr;      // [nlev_l + 1]
+  const int* ll_rows;     // [split]
+  const int* lbl_ptr;     // [nlev_lb + 1] backward levels of the leading rows
+  const int* lbl_rows;    // [split]
   const int* perm;        // [N] new -> old
   const int* rp;          // [N + 1]
   const int* ci;          // [nnz] new column
@@ -48,8 +56,8 @@ struct DevSparse {
 };
 
 struct SparsePlan {
-  int N = 0, nnz = 0, nlev_f = 0, nlev_b = 0, max_row = 0;
-  std::vector<int> perm, rp, ci, dpos, lf_ptr, lf_rows, lb_ptr, lb_rows, jt_ptr, pos_row,
+  int N = 0, nnz = 0, nlev_f = 0, nlev_b = 0, max_row = 0, split = 0, nlev_l = 0, nlev_lb = 0;
+  std::vector<int> ll_ptr, ll_rows, lbl_ptr, lbl_rows, perm, rp, ci, dpos, lf_ptr, lf_rows, lb_ptr, lb_rows, jt_ptr, pos_row,
       pos_col, cp_ptr, cp_pos, trow_pos, tcol_pos;
   std::vector<uint32_t> jt;
   long long factor_flops = 0;  // multiply-adds of one numeric factorisation

@@ -432,7 +432,8 @@ int ensure_sparse(MechHandle* h) {
                                                 &P->lf_rows, &P->lb_ptr, &P->lb_rows,
                                                 &P->jt_ptr, &P->pos_row, &P->pos_col,
                                                 &P->cp_ptr, &P->cp_pos, &P->trow_pos,
-                                                &P->tcol_pos};
+                                                &P->tcol_pos, &P->ll_ptr, &P->ll_rows,
+                                                &P->lbl_ptr, &P->lbl_rows};
   std::vector<size_t> off;
   size_t n = 0;
   for (auto* v : parts) {
@@ -479,6 +480,13 @@ int ensure_sparse(MechHandle* h) {
   S.cp_pos = b + off[12];
   S.trow_pos = b + off[13];
   S.tcol_pos = b + off[14];
+  S.ll_ptr = b + off[15];
+  S.ll_rows = b + off[16];
+  S.lbl_ptr = b + off[17];
+  S.lbl_rows = b + off[18];
+  S.nlev_lb = P->nlev_lb;
+  S.split = P->split;
+  S.nlev_l = P->nlev_l;
   S.jt = reinterpret_cast<const uint32_t*>(b + jt_off);
   h->sp_blob = d;
   h->plan = P;
@@ -499,7 +507,13 @@ int integrate_sparse(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
   if (dpdt || esrc || ysrc || o.delta_temperature_ignition > 0.0 || h->dev.n_ext > 0) return -4;
   if (n == 0) return 0;
   if (int rc = ensure_sparse(h)) return rc;
-  const int nb = std::max(1, std::min(h->sp_resident, n));
+  // the dense trailing blocks of the reactors in flight are kept within half
+  // of the 256 MB last-level cache: the block's right-looking factorisation
+  // re-reads it O(nd) times, so spilling them to HBM costs more than the
+  // concurrency gains
+  const double nd = (double)(h->sp_dev.N - h->sp_dev.split);
+  const int cap = nd > 0.0 ? std::max(1, (int)(128.0 * 1048576.0 / (4.0 * nd * nd))) : n;
+  const int nb = std::max(1, std::min(std::min(h->sp_resident, cap), n));
   const size_t per = sparse_slot_bytes(h->dev, h->sp_dev) / sizeof(double);
   if (int rc = ensure_workspace(h, (size_t)nb * per)) return rc;
   BdfOpts bo = to_bdf_opts(o);
