@@ -164,6 +164,7 @@ def main():
             restore()
             m.integrate(n, dt, T, P, Y, nsp, opts, steps, None, stream=sp)
         torch.cuda.synchronize()
+        log("warmup done")
         # the outer step (restore + integrate) captured once in a HIP graph
         graph = torch.cuda.CUDAGraph()
         cs = torch.cuda.Stream()
@@ -181,6 +182,7 @@ def main():
         e1.record(stream)
         torch.cuda.synchronize()
         total_ms = e0.elapsed_time(e1)
+        log(f"timed steps done: {total_ms / a.steps:.1f} ms/step")
         # the integrator kernel alone, eager, with HIP events on its stream
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
         restore()
