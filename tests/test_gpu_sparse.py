@@ -19,6 +19,8 @@ through zrk_integrate_device.
   needs minutes per reactor at this size, so it is compared here through
   these size-independent properties only.
 * Options the path does not take are refused with the documented codes.
+* The drop-in plugin takes the sparse path for the 2,000-species mechanism by
+  itself, with the device entry point's results bit for bit.
 """
 import time
 
@@ -133,3 +135,32 @@ def test_sparse_path_refuses_unsupported(torch_gpu):
                      (dict(fp32_preconditioner=1, delta_temperature_ignition=400.0), -4)):
         with pytest.raises(RuntimeError, match=rf"\({code}\)"):
             m.integrate(4, 1e-4, t(st.T), t(st.P), t(st.Y), 10, capi.default_options(**kw))
+
+
+def test_plugin_runs_large_mechanism_on_sparse_path(torch_gpu):
+    """The drop-in plugin (zerork_reactor_solve) takes the sparse path for a
+    2,000-species mechanism by itself and gives the device entry point's
+    results bit for bit."""
+    from zerork_hip import capi, mechgen
+    name = "synth_2000sp"
+    st = mechgen.workload_states("2000sp", mechgen.workload_mechanism("2000sp"), 4, seed=64)
+    n, nsp = st.Y.shape
+    dt = 1.0e-7
+    print("case plugin sparse 2000sp", flush=True)
+    p = capi.Plugin()
+    assert p.set_mechanism_files(*mech_paths(name)) == 0
+    for k, v in (("constant_volume", 0), ("rel_tol", RTOL), ("abs_tol", ATOL)):
+        assert (p.set_int(k, v) if isinstance(v, int) else p.set_double(k, v)) == 0, k
+    assert p.load_mechanism() == 0
+    T, P, Y = st.T.copy(), st.P.copy(), st.Y.copy()
+    cost = np.zeros(n)
+    assert p.set_aux("COST", cost) == 0
+    assert p.solve(0, 0.0, dt, T, P, Y) == 0
+    m = capi.Mech(*mech_paths(name))
+    opts = capi.default_options(rel_tol=RTOL, abs_tol=ATOL, constant_volume=0,
+                                fp32_preconditioner=1)
+    Td, Yd, sd, _ = _gpu_integrate(torch_gpu, m, st.T, st.P, st.Y, dt, opts)
+    assert np.all(sd > 0)
+    np.testing.assert_array_equal(T, Td)
+    np.testing.assert_array_equal(Y, Yd)
+    np.testing.assert_array_equal(cost, sd)
