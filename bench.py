@@ -78,14 +78,25 @@ def pmc_traffic(path):
     return None
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(mech, therm, st, dt, rtol, atol, seconds):
-    """Oracle CVODE restatement on the host cores over a bounded sample."""
+    """Oracle CVODE restatement on the host cores over a bounded sample: all
+    host threads (up to the box's 16-core share), then one thread."""
     import numpy as np
     import oracle
     o = oracle.Oracle(mech, therm)
     threads = max(1, min(16, os.cpu_count() or 1))
     opts = oracle.default_opts(rtol=rtol, atol=atol)
-    # probe cost on a small sample, then size the timed sample
+    # probe cost on a small sample, then size the timed samples
     n0 = min(len(st.T), 2 * threads)
     t0 = time.perf_counter()
     o.integrate(st.T[:n0], st.P[:n0], st.Y[:n0], dt, opts, nthreads=threads)
@@ -96,9 +107,18 @@ def cpu_baseline(mech, therm, st, dt, rtol, atol, seconds):
     t0 = time.perf_counter()
     _, _, stats = o.integrate(st.T[idx], st.P[idx], st.Y[idx], dt, opts, nthreads=threads)
     el = time.perf_counter() - t0
+    # one thread: a smaller evenly spaced sample (about a third of the time)
+    n1 = int(max(4, min(len(st.T), n * (seconds / 3.0) / (el * threads))))
+    idx1 = np.linspace(0, len(st.T) - 1, n1).astype(int)
+    t0 = time.perf_counter()
+    o.integrate(st.T[idx1], st.P[idx1], st.Y[idx1], dt, opts, nthreads=1)
+    el1 = time.perf_counter() - t0
     return {"value": n / el, "unit": "reactors/s", "cores": threads, "kind": "port",
             "sample": f"{n} evenly spaced reactors of the same {len(st.T):,}-reactor batch, "
                       f"{el:.1f} s on {threads} threads (OpenMP, one reactor per thread)",
+            "one_core": {"value": n1 / el1, "unit": "reactors/s",
+                         "sample": f"{n1} evenly spaced reactors, {el1:.1f} s on 1 thread"},
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
             "failed": int(np.sum(stats["nsteps"] < 0))}
 
 

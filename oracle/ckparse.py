@@ -51,13 +51,25 @@ UNITS_TACT = {
     "KJOULES/MOLE": 238.846 * CAL_PER_MOL_TACT,
     "JOULES/MOLE": 238.846 * CAL_PER_MOL_TACT / 1000.0,
 }
-# IUPAC 2007 atomic masses [kg/kmol], by element symbol (src/zerork/atomicMassDB.h)
+# IUPAC 2007 atomic masses [kg/kmol] of elements 1-92, by symbol
+# (src/zerork/atomicMassDB.h); looked up on the first two characters
 ATOMIC_MASS = {
-    "H": 1.00794, "HE": 4.002602, "LI": 6.941, "BE": 9.012182, "B": 10.811,
-    "C": 12.0107, "N": 14.0067, "O": 15.9994, "F": 18.9984032, "NE": 20.1797,
-    "NA": 22.98976928, "MG": 24.3050, "AL": 26.9815386, "SI": 28.0855,
-    "P": 30.973762, "S": 32.065, "CL": 35.453, "AR": 39.948, "K": 39.0983,
-    "CA": 40.078,
+    "H": 1.00794, "HE": 4.002602, "LI": 6.941, "BE": 9.012182, "B": 10.811, "C": 12.0107,
+    "N": 14.0067, "O": 15.9994, "F": 18.9984032, "NE": 20.1797, "NA": 22.98976928, "MG": 24.3050,
+    "AL": 26.9815386, "SI": 28.0855, "P": 30.973762, "S": 32.065, "CL": 35.453, "AR": 39.948,
+    "K": 39.0983, "CA": 40.078, "SC": 44.955912, "TI": 47.867, "V": 50.9415, "CR": 51.9961,
+    "MN": 54.938045, "FE": 55.845, "CO": 58.933195, "NI": 58.6934, "CU": 63.546, "ZN": 65.38,
+    "GA": 69.723, "GE": 72.64, "AS": 74.92160, "SE": 78.96, "BR": 79.904, "KR": 83.798,
+    "RB": 85.4678, "SR": 87.62, "Y": 88.90585, "ZR": 91.224, "NB": 92.90638, "MO": 95.96,
+    "TC": 98, "RU": 101.07, "RH": 102.90550, "PD": 106.42, "AG": 107.8682, "CD": 112.411,
+    "IN": 114.818, "SN": 118.710, "SB": 121.760, "TE": 127.60, "I": 126.90447, "XE": 131.293,
+    "CS": 132.9054519, "BA": 137.327, "LA": 138.90547, "CE": 140.116, "PR": 140.90765, "ND": 144.242,
+    "PM": 145, "SM": 150.36, "EU": 151.964, "GD": 157.25, "TB": 158.92535, "DY": 162.500,
+    "HO": 164.93032, "ER": 167.259, "TM": 168.93421, "YB": 173.054, "LU": 174.9668, "HF": 178.49,
+    "TA": 180.94788, "W": 183.84, "RE": 186.207, "OS": 190.23, "IR": 192.217, "PT": 195.084,
+    "AU": 196.966569, "HG": 200.59, "TL": 204.3833, "PB": 207.2, "BI": 208.98040, "PO": 209,
+    "AT": 210, "RN": 222, "FR": 223, "RA": 226, "AC": 227, "TH": 232.03806,
+    "PA": 231.03588, "U": 238.02891,
 }
 
 FALLOFF_LINDEMANN, FALLOFF_TROE3, FALLOFF_TROE4, FALLOFF_SRI = 0, 1, 2, 3
@@ -387,7 +399,7 @@ def build_tables(mech_path: str, therm_path: str | None, ru: float = NIST_RU) ->
         tmid, low, high, comp = th[s] if s in th else thl[s.upper()]
         w = 0.0
         for el, n in comp:
-            w += n * ATOMIC_MASS[el]
+            w += n * ATOMIC_MASS[el.upper()[:2]]
         mw[j] = w
         thermo[j, 0] = tmid
         thermo[j, 1:8] = low

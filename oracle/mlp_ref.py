@@ -43,33 +43,45 @@ def gelu_grad(z):
     return cdf + z * pdf
 
 
-def forward(x_bf16, Ws, bs):
+def _bf16_64(x):
+    """BF16 rounding of the kernels' GEMM operands, as FP64 values."""
+    return bf16(np.asarray(x).astype(np.float32)).astype(np.float64)
+
+
+def _exact(x):
+    """No rounding: the FP64 network, for derivative checks."""
+    return np.asarray(x, dtype=np.float64)
+
+
+def forward(x_bf16, Ws, bs, rnd=_bf16_64):
     """x: [B, K0] (values already BF16); Ws[l]: [K_l, N_l] FP32 masters (used as
-    BF16); returns (out, cache) with out FP64 [B, N_L]."""
+    BF16); returns (out, cache) with out FP64 [B, N_L].  rnd: the rounding of
+    GEMM operands (BF16 as the kernels do; _exact for the FP64 network)."""
     h = np.asarray(x_bf16, dtype=np.float64)
     zs, hs = [], [h]
     L = len(Ws)
     for i, (W, b) in enumerate(zip(Ws, bs)):
-        z = hs[-1] @ bf16(W).astype(np.float64) + np.asarray(b, np.float64)
+        z = hs[-1] @ rnd(W) + np.asarray(b, np.float64)
         zs.append(z)
         if i < L - 1:
-            hs.append(bf16(gelu(z).astype(np.float32)).astype(np.float64))
+            hs.append(rnd(gelu(z)))
     return zs[-1], (zs, hs)
 
 
-def backward(dout, Ws, cache):
+def backward(dout, Ws, cache, rnd=_bf16_64):
     """dout: dL/d(out) [B, N_L]; returns (dWs, dbs) in FP64, with the BF16
-    rounding of the backward GEMM operands the kernels apply."""
+    rounding of the backward GEMM operands the kernels apply (rnd as in
+    forward)."""
     zs, hs = cache
     L = len(Ws)
     dWs, dbs = [None] * L, [None] * L
     dz = np.asarray(dout, np.float64)
     for i in range(L - 1, -1, -1):
-        dzb = bf16(dz.astype(np.float32)).astype(np.float64)
+        dzb = rnd(dz)
         dWs[i] = hs[i].T @ dzb
         dbs[i] = dz.sum(axis=0)
         if i > 0:
-            dh = dzb @ bf16(Ws[i]).astype(np.float64).T
+            dh = dzb @ rnd(Ws[i]).T
             dz = dh * gelu_grad(zs[i - 1])
     return dWs, dbs
 

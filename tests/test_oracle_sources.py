@@ -16,9 +16,13 @@ reactor_constant_pressure_cpu.cpp:15-133, reactor_constant_volume_cpu.cpp:
 * the energy-to-temperature solve: at t = 0 the source-term RHS equals the
   plain RHS plus e_src / cp in the temperature row (the solve returns the
   state's own temperature to its 1e-6 K tolerance).
-The constant-volume form with dpdt (an effective density from Ru T/(v W) +
-dpdt t, reactor_constant_volume_cpu.cpp:94-98) has no such invariant and is
-pinned only by restatement (parity unpinned beyond that).
+* the constant-volume pressure rate (reactor_constant_volume_cpu.cpp:94-98):
+  at time t the reactor of fixed volume v0 evaluates its rates at the
+  density of the pressure Ru T/(v0 W) + dpdt t, so its RHS equals the plain
+  constant-volume RHS of a reactor whose volume gives that pressure, plus
+  dpdt v_t / cv in the temperature row (v_t the volume of that density); at
+  t = 0 this is the plain RHS of the reactor itself plus dpdt v0 / cv.  A
+  density taken at v0, a dropped dpdt t or a missing energy term breaks it.
 """
 import numpy as np
 import pytest
@@ -122,3 +126,32 @@ def test_energy_source_rhs_at_t0(oracle_mod, o10, cvol):
         cp_R, _, _ = o10.thermo(T[k])
         cx = np.sum((cp_R - (1.0 if cvol else 0.0)) * RU / o10.tables.mw * Y[k])
         np.testing.assert_allclose(f1[-1] - f0[-1], 3e6 / cx, rtol=1e-6)
+
+
+@pytest.mark.parametrize("t", [0.0, 1e-4, 3e-3])
+def test_constant_volume_pressure_rate_rhs(oracle_mod, o10, t):
+    T, P, Y = _states(8, 9)
+    oo = oracle_mod.default_opts(constant_volume=1)
+    mw = o10.tables.mw
+    rng = np.random.default_rng(10)
+    for k in range(8):
+        y0 = np.append(Y[k], T[k])
+        # the state at time t differs from the initial one (T and Y moved)
+        Yt = Y[k] * (1.0 + 0.2 * rng.standard_normal(len(mw)))
+        Yt = np.abs(Yt) / np.sum(np.abs(Yt))
+        Tt = T[k] * 1.07
+        y = np.append(Yt, Tt)
+        dpdt = 3.0e8 if k % 2 == 0 else -1.5e8
+        f1, fl = o10.rhs_src(P[k], y0, t, y, dpdt=dpdt, opts=oo)
+        assert fl == 0
+        v0 = RU * T[k] * np.sum(Y[k] / mw) / P[k]        # the reactor's fixed volume
+        Pt = RU * Tt * np.sum(Yt / mw) / v0 + dpdt * t    # its pressure at t
+        vt = RU * Tt * np.sum(Yt / mw) / Pt              # the volume of that density
+        f0, fl = o10.rhs(Pt, y, oo)                      # plain RHS at volume vt
+        assert fl == 0
+        cp_R, _, _ = o10.thermo(Tt)
+        cv = np.sum((cp_R - 1.0) * RU / mw * Yt)
+        np.testing.assert_allclose(f1[:-1], f0[:-1], rtol=1e-10,
+                                   atol=1e-12 * np.max(np.abs(f0[:-1])))
+        np.testing.assert_allclose(f1[-1], f0[-1] + dpdt * vt / cv, rtol=1e-10,
+                                   atol=1e-12 * abs(dpdt * vt / cv))

@@ -167,19 +167,78 @@ std::string trim(const std::string& s) {
   return s.substr(a, b - a + 1);
 }
 
-// Flat "key: value" YAML as written for the reference's options file
-// (applications/cfd_plugin/generateZeroRKCFDPluginIFP.py keys).
+// One key of the options file: the schema of ZeroRKCFDPluginIFP
+// (applications/cfd_plugin/generateZeroRKCFDPluginIFP.py), i.e. the YAML key,
+// the option it sets (zerork_reactor_manager.cpp:135-191), its type, default
+// and allowed values.  A key without a default is required.
+struct IfpKey {
+  const char* yaml;
+  const char* option;
+  char type;  // 'i' int, 'd' double, 's' string
+  bool has_default;
+  double def;
+  const char* sdef;
+  double lo, hi;        // bounds (inclusive); NaN: none
+  const char* allowed;  // discrete int values, "" for any
+};
+
+const double kNone = std::nan("");
+
+const IfpKey kIfp[] = {
+    {"mechanism_parsing_log", "mechanism_parsing_log_filename", 's', true, 0, "/dev/null", kNone, kNone, ""},
+    {"reactor_timing_log", "reactor_timing_log_filename", 's', true, 0, "/dev/null", kNone, kNone, ""},
+    {"n_reactors_max", "n_reactors_max", 'i', true, 2048, nullptr, 1, kNone, ""},
+    {"n_reactors_min", "n_reactors_min", 'i', true, 256, nullptr, 1, kNone, ""},
+    {"max_dt", "max_dt", 'd', true, 0.05, nullptr, kNone, kNone, ""},
+    {"max_steps", "max_steps", 'i', true, 5000, nullptr, kNone, kNone, ""},
+    {"relative_tolerance", "rel_tol", 'd', true, 1.0e-8, nullptr, kNone, kNone, ""},
+    {"absolute_tolerance", "abs_tol", 'd', true, 1.0e-20, nullptr, kNone, kNone, ""},
+    {"abstol_dens", "abstol_dens", 'i', true, 0, nullptr, kNone, kNone, ""},
+    {"reference_temperature", "reference_temperature", 'd', true, 1.0, nullptr, kNone, kNone, ""},
+    {"preconditioner_threshold", "preconditioner_threshold", 'd', false, 0, nullptr, kNone, kNone, ""},
+    {"eps_lin", "eps_lin", 'd', true, 0.1, nullptr, kNone, kNone, ""},
+    {"nonlinear_convergence_coeff", "nonlinear_convergence_coeff", 'd', true, 0.05, nullptr, kNone, kNone, ""},
+    {"integrator", "integrator", 'i', true, 0, nullptr, kNone, kNone, "0123"},
+    {"dense", "dense", 'i', true, 1, nullptr, kNone, kNone, "01"},
+    {"analytic", "analytic", 'i', true, 1, nullptr, kNone, kNone, "01"},
+    {"iterative", "iterative", 'i', true, 1, nullptr, kNone, kNone, "01"},
+    {"gpu", "gpu", 'i', true, 0, nullptr, kNone, kNone, "01"},
+    {"initial_gpu_multiplier", "initial_gpu_multiplier", 'd', true, 20.0, nullptr, 1.0, kNone, ""},
+    {"load_balance", "load_balance", 'i', true, 1, nullptr, kNone, kNone, "012"},
+    {"load_balance_noise", "load_balance_noise", 'i', true, 0, nullptr, 0, kNone, ""},
+    {"load_balance_mem", "load_balance_mem", 'i', true, 1, nullptr, 0, kNone, ""},
+    {"reactor_weight_mult", "reactor_weight_mult", 'i', true, 1, nullptr, 1, kNone, ""},
+    {"sort_reactors", "sort_reactors", 'i', true, 1, nullptr, kNone, kNone, "01"},
+    {"verbosity", "verbosity", 'i', true, 4, nullptr, kNone, kNone, "01234"},
+    {"constant_volume", "constant_volume", 'i', true, 1, nullptr, kNone, kNone, "01"},
+    {"delta_temperature_ignition", "delta_temperature_ignition", 'd', true, 0.0, nullptr, 0.0, 100000.0, ""},
+    {"min_mass_fraction", "min_mass_fraction", 'd', true, 1.0e-30, nullptr, kNone, kNone, ""},
+    {"step_limiter", "step_limiter", 'd', true, 1.0e22, nullptr, kNone, kNone, ""},
+    {"always_solve_temperature", "always_solve_temperature", 'i', true, 1, nullptr, kNone, kNone, "01"},
+    {"solve_temperature_threshold", "solve_temperature_threshold", 'd', true, 2.0, nullptr, 0.0, kNone, ""},
+    {"dump_reactors", "dump_reactors", 'i', true, 0, nullptr, kNone, kNone, "01"},
+    {"dump_failed_reactors", "dump_failed_reactors", 'i', true, 0, nullptr, kNone, kNone, "01"},
+    {"output_performance_log", "output_performance_log", 'i', true, 1, nullptr, kNone, kNone, "01"},
+    {"cvode_num_retries", "cvode_num_retries", 'i', true, 3, nullptr, 0, 20, ""},
+    {"cvode_retry_absolute_tolerance_adjustment", "cvode_retry_absolute_tolerance_adjustment", 'd', true, 0.1, nullptr, 1.0e-6, 1.0, ""},
+    {"cvode_retry_relative_tolerance_adjustment", "cvode_retry_relative_tolerance_adjustment", 'd', true, 1.0, nullptr, 1.0e-6, 1.0, ""},
+};
+
+// Options file (ZeroRKReactorManager::ReadOptionsFile,
+// zerork_reactor_manager.cpp:135-191): a flat "key: value" YAML document
+// validated against the schema above.  Every schema key sets its option,
+// from the file or from the schema's default (which differs from the
+// manager's constructor default for dense, eps_lin, cvode_num_retries,
+// n_reactors_min/max and initial_gpu_multiplier); preconditioner_threshold
+// has no default and must be present; a value outside its type, bounds or
+// allowed set fails the parse and leaves the handle's options unchanged.
+// Keys outside the schema are kept: an int option of this library's own
+// (fp32_preconditioner, stop_after_ignition, num_species_stride) takes an
+// integer, anything else is stored as a string.
 zerork_status_t read_options(zerork_handle h, const char* fn) {
   std::ifstream f(fn);
   if (!f) return ZERORK_STATUS_FAILED_OPTIONS_PARSE;
-  // YAML key -> internal option name where they differ
-  static const std::map<std::string, std::string> alias = {
-      {"absolute_tolerance", "abs_tol"},
-      {"relative_tolerance", "rel_tol"},
-      {"reactor_timing_log", "reactor_timing_log_filename"},
-      {"mechanism_parsing_log", "mechanism_parsing_log_filename"},
-      {"mech_file", "mech_filename"},
-      {"therm_file", "therm_filename"}};
+  std::map<std::string, std::string> kv;
   std::string line;
   while (std::getline(f, line)) {
     auto hash = line.find('#');
@@ -187,31 +246,71 @@ zerork_status_t read_options(zerork_handle h, const char* fn) {
     if (trim(line).empty() || trim(line) == "---" || trim(line) == "...") continue;
     auto colon = line.find(':');
     if (colon == std::string::npos) return ZERORK_STATUS_FAILED_OPTIONS_PARSE;
-    std::string key = trim(line.substr(0, colon));
-    std::string val = trim(line.substr(colon + 1));
-    auto it = alias.find(key);
-    if (it != alias.end()) key = it->second;
-    if (h->iopt.count(key)) {
-      char* end = nullptr;
-      long v = std::strtol(val.c_str(), &end, 10);
-      if (val.empty() || *end) {
-        if (val == "true" || val == "True") v = 1;
-        else if (val == "false" || val == "False") v = 0;
-        else return ZERORK_STATUS_FAILED_OPTIONS_PARSE;
+    kv[trim(line.substr(0, colon))] = trim(line.substr(colon + 1));
+  }
+  auto parse_int = [](const std::string& v, long* out) {
+    if (v == "true" || v == "True") { *out = 1; return true; }
+    if (v == "false" || v == "False") { *out = 0; return true; }
+    char* end = nullptr;
+    *out = std::strtol(v.c_str(), &end, 10);
+    return !v.empty() && !*end;
+  };
+  auto parse_dbl = [](const std::string& v, double* out) {
+    char* end = nullptr;
+    *out = std::strtod(v.c_str(), &end);
+    return !v.empty() && !*end;
+  };
+  std::map<std::string, int> iset;
+  std::map<std::string, double> dset;
+  std::map<std::string, std::string> sset;
+  for (const IfpKey& k : kIfp) {
+    auto it = kv.find(k.yaml);
+    if (it == kv.end() && !k.has_default) return ZERORK_STATUS_FAILED_OPTIONS_PARSE;
+    if (k.type == 's') {
+      sset[k.option] = it == kv.end() ? std::string(k.sdef) : it->second;
+      continue;
+    }
+    double v = k.def;
+    if (it != kv.end()) {
+      if (k.type == 'i') {
+        long iv;
+        if (!parse_int(it->second, &iv)) return ZERORK_STATUS_FAILED_OPTIONS_PARSE;
+        v = (double)iv;
+      } else if (!parse_dbl(it->second, &v)) {
+        return ZERORK_STATUS_FAILED_OPTIONS_PARSE;
       }
-      h->iopt[key] = (int)v;
-    } else if (h->dopt.count(key)) {
-      char* end = nullptr;
-      double v = std::strtod(val.c_str(), &end);
-      if (val.empty() || *end) return ZERORK_STATUS_FAILED_OPTIONS_PARSE;
-      h->dopt[key] = v;
-    } else if (h->sopt.count(key)) {
-      h->sopt[key] = val;
+    }
+    if ((!std::isnan(k.lo) && v < k.lo) || (!std::isnan(k.hi) && v > k.hi))
+      return ZERORK_STATUS_FAILED_OPTIONS_PARSE;
+    if (k.allowed[0]) {
+      bool ok = false;
+      for (const char* c = k.allowed; *c; ++c) ok = ok || v == (double)(*c - '0');
+      if (!ok) return ZERORK_STATUS_FAILED_OPTIONS_PARSE;
+    }
+    // initial_gpu_multiplier is a double in the schema and an int option of
+    // the manager (the value is truncated, as the manager's assignment does)
+    if (k.type == 'i' || h->iopt.count(k.option)) iset[k.option] = (int)v;
+    else dset[k.option] = v;
+  }
+  for (const auto& e : kv) {
+    bool schema = false;
+    for (const IfpKey& k : kIfp) schema = schema || e.first == k.yaml;
+    if (schema) continue;
+    if (h->iopt.count(e.first)) {
+      long iv;
+      if (!parse_int(e.second, &iv)) return ZERORK_STATUS_FAILED_OPTIONS_PARSE;
+      iset[e.first] = (int)iv;
+    } else if (h->dopt.count(e.first)) {
+      double dv;
+      if (!parse_dbl(e.second, &dv)) return ZERORK_STATUS_FAILED_OPTIONS_PARSE;
+      dset[e.first] = dv;
     } else {
-      // unknown keys are kept as strings, like an Optionable map entry
-      h->sopt[key] = val;
+      sset[e.first] = e.second;
     }
   }
+  for (auto& e : iset) h->iopt[e.first] = e.second;
+  for (auto& e : dset) h->dopt[e.first] = e.second;
+  for (auto& e : sset) h->sopt[e.first] = e.second;
   return ZERORK_STATUS_SUCCESS;
 }
 

@@ -149,6 +149,28 @@ def _ptr(x):
     return x.data_ptr()
 
 
+_F64, _I32 = "float64", "int32"
+
+
+def _dev(x, kind, numel, name):
+    """Device pointer of x after checking on the host that the kernels' reads
+    and writes stay inside it: a torch tensor must live on the GPU, be
+    contiguous, have the expected element type (kind "float64" or "int32")
+    and hold at least numel elements.  None passes as NULL; a plain int is
+    taken as a raw device address (the caller vouches for it)."""
+    if x is None or isinstance(x, int):
+        return x
+    if not getattr(x, "is_cuda", False):
+        raise ValueError(f"{name}: expected a device (cuda) tensor")
+    if str(x.dtype).rsplit(".", 1)[-1] != kind:
+        raise ValueError(f"{name}: expected {kind}, got {x.dtype}")
+    if not x.is_contiguous():
+        raise ValueError(f"{name}: tensor must be contiguous")
+    if x.numel() < numel:
+        raise ValueError(f"{name}: {x.numel()} elements, at least {numel} needed")
+    return x.data_ptr()
+
+
 class Mech:
     """A mechanism loaded into the library (parsed on the host; uploaded to the
     device on first device call)."""
@@ -212,16 +234,27 @@ class Mech:
     # -- device entry points ---------------------------------------------------
     def rates(self, n, T, P, Y, stride, step_limiter=1.0e22, ydot=None, rop=None, create=None,
               destroy=None, net=None, stream=0):
-        rc = lib().zrk_rates_device(self.h, n, _ptr(T), _ptr(P), _ptr(Y), stride, step_limiter,
-                                    _ptr(ydot), _ptr(rop), _ptr(create), _ptr(destroy), _ptr(net),
-                                    stream)
+        ns = self.nsp
+        if stride < ns:
+            raise ValueError(f"stride {stride} < {ns} species")
+        rc = lib().zrk_rates_device(
+            self.h, n, _dev(T, _F64, n, "T"), _dev(P, _F64, n, "P"),
+            _dev(Y, _F64, (n - 1) * stride + ns if n else 0, "Y"), stride, step_limiter,
+            _dev(ydot, _F64, n * (ns + 1), "ydot"), _dev(rop, _F64, n * self.nstep, "rop"),
+            _dev(create, _F64, n * ns, "create"), _dev(destroy, _F64, n * ns, "destroy"),
+            _dev(net, _F64, n * ns, "net"), stream)
         if rc != 0:
             raise RuntimeError(f"zrk_rates_device failed ({rc})")
 
     def jacobian(self, n, T, P, Y, stride, J, opts: BdfOptions | None = None, stream=0):
         o = opts or default_options()
-        rc = lib().zrk_jacobian_device(self.h, C.byref(o), n, _ptr(T), _ptr(P), _ptr(Y), stride,
-                                       _ptr(J), stream)
+        ns = self.nsp
+        if stride < ns:
+            raise ValueError(f"stride {stride} < {ns} species")
+        rc = lib().zrk_jacobian_device(
+            self.h, C.byref(o), n, _dev(T, _F64, n, "T"), _dev(P, _F64, n, "P"),
+            _dev(Y, _F64, (n - 1) * stride + ns if n else 0, "Y"), stride,
+            _dev(J, _F64, n * (ns + 1) ** 2, "J"), stream)
         if rc != 0:
             raise RuntimeError(f"zrk_jacobian_device failed ({rc})")
 
@@ -232,14 +265,20 @@ class Mech:
         of ignition times (with opts.delta_temperature_ignition > 0), all
         device tensors."""
         o = opts or default_options()
+        ns = self.nsp
+        if stride < ns:
+            raise ValueError(f"stride {stride} < {ns} species")
+        ny = (n - 1) * stride + ns if n else 0
+        T, P, Y = _dev(T, _F64, n, "T"), _dev(P, _F64, n, "P"), _dev(Y, _F64, ny, "Y")
+        nsteps, stats = _dev(nsteps, _I32, n, "nsteps"), _dev(stats, _I32, 8 * n, "stats")
         if dpdt is None and esrc is None and ysrc is None and ignition_time is None:
-            rc = lib().zrk_integrate_device(self.h, C.byref(o), n, dt, _ptr(T), _ptr(P), _ptr(Y),
-                                            stride, _ptr(nsteps), _ptr(stats), stream)
+            rc = lib().zrk_integrate_device(self.h, C.byref(o), n, dt, T, P, Y, stride, nsteps,
+                                            stats, stream)
         else:
-            rc = lib().zrk_integrate_device_aux(self.h, C.byref(o), n, dt, _ptr(T), _ptr(P),
-                                                _ptr(Y), stride, _ptr(dpdt), _ptr(esrc),
-                                                _ptr(ysrc), _ptr(ignition_time), _ptr(nsteps),
-                                                _ptr(stats), stream)
+            rc = lib().zrk_integrate_device_aux(
+                self.h, C.byref(o), n, dt, T, P, Y, stride, _dev(dpdt, _F64, n, "dpdt"),
+                _dev(esrc, _F64, n, "esrc"), _dev(ysrc, _F64, ny, "ysrc"),
+                _dev(ignition_time, _F64, n, "ignition_time"), nsteps, stats, stream)
         if rc != 0:
             raise RuntimeError(f"zrk_integrate_device failed ({rc})")
 
@@ -248,8 +287,13 @@ class Mech:
         (wavefronts per reactor, {size: ms})."""
         choice = C.c_int(0)
         ms = (C.c_float * 4)()
-        rc = lib().zrk_autotune_integrator(self.h, C.byref(opts), n, dt, _ptr(T), _ptr(P),
-                                           _ptr(Y), stride, C.byref(choice), ms, stream)
+        ns = self.nsp
+        if stride < ns:
+            raise ValueError(f"stride {stride} < {ns} species")
+        rc = lib().zrk_autotune_integrator(
+            self.h, C.byref(opts), n, dt, _dev(T, _F64, n, "T"), _dev(P, _F64, n, "P"),
+            _dev(Y, _F64, (n - 1) * stride + ns if n else 0, "Y"), stride, C.byref(choice), ms,
+            stream)
         if rc != 0:
             raise RuntimeError(f"zrk_autotune_integrator failed ({rc})")
         return choice.value, {w: ms[w] for w in (1, 2, 3) if ms[w] >= 0}
