@@ -68,6 +68,27 @@ typedef struct {
   /* 1: end a reactor's integration at its ignition time (the plugin option
    * stop_after_ignition, solver_cvode.cpp:262-264) */
   int stop_after_ignition;
+  /* 1: absolute tolerance of species j scaled by 1 / (rho0 / MW_j 1e-3), rho0
+   * the initial density (the plugin option abstol_dens,
+   * reactor_nvector_serial.cpp:107-120) */
+  int abstol_dens;
+  /* The plugin's linear-solver options (solver_cvode.cpp:93-214,
+   * reactor_nvector_serial.cpp:363-507).  dense = 1 with iterative = 0: the
+   * direct solve with M = I - gamma J, J analytic (analytic = 1) or CVODE's
+   * difference quotients.  Otherwise SPGMR (5 Krylov vectors, left
+   * preconditioning, modified Gram-Schmidt, tolerance eps_lin * the Newton
+   * tolerance) with Jacobian-vector products by difference quotients,
+   * preconditioned by I - gamma J (J analytic, or the reactor's own
+   * divided differences with analytic = 0) keeping the diagonal and, with
+   * iterative = 1, only the off-diagonal entries of the Jacobian's pattern
+   * whose magnitude exceeds preconditioner_threshold.  Device defaults:
+   * dense 1, analytic 1, iterative 0 (the plugin's defaults are the
+   * reference's: dense 0, analytic 1, iterative 1). */
+  int dense;
+  int analytic;
+  int iterative;
+  double eps_lin;
+  double preconditioner_threshold;
 } zrk_bdf_options;
 
 ZRK_EXPORT void zrk_bdf_default_options(zrk_bdf_options* o);

@@ -73,7 +73,10 @@ class Opts(C.Structure):
                 ("retry_atol_adj", C.c_double), ("retry_rtol_adj", C.c_double),
                 ("min_mass_fraction", C.c_double), ("step_limiter", C.c_double),
                 ("Tref", C.c_double), ("constant_volume", C.c_int),
-                ("delta_temperature_ignition", C.c_double), ("stop_after_ignition", C.c_int)]
+                ("delta_temperature_ignition", C.c_double), ("stop_after_ignition", C.c_int),
+                ("abstol_dens", C.c_int), ("solve_temperature", C.c_int), ("dense", C.c_int),
+                ("analytic", C.c_int), ("iterative", C.c_int), ("eps_lin", C.c_double),
+                ("preconditioner_threshold", C.c_double)]
 
 
 class Stats(C.Structure):

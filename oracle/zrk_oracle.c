@@ -32,6 +32,13 @@ void zrko_default_opts(zrko_opts* o) {
   o->constant_volume = 0;
   o->delta_temperature_ignition = 0.0;
   o->stop_after_ignition = 0;
+  o->abstol_dens = 0;
+  o->solve_temperature = 1;
+  o->dense = 1;
+  o->analytic = 1;
+  o->iterative = 0;
+  o->eps_lin = 1.0e-3;
+  o->preconditioner_threshold = 1.0e-3;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -282,6 +289,9 @@ typedef struct {
    * internal energy (constant volume), vinit the initial 1/density */
   double dpdt, esrc, e0, vinit;
   const double* ysrc;
+  /* > 0: the temperature is held at fixT * Tref and is not a state variable
+   * (SetSolveTemperature(false)); the state is then the nsp mass fractions */
+  double fixT;
   double *C, *h, *cpm, *G, *rop, *cr, *de, *net, *esc, *tmp1, *tmp2, *tmp3;
 } reactor_t;
 
@@ -295,6 +305,7 @@ static void reactor_init(reactor_t* r, const zrko_mech* m, const zrko_opts* o, d
   r->cp_mix = 0.0;
   r->dpdt = r->esrc = r->e0 = r->vinit = 0.0;
   r->ysrc = NULL;
+  r->fixT = 0.0;
   double* w = (double*)calloc((size_t)(8 * nsp + m->nstep + 3 * N), sizeof(double));
   r->C = w;
   r->h = w + nsp;
@@ -402,7 +413,10 @@ static int reactor_rhs(reactor_t* r, double t, const double* y, double* ydot) {
   const zrko_mech* m = r->m;
   const int nsp = m->nsp;
   const int cvol = r->o->constant_volume;
-  double T = y[nsp] * r->o->Tref;
+  /* the initial temperature when it is not solved for (GetTimeDerivative's
+   * initial_temperature_, reactor_constant_volume_cpu.cpp:79-82) */
+  const double yT = r->fixT > 0.0 ? r->fixT : y[nsp];
+  double T = yT * r->o->Tref;
   if (T <= 0.0) return 1;
   if (T > TLIMIT) T = TLIMIT;
   if (!cvol && !r->ysrc) {
@@ -451,6 +465,7 @@ static int reactor_rhs(reactor_t* r, double t, const double* y, double* ydot) {
       esum += r->h[j] * r->net[j] * vd;
     }
   }
+  if (r->fixT > 0.0) return 0;  /* no temperature equation */
   const double dTdt = -esum * NIST_RU * y[nsp] / cp;
   ydot[nsp] = dTdt + (r->esrc + r->dpdt * vd) / (cp * r->o->Tref);
   return 0;
@@ -531,6 +546,13 @@ static void reactor_jacobian(reactor_t* r, double t, const double* y, const doub
     }
     for (int i = 0; i < N * N; ++i) J[i] += nj[i];
     free(nj);
+  }
+  if (r->fixT > 0.0) {
+    /* no temperature row or column: the species block only
+     * (reactor_nvector_serial.cpp:641-643, 691-750 under solve_temperature_) */
+    for (int j = 0; j < nsp; ++j)
+      for (int k = 0; k < nsp; ++k) J[j * N + k] *= m->mw[k] * (1.0 / m->mw[j]);
+    return;
   }
   /* d(Tdot)/dy_j */
   const double RuTemp = NIST_RU * y[nsp];
@@ -668,13 +690,24 @@ enum { NLS_SUCCESS = 0, NLS_CONTINUE = 901, NLS_CONV_RECVR = 902, RHSFUNC_RECVR 
 #define CV_REPTD_RHSFUNC_ERR (-10)
 #define CV_ILL_INPUT (-22)
 
+#define SPGMR_MAXL 5 /* SUNLinSol_SPGMR(state, PREC_LEFT, 0): maxl 0 -> 5 */
+
 typedef struct {
   reactor_t* r;
-  int N;
+  const zrko_opts* o;
+  int N;      /* state variables: nsp + 1, or nsp at fixed temperature */
+  int nsp;
+  int direct; /* CVODE's dense direct solver (dense = 1, iterative = 0) */
   double* zn[L_MAX];
   double *ewt, *acor, *tempv, *ftemp, *y, *atolv, *delta;
   double *savedJ, *M;
   int* piv;
+  const char* pat; /* [N*N] the analytic Jacobian's pattern (preconditioner) */
+  /* SPGMR work: Krylov basis V[0..maxl], its Hessenberg matrix and rotations */
+  double* V[SPGMR_MAXL + 1];
+  double *xcor, *vtemp, *work, *jv;
+  double hes[SPGMR_MAXL + 1][SPGMR_MAXL];
+  double givens[2 * SPGMR_MAXL];
   double rtol;
   double tn, h, hscale, hprime, next_h, eta, etamax, hmin, hmax_inv, hu, h0u;
   double tau[L_MAX + 1], tq[6], l[L_MAX];
@@ -941,29 +974,318 @@ static void cv_set(cvmem* cv) {
   cv->gamrat = (cv->nst > 0) ? cv->gamma / cv->gammap : 1.0;
 }
 
-/* cvLsSetup + cvLsLinSys + SUNLinSolSetup_Dense; returns 0, >0 recoverable. */
-static int cv_lsetup(cvmem* cv, int convfail, const double* ypred, const double* fpred,
-                     int* jcur) {
+/* cvLsDenseDQJac (SUNDIALS 5.8.0 cvode_ls.c): column j is
+ * (f(y + inc e_j) - f(y)) / inc with inc = max(sqrt(uround) |y_j|,
+ * minInc / ewt_j), minInc = 1000 |h| uround N wrms(f(y)) (1 when that norm is
+ * 0); y is restored exactly.  J column-major with stride N. */
+static int cv_dq_jac(cvmem* cv, double* y, const double* fy, double* J) {
   const int N = cv->N;
+  const double srur = sqrt(DBL_EPSILON);
+  const double fnorm = wrms(fy, cv->ewt, N);
+  const double minInc = (fnorm != 0.0) ? (1000.0 * fabs(cv->h) * DBL_EPSILON * N * fnorm) : 1.0;
+  double* ftemp = cv->work;
+  for (int j = 0; j < N; ++j) {
+    const double yjsaved = y[j];
+    const double inc = fmax(srur * fabs(yjsaved), minInc / cv->ewt[j]);
+    y[j] += inc;
+    const int rv = cv_f(cv, cv->tn, y, ftemp);
+    y[j] = yjsaved;
+    if (rv != 0) return rv;
+    const double inc_inv = 1.0 / inc;
+    for (int i = 0; i < N; ++i) J[(size_t)j * N + i] = inc_inv * (ftemp[i] - fy[i]);
+  }
+  return 0;
+}
+
+/* ReactorNVectorSerial::DividedDifferenceJacobian
+ * (reactor_nvector_serial.cpp:816-835): a fresh f(y), then column i is
+ * (f(y + delta e_i) - f(y)) / delta with delta = sqrt(1e-16 max(1e-5, |y_i|)). */
+static void reactor_dq_jac(cvmem* cv, double* y, double* J) {
+  const int N = cv->N;
+  double* f0 = cv->work;
+  double* f1 = cv->jv;
+  reactor_rhs(cv->r, cv->tn, y, f0);
+  for (int i = 0; i < N; ++i) {
+    const double ysafe = y[i];
+    const double delta = sqrt(1.0e-16 * fmax(1.0e-5, fabs(ysafe)));
+    y[i] = ysafe + delta;
+    reactor_rhs(cv->r, cv->tn, y, f1);
+    for (int j = 0; j < N; ++j) J[(size_t)i * N + j] = (f1[j] - f0[j]) / delta;
+    y[i] = ysafe;
+  }
+}
+
+/* The analytic Jacobian (reactor_jacobian) in CVODE's N x N layout: at fixed
+ * temperature the species block of the (nsp+1)-strided matrix. */
+static void cv_analytic_jac(cvmem* cv, const double* y, const double* fy, double* J) {
+  const int nsp = cv->nsp;
+  if (cv->N == nsp + 1) {
+    reactor_jacobian(cv->r, cv->tn, y, fy, J);
+    return;
+  }
+  const int Nf = nsp + 1;
+  double* full = (double*)calloc((size_t)Nf * Nf, sizeof(double));
+  double* yf = (double*)malloc(sizeof(double) * (size_t)Nf);
+  memcpy(yf, y, sizeof(double) * (size_t)nsp);
+  yf[nsp] = cv->r->fixT;
+  reactor_jacobian(cv->r, cv->tn, yf, NULL, full);
+  for (int j = 0; j < nsp; ++j)
+    for (int i = 0; i < nsp; ++i) J[(size_t)j * nsp + i] = full[(size_t)j * Nf + i];
+  free(yf);
+  free(full);
+}
+
+/* cvLsSetup with the dense direct solver (cvLsLinSys + SUNLinSolSetup_Dense)
+ * or, for SPGMR, the preconditioner setup ReactorJacobianSetupAndFactor
+ * (reactor_base.cpp:96-109: Jacobian when CVODE does not vouch for the saved
+ * one, then JacobianFactor / FormPreconditioner, reactor_nvector_serial.cpp:
+ * 373-458).  The jbad test and its bookkeeping are CVODE's for both.
+ * Returns 0, >0 recoverable. */
+static int cv_lsetup(cvmem* cv, int convfail, double* ypred, const double* fpred, int* jcur) {
+  const int N = cv->N;
+  const zrko_opts* o = cv->o;
   const double dgamma = fabs(cv->gamma / cv->gammap - 1.0);
   const int jbad = (cv->nst == 0) || (cv->nst >= cv->nstlj + CVLS_MSBJ) ||
                    ((convfail == CV_FAIL_BAD_J) && (dgamma < CVLS_DGMAX)) ||
                    (convfail == CV_FAIL_OTHER);
   if (!jbad) {
     *jcur = 0;
-    memcpy(cv->M, cv->savedJ, sizeof(double) * (size_t)N * N);
   } else {
     *jcur = 1;
     cv->nje++;
     cv->nstlj = cv->nst;
-    reactor_jacobian(cv->r, cv->tn, ypred, fpred, cv->M);
-    memcpy(cv->savedJ, cv->M, sizeof(double) * (size_t)N * N);
+    if (o->analytic) {
+      cv_analytic_jac(cv, ypred, fpred, cv->savedJ);
+    } else if (cv->direct) {
+      if (cv_dq_jac(cv, ypred, fpred, cv->savedJ) != 0) return 1;
+    } else {
+      reactor_dq_jac(cv, ypred, cv->savedJ);
+    }
   }
-  for (int j = 0; j < N; ++j) {
-    for (int i = 0; i < N; ++i) cv->M[j * N + i] *= -cv->gamma;
-    cv->M[j * N + j] += 1.0;
+  const double g = cv->gamma;
+  if (cv->direct || (o->dense && !o->analytic)) {
+    /* M = I - gamma J, every entry */
+    for (int j = 0; j < N; ++j) {
+      for (int i = 0; i < N; ++i) cv->M[j * N + i] = -g * cv->savedJ[j * N + i];
+      cv->M[j * N + j] += 1.0;
+    }
+  } else {
+    /* FormPreconditioner: the Jacobian's pattern, diagonal kept, other
+     * entries only when |-gamma J_ij| > threshold (0 when iterative = 0) */
+    const double thr = o->iterative ? o->preconditioner_threshold : 0.0;
+    for (int j = 0; j < N; ++j)
+      for (int i = 0; i < N; ++i) {
+        double v = 0.0;
+        if (cv->pat[j * N + i]) {
+          v = -g * cv->savedJ[j * N + i];
+          if (i == j) v += 1.0;
+          else if (!(fabs(v) > thr)) v = 0.0;
+        }
+        cv->M[j * N + i] = v;
+      }
   }
   return zrko_dense_getrf(N, cv->M, cv->piv) != 0 ? 1 : 0;
+}
+
+/* cvLsATimes with cvLsDQJtimes: z = v - gamma J v, J v by the difference
+ * quotient (f(y + sig v) - f(y)) / sig at the Newton iterate, sig = 1 /
+ * wrms(v), shrunk by 4 on a recoverable RHS failure (3 tries). */
+static int cv_atimes(cvmem* cv, const double* v, double* z) {
+  const int N = cv->N;
+  double sig = 1.0 / wrms(v, cv->ewt, N);
+  int rv = 1;
+  for (int it = 0; it < 3; ++it) {
+    for (int i = 0; i < N; ++i) cv->work[i] = sig * v[i] + cv->y[i];
+    rv = cv_f(cv, cv->tn, cv->work, cv->jv);
+    if (rv == 0) break;
+    if (rv < 0) return -1;
+    sig *= 0.25;
+  }
+  if (rv > 0) return 1;
+  const double siginv = 1.0 / sig;
+  for (int i = 0; i < N; ++i) cv->jv[i] = siginv * (cv->jv[i] - cv->ftemp[i]);
+  for (int i = 0; i < N; ++i) z[i] = -cv->gamma * cv->jv[i] + v[i];
+  return 0;
+}
+
+static double dotp(const double* a, const double* b, int n) {
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+
+/* SUNModifiedGS: orthogonalise V[k] against V[0..k-1] into column k-1 of hes,
+ * reorthogonalising when the new norm lost more than 1000 x roundoff. */
+static void cv_mgs(cvmem* cv, int k, double* new_norm) {
+  const int N = cv->N;
+  double* vk = cv->V[k];
+  const double vk_norm = sqrt(dotp(vk, vk, N));
+  for (int i = 0; i < k; ++i) {
+    cv->hes[i][k - 1] = dotp(cv->V[i], vk, N);
+    const double h = cv->hes[i][k - 1];
+    for (int e = 0; e < N; ++e) vk[e] = vk[e] - h * cv->V[i][e];
+  }
+  *new_norm = sqrt(dotp(vk, vk, N));
+  double temp = 1000.0 * vk_norm;
+  if ((temp + *new_norm) != temp) return;
+  double new_norm_2 = 0.0;
+  for (int i = 0; i < k; ++i) {
+    const double np = dotp(cv->V[i], vk, N);
+    temp = 1000.0 * cv->hes[i][k - 1];
+    if ((temp + np) == temp) continue;
+    cv->hes[i][k - 1] += np;
+    for (int e = 0; e < N; ++e) vk[e] = vk[e] - np * cv->V[i][e];
+    new_norm_2 += np * np;
+  }
+  if (new_norm_2 != 0.0) {
+    const double np = (*new_norm) * (*new_norm) - new_norm_2;
+    *new_norm = (np > 0.0) ? sqrt(np) : 0.0;
+  }
+}
+
+/* one Givens rotation annihilating b against a: (c, s) with c a - s b = r */
+static void givens_of(double a, double b, double* c, double* s) {
+  if (b == 0.0) {
+    *c = 1.0;
+    *s = 0.0;
+  } else if (fabs(b) >= fabs(a)) {
+    const double t = a / b;
+    *s = -1.0 / sqrt(1.0 + t * t);
+    *c = -*s * t;
+  } else {
+    const double t = b / a;
+    *c = 1.0 / sqrt(1.0 + t * t);
+    *s = -*c * t;
+  }
+}
+
+/* SUNQRfact for column l of the Hessenberg matrix (the earlier columns are
+ * already factored): apply the previous rotations, then a new one; nonzero
+ * when the new diagonal entry vanishes. */
+static int cv_qrfact(cvmem* cv, int l) {
+  for (int k = 0; k < l; ++k) {
+    const double c = cv->givens[2 * k], s = cv->givens[2 * k + 1];
+    const double t1 = cv->hes[k][l], t2 = cv->hes[k + 1][l];
+    cv->hes[k][l] = c * t1 - s * t2;
+    cv->hes[k + 1][l] = s * t1 + c * t2;
+  }
+  double c, s;
+  const double t1 = cv->hes[l][l], t2 = cv->hes[l + 1][l];
+  givens_of(t1, t2, &c, &s);
+  cv->givens[2 * l] = c;
+  cv->givens[2 * l + 1] = s;
+  cv->hes[l][l] = c * t1 - s * t2;
+  return cv->hes[l][l] == 0.0 ? l + 1 : 0;
+}
+
+/* SUNQRsol: y = R^{-1} Q b for the factored n-column Hessenberg matrix */
+static int cv_qrsol(cvmem* cv, int n, double* b) {
+  for (int k = 0; k < n; ++k) {
+    const double c = cv->givens[2 * k], s = cv->givens[2 * k + 1];
+    const double t1 = b[k], t2 = b[k + 1];
+    b[k] = c * t1 - s * t2;
+    b[k + 1] = s * t1 + c * t2;
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    if (cv->hes[k][k] == 0.0) return k + 1;
+    b[k] /= cv->hes[k][k];
+    for (int i = 0; i < k; ++i) b[i] -= b[k] * cv->hes[i][k];
+  }
+  return 0;
+}
+
+/* ReactorJacobianSolveCVODE: z = P^{-1} r with the factored preconditioner */
+static void cv_psolve(cvmem* cv, const double* r, double* z) {
+  memcpy(z, r, sizeof(double) * (size_t)cv->N);
+  zrko_dense_getrs(cv->N, cv->M, cv->piv, z);
+}
+
+enum { LS_SUCCESS = 0, LS_RES_REDUCED, LS_CONV_FAIL, LS_REC_FAIL, LS_UNREC_FAIL };
+
+/* SUNLinSolSolve_SPGMR with left preconditioning, scaling s1 = s2 = ewt, no
+ * restarts, initial guess 0: x (returned in b) for A x = b to within delta in
+ * the scaled preconditioned 2-norm. */
+static int cv_spgmr(cvmem* cv, double* b, double delta) {
+  const int N = cv->N;
+  const double* s = cv->ewt;
+  /* V[0] = s1 P^{-1} b (the initial guess is 0, so the residual is b) */
+  cv_psolve(cv, b, cv->vtemp);
+  for (int i = 0; i < N; ++i) cv->V[0][i] = s[i] * cv->vtemp[i];
+  const double beta = sqrt(dotp(cv->V[0], cv->V[0], N));
+  double rho = beta;
+  if (beta <= delta) {  /* the initial guess 0 is returned */
+    for (int i = 0; i < N; ++i) b[i] = 0.0;
+    return LS_SUCCESS;
+  }
+  for (int i = 0; i <= SPGMR_MAXL; ++i)
+    for (int j = 0; j < SPGMR_MAXL; ++j) cv->hes[i][j] = 0.0;
+  double rotation_product = 1.0;
+  const double rinv = 1.0 / beta;
+  for (int i = 0; i < N; ++i) cv->V[0][i] *= rinv;
+  int krydim = 0, converged = 0;
+  for (int l = 0; l < SPGMR_MAXL; ++l) {
+    krydim = l + 1;
+    for (int i = 0; i < N; ++i) cv->vtemp[i] = cv->V[l][i] / s[i];
+    const int ar = cv_atimes(cv, cv->vtemp, cv->V[l + 1]);
+    if (ar != 0) return ar < 0 ? LS_UNREC_FAIL : LS_REC_FAIL;
+    cv_psolve(cv, cv->V[l + 1], cv->vtemp);
+    for (int i = 0; i < N; ++i) cv->V[l + 1][i] = s[i] * cv->vtemp[i];
+    cv_mgs(cv, l + 1, &cv->hes[l + 1][l]);
+    if (cv_qrfact(cv, l) != 0) return LS_REC_FAIL;
+    rotation_product *= cv->givens[2 * l + 1];
+    rho = fabs(rotation_product * beta);
+    if (rho <= delta) {
+      converged = 1;
+      break;
+    }
+    const double hinv = 1.0 / cv->hes[l + 1][l];
+    for (int i = 0; i < N; ++i) cv->V[l + 1][i] *= hinv;
+  }
+  double yg[SPGMR_MAXL + 1];
+  yg[0] = beta;
+  for (int i = 1; i <= krydim; ++i) yg[i] = 0.0;
+  if (cv_qrsol(cv, krydim, yg) != 0) return LS_UNREC_FAIL;
+  double* xc = cv->xcor;
+  for (int i = 0; i < N; ++i) xc[i] = 0.0;
+  for (int k = 0; k < krydim; ++k)
+    for (int i = 0; i < N; ++i) xc[i] += yg[k] * cv->V[k][i];
+  if (converged || rho < beta) {
+    for (int i = 0; i < N; ++i) b[i] = xc[i] / s[i];
+    return converged ? LS_SUCCESS : LS_RES_REDUCED;
+  }
+  for (int i = 0; i < N; ++i) b[i] = 0.0;
+  return LS_CONV_FAIL;
+}
+
+/* cvLsSolve: b <- the Newton correction.  Dense direct: the LU solve and the
+ * BDF rescaling 2/(1 + gamrat) for a lagged gamma; SPGMR: the small-residual
+ * shortcut, tolerance eps_lin tq[4] sqrt(N), and the solver's outcome as
+ * CVODE reads it (a reduced residual is accepted on the first Newton
+ * iteration only).  Returns 0, >0 recoverable, <0 unrecoverable. */
+static int cv_lsolve(cvmem* cv, double* b, int curiter) {
+  const int N = cv->N;
+  if (cv->direct) {
+    zrko_dense_getrs(N, cv->M, cv->piv, b);
+    if (cv->gamrat != 1.0) {
+      const double sc = 2.0 / (1.0 + cv->gamrat);
+      for (int i = 0; i < N; ++i) b[i] *= sc;
+    }
+    return 0;
+  }
+  const double deltar = cv->o->eps_lin * cv->tq[4];
+  const double bnorm = wrms(b, cv->ewt, N);
+  if (bnorm <= deltar) {
+    if (curiter > 0)
+      for (int i = 0; i < N; ++i) b[i] = 0.0;
+    return 0;
+  }
+  const int rv = cv_spgmr(cv, b, deltar * sqrt((double)N));
+  switch (rv) {
+    case LS_SUCCESS: return 0;
+    case LS_RES_REDUCED: return curiter == 0 ? 0 : 1;
+    case LS_UNREC_FAIL: return -1;
+    default: return 1;
+  }
 }
 
 /* cvNlsLSetup */
@@ -1024,10 +1346,10 @@ static int cv_newton(cvmem* cv, double* ycor, double tol, int callLSetup) {
       for (;;) {
         cv->nni++;
         for (int i = 0; i < N; ++i) delta[i] = -delta[i];
-        zrko_dense_getrs(N, cv->M, cv->piv, delta);
-        if (cv->gamrat != 1.0) {
-          const double s = 2.0 / (1.0 + cv->gamrat);
-          for (int i = 0; i < N; ++i) delta[i] *= s;
+        const int ls = cv_lsolve(cv, delta, curiter);
+        if (ls != 0) {  /* (an unrecoverable solver failure is treated alike) */
+          rv = NLS_CONV_RECVR;
+          break;
         }
         for (int i = 0; i < N; ++i) ycor[i] += delta[i];
         rv = cv_nls_convtest(cv, curiter, ycor, delta, tol);
@@ -1390,38 +1712,89 @@ static void root_check2(const cvmem* cv, rootst* rs) {
   rs->glo = g;
 }
 
+/* The analytic Jacobian's pattern (ReactorNVectorSerial::
+ * SetupSparseJacobianArrays, reactor_nvector_serial.cpp:122-281): diagonal,
+ * temperature row and column, every step's reactant columns against its
+ * reactant and product rows, the non-integer terms; pat[col*N + row], N = nsp
+ * (no temperature) or nsp + 1. */
+static char* jacobian_pattern(const zrko_mech* m, int N) {
+  const int nsp = m->nsp;
+  char* pat = (char*)calloc((size_t)N * N, 1);
+  for (int j = 0; j < N; ++j) pat[j * N + j] = 1;
+  if (N == nsp + 1)
+    for (int k = 0; k < N; ++k) pat[k * N + nsp] = pat[nsp * N + k] = 1;
+  for (int s = 0; s < m->nstep; ++s)
+    for (int k = m->reac_ptr[s]; k < m->reac_ptr[s + 1]; ++k) {
+      const int col = m->reac_sp[k];
+      for (int q = m->reac_ptr[s]; q < m->reac_ptr[s + 1]; ++q) pat[col * N + m->reac_sp[q]] = 1;
+      for (int q = m->prod_ptr[s]; q < m->prod_ptr[s + 1]; ++q) pat[col * N + m->prod_sp[q]] = 1;
+    }
+  for (int q = 0; q < m->n_ni; ++q)
+    for (int k = m->ni_rop_ptr[q]; k < m->ni_rop_ptr[q + 1]; ++k) {
+      const int col = m->ni_rop_sp[k];
+      for (int e = m->ni_re_ptr[q]; e < m->ni_re_ptr[q + 1]; ++e) pat[col * N + m->ni_re_sp[e]] = 1;
+      for (int e = m->ni_pr_ptr[q]; e < m->ni_pr_ptr[q + 1]; ++e) pat[col * N + m->ni_pr_sp[e]] = 1;
+    }
+  return pat;
+}
+
 static int integrate_one(const zrko_mech* m, const zrko_opts* o, double* P, double dt,
                          double* y, zrko_stats* st, double dpdt, double esrc,
                          const double* ysrc, double T0, double* ign_time) {
-  const int N = m->nsp + 1;
+  const int nsp = m->nsp;
+  const int Nf = nsp + 1;                            /* storage of every vector */
+  const int N = o->solve_temperature ? nsp + 1 : nsp; /* CVODE's state size */
   reactor_t r;
   reactor_init(&r, m, o, *P);
   /* ReactorConstantVolumeCPU::InitializeState (reactor_constant_volume_cpu.cpp:15-42) */
   if (o->constant_volume) r.v0 = inv_density_tpy(m, y[m->nsp] * o->Tref, *P, y);
   reactor_set_sources(&r, y, *P, dpdt, esrc, ysrc);
+  if (!o->solve_temperature) r.fixT = y[nsp];
   cvmem cv;
   memset(&cv, 0, sizeof(cv));
   cv.r = &r;
+  cv.o = o;
   cv.N = N;
-  double* w = (double*)calloc((size_t)(L_MAX + 8) * N + 2 * (size_t)N * N, sizeof(double));
-  for (int j = 0; j < L_MAX; ++j) cv.zn[j] = w + (size_t)j * N;
-  cv.ewt = w + (size_t)L_MAX * N;
-  cv.acor = cv.ewt + N;
-  cv.tempv = cv.acor + N;
-  cv.ftemp = cv.tempv + N;
-  cv.y = cv.ftemp + N;
-  cv.atolv = cv.y + N;
-  cv.delta = cv.atolv + N;
-  cv.savedJ = cv.delta + 2 * N;
-  cv.M = cv.savedJ + (size_t)N * N;
-  cv.piv = (int*)calloc((size_t)N, sizeof(int));
-  for (int i = 0; i < N; ++i) cv.atolv[i] = o->atol; /* abstol_dens = 0 */
+  cv.nsp = nsp;
+  cv.direct = o->dense == 1 && o->iterative == 0;
+  double* w = (double*)calloc((size_t)(L_MAX + 8 + SPGMR_MAXL + 1 + 4) * Nf +
+                                  2 * (size_t)Nf * Nf, sizeof(double));
+  for (int j = 0; j < L_MAX; ++j) cv.zn[j] = w + (size_t)j * Nf;
+  cv.ewt = w + (size_t)L_MAX * Nf;
+  cv.acor = cv.ewt + Nf;
+  cv.tempv = cv.acor + Nf;
+  cv.ftemp = cv.tempv + Nf;
+  cv.y = cv.ftemp + Nf;
+  cv.atolv = cv.y + Nf;
+  cv.delta = cv.atolv + Nf;
+  for (int j = 0; j <= SPGMR_MAXL; ++j) cv.V[j] = cv.delta + (size_t)(2 + j) * Nf;
+  cv.xcor = cv.V[SPGMR_MAXL] + Nf;
+  cv.vtemp = cv.xcor + Nf;
+  cv.work = cv.vtemp + Nf;
+  cv.jv = cv.work + Nf;
+  cv.savedJ = cv.jv + Nf;
+  cv.M = cv.savedJ + (size_t)Nf * Nf;
+  cv.piv = (int*)calloc((size_t)Nf, sizeof(int));
+  char* pat = jacobian_pattern(m, N);
+  cv.pat = pat;
+  /* absolute tolerances (CvodeSolver::Integrate, solver_cvode.cpp:74-80) times
+   * GetAbsoluteToleranceCorrection: with abstol_dens 1 / (rho0 / MW_j 1e-3)
+   * for the species, rho0 the density of the initial state */
+  for (int i = 0; i < N; ++i) cv.atolv[i] = o->atol;
+  if (o->abstol_dens) {
+    const double rho0 = 1.0 / inv_density_tpy(m, y[nsp] * o->Tref, *P, y);
+    for (int j = 0; j < nsp; ++j) {
+      const double molar_density = rho0 * (1.0 / m->mw[j]) * 1.0e-3;
+      cv.atolv[j] = o->atol * (1.0 / molar_density);
+    }
+  }
   cv.rtol = o->rtol;
   cv.nlscoef = o->nlscoef;
   cv.hmax_inv = (o->max_dt > 0.0) ? 1.0 / o->max_dt : 0.0;
   cv_reinit(&cv, 0.0, y);
-  /* ignition root function: the crossing of T0 + delta_T, T0 the input T */
-  const int ign = o->delta_temperature_ignition > 0.0;
+  /* ignition root function: the crossing of T0 + delta_T, T0 the input T
+   * (none when the temperature is held) */
+  const int ign = o->delta_temperature_ignition > 0.0 && o->solve_temperature;
   rootst rs;
   rs.Tign = T0 + o->delta_temperature_ignition;
   rs.Tref = o->Tref;
@@ -1522,6 +1895,7 @@ static int integrate_one(const zrko_mech* m, const zrko_opts* o, double* P, doub
     st->status = flag;
   }
   free(cv.piv);
+  free(pat);
   free(w);
   reactor_free(&r);
   return flag;

@@ -103,6 +103,21 @@ typedef struct {
   int constant_volume;     /* 0: constant pressure, 1: constant volume */
   double delta_temperature_ignition; /* > 0: locate T crossing T0 + this */
   int stop_after_ignition; /* 1: end the integration at the crossing */
+  /* 1: absolute tolerances scaled by the initial molar density of each
+   * species, atol MW_j 1e3 / rho0 (reactor_nvector_serial.cpp:107-120) */
+  int abstol_dens;
+  /* 0: the temperature is held at its initial value and is not a state
+   * variable (SetSolveTemperature(false), reactor_nvector_serial.cpp:69-101) */
+  int solve_temperature;
+  /* linear solver (solver_cvode.cpp:93-214, reactor_nvector_serial.cpp:
+   * 363-507): dense = 1 with iterative = 0 is CVODE's dense direct solver with
+   * the analytic Jacobian (analytic = 1) or CVODE's difference-quotient one;
+   * otherwise SPGMR (maxl 5, left preconditioning, modified Gram-Schmidt,
+   * tolerance factor eps_lin) preconditioned by the LU of I - gamma J, with
+   * off-diagonal entries |.| <= preconditioner_threshold dropped when
+   * iterative = 1, J analytic or the reactor's own divided differences */
+  int dense, analytic, iterative;
+  double eps_lin, preconditioner_threshold;
 } zrko_opts;
 
 typedef struct {

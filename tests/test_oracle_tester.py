@@ -114,3 +114,58 @@ def test_oracle_reproduces_reference_tester(oracle_mod, name):
     w2 = compare(o2, name, RUNS[name][0], traj2, fine2)
     print(f"{name} with the current gas constant: {w2:.2f}")
     assert w2 > 1.0
+
+
+# The tester's runs used the options file inputs/base_plugin.yml with, per run,
+# dense/analytic/iterative set by run.sh.in: "dense" = CVODE's dense direct
+# solver with its difference-quotient Jacobian, "sparse" = SPGMR (eps_lin 0.1)
+# preconditioned by the thresholded analytic Newton matrix.  Their timing logs
+# (reactor_timing_log) hold the CVode call count of every solve
+# (n_steps_avg: all 32 reactors are identical).
+SOLVER = {"dense": dict(dense=1, analytic=0, iterative=0),
+          "sparse": dict(dense=0, analytic=1, iterative=1)}
+THRESHOLD = {"h2": 3.2e-5, "dme": 6.4e-5, "nc7h16-skel": 1.28e-4}  # run.sh.in
+
+
+def solve_counts(oracle_mod, name, solver):
+    mech = (os.path.join(TESTER, f"{name}.mech"), os.path.join(TESTER, f"{name}.therm"))
+    o = oracle_mod.Oracle(*mech, ru=RU_2014)
+    Y0 = initial_mass_fractions(o, name, mech)
+    dt = 2.0 * IDT[name] / 10.0
+    oo = oracle_mod.default_opts(rtol=1e-8, atol=1e-18, constant_volume=1, max_steps=3000,
+                                 num_retries=3, delta_temperature_ignition=400.0, eps_lin=0.1,
+                                 preconditioner_threshold=THRESHOLD[name], **SOLVER[solver])
+    T, P, Y = np.array([1450.0]), np.array([2.02650e6]), Y0[None].copy()
+    out = []
+    for _ in range(10):
+        T, Y, st = o.integrate(T, P, Y, dt, oo)
+        P = st["P"]
+        out.append(int(st["nsteps"][0]))
+    return np.array(out)
+
+
+def reference_counts(run):
+    rows = [l.split() for l in open(os.path.join(TESTER, f"{run}_log.txt")) if l[0] != "#"]
+    return np.array([int(float(r[5])) for r in rows])
+
+
+@pytest.mark.parametrize("run", ["h2_dense", "h2_sparse", "dme_dense", "dme_sparse",
+                                 "nc7h16-skel_sparse"])
+def test_oracle_solver_modes_take_the_reference_steps(oracle_mod, run):
+    """The oracle's restatements of the reference's two solver configurations
+    take the reference's own number of CVode calls per solve: every solve
+    within max(4, 12 %) of the logged count and the ten-solve total within
+    5 % (the SPGMR runs match 5 to 9 of the 10 solves exactly; the largest
+    differences are in the solve that ignites, where step selection reacts to
+    round-off, and the logs come from an unstated SUNDIALS release and
+    LAPACK/SuperLU build).  The same scenario in the other configuration
+    misses the log by far more, so the counts tell the modes apart."""
+    name, solver = run.rsplit("_", 1)
+    ref = reference_counts(run)
+    got = solve_counts(oracle_mod, name, solver)
+    print(run, "oracle", got.tolist(), "reference", ref.tolist())
+    assert np.all(np.abs(got - ref) <= np.maximum(4, 0.12 * ref)), (got, ref)
+    assert abs(got.sum() - ref.sum()) <= 0.05 * ref.sum()
+    other = solve_counts(oracle_mod, name, "sparse" if solver == "dense" else "dense")
+    assert abs(other.sum() - ref.sum()) > 0.1 * ref.sum() or \
+        np.any(np.abs(other - ref) > np.maximum(4, 0.12 * ref)), other

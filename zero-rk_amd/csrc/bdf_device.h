@@ -48,12 +48,18 @@ constexpr int kQmax = 5;
 // Every entry is accumulated in FP64; MT = float stores the finished entries
 // in FP32 (the FP32 preconditioner keeps its Jacobian in FP32).
 // ---------------------------------------------------------------------------
+//
+// solveT = false (temperature held, not a state variable): the temperature
+// row and column are zero and the finite-difference evaluation is skipped,
+// as the reference's Jacobian without temperature has neither
+// (reactor_nvector_serial.cpp:641-643, 691-750).
 template <typename MT, bool kExt = false>
 __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s, double P,
                                                double v0, const BdfOpts& o, double y, double fy,
                                                double inv_rho_prev, MT* __restrict__ Jl, int ld,
                                                MT* __restrict__ saved,
-                                               double* __restrict__ t2l, RhsSrc src = {}) {
+                                               double* __restrict__ t2l, RhsSrc src = {},
+                                               bool solveT = true) {
   const int lane = lane_id();
   const int nsp = m.nsp;
   const bool sp = lane < nsp;
@@ -71,14 +77,18 @@ __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s
   const double Tp = yT + dT;
   dT = Tp - yT;
   const double t4in = (lane == nsp) ? Tp : (sp ? y : 0.0);
-  double t4;
+  double colT = 0.0;
   RhsAux aux2;
-  rhs64<kExt>(m, s, P, v0, o.Tref, o.step_limiter, t4in, t4, aux2, src);
-  const double colT = act ? (t4 - fy) * (1.0 / dT) : 0.0;
+  if (solveT) {
+    double t4;
+    rhs64<kExt>(m, s, P, v0, o.Tref, o.step_limiter, t4in, t4, aux2, src);
+    colT = act ? (t4 - fy) * (1.0 / dT) : 0.0;
+  }
   // floored evaluation: rates of progress for every species term
   double t3;
   RhsAux aux;
   rhs64<kExt>(m, s, P, v0, o.Tref, o.step_limiter, t1, t3, aux, src);
+  if (!solveT) aux2.inv_rho = aux.inv_rho;  // the last evaluation's
   const double Tdot = bcast(t3, nsp);
   const double RuTemp = kRu * yT;
   const double mult = -1.0 / aux.cp_mix;
@@ -129,10 +139,13 @@ __device__ __forceinline__ double jacobian_lds(const DevMech& m, const RhsLds& s
   for (int c = 0; c < nsp; ++c) {
     const double v = sp ? (double)Jl[c * ld + lane] : 0.0;
     // d(Tdot)/dy_c from the concentration-space column
-    const double sc = wave_sum(v * aux.h);
-    double val = sc * RuTemp;
-    val += Tdot * bcast(aux.cpm, c) * m.mw[c];
-    val *= m.inv_mw[c] * mult;
+    double val = 0.0;
+    if (solveT) {
+      const double sc = wave_sum(v * aux.h);
+      val = sc * RuTemp;
+      val += Tdot * bcast(aux.cpm, c) * m.mw[c];
+      val *= m.inv_mw[c] * mult;
+    }
     const double out = sp ? v * (mwi * m.inv_mw[c]) : (lane == nsp ? val : 0.0);
     if (act) Jl[c * ld + lane] = (MT)out;
     saved[c * kNP + lane] = (MT)out;
@@ -157,7 +170,8 @@ __device__ __forceinline__ double jacobian_team(const Team& team, const DevMech&
                                                 double inv_rho_prev, MT* __restrict__ J,
                                                 MT* __restrict__ saved, int js,
                                                 double* __restrict__ t2l,
-                                                double* __restrict__ hl, RhsSrc src = {}) {
+                                                double* __restrict__ hl, RhsSrc src = {},
+                                                bool solveT = true) {
   const int tid = team.tid();
   const int nsp = m.nsp;
   const bool sp = tid < nsp;
@@ -170,14 +184,18 @@ __device__ __forceinline__ double jacobian_team(const Team& team, const DevMech&
   double dT = yT * sqrt(DBL_EPSILON);
   const double Tp = yT + dT;
   dT = Tp - yT;
-  double t4;
+  double colT = 0.0;
   RhsAux aux2;
-  rhs_team(team, m, s, P, v0, o.Tref, o.step_limiter, (tid == nsp) ? Tp : (sp ? y : 0.0), t4,
-           aux2, src);
-  const double colT = act ? (t4 - fy) * (1.0 / dT) : 0.0;
+  if (solveT) {
+    double t4;
+    rhs_team(team, m, s, P, v0, o.Tref, o.step_limiter, (tid == nsp) ? Tp : (sp ? y : 0.0), t4,
+             aux2, src);
+    colT = act ? (t4 - fy) * (1.0 / dT) : 0.0;
+  }
   double t3;
   RhsAux aux;
   rhs_team(team, m, s, P, v0, o.Tref, o.step_limiter, t1, t3, aux, src);
+  if (!solveT) aux2.inv_rho = aux.inv_rho;  // the last evaluation's
   const double Tdot = team.from(t3, nsp);
   hl[tid] = sp ? aux.h : 0.0;
   if (tid < js)
@@ -220,7 +238,7 @@ __device__ __forceinline__ double jacobian_team(const Team& team, const DevMech&
   team.sync();
   // temperature row: thread c forms d(Tdot)/dy_c from column c
   double trow = 0.0;
-  if (sp) {
+  if (sp && solveT) {
     double acc = 0.0;
     for (int i = 0; i < nsp; ++i) acc += (double)J[tid * js + i] * hl[i];
     trow = acc * (kRu * yT);

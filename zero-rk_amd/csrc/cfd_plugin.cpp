@@ -1,21 +1,31 @@
 // The reference plugin API (include/zerork_cfd_plugin.h) over the HIP path.
 //
 // Option names, defaults and status codes follow
-// applications/cfd_plugin/zerork_reactor_manager.cpp:30-133 and
+// applications/cfd_plugin/zerork_reactor_manager.cpp:30-191 and
 // zerork_cfd_plugin.cpp:21-216.  A solve copies T/P/mf to HBM once, runs the
 // persistent BDF kernel over the whole batch (reactors handed out in
 // descending previous-cost order when sort_reactors is set, the reference's
 // cost sort of :522-524), copies back, and fills the aux outputs
 // (COST = CVode one-step calls, GPU = seconds per reactor of the batched
 // solve, TEMPERATURE_DELTA = T change above solve_temperature_threshold).
+// The linear solver follows the dense / analytic / iterative options (direct
+// or SPGMR with the thresholded preconditioner, analytic or difference-
+// quotient Jacobians), abstol_dens scales the absolute tolerances, and with
+// always_solve_temperature = 0 reactors whose temperature changed by less
+// than the threshold in their previous solve are integrated at fixed
+// temperature (:1109-1113).
 // The DPDT, E_SRC and Y_SRC aux fields are passed to the integrator as
 // per-reactor source terms (P is then returned as the end pressure); with
 // delta_temperature_ignition > 0 the IGNITION_TIME field receives each
 // reactor's time of crossing T0 + delta_temperature_ignition (0 if none), and
 // stop_after_ignition ends a reactor's integration there.
-// Reactors whose integration fails keep their input state and report a
-// negative cost; the call then returns ZERORK_STATUS_FAILED_SOLVE
-// (:1123-1139).  No C++ exception or exit() crosses the ABI.
+// Reactors whose batched integration fails are integrated again on their own
+// with the FP64 Newton matrix when the batch used the FP32 one (the manager's
+// CPU re-solve of a failed GPU batch, :1021-1034, 1090-1147); reactors that
+// still fail keep their input state and report a negative cost, and the call
+// returns ZERORK_STATUS_FAILED_SOLVE (:1123-1139).  Each solve appends a line
+// to the reactor_timing_log when output_performance_log is set.  No C++
+// exception or exit() crosses the ABI.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -24,6 +34,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <iomanip>
 #include <map>
 #include <memory>
 #include <numeric>
@@ -61,6 +72,14 @@ struct zerork_handle_impl {
   double* dysrc = nullptr;   // y_src (n*stride)
   size_t cap_n = 0, cap_y = 0, cap_src_n = 0, cap_src_y = 0;
   int n_calls = 0;           // solves so far (names the reactor dump files)
+  // the manager's own cost and temperature-change arrays when the caller
+  // supplies none (zerork_reactor_manager.cpp:503-518: kept across calls,
+  // new entries cost 1 and change 0)
+  std::vector<double> rc_own, tdelta_own;
+  double* dtdelta = nullptr;  // device copy of the temperature changes
+  size_t cap_td = 0;
+  std::ofstream timing_log;   // reactor_timing_log (zerork_reactor_manager.cpp:614-636)
+  bool timing_open = false;
 
   zerork_handle_impl() {
     sopt["mech_filename"] = "mech.dat";
@@ -114,6 +133,7 @@ struct zerork_handle_impl {
     if (dorder) (void)hipFree(dorder);
     if (dsrc_n) (void)hipFree(dsrc_n);
     if (dysrc) (void)hipFree(dysrc);
+    if (dtdelta) (void)hipFree(dtdelta);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -146,6 +166,21 @@ zerork_status_t finish_init(zerork_handle h) {
     if (!h->mech) {
       zerork_status_t f = load_mechanism(h);
       if (f != ZERORK_STATUS_SUCCESS) return f;
+    }
+    if (h->iopt["output_performance_log"] != 0) {
+      // the timing log's header (zerork_reactor_manager.cpp:614-636)
+      h->timing_log.open(h->sopt["reactor_timing_log_filename"]);
+      h->timing_open = h->timing_log.good();
+      if (h->timing_open) {
+        static const char* cols[] = {
+            "reactors_solved", "n_cpu", "n_cpu_notemp", "n_gpu", "n_gpu_notemp",
+            "n_gpu_groups", "n_steps_avg", "n_steps_avg_cpu", "n_steps_avg_gpu",
+            "max_time_cpu", "max_time_gpu", "step_time_cpu", "step_time_gpu",
+            "avg_time_total", "max_time_total"};
+        h->timing_log << "#" << std::setw(11) << "solve_number";
+        for (const char* c : cols) h->timing_log << std::setw(17) << c;
+        h->timing_log << std::endl;
+      }
     }
   } else if (!h->mech) {
     return ZERORK_STATUS_FAILED_MECHANISM_PARSE;
@@ -329,9 +364,48 @@ void dump_reactor(const zerork_handle_impl* h, const char* tag, int id, double T
   std::fclose(f);
 }
 
-zerork_status_t solve(zerork_handle h, double dt, int n, double* T, double* P, double* mf) {
+// The device options of a solve from the handle's plugin options.
+zrk_bdf_options solve_options(zerork_handle h, int nsp) {
+  zrk_bdf_options o;
+  zrk_bdf_default_options(&o);
+  o.rel_tol = h->dopt["rel_tol"];
+  o.abs_tol = h->dopt["abs_tol"];
+  o.max_dt = h->dopt["max_dt"];
+  o.nonlinear_convergence_coeff = h->dopt["nonlinear_convergence_coeff"];
+  o.step_limiter = h->dopt["step_limiter"];
+  o.reference_temperature = h->dopt["reference_temperature"];
+  o.min_mass_fraction = h->dopt["min_mass_fraction"];
+  o.cvode_retry_absolute_tolerance_adjustment = h->dopt["cvode_retry_absolute_tolerance_adjustment"];
+  o.cvode_retry_relative_tolerance_adjustment = h->dopt["cvode_retry_relative_tolerance_adjustment"];
+  o.max_steps = h->iopt["max_steps"];
+  o.cvode_num_retries = h->iopt["cvode_num_retries"];
+  o.constant_volume = h->iopt["constant_volume"] != 0;
+  // mechanisms above 63 species run on multi-wavefront teams, which keep the
+  // Newton matrix in FP32 only
+  o.fp32_preconditioner = h->iopt["fp32_preconditioner"] != 0 || nsp > 63;
+  o.delta_temperature_ignition = h->dopt["delta_temperature_ignition"];
+  o.stop_after_ignition = h->iopt["stop_after_ignition"] != 0;
+  // the linear solver (solver_cvode.cpp:93-214) and tolerances
+  // (reactor_nvector_serial.cpp:107-120, 414-493)
+  o.abstol_dens = h->iopt["abstol_dens"] != 0;
+  o.dense = h->iopt["dense"];
+  o.analytic = h->iopt["analytic"];
+  o.iterative = h->iopt["iterative"];
+  o.eps_lin = h->dopt["eps_lin"];
+  o.preconditioner_threshold = h->dopt["preconditioner_threshold"];
+  return o;
+}
+
+zerork_status_t solve(zerork_handle h, int n_cycle, double dt, int n, double* T, double* P,
+                      double* mf) {
   if (n < 0 || !T || !P || (n > 0 && !mf)) return ZERORK_STATUS_INVALID_POINTER;
   if (n == 0) return ZERORK_STATUS_SUCCESS;
+  // only CVODE (integrator 0) is restated; SEULEX, SODEX and RADAU are not
+  if (h->iopt["integrator"] != 0) {
+    std::fprintf(stderr, "zerork-hip: integrator %d is not available (only 0, CVODE)\n",
+                 h->iopt["integrator"]);
+    return ZERORK_STATUS_UNKNOWN_ERROR;
+  }
   zrk::MechHandle* m = h->mech.get();
   const int nsp = m->host.nsp;
   const int stride = h->iopt.count("num_species_stride") ? h->iopt["num_species_stride"] : nsp;
@@ -348,13 +422,20 @@ zerork_status_t solve(zerork_handle h, double dt, int n, double* T, double* P, d
     h->cap_y = ny;
   }
   std::vector<double> T0(T, T + n);
-  // work order: descending previous cost (LPT) when costs are supplied
+  // the manager's own cost and temperature-change arrays when the caller
+  // gives none: resized, keeping earlier values (SetInputVariables,
+  // zerork_reactor_manager.cpp:503-518)
+  if (!h->cost) h->rc_own.resize(n, 1.0);
+  if (!h->tdelta) h->tdelta_own.resize(n, 0.0);
+  double* rc = h->cost ? h->cost : h->rc_own.data();
+  double* tdelta = h->tdelta ? h->tdelta : h->tdelta_own.data();
+  // work order: descending previous cost (the reference's sort_reactors,
+  // :522-524, as longest-processing-time scheduling)
   std::vector<int> order(n);
   std::iota(order.begin(), order.end(), 0);
-  const bool sorted = h->iopt["sort_reactors"] != 0 && h->cost != nullptr;
+  const bool sorted = h->iopt["sort_reactors"] != 0;
   if (sorted)
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int a, int b) { return h->cost[a] > h->cost[b]; });
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return rc[a] > rc[b]; });
   // reactor dumps (zerork_reactor_manager.cpp:837, 904-906, 1042-1045, 1124-1128):
   // ids are positions in the solve order
   h->n_calls++;
@@ -362,16 +443,32 @@ zerork_status_t solve(zerork_handle h, double dt, int n, double* T, double* P, d
   if (dump)
     for (int j = 0; j < n; ++j) {
       const int k = order[j];
-      dump_reactor(h, "pre", j, T[k], P[k], h->cost ? h->cost[k] : 0.0, h->gpu ? h->gpu[k] : 0.0,
+      dump_reactor(h, "pre", j, T[k], P[k], rc[k], h->gpu ? h->gpu[k] : 0.0,
                    mf + (size_t)k * stride, nsp);
     }
   hipStream_t st = h->stream;
+  std::vector<double> P0(P, P + n);
   bool ok = hipMemcpyAsync(h->dT, T, sizeof(double) * n, hipMemcpyHostToDevice, st) == hipSuccess;
   ok = ok && hipMemcpyAsync(h->dP, P, sizeof(double) * n, hipMemcpyHostToDevice, st) == hipSuccess;
   ok = ok && hipMemcpyAsync(h->dY, mf, sizeof(double) * ny, hipMemcpyHostToDevice, st) == hipSuccess;
   if (sorted)
     ok = ok && hipMemcpyAsync(h->dorder, order.data(), sizeof(int) * n, hipMemcpyHostToDevice,
                               st) == hipSuccess;
+  // solve_temperature (zerork_reactor_manager.cpp:1109-1113): with
+  // always_solve_temperature = 0 a reactor is integrated at fixed temperature
+  // unless its temperature change of the previous solve is above 0
+  const int always_T = h->iopt["always_solve_temperature"] != 0;
+  if (!always_T) {
+    if ((size_t)n > h->cap_td) {
+      if (!grow(&h->dtdelta, n)) return ZERORK_STATUS_UNKNOWN_ERROR;
+      h->cap_td = n;
+    }
+    ok = ok && hipMemcpyAsync(h->dtdelta, tdelta, sizeof(double) * n, hipMemcpyHostToDevice,
+                              st) == hipSuccess;
+  }
+  int n_notemp = 0;
+  if (!always_T)
+    for (int k = 0; k < n; ++k) n_notemp += !(tdelta[k] > 0.0);
   // source terms from the DPDT, E_SRC and Y_SRC aux fields (per reactor; Y_SRC
   // in the layout of mf), as the manager passes them to InitializeState
   // (zerork_reactor_manager.cpp:909-1053)
@@ -413,37 +510,45 @@ zerork_status_t solve(zerork_handle h, double dt, int n, double* T, double* P, d
     }
   }
   if (!ok) return ZERORK_STATUS_UNKNOWN_ERROR;
-  zrk_bdf_options o;
-  zrk_bdf_default_options(&o);
-  o.rel_tol = h->dopt["rel_tol"];
-  o.abs_tol = h->dopt["abs_tol"];
-  o.max_dt = h->dopt["max_dt"];
-  o.nonlinear_convergence_coeff = h->dopt["nonlinear_convergence_coeff"];
-  o.step_limiter = h->dopt["step_limiter"];
-  o.reference_temperature = h->dopt["reference_temperature"];
-  o.min_mass_fraction = h->dopt["min_mass_fraction"];
-  o.cvode_retry_absolute_tolerance_adjustment = h->dopt["cvode_retry_absolute_tolerance_adjustment"];
-  o.cvode_retry_relative_tolerance_adjustment = h->dopt["cvode_retry_relative_tolerance_adjustment"];
-  o.max_steps = h->iopt["max_steps"];
-  o.cvode_num_retries = h->iopt["cvode_num_retries"];
-  o.constant_volume = h->iopt["constant_volume"] != 0;
-  // mechanisms above 63 species run on multi-wavefront teams, which keep the
-  // Newton matrix in FP32 only
-  o.fp32_preconditioner = h->iopt["fp32_preconditioner"] != 0 || nsp > 63;
-  o.delta_temperature_ignition = h->dopt["delta_temperature_ignition"];
-  o.stop_after_ignition = h->iopt["stop_after_ignition"] != 0;
+  zrk_bdf_options o = solve_options(h, nsp);
   const auto t0 = std::chrono::steady_clock::now();
-  int rc = zrk::integrate_device(m, o, n, dt, nullptr, h->dT, h->dP, h->dY, stride, h->dsteps,
-                                 nullptr, sorted ? h->dorder : nullptr, st, ddpdt, desrc, dysrc,
-                                 dign);
-  if (rc != 0) return ZERORK_STATUS_UNKNOWN_ERROR;
+  int rc_dev = zrk::integrate_device(m, o, n, dt, nullptr, h->dT, h->dP, h->dY, stride,
+                                     h->dsteps, nullptr, sorted ? h->dorder : nullptr, st, ddpdt,
+                                     desrc, dysrc, dign, always_T ? nullptr : h->dtdelta,
+                                     always_T);
+  if (rc_dev != 0) return ZERORK_STATUS_UNKNOWN_ERROR;
   std::vector<int> steps(n);
+  ok = hipMemcpyAsync(steps.data(), h->dsteps, sizeof(int) * n, hipMemcpyDeviceToHost, st) ==
+       hipSuccess;
+  if (hipStreamSynchronize(st) != hipSuccess || !ok) return ZERORK_STATUS_UNKNOWN_ERROR;
+  // a reactor whose batched integration failed is solved again on its own,
+  // as the manager re-solves a failed GPU batch on the CPU
+  // (zerork_reactor_manager.cpp:1021-1034, 1090-1147): here with the
+  // reference's own arithmetic, the FP64 Newton matrix, when the batch used
+  // the FP32 one (failed reactors kept their input state)
+  std::vector<int> failed;
+  for (int k = 0; k < n; ++k)
+    if (steps[k] < 0) failed.push_back(k);
+  int n_resolved = 0;
+  if (!failed.empty() && o.fp32_preconditioner && nsp <= 63) {
+    zrk_bdf_options o64 = o;
+    o64.fp32_preconditioner = 0;
+    ok = hipMemcpyAsync(h->dorder, failed.data(), sizeof(int) * failed.size(),
+                        hipMemcpyHostToDevice, st) == hipSuccess;
+    if (!ok) return ZERORK_STATUS_UNKNOWN_ERROR;
+    rc_dev = zrk::integrate_device(m, o64, (int)failed.size(), dt, nullptr, h->dT, h->dP, h->dY,
+                                   stride, h->dsteps, nullptr, h->dorder, st, ddpdt, desrc,
+                                   dysrc, dign, always_T ? nullptr : h->dtdelta, always_T);
+    if (rc_dev != 0) return ZERORK_STATUS_UNKNOWN_ERROR;
+    ok = hipMemcpyAsync(steps.data(), h->dsteps, sizeof(int) * n, hipMemcpyDeviceToHost, st) ==
+         hipSuccess;
+    if (hipStreamSynchronize(st) != hipSuccess || !ok) return ZERORK_STATUS_UNKNOWN_ERROR;
+    n_resolved = (int)failed.size();
+  }
   ok = hipMemcpyAsync(T, h->dT, sizeof(double) * n, hipMemcpyDeviceToHost, st) == hipSuccess;
   ok = ok && hipMemcpyAsync(mf, h->dY, sizeof(double) * ny, hipMemcpyDeviceToHost, st) == hipSuccess;
   if (o.constant_volume || srcs)  // end pressure (GetState)
     ok = ok && hipMemcpyAsync(P, h->dP, sizeof(double) * n, hipMemcpyDeviceToHost, st) == hipSuccess;
-  ok = ok && hipMemcpyAsync(steps.data(), h->dsteps, sizeof(int) * n, hipMemcpyDeviceToHost,
-                            st) == hipSuccess;
   if (ign)
     ok = ok && hipMemcpyAsync(h->ign, dign, sizeof(double) * n, hipMemcpyDeviceToHost, st) ==
                    hipSuccess;
@@ -452,13 +557,18 @@ zerork_status_t solve(zerork_handle h, double dt, int n, double* T, double* P, d
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   zerork_status_t flag = ZERORK_STATUS_SUCCESS;
   const double thr = h->dopt["solve_temperature_threshold"];
+  long long nsteps_total = 0;
+  int nsolved = 0;
   for (int k = 0; k < n; ++k) {
     if (steps[k] < 0) flag = ZERORK_STATUS_FAILED_SOLVE;
-    if (h->cost) h->cost[k] = steps[k];
+    rc[k] = steps[k];
     if (h->gpu) h->gpu[k] = secs / n;
-    if (h->tdelta && steps[k] >= 0) {
-      double d = T[k] - T0[k];
-      h->tdelta[k] = d < thr ? 0.0 : d;
+    if (steps[k] >= 0) {
+      // TEMPERATURE_DELTA: the change if at least the threshold, else 0
+      const double d = T[k] - T0[k];
+      tdelta[k] = d < thr ? 0.0 : d;
+      nsteps_total += steps[k];
+      ++nsolved;
     }
   }
   const bool dump_failed = h->iopt["dump_failed_reactors"] != 0;
@@ -467,9 +577,22 @@ zerork_status_t solve(zerork_handle h, double dt, int n, double* T, double* P, d
       const int k = order[j];
       const double* y = mf + (size_t)k * stride;
       if (dump_failed && steps[k] < 0)  // the kept input state
-        dump_reactor(h, "failed_state", j, T[k], P[k], steps[k], secs / n, y, nsp);
+        dump_reactor(h, "failed_state", j, T[k], P0[k], steps[k], secs / n, y, nsp);
       if (dump) dump_reactor(h, "postg", j, T[k], P[k], steps[k], secs / n, y, nsp);
     }
+  if (h->timing_open) {
+    // one line per solve (zerork_reactor_manager.cpp:1340-1376); the whole
+    // batch counts as one GPU group, the re-solved failures as CPU solves
+    const int n_gpu = nsolved, n_cpu = n_resolved;
+    const long long avg = nsolved > 0 ? nsteps_total / nsolved : 0;
+    h->timing_log << std::setw(13) << n_cycle << std::setw(17) << nsolved
+                  << std::setw(17) << n_cpu << std::setw(17) << 0 << std::setw(17) << n_gpu
+                  << std::setw(17) << n_notemp << std::setw(17) << 1 << std::setw(17) << avg
+                  << std::setw(17) << 0 << std::setw(17) << avg << std::setw(17) << 0
+                  << std::setw(17) << secs << std::setw(17) << 0
+                  << std::setw(17) << (nsteps_total > 0 ? secs / nsteps_total : 0.0)
+                  << std::setw(17) << secs << std::setw(17) << secs << std::endl;
+  }
   return flag;
 }
 
@@ -516,13 +639,12 @@ zerork_status_t zerork_reactor_load_mechanism(zerork_handle h) {
 zerork_status_t zerork_reactor_solve(const int n_cycle, const double time, const double dt,
                                      const int n_reactors, double* T, double* P, double* mf,
                                      zerork_handle h) {
-  (void)n_cycle;
   (void)time;
   if (!h) return ZERORK_STATUS_INVALID_HANDLE;
   try {
     zerork_status_t f = finish_init(h);
     if (f != ZERORK_STATUS_SUCCESS) return f;
-    return solve(h, dt, n_reactors, T, P, mf);
+    return solve(h, n_cycle, dt, n_reactors, T, P, mf);
   } catch (...) {
     return ZERORK_STATUS_UNKNOWN_ERROR;
   }

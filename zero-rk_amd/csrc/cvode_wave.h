@@ -59,7 +59,9 @@ static_assert(sizeof(CvUni) % 16 == 0, "CvUni keeps the LDS layout 16-byte align
 // residual, error test and state update still in FP64).
 // Team: TeamWave (one wavefront per reactor, nsp <= 63) or TeamBlock<W> (W
 // wavefronts per reactor, nsp < 64W, FP32 matrices only).
-template <typename MT, class Team = TeamWave>
+// kIter: the Krylov solver (SPGMR, the plugin's iterative option) is compiled
+// in; its instantiations are used exactly when BdfOpts::krylov is set.
+template <typename MT, class Team = TeamWave, bool kIter = false>
 struct Cvode {
   static constexpr int kL = Team::kThreads;        // state slots = matrix row registers
   static constexpr bool kWaveTeam = (kL == kWave);
@@ -95,12 +97,16 @@ struct Cvode {
   double* t2l;     // LDS [kL]: Jacobian column factors
   double* hl;      // LDS [kL]: energies for the Jacobian's temperature row (teams of
                    // several wavefronts)
+  double* kry;     // LDS [kKry] per wavefront (kIter): the Krylov solver's Givens
+                   // rotations, triangular factor and least-squares right side
   // lane vectors
   double zn[kQmax + 1];
   double ewt, acor, tempv, ftemp, y, atolv, delta;
   double ysrc;  // this lane's mass-fraction source (when has_ysrc)
-  bool active;  // species lane or temperature lane
+  bool active;  // species lane, or the temperature lane when it is solved for
   bool has_src, has_ysrc;  // uniform: any source term / mass-fraction sources
+  bool solveT;  // uniform: the temperature is a state variable (else held fixed)
+  int nvar;     // uniform: state variables (nsp + 1, or nsp when T is held)
   // uniform FP64 scalars: kept in this wavefront's CvUni block in LDS (every
   // lane holds the same values), so they occupy no vector registers across
   // the rate evaluation and the matrix inversion
@@ -472,9 +478,13 @@ struct Cvode {
       nje++;
       nstlj = nst;
       ZRK_TIC(t0);
-      inv_rho_last = jacobian_lds<MT, Team::kExt>(mech, s, P, v0, opt, ypred, fpred,
-                                                  inv_rho_last, Mlds, N, savedJ, t2l,
-                                                  sources(tn));
+      if (opt.jac == kJacAnalytic) {
+        inv_rho_last = jacobian_lds<MT, Team::kExt>(mech, s, P, v0, opt, ypred, fpred,
+                                                    inv_rho_last, Mlds, N, savedJ, t2l,
+                                                    sources(tn), solveT);
+      } else if (dq_jacobian(ypred, fpred)) {
+        return 1;  // the RHS failed at a perturbed state: recoverable
+      }
       ZRK_TOC(t0, 1);
     }
     if constexpr (kWaveMfmaInverse) {
@@ -485,8 +495,7 @@ struct Cvode {
       team.sync();  // every lane is done with the Jacobian workspace
       if (lane < N) {
         for (int c = 0; c < N; ++c) {
-          double v = ng * (double)savedJ[c * kNP + lane];
-          if (c == lane) v += 1.0;
+          const double v = newton_entry(ng * (double)savedJ[c * kNP + lane], lane, c);
           Mlds[c * N + lane] = (MT)v;
         }
       }
@@ -505,7 +514,8 @@ struct Cvode {
 #pragma unroll
     for (int c = 0; c < kNP; ++c) {
       double v = (c <= nsp && lane <= nsp) ? ng * (double)R[c] : 0.0;
-      if (c == lane) v += 1.0;
+      if (c <= nsp && lane <= nsp) v = newton_entry(v, lane, c);
+      else if (c == lane) v += 1.0;
       R[c] = (MT)v;
     }
     const bool ok = gj_invert<MT>(R, N);
@@ -530,25 +540,33 @@ struct Cvode {
     const int nsp = mech.nsp;
     const int N = nsp + 1;
     const int tid = team.tid();
+    bool copy = true;
     if (!jacobian_is_bad(cf)) {
       jc = 0;
-      if (tid < N)
-        for (int c = 0; c < N; ++c) Mlds[c * kJS + tid] = savedJ[c * kJS + tid];
     } else {
       jc = 1;
       nje++;
       nstlj = nst;
       ZRK_TIC(t0);
-      inv_rho_last = jacobian_team<MT>(team, mech, s, P, v0, opt, ypred, fpred, inv_rho_last,
-                                       Mlds, savedJ, kJS, t2l, hl, sources(tn));
+      if (opt.jac == kJacAnalytic) {
+        inv_rho_last = jacobian_team<MT>(team, mech, s, P, v0, opt, ypred, fpred, inv_rho_last,
+                                         Mlds, savedJ, kJS, t2l, hl, sources(tn), solveT);
+        copy = false;
+      } else if (dq_jacobian(ypred, fpred)) {
+        return 1;  // the RHS failed at a perturbed state: recoverable
+      }
       ZRK_TOC(t0, 1);
+    }
+    if (copy) {
+      team.sync();
+      if (tid < N)
+        for (int c = 0; c < N; ++c) Mlds[c * kJS + tid] = savedJ[c * kJS + tid];
     }
     ZRK_TIC(t1);
     const double ng = -gamma;
     if (tid < N)
       for (int c = 0; c < N; ++c) {
-        double v = ng * (double)Mlds[c * kJS + tid];
-        if (c == tid) v += 1.0;
+        const double v = newton_entry(ng * (double)Mlds[c * kJS + tid], tid, c);
         Mlds[c * kJS + tid] = (MT)v;
       }
     bool ok;
@@ -558,6 +576,57 @@ struct Cvode {
       ok = gj_invert_lds<MT>(team, Mlds, kJS, N, prow);
     ZRK_TOC(t1, 2);
     return ok ? 0 : 1;
+  }
+
+  // Entry (row, col) of the Newton matrix from v = -gamma J_row,col: the
+  // direct solver's I - gamma J, or the Krylov solver's preconditioner
+  // (FormPreconditioner, reactor_nvector_serial.cpp:414-493): the diagonal
+  // and, unless every entry is kept, only entries of the Jacobian's pattern
+  // with |v| > threshold.  Rows and columns < nsp + 1.
+  __device__ __forceinline__ double newton_entry(double v, int row, int col) const {
+    if (row == col) return v + 1.0;
+    if (opt.pfull) return v;
+    const bool in = (mech.jpat[row * 3 + (col >> 6)] >> (col & 63)) & 1ull;
+    return (in && fabs(v) > opt.pthr) ? v : 0.0;
+  }
+
+  // Jacobian by difference quotients into savedJ (column c at savedJ[c * kJS
+  // + row]), one RHS per column: CVODE's dense quotients (cvLsDenseDQJac:
+  // inc_c = max(sqrt(uround) |y_c|, minInc / ewt_c), minInc = 1000 |h|
+  // uround N wrms(f), column (f(y + inc e_c) - f) / inc as (1/inc)(f' - f)),
+  // or the reactor's divided differences (reactor_nvector_serial.cpp:
+  // 816-835: a fresh f(y), delta_c = sqrt(1e-16 max(1e-5, |y_c|)), column
+  // (f' - f) / delta).  Columns of held variables are zero.  Returns nonzero
+  // when the RHS fails.
+  __device__ __noinline__ int dq_jacobian(double ypred, double fpred) {
+    const int lane = team.tid();
+    const int N = mech.nsp + 1;
+    double f0 = fpred;
+    double minInc = 1.0;
+    if (opt.jac == kJacCvodeDQ) {
+      const double fnorm = wrms(fpred);
+      if (fnorm != 0.0) minInc = 1000.0 * fabs(h) * DBL_EPSILON * (double)nvar * fnorm;
+    } else {
+      if (f(tn, ypred, f0) > 0) return 1;
+    }
+    const double srur = sqrt(DBL_EPSILON);
+    for (int c = 0; c < N; ++c) {
+      double col = 0.0;
+      if (c < nvar) {
+        double incl = 0.0;
+        if (lane == c)
+          incl = (opt.jac == kJacCvodeDQ) ? fmax(srur * fabs(ypred), minInc / ewt)
+                                          : sqrt(1.0e-16 * fmax(1.0e-5, fabs(ypred)));
+        const double inc = team.from(incl, c);
+        double fp;
+        if (f(tn, lane == c ? ypred + inc : ypred, fp) > 0) return 1;
+        if (active)
+          col = (opt.jac == kJacCvodeDQ) ? (1.0 / inc) * (fp - f0) : (fp - f0) / inc;
+      }
+      if (lane < N) savedJ[c * kJS + lane] = (MT)col;
+    }
+    team.sync();
+    return 0;
   }
 
   // x = M^{-1} b with the inverse in LDS (for one wavefront the entries of b
@@ -590,6 +659,181 @@ struct Cvode {
     team.sync();
     ZRK_TOC(t0, 3);
     return (double)acc;
+  }
+
+  // ---- the Krylov solver (SUNLinSol_SPGMR as the plugin configures it:
+  // maxl 5, left preconditioning, modified Gram-Schmidt, no restarts, scaling
+  // by ewt; cvLsSolve's small-residual shortcut and tolerance; cvLsATimes
+  // with cvLsDQJtimes).  The Krylov vectors are lane registers; the Givens
+  // rotations, the triangular factor and the least-squares right side are
+  // uniform and kept in this wavefront's kry block of LDS.
+  static constexpr int kMaxl = 5;
+  static constexpr int kKry = 2 * kMaxl + (kMaxl + 1) * kMaxl + kMaxl + 1;
+
+  // z = v - gamma J v, J v = (f(y + sig v) - f(y)) / sig at the Newton
+  // iterate (y, ftemp), sig = 1 / wrms(v) shrunk by 4 on a failed RHS (3
+  // tries); nonzero on failure
+  __device__ __forceinline__ int atimes(double v, double& z) {
+    double sig = 1.0 / wrms(v);
+    double fv = 0.0;
+    int rv = 1;
+    for (int it = 0; it < 3; ++it) {
+      rv = f(tn, sig * v + y, fv);
+      if (rv == 0) break;
+      sig *= 0.25;
+    }
+    if (rv != 0) return 1;
+    const double jv = (1.0 / sig) * (fv - ftemp);
+    z = active ? -gamma * jv + v : 0.0;
+    return 0;
+  }
+
+  // b <- the Newton correction; lflag: 0, or 1 for a recoverable failure
+  // (no convergence, or a reduced residual after the first Newton iteration)
+  __device__ double lsolve_krylov(double b, int curiter, int& lflag) {
+    const int lane = team.tid();
+    double* G = kry;                        // rotations (c, s) [2 kMaxl]
+    double* R = kry + 2 * kMaxl;            // R[i * kMaxl + j]
+    double* Yg = R + (kMaxl + 1) * kMaxl;   // [kMaxl + 1]
+    lflag = 0;
+    const double deltar = opt.eps_lin * tq[4];
+    if (wrms(b) <= deltar) return curiter > 0 ? 0.0 : b;
+    const double delta = deltar * sqrt((double)nvar);
+    const double w = active ? ewt : 0.0;
+    double V[kMaxl + 1] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    V[0] = w * lsolve(b);
+    const double beta = sqrt(team.sum(V[0] * V[0]));
+    if (beta <= delta) return 0.0;
+    V[0] *= 1.0 / beta;
+    double rot = 1.0, rho = beta;
+    int krydim = 0;
+    bool conv = false, fail = false;
+    // the iterations run as a loop (one copy of the inlined RHS); every
+    // register array is indexed at compile time through predicated loops
+    for (int l = 0; l < kMaxl && !conv && !fail; ++l) {
+      krydim = l + 1;
+      double vl = 0.0;
+#pragma unroll
+      for (int k = 0; k < kMaxl; ++k)
+        if (k == l) vl = V[k];
+      double z = 0.0;
+      fail = atimes(active ? vl / ewt : 0.0, z) != 0;
+      double vk = fail ? 0.0 : w * lsolve(z);
+      // SUNModifiedGS: column l of the Hessenberg matrix
+      const double vk_norm = sqrt(team.sum(vk * vk));
+      double hc[kMaxl + 1];
+#pragma unroll
+      for (int i = 0; i <= kMaxl; ++i) {
+        hc[i] = 0.0;
+        if (i <= l) {
+          hc[i] = team.sum(V[i] * vk);
+          vk = vk - hc[i] * V[i];
+        }
+      }
+      double nn = sqrt(team.sum(vk * vk));
+      double t = 1000.0 * vk_norm;
+      if ((t + nn) == t) {  // reorthogonalise
+        double nn2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < kMaxl; ++i) {
+          if (i <= l) {
+            const double np = team.sum(V[i] * vk);
+            t = 1000.0 * hc[i];
+            if ((t + np) != t) {
+              hc[i] += np;
+              vk = vk - np * V[i];
+              nn2 += np * np;
+            }
+          }
+        }
+        if (nn2 != 0.0) {
+          const double np = nn * nn - nn2;
+          nn = np > 0.0 ? sqrt(np) : 0.0;
+        }
+      }
+      // SUNQRfact: the earlier rotations on this column, then a new one
+      // annihilating its subdiagonal entry nn
+#pragma unroll
+      for (int k = 0; k < kMaxl - 1; ++k) {
+        if (k < l) {
+          const double c = G[2 * k], sn = G[2 * k + 1];
+          const double t1 = hc[k], t2 = hc[k + 1];
+          hc[k] = c * t1 - sn * t2;
+          hc[k + 1] = sn * t1 + c * t2;
+        }
+      }
+      double t1 = 0.0;
+#pragma unroll
+      for (int k = 0; k < kMaxl; ++k)
+        if (k == l) t1 = hc[k];
+      const double t2 = nn;
+      double c, sn;
+      if (t2 == 0.0) {
+        c = 1.0;
+        sn = 0.0;
+      } else if (fabs(t2) >= fabs(t1)) {
+        const double q = t1 / t2;
+        sn = -1.0 / sqrt(1.0 + q * q);
+        c = -sn * q;
+      } else {
+        const double q = t2 / t1;
+        c = 1.0 / sqrt(1.0 + q * q);
+        sn = -c * q;
+      }
+      const double rll = c * t1 - sn * t2;
+      team.sync();
+      if (lane == 0) {
+        G[2 * l] = c;
+        G[2 * l + 1] = sn;
+#pragma unroll
+        for (int i = 0; i < kMaxl; ++i)
+          if (i < l) R[i * kMaxl + l] = hc[i];
+        R[l * kMaxl + l] = rll;
+      }
+      team.sync();
+      fail = fail || rll == 0.0;  // SUNLS_QRFACT_FAIL: recoverable
+      rot *= sn;
+      rho = fabs(rot * beta);
+      if (rho <= delta) {
+        conv = true;
+      } else if (!fail) {
+        vk *= 1.0 / nn;
+      }
+#pragma unroll
+      for (int k = 1; k <= kMaxl; ++k)
+        if (k == l + 1) V[k] = vk;
+    }
+    if (fail) {
+      lflag = 1;
+      return 0.0;
+    }
+    // SUNQRsol: y = R^{-1} Q (beta e_0), uniform work on one lane, then shared
+    if (lane == 0) {
+      Yg[0] = beta;
+      for (int i = 1; i <= krydim; ++i) Yg[i] = 0.0;
+      for (int k = 0; k < krydim; ++k) {
+        const double c = G[2 * k], sn = G[2 * k + 1];
+        const double t1 = Yg[k], t2 = Yg[k + 1];
+        Yg[k] = c * t1 - sn * t2;
+        Yg[k + 1] = sn * t1 + c * t2;
+      }
+      for (int k = krydim - 1; k >= 0; --k) {
+        Yg[k] /= R[k * kMaxl + k];
+        for (int i = 0; i < k; ++i) Yg[i] -= Yg[k] * R[i * kMaxl + k];
+      }
+    }
+    team.sync();
+    double xc = 0.0;
+#pragma unroll
+    for (int k = 0; k < kMaxl; ++k)
+      if (k < krydim) xc += Yg[k] * V[k];
+    team.sync();
+    if (conv || rho < beta) {
+      lflag = conv ? 0 : (curiter == 0 ? 0 : 1);
+      return active ? xc / ewt : 0.0;
+    }
+    lflag = 1;
+    return 0.0;
   }
 
   // cvNlsLSetup
@@ -643,8 +887,18 @@ struct Cvode {
         for (;;) {
           nni++;
           double b = -delta;
-          b = lsolve(b);
-          if (gamrat != 1.0) b *= 2.0 / (1.0 + gamrat);
+          if constexpr (kIter) {
+            // cvLsSolve with SPGMR: no rescaling for a lagged gamma
+            int lflag;
+            b = lsolve_krylov(b, curiter, lflag);
+            if (lflag) {
+              rv = kNlsConvRecvr;
+              break;
+            }
+          } else {
+            b = lsolve(b);
+            if (gamrat != 1.0) b *= 2.0 / (1.0 + gamrat);
+          }
           if (!active) b = 0.0;
           delta = b;
           ycor += delta;
@@ -1005,7 +1259,7 @@ struct Cvode {
   }
 };
 
-template <typename MT, bool E = false>
-using CvWave = Cvode<MT, TeamWaveX<E>>;
+template <typename MT, bool E = false, bool I = false>
+using CvWave = Cvode<MT, TeamWaveX<E>, I>;
 
 }  // namespace zrk

@@ -33,6 +33,25 @@ struct BdfOpts {
   double ign_dT = 0.0;
   int stop_after_ignition = 0;
   double* ign_time = nullptr;
+  // absolute tolerances scaled by the initial molar densities
+  int abstol_dens = 0;
+  // linear solver: 0 the direct solve of I - gamma J; 1 SPGMR preconditioned
+  // by it (pfull: every entry kept, else the Jacobian's pattern and
+  // |entry| > pthr off the diagonal), tolerance factor eps_lin
+  int krylov = 0;
+  int jac = 0;  // kJacAnalytic, kJacCvodeDQ or kJacReactorDQ
+  int pfull = 1;
+  double pthr = 0.0, eps_lin = 0.0;
+  // fixed temperature: with always_T = 0 a reactor whose tdelta entry is not
+  // above 0 keeps its temperature (not a state variable)
+  int always_T = 1;
+  const double* tdelta = nullptr;
+};
+
+enum JacKind : int {
+  kJacAnalytic = 0,   // reactor_nvector_serial.cpp:610-771
+  kJacCvodeDQ = 1,    // CVODE's dense difference quotients (cvLsDenseDQJac)
+  kJacReactorDQ = 2,  // the reactor's divided differences (:816-835)
 };
 
 enum RecFlags : int {
@@ -104,6 +123,11 @@ struct DevMech {
   // by column, then destruction before creation, each in step order, then the
   // non-integer terms (the reference's accumulation order)
   const uint4* jr;
+  // the analytic Jacobian's pattern (nsp <= 191), for the Krylov solver's
+  // preconditioner: bit c % 64 of jpat[row * 3 + c / 64] is set when column c
+  // of that row is in the pattern (diagonal, temperature row and column,
+  // reactant columns against reactant and product rows, non-integer terms)
+  const unsigned long long* jpat;
 };
 
 }  // namespace zrk

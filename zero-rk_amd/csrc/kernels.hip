@@ -4,25 +4,20 @@
 //                    for a batch of reactor states (any mechanism size):
 //                    one wavefront per reactor, 4 per workgroup.
 //   jacobian_kernel  reference analytic Jacobian for nsp <= 63 (parity tests).
-//   bdf_kernel       persistent per-reactor BDF integrator (nsp <= 63): each
-//                    wavefront takes reactors from a device work counter until
-//                    the batch is exhausted.
+//   launch_bdf       the persistent per-reactor BDF integrator (bdf_kernels.h;
+//                    its instantiations are compiled in bdf_{wave,team}_
+//                    {direct,krylov}.hip).
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
 
+#include "bdf_launch.h"
 #include "cvode_wave.h"
 #include "launch.h"
 
 namespace zrk {
 
 constexpr int kRatesWaves = 4;  // wavefronts per workgroup in rates_kernel
-
-// LDS bytes of the one-wavefront integrator's (nsp+1) x (nsp+1) matrix of
-// element size mt, padded to 16 bytes.
-__host__ __device__ constexpr size_t wave_matrix_bytes(int nsp, size_t mt) {
-  return ((size_t)(nsp + 1) * (size_t)(nsp + 1) * mt + 15) & ~(size_t)15;
-}
 
 // ---------------------------------------------------------------------------
 // rates_kernel: inputs T[n], P[n], Y[n*stride] (reactor-major AoS, as the
@@ -167,282 +162,6 @@ jacobian_team_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ Tin
   }
 }
 
-// ---------------------------------------------------------------------------
-// Persistent per-reactor BDF: each team (one wavefront, or W wavefronts of a
-// workgroup) takes reactors from *counter until the batch is exhausted.
-// T[n], Y[n*stride] updated in place (and P[n] at constant volume);
-// nsteps[n] = plugin cost (CVode one-step calls, negative on failure);
-// stats[n*8] (optional): nst nfe nje nsetups nni ncfn netf status.
-// ---------------------------------------------------------------------------
-template <typename MT, class Team>
-__device__ __forceinline__ void bdf_driver(Cvode<MT, Team>& cv, const DevMech& m,
-                                           const BdfOpts& o, int n, double dt_all,
-                                           const double* __restrict__ dts,
-                                           double* __restrict__ Tio, double* __restrict__ Pio,
-                                           double* __restrict__ Yio, int stride,
-                                           int* __restrict__ nsteps_out,
-                                           int* __restrict__ stats_out,
-                                           const int* __restrict__ order,
-                                           unsigned int* __restrict__ counter) {
-  const int lane = cv.team.tid();
-  const int nsp = m.nsp;
-  cv.mech = m;
-  cv.opt = o;
-  cv.active = lane <= nsp;
-  cv.has_src = o.dpdt || o.esrc || o.ysrc;
-  cv.has_ysrc = o.ysrc != nullptr;
-  cv.inv_nvar = 1.0 / (double)(nsp + 1);
-  cv.nlscoef = o.nlscoef;
-  cv.hmax_inv = o.hmax_inv;
-  for (;;) {
-    unsigned int r = 0;
-    if (lane == 0) r = atomicAdd(counter, 1u);
-    r = (unsigned int)cv.team.from((double)r, 0);
-    if (r >= (unsigned int)n) break;
-    if (order) r = (unsigned int)order[r];
-    const double dt = dts ? dts[r] : dt_all;
-    double y0 = 0.0;
-    if (lane < nsp) y0 = Yio[(size_t)r * stride + lane];
-    if (lane == nsp) y0 = Tio[r] / o.Tref;
-    cv.P = Pio[r];
-    cv.v0 = 0.0;
-    if (o.constant_volume) {
-      // ReactorConstantVolumeCPU::InitializeState: the volume of the initial
-      // state (reactor_constant_volume_cpu.cpp:28-29)
-      const double mws = cv.team.sum(lane < nsp ? y0 * m.inv_mw[lane] : 0.0);
-      cv.v0 = 1.0 / (cv.P * (1.0 / mws) / (kRu * (cv.team.from(y0, nsp) * o.Tref)));
-    }
-    // source terms (InitializeState, reactor_constant_pressure_cpu.cpp:15-42,
-    // reactor_constant_volume_cpu.cpp:15-42): the initial mass enthalpy or
-    // internal energy and 1/density
-    cv.dpdt = (o.dpdt ? o.dpdt[r] : 0.0);
-    cv.esrc = (o.esrc ? o.esrc[r] : 0.0);
-    cv.ysrc = (o.ysrc && lane < nsp) ? o.ysrc[(size_t)r * stride + lane] : 0.0;
-    cv.e0 = cv.vinit = 0.0;
-    if (cv.has_src) {
-      const double T0 = cv.team.from(y0, nsp) * o.Tref;
-      const double Yj = lane < nsp ? y0 : 0.0;
-      const double mws = cv.team.sum(lane < nsp ? y0 * m.inv_mw[lane] : 0.0);
-      cv.vinit = 1.0 / (cv.P * (1.0 / mws) / (kRu * T0));
-      double e0, c;
-      team_mass_energy(cv.team, m, o.constant_volume != 0, false, T0, Yj, e0, c);
-      cv.e0 = e0;
-    }
-    cv.atolv = cv.active ? o.atol : 0.0;
-    cv.rtol = o.rtol;
-    cv.inv_rho_last = 0.0;
-#ifdef ZRK_PHASE_CYCLES
-    for (int k = 0; k < 12; ++k) cv.cyc[k] = 0;
-    const long long tr0 = (long long)__builtin_amdgcn_s_memtime();
-#endif
-    cv.reinit(0.0, y0);
-    // ignition root function (reactor_nvector_serial.cpp:774-784): the
-    // crossing of T0 + delta_T, T0 the input temperature (compiled into the
-    // kExt variants only)
-    const bool ign = Team::kExt && o.ign_dT > 0.0;
-    double troot_last = 0.0;
-    bool irfnd = false;
-    // root returns per reactor are bounded (an interpolant of degree <= 5 has
-    // at most 5 roots per step; round-off flips of g near a double root must
-    // not keep a wavefront here)
-    constexpr int kMaxRootReturns = 64;
-    int nroots = 0;
-    if (ign) {
-      cv.rt_Tign = Tio[r] + o.ign_dT;
-      cv.root_init();
-    }
-    int nsteps = 0, num_tries = 0, flag = kCvSuccess;
-    int tot[7] = {0, 0, 0, 0, 0, 0, 0};
-    double tcurr = 0.0;
-    // one CVode call per iteration, CV_ONE_STEP (solver_cvode.cpp:228-266)
-    while (tcurr < dt) {
-      if (ign && cv.nst > 0) {
-        // a later call: after a root return, roots in the rest of the last
-        // step, else the step's end tn, which was not returned yet
-        if (irfnd) cv.root_check2();
-        const double troundoff = 100.0 * DBL_EPSILON * (fabs(cv.tn) + fabs(cv.h));
-        if (fabs(cv.tn - cv.rt_tret) > troundoff) {
-          nsteps += 1;
-          double troot;
-          if (nroots < kMaxRootReturns && cv.root_check3(troot)) {
-            ++nroots;
-            irfnd = true;
-            tcurr = cv.rt_tret = troot_last = troot;
-            if (o.stop_after_ignition) break;
-            continue;
-          }
-          irfnd = false;
-          tcurr = cv.rt_tret = cv.tn;
-          continue;
-        }
-      }
-      flag = cv.one_step(dt, tcurr);
-      nsteps += 1;
-      if (cv.nst >= o.max_steps) flag = kCvTooMuchWork;
-      if (flag < 0) {
-        num_tries += 1;
-        cv.atolv *= o.retry_atol_adj;
-        cv.rtol *= o.retry_rtol_adj;
-        tot[0] += cv.nst; tot[1] += cv.nfe; tot[2] += cv.nje; tot[3] += cv.nsetups;
-        tot[4] += cv.nni; tot[5] += cv.ncfn; tot[6] += cv.netf;
-        if (num_tries == o.num_retries) break;
-        cv.reinit(tcurr, cv.zn[0]);
-        if (ign) {
-          cv.root_init();
-          irfnd = false;
-        }
-      } else if (ign) {
-        cv.rt_tret = cv.tn;
-        double troot;
-        if (nroots < kMaxRootReturns && cv.root_check3(troot)) {  // returned before tn
-          ++nroots;
-          irfnd = true;
-          tcurr = cv.rt_tret = troot_last = troot;
-          if (o.stop_after_ignition) break;
-        }
-      }
-    }
-    // failed reactors keep their input state (zerork_reactor_manager.cpp:1123-1139)
-    if (flag == kCvSuccess) {
-      if (ign && o.ign_time && lane == 0) o.ign_time[r] = troot_last;
-      // the state at dt, or at the ignition time when stopped there
-      const double yout = cv.dky0(tcurr < dt ? tcurr : dt);
-      tot[0] += cv.nst; tot[1] += cv.nfe; tot[2] += cv.nje; tot[3] += cv.nsetups;
-      tot[4] += cv.nni; tot[5] += cv.ncfn; tot[6] += cv.netf;
-      if (lane < nsp) Yio[(size_t)r * stride + lane] = yout;
-      if (lane == nsp) Tio[r] = yout * o.Tref;
-      // GetState (reactor_constant_volume_cpu.cpp:44-63,
-      // reactor_constant_pressure_cpu.cpp:44-66): constant volume P = Ru T /
-      // (v Wmix) + dpdt t; constant pressure P + dpdt t, or with mass sources
-      // Ru T / (v Wmix) at the initial v; t = dt (the manager passes the
-      // solve's dt, also when the integration stopped at ignition)
-      const double tend = dt;
-      if (o.constant_volume || cv.has_ysrc) {
-        const double mws = cv.team.sum(lane < nsp ? yout * m.inv_mw[lane] : 0.0);
-        const double Tend = cv.team.from(yout, nsp) * o.Tref;
-        const double v = o.constant_volume ? cv.v0 : cv.vinit;
-        const double Pend = kRu * Tend / (v * (1.0 / mws)) + (o.constant_volume ? cv.dpdt * tend : 0.0);
-        if (lane == 0) Pio[r] = Pend;
-      } else if (cv.has_src) {
-        const double Pend = cv.P + cv.dpdt * tend;
-        if (lane == 0) Pio[r] = Pend;
-      }
-    } else {
-      nsteps = (nsteps <= 0) ? -1 : -nsteps;
-    }
-#ifdef ZRK_PHASE_CYCLES
-    cv.cyc[4] = (long long)__builtin_amdgcn_s_memtime() - tr0;
-#endif
-    if (lane == 0) {
-      if (nsteps_out) nsteps_out[r] = nsteps;
-      if (stats_out) {
-#ifdef ZRK_PHASE_CYCLES
-        // profiling layout: 20 ints per reactor, cycles / 16 in slots 8..19
-#pragma unroll
-        for (int k = 0; k < 7; ++k) stats_out[(size_t)r * 20 + k] = tot[k];
-        stats_out[(size_t)r * 20 + 7] = flag;
-#pragma unroll
-        for (int k = 0; k < 12; ++k) stats_out[(size_t)r * 20 + 8 + k] = (int)(cv.cyc[k] >> 4);
-#else
-#pragma unroll
-        for (int k = 0; k < 7; ++k) stats_out[(size_t)r * 8 + k] = tot[k];
-        stats_out[(size_t)r * 8 + 7] = flag;
-#endif
-      }
-    }
-    cv.team.sync();
-  }
-}
-
-// One wavefront per reactor (nsp <= 63).  MT = double: plain FP64 method;
-// MT = float: FP32 preconditioner.  LDS (bdf_lds_bytes): CvUni, Mlds[N*N] MT
-// (N = nsp + 1, padded to 16 bytes), t2l[kNP], then C[nsp+1], G[nsp+1],
-// rop[nstep+1] doubles.
-template <typename MT, bool kExt>
-__device__ __forceinline__ void bdf_wave(DevMech m, BdfOpts o, int n, double dt,
-                                         const double* __restrict__ dts, double* __restrict__ Tio,
-                                         double* __restrict__ Pio, double* __restrict__ Yio,
-                                         int stride, int* __restrict__ nsteps_out,
-                                         int* __restrict__ stats_out,
-                                         const int* __restrict__ order,
-                                         unsigned int* __restrict__ counter,
-                                         double* __restrict__ jscratch) {
-  extern __shared__ double lds[];
-  const int nsp = m.nsp;
-  CvWave<MT, kExt> cv(reinterpret_cast<CvUni*>(lds));
-  char* base = reinterpret_cast<char*>(lds) + sizeof(CvUni);
-  cv.Mlds = reinterpret_cast<MT*>(base);
-  cv.prow = nullptr;
-  cv.rvec = nullptr;
-  cv.t2l = reinterpret_cast<double*>(base + wave_matrix_bytes(nsp, sizeof(MT)));
-  cv.hl = nullptr;
-  double* rl = cv.t2l + kNP;
-  cv.s = rhs_lds_at(rl, nsp);
-  cv.savedJ = reinterpret_cast<MT*>(jscratch + (size_t)blockIdx.x * kNP * kNP);
-  bdf_driver(cv, m, o, n, dt, dts, Tio, Pio, Yio, stride, nsteps_out, stats_out, order, counter);
-}
-
-// Minimum waves per SIMD each instantiation's register allocation targets
-// (build-time tuning knobs).  The FP32 kernel's LDS (N*N floats + the rate
-// arrays, about 18 KB at 53 species) allows 2 waves per SIMD, and holding
-// registers to that budget is faster than one unconstrained wave; the FP64
-// kernel's matrix (about 23 KB) does not leave room for a second wave.
-#ifndef ZRK_BDF_F64_WAVES_PER_EU
-#define ZRK_BDF_F64_WAVES_PER_EU 1
-#endif
-#ifndef ZRK_BDF_F32_WAVES_PER_EU
-#define ZRK_BDF_F32_WAVES_PER_EU 2
-#endif
-#define ZRK_BDF_KERNEL_ARGS                                                                     \
-  DevMech m, BdfOpts o, int n, double dt, const double* __restrict__ dts,                      \
-      double* __restrict__ Tio, double* __restrict__ Pio, double* __restrict__ Yio, int stride, \
-      int* __restrict__ nsteps_out, int* __restrict__ stats_out, const int* __restrict__ order, \
-      unsigned int* __restrict__ counter, double* __restrict__ jscratch
-template <typename MT>
-constexpr int bdf_waves_per_eu() {
-  return sizeof(MT) == sizeof(float) ? ZRK_BDF_F32_WAVES_PER_EU : ZRK_BDF_F64_WAVES_PER_EU;
-}
-template <typename MT, bool kExt>
-__global__ void __launch_bounds__(kWave, bdf_waves_per_eu<MT>())
-bdf_kernel(ZRK_BDF_KERNEL_ARGS) {
-  bdf_wave<MT, kExt>(m, o, n, dt, dts, Tio, Pio, Yio, stride, nsteps_out, stats_out, order,
-                     counter, jscratch);
-}
-#undef ZRK_BDF_KERNEL_ARGS
-
-// W wavefronts per reactor (64 <= nsp < 64W), FP32 preconditioner.  LDS
-// (bdf_team_lds_bytes): CvUni[W] (one per wavefront), Mlds[(nsp+1)*(64W+1)]
-// float, prow[64W], rvec[64W] float, t2l[64W], hl[64W], red[W+1] double,
-// then C[nsp+1], G[nsp+1], rop[nstep+1] doubles.  savedJ slot: slot_doubles
-// doubles per workgroup.
-template <int W, bool kExt>
-__global__ void __launch_bounds__(kWave * W)
-bdf_team_kernel(DevMech m, BdfOpts o, int n, double dt, const double* __restrict__ dts,
-                double* __restrict__ Tio,
-                double* __restrict__ Pio, double* __restrict__ Yio, int stride,
-                int* __restrict__ nsteps_out, int* __restrict__ stats_out,
-                const int* __restrict__ order, unsigned int* __restrict__ counter,
-                double* __restrict__ jscratch, size_t slot_doubles) {
-  extern __shared__ double lds[];
-  constexpr int L = kWave * W;
-  using CV = Cvode<float, TeamBlock<W, kExt>>;
-  const int nsp = m.nsp;
-  CV cv(reinterpret_cast<CvUni*>(lds) + threadIdx.x / kWave);
-  char* p = reinterpret_cast<char*>(lds) + W * sizeof(CvUni);
-  cv.Mlds = reinterpret_cast<float*>(p);
-  p += ((size_t)(nsp + 1) * CV::kJS * sizeof(float) + 15) & ~(size_t)15;
-  cv.prow = reinterpret_cast<float*>(p);
-  cv.rvec = cv.prow + L;
-  cv.t2l = reinterpret_cast<double*>(cv.rvec + L);
-  cv.hl = cv.t2l + L;
-  double* red = cv.hl + L;
-  cv.team.red = red;
-  double* rl = red + ((W + 2) & ~1);
-  cv.s = rhs_lds_at(rl, nsp);
-  cv.savedJ = reinterpret_cast<float*>(jscratch + (size_t)blockIdx.x * slot_doubles);
-  bdf_driver(cv, m, o, n, dt, dts, Tio, Pio, Yio, stride, nsteps_out, stats_out, order, counter);
-}
-
 __global__ void fill_u32_kernel(unsigned int* p, unsigned int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
@@ -524,65 +243,25 @@ size_t bdf_slot_doubles(const DevMech& m, int W) {
   return ((floats + 1) / 2 + 31) & ~(size_t)31;  // 256-byte aligned slots
 }
 
-size_t bdf_lds_bytes(const DevMech& m, bool f32, int W) {
+size_t bdf_lds_bytes(const DevMech& m, bool f32, int W, bool iter) {
+  const size_t kry = iter ? sizeof(double) * kKryDoubles : 0;
   if (W <= 1)
     return sizeof(CvUni) + wave_matrix_bytes(m.nsp, f32 ? sizeof(float) : sizeof(double)) +
-           sizeof(double) * kNP + rhs_lds_bytes(m);
+           sizeof(double) * kNP + rhs_lds_bytes(m) + kry;
   const size_t L = (size_t)kWave * W;
   const size_t mat = ((size_t)(m.nsp + 1) * (L + 1) * sizeof(float) + 15) & ~(size_t)15;
   return W * sizeof(CvUni) + mat + 2 * L * sizeof(float) + 2 * L * sizeof(double) +
-         (size_t)((W + 2) & ~1) * sizeof(double) + rhs_lds_bytes(m);
+         (size_t)((W + 2) & ~1) * sizeof(double) + rhs_lds_bytes(m) + kry;
 }
 
-namespace {
 
-size_t max_lds_per_block() {
-  int dev = 0, v = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return 0;
-  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
-    return 0;
-  return (size_t)v;
-}
-
-template <class K>
-int resident_blocks(K kernel, int threads, size_t lds) {
-  int dev = 0, ncu = 0, per_cu = 0;
-  if (lds > max_lds_per_block()) return 0;
-  if (hipGetDevice(&dev) != hipSuccess) return 0;
-  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-    return 0;
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds) != hipSuccess)
-    return 0;
-  return ncu * (per_cu > 0 ? per_cu : 1);
-}
-
-}  // namespace
-
-int bdf_max_resident_teams(const DevMech& m, bool f32, int W) {
+int bdf_max_resident_teams(const DevMech& m, bool f32, int W, bool iter) {
   if (!bdf_team_ok(m.nsp, f32, W)) return 0;
-  const size_t lds = bdf_lds_bytes(m, f32, W);
-  int r = 0;
-  (void)with_ext(m.n_ext > 0, [&](auto e) {
-    constexpr bool E = decltype(e)::value;
-    switch (W) {
-      case 1:
-        r = f32 ? resident_blocks(bdf_kernel<float, E>, kWave, lds)
-                : resident_blocks(bdf_kernel<double, E>, kWave, lds);
-        break;
-      case 2:
-        r = f32 ? resident_blocks(bdf_team_kernel<2, E>, 2 * kWave, lds) : 0;
-        break;
-      case 3:
-        r = f32 ? resident_blocks(bdf_team_kernel<3, E>, 3 * kWave, lds) : 0;
-        break;
-      default:
-        r = 0;
-    }
-    return hipSuccess;
-  });
-  return r;
+  const size_t lds = bdf_lds_bytes(m, f32, W, iter);
+  const bool ext = m.n_ext > 0;
+  if (W == 1) return iter ? resident_bdf_wave<true>(ext, f32, lds)
+                          : resident_bdf_wave<false>(ext, f32, lds);
+  return iter ? resident_bdf_team<true>(ext, W, lds) : resident_bdf_team<false>(ext, W, lds);
 }
 
 hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, const double* dts,
@@ -591,29 +270,18 @@ hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, cons
                       int W, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   const bool f32 = o.fp32_preconditioner != 0;
+  const bool iter = o.krylov != 0;
   if (!bdf_team_ok(m.nsp, f32, W)) return hipErrorInvalidValue;
-  const size_t lds = bdf_lds_bytes(m, f32, W);
+  const size_t lds = bdf_lds_bytes(m, f32, W, iter);
   if (lds > max_lds_per_block()) return hipErrorInvalidValue;
   hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, counter, 0u);
   const int grid = nwaves < n ? nwaves : n;
-  return with_ext(m.n_ext > 0 || o.ign_dT > 0.0, [&](auto e) {
-    constexpr bool E = decltype(e)::value;
-    if (W == 1 && f32)
-      hipLaunchKernelGGL((bdf_kernel<float, E>), dim3(grid), dim3(kWave), lds, st, m, o, n, dt,
-                         dts, T, P, Y, stride, nsteps, stats, order, counter, jscratch);
-    else if (W == 1)
-      hipLaunchKernelGGL((bdf_kernel<double, E>), dim3(grid), dim3(kWave), lds, st, m, o, n, dt,
-                         dts, T, P, Y, stride, nsteps, stats, order, counter, jscratch);
-    else if (W == 2)
-      hipLaunchKernelGGL((bdf_team_kernel<2, E>), dim3(grid), dim3(2 * kWave), lds, st, m, o, n,
-                         dt, dts, T, P, Y, stride, nsteps, stats, order, counter, jscratch,
-                         bdf_slot_doubles(m, W));
-    else
-      hipLaunchKernelGGL((bdf_team_kernel<3, E>), dim3(grid), dim3(3 * kWave), lds, st, m, o, n,
-                         dt, dts, T, P, Y, stride, nsteps, stats, order, counter, jscratch,
-                         bdf_slot_doubles(m, W));
-    return hipGetLastError();
-  });
+  const bool ext = m.n_ext > 0 || o.ign_dT > 0.0;
+  const BdfLaunch a{m, o, n, dt, dts, T, P, Y, stride, nsteps, stats, order, counter, jscratch,
+                    grid, lds, st};
+  if (W == 1) return iter ? launch_bdf_wave<true>(a, ext, f32) : launch_bdf_wave<false>(a, ext, f32);
+  const size_t slot = bdf_slot_doubles(m, W);
+  return iter ? launch_bdf_team<true>(a, ext, W, slot) : launch_bdf_team<false>(a, ext, W, slot);
 }
 
 }  // namespace zrk

@@ -20,9 +20,10 @@ int bdf_team_width(int nsp);
 bool bdf_team_ok(int nsp, bool f32, int W);
 // doubles of saved-Jacobian workspace per resident team of W wavefronts
 size_t bdf_slot_doubles(const DevMech& m, int W);
-size_t bdf_lds_bytes(const DevMech& m, bool f32, int W);
+// iter: the Krylov solver's kernels (their LDS holds its uniform block too)
+size_t bdf_lds_bytes(const DevMech& m, bool f32, int W, bool iter = false);
 // teams the persistent integrator keeps resident (0: does not fit)
-int bdf_max_resident_teams(const DevMech& m, bool f32, int W);
+int bdf_max_resident_teams(const DevMech& m, bool f32, int W, bool iter = false);
 hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, const double* dts,
                       double* T, double* P, double* Y, int stride, int* nsteps, int* stats,
                       const int* order, unsigned int* counter, double* jscratch, int nwaves,

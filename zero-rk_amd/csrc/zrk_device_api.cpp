@@ -280,6 +280,31 @@ int build_device_mech(MechHandle* h, std::string* err) {
         jrt[((k / 4) * team + i) * 4 + k % 4] = rows[i][k];
   }
   if (jrt.empty()) jrt.assign(4, 0u);
+  // the Jacobian's pattern by row, 3 x 64 bits (reactor_nvector_serial.cpp:122-200)
+  std::vector<unsigned long long> jpat;
+  if (nsp <= 191) {
+    const int N = nsp + 1;
+    jpat.assign((size_t)N * 3, 0ull);
+    auto mark = [&](int row, int col) { jpat[(size_t)row * 3 + col / 64] |= 1ull << (col % 64); };
+    for (int k = 0; k < N; ++k) {
+      mark(k, k);
+      mark(nsp, k);
+      mark(k, nsp);
+    }
+    for (int s = 0; s < nstep; ++s)
+      for (int k = M.reac_ptr[s]; k < M.reac_ptr[s + 1]; ++k) {
+        const int c = M.reac_sp[k];
+        for (int q = M.reac_ptr[s]; q < M.reac_ptr[s + 1]; ++q) mark(M.reac_sp[q], c);
+        for (int q = M.prod_ptr[s]; q < M.prod_ptr[s + 1]; ++q) mark(M.prod_sp[q], c);
+      }
+    for (const auto& st : M.ni)
+      for (size_t k = 0; k < st.rop_sp.size(); ++k) {
+        const int c = st.rop_sp[k];
+        for (int r : st.re_sp) mark(r, c);
+        for (int r : st.pr_sp) mark(r, c);
+      }
+  }
+  if (jpat.empty()) jpat.assign(3, 0ull);
   const int n_ext = (int)(ext.size() / 4);
   const bool has_nig = !nig.empty();
   // keep every table non-empty so that each device pointer is valid
@@ -299,7 +324,7 @@ int build_device_mech(MechHandle* h, std::string* err) {
                o_fo = b.add(fo), o_gd = b.add(gd), o_gc = b.add(gc), o_jr = b.add(jrt),
                o_ext = b.add(ext), o_pl = b.add(plog_t), o_nsp = b.add(ni_sp),
                o_npw = b.add(ni_pow), o_ndp = b.add(nig_dptr), o_ncp = b.add(nig_cptr),
-               o_nig = b.add(nig), o_njt = b.add(ni_jt);
+               o_nig = b.add(nig), o_njt = b.add(ni_jt), o_jpat = b.add(jpat);
   void* base = nullptr;
   hipError_t e = hipMalloc(&base, b.bytes.size());
   if (e != hipSuccess) {
@@ -346,6 +371,7 @@ int build_device_mech(MechHandle* h, std::string* err) {
   d.nig_cptr = (const int*)(p + o_ncp);
   d.nig = (const double2*)(p + o_nig);
   d.ni_jt = (const double2*)(p + o_njt);
+  d.jpat = (const unsigned long long*)(p + o_jpat);
   e = hipMalloc((void**)&h->dev_self, sizeof(DevMech));
   if (e == hipSuccess) {
     d.self = h->dev_self;
@@ -398,6 +424,15 @@ BdfOpts to_bdf_opts(const zrk_bdf_options& o) {
   b.fp32_preconditioner = o.fp32_preconditioner != 0;
   b.ign_dT = o.delta_temperature_ignition > 0.0 ? o.delta_temperature_ignition : 0.0;
   b.stop_after_ignition = o.stop_after_ignition != 0;
+  b.abstol_dens = o.abstol_dens != 0;
+  b.krylov = !(o.dense == 1 && o.iterative == 0);
+  b.jac = o.analytic ? kJacAnalytic : (b.krylov ? kJacReactorDQ : kJacCvodeDQ);
+  // the preconditioner of the Krylov solver: the Jacobian's pattern with the
+  // threshold (0 without iterative), or every entry for dense divided
+  // differences (JacobianFactor, reactor_nvector_serial.cpp:373-397)
+  b.pthr = (o.iterative ? o.preconditioner_threshold : 0.0);
+  b.pfull = !b.krylov || (o.dense == 1 && !o.analytic);
+  b.eps_lin = o.eps_lin;
   return b;
 }
 
@@ -524,20 +559,26 @@ int integrate_sparse(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
 
 }  // namespace
 
-int team_width_for(MechHandle* h, bool f32) {
+int team_width_for(MechHandle* h, bool f32, bool iter) {
   int W = h->team_pref[f32] ? h->team_pref[f32] : bdf_team_width(h->host.nsp);
   if (!bdf_team_ok(h->host.nsp, f32, W)) return 0;
-  if (h->resident[f32][W] <= 0) h->resident[f32][W] = bdf_max_resident_teams(h->dev, f32, W);
-  return h->resident[f32][W] > 0 ? W : 0;
+  int& res = h->resident[f32][iter][W];
+  if (res <= 0) res = bdf_max_resident_teams(h->dev, f32, W, iter);
+  return res > 0 ? W : 0;
 }
 
 int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
                      const double* dts, double* T, double* P, double* Y, int stride, int* nsteps, int* stats,
                      const int* order, hipStream_t st, const double* dpdt, const double* esrc,
-                     const double* ysrc, double* ign_time) {
+                     const double* ysrc, double* ign_time, const double* tdelta, int always_T) {
   if (n < 0 || !T || !P || !Y || stride < h->host.nsp || (!dts && !(dt > 0.0))) return -1;
+  BdfOpts bo = to_bdf_opts(o);
   const int W0 = bdf_team_width(h->host.nsp);
   if (W0 == 0 || h->force_sparse) {  // above 191 species: the sparse path
+    // the sparse path has the direct analytic solver with solved temperature
+    // and plain absolute tolerances only
+    if (bo.krylov || bo.jac != kJacAnalytic || bo.abstol_dens || (!always_T && tdelta))
+      return -4;
     if (n > 0)
       if (int rc = ensure_device(h)) return rc;
     return integrate_sparse(h, o, n, dt, dts, T, P, Y, stride, nsteps, stats, order, st, dpdt,
@@ -547,16 +588,18 @@ int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
   if (n == 0) return 0;
   if (int rc = ensure_device(h)) return rc;
   const int f32 = o.fp32_preconditioner != 0;
-  const int W = team_width_for(h, f32);
+  const int it = bo.krylov ? 1 : 0;
+  const int W = team_width_for(h, f32, it);
   if (W == 0) return -2;                                // does not fit in LDS
-  const int nw = std::max(1, std::min(h->resident[f32][W], n));
+  const int nw = std::max(1, std::min(h->resident[f32][it][W], n));
   int rc = ensure_workspace(h, (size_t)nw * bdf_slot_doubles(h->dev, W));
   if (rc) return rc;
-  BdfOpts bo = to_bdf_opts(o);
   bo.dpdt = dpdt;
   bo.esrc = esrc;
   bo.ysrc = ysrc;
   bo.ign_time = bo.ign_dT > 0.0 ? ign_time : nullptr;
+  bo.always_T = always_T ? 1 : 0;
+  bo.tdelta = always_T ? nullptr : tdelta;
   hipError_t e = launch_bdf(h->dev, bo, n, dt, dts, T, P, Y, stride, nsteps, stats, order,
                             h->ws_counter, h->ws_jac, nw, W, st);
   return e == hipSuccess ? 0 : (int)e;
@@ -587,6 +630,12 @@ void zrk_bdf_default_options(zrk_bdf_options* o) {
   o->fp32_preconditioner = 0;
   o->delta_temperature_ignition = 0.0;
   o->stop_after_ignition = 0;
+  o->abstol_dens = 0;
+  o->dense = 1;
+  o->analytic = 1;
+  o->iterative = 0;
+  o->eps_lin = 1.0e-3;
+  o->preconditioner_threshold = 1.0e-3;
 }
 
 int zrk_mech_load(const char* mech_file, const char* therm_file, zrk_mech* out, char* err,
@@ -797,8 +846,9 @@ int zrk_integrator_resident_waves(zrk_mech m, const zrk_bdf_options* o) {
     if (!f32 || zrk::ensure_sparse(h)) return 0;
     return h->sp_resident * zrk::kSparseWaves;
   }
-  const int W = zrk::team_width_for(h, f32);
-  return W ? h->resident[f32][W] * W : 0;
+  const int it = (o && !(o->dense == 1 && o->iterative == 0)) ? 1 : 0;
+  const int W = zrk::team_width_for(h, f32, it);
+  return W ? h->resident[f32][it][W] * W : 0;
 }
 
 }  // extern "C"

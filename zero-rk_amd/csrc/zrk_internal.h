@@ -19,8 +19,9 @@ struct MechHandle {
   void* dev_blob = nullptr;
   size_t dev_bytes = 0;
   int device = 0;
-  // persistent grid size in teams, by [FP32 preconditioner][wavefronts per team]
-  int resident[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+  // persistent grid size in teams, by [FP32 preconditioner][Krylov solver]
+  // [wavefronts per team]
+  int resident[2][2][4] = {};
   // wavefronts per team chosen by the autotuner for [FP32 preconditioner] (0: smallest)
   int team_pref[2] = {0, 0};
   // integrator workspace
@@ -39,7 +40,7 @@ struct MechHandle {
 
 int build_device_mech(MechHandle* h, std::string* err);
 // team width the integrator uses for these options (0 if none fits)
-int team_width_for(MechHandle* h, bool f32);
+int team_width_for(MechHandle* h, bool f32, bool iter = false);
 int ensure_device(MechHandle* h);
 int ensure_workspace(MechHandle* h, size_t doubles);
 // builds and uploads the sparse plan once (0 on success)
@@ -47,10 +48,14 @@ int ensure_sparse(MechHandle* h);
 BdfOpts to_bdf_opts(const zrk_bdf_options& o);
 // dpdt[n], esrc[n], ysrc[n*stride]: optional device arrays of source terms;
 // ign_time[n]: optional ignition-time output (with o.delta_temperature_ignition > 0)
+// tdelta[n] (optional, with always_T = 0): the manager's temperature change
+// of each reactor's previous solve; a reactor whose entry is not above 0 is
+// integrated at fixed temperature (zerork_reactor_manager.cpp:1109-1113)
 int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
                      const double* dts, double* T, double* P, double* Y, int stride, int* nsteps, int* stats,
                      const int* order, hipStream_t st, const double* dpdt = nullptr,
                      const double* esrc = nullptr, const double* ysrc = nullptr,
-                     double* ign_time = nullptr);
+                     double* ign_time = nullptr, const double* tdelta = nullptr,
+                     int always_T = 1);
 
 }  // namespace zrk

@@ -55,7 +55,9 @@ class BdfOptions(C.Structure):
         ("cvode_retry_relative_tolerance_adjustment", C.c_double),
         ("max_steps", C.c_int), ("cvode_num_retries", C.c_int), ("constant_volume", C.c_int),
         ("fp32_preconditioner", C.c_int), ("delta_temperature_ignition", C.c_double),
-        ("stop_after_ignition", C.c_int),
+        ("stop_after_ignition", C.c_int), ("abstol_dens", C.c_int), ("dense", C.c_int),
+        ("analytic", C.c_int), ("iterative", C.c_int), ("eps_lin", C.c_double),
+        ("preconditioner_threshold", C.c_double),
     ]
 
 
