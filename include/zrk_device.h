@@ -114,7 +114,9 @@ ZRK_EXPORT int zrk_mech_reactions(zrk_mech m, int* fwd_step, int* rev_step, int*
 /* Rates of progress (limiter step_limiter, as perf_net.cpp:190-275), species
  * creation/destruction/net rates and the constant-pressure derivative
  * ydot[r*(nsp+1) + k] ([dY_0/dt .. dY_{nsp-1}/dt, dT/dt]).  Any output may be
- * NULL. */
+ * NULL.  Any mechanism size: one wavefront per reactor while a reactor's rate
+ * arrays fit in its share of LDS, else one workgroup per reactor with the
+ * rates of progress in device scratch of the handle. */
 ZRK_EXPORT int zrk_rates_device(zrk_mech m, int n, const double* T, const double* P,
                                 const double* Y, int stride, double step_limiter, double* ydot,
                                 double* rop, double* create, double* destroy, double* net,

@@ -379,8 +379,11 @@ def build_tables(mech_path: str, therm_path: str | None, ru: float = NIST_RU) ->
                     if t:
                         elements.append(t.upper())
         elif key == "SPEC":
+            # a repeated name is ignored (CKParser.cpp:647-657)
             for l in [" ".join(rest)] + lines:
-                species += l.split()
+                for t in l.split():
+                    if t not in species:
+                        species.append(t)
         elif key == "THER":
             thermo_lines += lines
         elif key == "REAC":

@@ -14,7 +14,8 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 MECHS = os.path.join(GOLDEN, "mech")
 GOLDEN_CASES = ["synth_10sp", "synth_53sp", "synth_160sp", "ref_h2", "ref_dme",
                 "ref_h2_no_falloff", "ref_troe_test", "ref_sri", "ref_plog", "ref_non_integer",
-                "ref_ford_propane", "ref_rord_propane", "ref_h2_features"]
+                "ref_ford_propane", "ref_rord_propane", "ref_h2_features", "synth_2000sp",
+                "ref_ic8"]
 # the reference's CHEMKIN-feature test mechanisms (SRI, PLOG, fractional
 # stoichiometry, FORD, RORD) and the H2 mechanism with all of them added
 FEATURE_CASES = ["ref_sri", "ref_plog", "ref_non_integer", "ref_ford_propane",

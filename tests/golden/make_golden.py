@@ -140,8 +140,7 @@ def main():
     for wl in mechgen.WORKLOADS:
         m = mechgen.workload_mechanism(wl)
         mp, tp = m.write(MECH_DIR, f"synth_{wl}")
-        if len(m.species) <= 200:  # the 2,000-species files only (dense Jacobians too large)
-            cases.append((f"synth_{wl}", mp, tp, m, wl))
+        cases.append((f"synth_{wl}", mp, tp, m, wl))
     refdata = [
         ("ref_h2", "data/mechanisms/hydrogen/h2_v1b_mech.txt",
          "data/mechanisms/hydrogen/h2_v1a_therm.txt"),
@@ -160,6 +159,9 @@ def main():
          "data/mechanisms/ideal/const_specific_heat.therm"),
         ("ref_rord_propane", "data/mechanisms/real_order/rord_propane.mech",
          "data/mechanisms/ideal/const_specific_heat.therm"),
+        # a real large mechanism (874 species; its SPECIES list repeats one name)
+        ("ref_ic8", "data/mechanisms/iso-octane/species874/ic8_ver3_mech.txt",
+         "data/mechanisms/iso-octane/species874/prf_v3_therm_dat.txt"),
     ]
     for name, mrel, trel in refdata:
         mp = os.path.join(MECH_DIR, name + ".mech")
@@ -188,14 +190,17 @@ def main():
         nsp = t.nsp
         seed = SEEDS.get(name, 4242 + sum(map(ord, name)))
         rng = np.random.default_rng(seed)
-        n = {"synth_160sp": 6, "ref_dme": 6, "ref_plog": 48}.get(name, 16)
+        # few states for the large mechanisms (their vectors are 16,000 and
+        # 7,000 steps long)
+        n = {"synth_160sp": 6, "ref_dme": 6, "ref_plog": 48, "synth_2000sp": 2,
+             "ref_ic8": 3}.get(name, 16)
         T = rng.uniform(700.0, 2400.0, n)
         # PLOG tables span 0.01-100 atm: sample beyond both ends
         plo, phi = (0.003, 300.0) if name == "ref_plog" else (0.3, 60.0)
         P = np.exp(rng.uniform(np.log(plo), np.log(phi), n)) * 101325.0
         Y = rng.dirichlet(np.ones(nsp), n)
         if wl is not None:  # add states from the workload distribution (zeros present)
-            st = mechgen.workload_states(wl, m, 4, seed + 1)
+            st = mechgen.workload_states(wl, m, 1 if wl == "2000sp" else 4, seed + 1)
             T = np.concatenate([T, st.T])
             P = np.concatenate([P, st.P])
             Y = np.concatenate([Y, st.Y])

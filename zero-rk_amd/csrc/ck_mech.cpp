@@ -406,9 +406,15 @@ bool load_mechanism(const std::string& mech_file, const std::string& therm_file,
       for (auto& l : sec.lines)
         for (auto& t : tokens(l)) M.elements.push_back(upper(t.substr(0, t.find('/'))));
     } else if (sec.key == "SPEC") {
-      for (auto& t : sec.rest) species_names.push_back(t);
+      // a repeated name is dropped, keeping the first declaration
+      // (CKParser.cpp:647-657 warns and ignores it)
+      auto add = [&](const std::string& t) {
+        if (std::find(species_names.begin(), species_names.end(), t) == species_names.end())
+          species_names.push_back(t);
+      };
+      for (auto& t : sec.rest) add(t);
       for (auto& l : sec.lines)
-        for (auto& t : tokens(l)) species_names.push_back(t);
+        for (auto& t : tokens(l)) add(t);
     } else if (sec.key == "THER") {
       thermo_lines.insert(thermo_lines.end(), sec.lines.begin(), sec.lines.end());
     } else if (sec.key == "REAC") {

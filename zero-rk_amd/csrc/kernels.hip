@@ -9,6 +9,7 @@
 //                    {direct,krylov}.hip).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "bdf_launch.h"
@@ -89,6 +90,94 @@ rates_kernel(DevMech m, int n, const double* __restrict__ Tin, const double* __r
       for (int s = lane; s < nstep; s += kWave) rop_out[(size_t)r * nstep + s] = rop[s];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// rates_block_kernel: rates_kernel for mechanisms whose per-reactor arrays do
+// not fit in one wavefront's share of LDS (the 874-species iso-octane and
+// 2,000-species mechanisms): one reactor per workgroup of kRatesBlockWaves
+// wavefronts at a time, C, g/RT and h/RT in LDS, the rates of progress in
+// this workgroup's slot of scratch (nstep + 1 doubles: the last is the zero
+// that absent participants read), copied to rop_out when it is requested.
+// Sums are reduced per wavefront, then across wavefronts in a fixed order.
+// ---------------------------------------------------------------------------
+constexpr int kRatesBlockWaves = 4;
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  const int wave = threadIdx.x / kWave;
+  v = wave_sum(v);
+  if (lane_id() == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int w = 0; w < kRatesBlockWaves; ++w) s += red[w];
+  __syncthreads();
+  return s;
+}
+
+template <bool kExt>
+__global__ void __launch_bounds__(kWave* kRatesBlockWaves)
+rates_block_kernel(DevMech m, int n, const double* __restrict__ Tin,
+                   const double* __restrict__ Pin, const double* __restrict__ Yin, int stride,
+                   double L, double* __restrict__ ydot, double* __restrict__ rop_out,
+                   double* __restrict__ create_out, double* __restrict__ destroy_out,
+                   double* __restrict__ net_out, double* __restrict__ scratch) {
+  constexpr int S = kWave * kRatesBlockWaves;
+  extern __shared__ double lds[];
+  const int tid = threadIdx.x;
+  const int nsp = m.nsp, nstep = m.nstep;
+  double* red = lds;  // kRatesBlockWaves (padded to 8)
+  double* C = lds + 8;
+  double* G = C + nsp + 1;
+  double* H = G + nsp + 1;
+  double* rop = scratch + (size_t)blockIdx.x * (nstep + 1);
+  if (tid == 0) {  // entries read by absent reaction participants
+    C[nsp] = 1.0;
+    G[nsp] = 0.0;
+  }
+  for (int r = blockIdx.x; r < n; r += gridDim.x) {
+    const double* Y = Yin + (size_t)r * stride;
+    const double T = fmin(Tin[r], kTlimit);
+    const double P = Pin[r];
+    const TPow p = make_tpow(T);
+    double part = 0.0;
+    for (int j = tid; j < nsp; j += S) part += Y[j] * m.inv_mw[j];
+    const double wmix = 1.0 / block_sum(part, red);
+    const double inv_rho = 1.0 / (P * wmix / (kRu * T));
+    const double rho_c = 1.0 / inv_rho;
+    double cps = 0.0, cs = 0.0;
+    for (int j = tid; j < nsp; j += S) {
+      double cp, h, g;
+      thermo_one(m.thermo + 16 * j, p, cp, h, g);
+      const double Cj = rho_c * m.inv_mw[j] * Y[j];
+      C[j] = Cj;
+      G[j] = g;
+      H[j] = h;
+      cps += cp * m.ru_mw[j] * Y[j];
+      cs += Cj;
+    }
+    const double cp_mix = block_sum(cps, red);
+    const double Csum = block_sum(cs, red);  // ends with a barrier: C, G visible
+    team_rates<S, kExt>(tid, m, C, G, rop, p, Csum, L);
+    __syncthreads();  // the rates of progress (this workgroup's slot) visible
+    double e = 0.0;
+    for (int j = tid; j < nsp; j += S) {
+      double cr, de;
+      species_cd<kExt>(m, rop, j, cr, de);
+      const double net = cr - de;
+      if (create_out) create_out[(size_t)r * nsp + j] = cr;
+      if (destroy_out) destroy_out[(size_t)r * nsp + j] = de;
+      if (net_out) net_out[(size_t)r * nsp + j] = net;
+      if (ydot) ydot[(size_t)r * (nsp + 1) + j] = (net * m.mw[j]) * inv_rho;
+      e += H[j] * net * inv_rho;
+    }
+    const double esum = block_sum(e, red);
+    if (ydot && tid == 0)
+      ydot[(size_t)r * (nsp + 1) + nsp] = -esum * kRu * Tin[r] / cp_mix + 0.0 / cp_mix;
+    if (rop_out)
+      for (int s2 = tid; s2 < nstep; s2 += S) rop_out[(size_t)r * nstep + s2] = rop[s2];
+    __syncthreads();  // the slot and the LDS arrays are reused by the next reactor
   }
 }
 
@@ -180,12 +269,45 @@ hipError_t with_ext(bool ext, F&& f) {
 }
 }  // namespace
 
+bool rates_need_block(const DevMech& m) {
+  return sizeof(double) * (size_t)kRatesWaves * (3 * m.nsp + 2 + m.nstep + 1) > 160 * 1024;
+}
+
+size_t rates_block_lds_bytes(const DevMech& m) {
+  return sizeof(double) * (size_t)(8 + 3 * m.nsp + 2);
+}
+
+int rates_block_grid(const DevMech& m, int n) {
+  int dev = 0, ncu = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 0;
+  const size_t lds = rates_block_lds_bytes(m);
+  hipError_t e = m.n_ext > 0
+      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rates_block_kernel<true>,
+                                                     kWave * kRatesBlockWaves, lds)
+      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rates_block_kernel<false>,
+                                                     kWave * kRatesBlockWaves, lds);
+  if (e != hipSuccess || per_cu <= 0) return 0;
+  return std::max(1, std::min(n, per_cu * ncu));
+}
+
 hipError_t launch_rates(const DevMech& m, int n, const double* T, const double* P,
                         const double* Y, int stride, double L, double* ydot, double* rop,
-                        double* create, double* destroy, double* net, hipStream_t st) {
+                        double* create, double* destroy, double* net, hipStream_t st,
+                        double* scratch, int grid) {
   if (n <= 0) return hipSuccess;
+  if (rates_need_block(m)) {
+    const size_t lds = rates_block_lds_bytes(m);
+    if (lds > 160 * 1024 || !scratch || grid <= 0) return hipErrorInvalidValue;
+    return with_ext(m.n_ext > 0, [&](auto e) {
+      hipLaunchKernelGGL((rates_block_kernel<decltype(e)::value>), dim3(std::min(grid, n)),
+                         dim3(kWave * kRatesBlockWaves), lds, st, m, n, T, P, Y, stride, L, ydot,
+                         rop, create, destroy, net, scratch);
+      return hipGetLastError();
+    });
+  }
   const size_t lds = sizeof(double) * (size_t)kRatesWaves * (3 * m.nsp + 2 + m.nstep + 1);
-  if (lds > 160 * 1024) return hipErrorInvalidValue;
   int blocks = (n + kRatesWaves - 1) / kRatesWaves;
   return with_ext(m.n_ext > 0, [&](auto e) {
     hipLaunchKernelGGL((rates_kernel<decltype(e)::value>), dim3(blocks),

@@ -7,9 +7,15 @@
 
 namespace zrk {
 
+// Mechanisms whose per-reactor rate arrays exceed a wavefront's LDS share
+// (rates_need_block) run one reactor per workgroup on a persistent grid of
+// rates_block_grid workgroups, each with nstep + 1 doubles of scratch.
+bool rates_need_block(const DevMech& m);
+int rates_block_grid(const DevMech& m, int n);
 hipError_t launch_rates(const DevMech& m, int n, const double* T, const double* P,
                         const double* Y, int stride, double L, double* ydot, double* rop,
-                        double* create, double* destroy, double* net, hipStream_t st);
+                        double* create, double* destroy, double* net, hipStream_t st,
+                        double* scratch = nullptr, int grid = 0);
 hipError_t launch_jacobian(const DevMech& m, const BdfOpts& o, int n, const double* T,
                            const double* P, const double* Y, int stride, double* J,
                            double* jscratch, int nblocks, hipStream_t st);

@@ -716,8 +716,16 @@ int zrk_rates_device(zrk_mech m, int n, const double* T, const double* P, const 
   auto* h = reinterpret_cast<MechHandle*>(m);
   if (stride < h->host.nsp) return -1;
   if (int rc = zrk::ensure_device(h)) return rc;
+  double* scratch = nullptr;
+  int grid = 0;
+  if (n > 0 && zrk::rates_need_block(h->dev)) {  // large mechanisms: one reactor per workgroup
+    grid = zrk::rates_block_grid(h->dev, n);
+    if (grid <= 0) return -2;
+    if (int rc = zrk::ensure_workspace(h, (size_t)grid * (h->host.nstep + 1))) return rc;
+    scratch = h->ws_jac;
+  }
   hipError_t e = zrk::launch_rates(h->dev, n, T, P, Y, stride, step_limiter, ydot, rop, create,
-                                   destroy, net, (hipStream_t)stream);
+                                   destroy, net, (hipStream_t)stream, scratch, grid);
   return e == hipSuccess ? 0 : (int)e;
 }
 

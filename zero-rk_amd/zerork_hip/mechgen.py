@@ -403,37 +403,243 @@ def generate(n_species: int, n_reactions: int, seed: int, elements=("H", "O", "N
     order = rng.permutation(n_reactions)
     # standard enthalpy of formation at 298.15 K from the generated NASA-7 data
     h298 = np.array([_h298(lo) for lo, _ in thermo])
-    reactions = []
-    for idx in order:
-        (a, b), kind = pairs[idx], kinds[idx]
-        # write every reaction in its exothermic direction: the random
-        # activation energy then belongs to the downhill direction and the
-        # reverse (from Keq) carries Ea + dH, as in physical mechanisms
-        if h298[list(b)].sum() > h298[list(a)].sum():
-            a, b = b, a
-        A = 10.0 ** rng.uniform(8.0, 14.0)
-        beta = rng.uniform(-1.0, 2.0)
-        Ea = rng.uniform(0.0, 40.0) * 1000.0
-        r = Reaction([species[i] for i in a], [species[i] for i in b], kind, A, beta, Ea)
-        if kind in ("tb", "falloff"):
-            ne = int(rng.integers(1, 5))
-            cand = [s for s in species]
-            for j in rng.choice(len(cand), size=min(ne, len(cand)), replace=False):
-                r.eff[cand[j]] = float(rng.uniform(0.5, 3.0))
-        if kind == "falloff":
-            r.low = (A * 10.0 ** rng.uniform(3.0, 7.0), rng.uniform(-2.0, 1.0),
-                     rng.uniform(0.0, 40.0) * 1000.0)
-            alpha = rng.uniform(0.1, 0.9)
-            t3 = rng.uniform(50.0, 500.0)
-            t1 = rng.uniform(500.0, 5000.0)
-            if rng.random() < 0.5:
-                r.troe = (alpha, t3, t1, rng.uniform(1000.0, 10000.0))
-            else:
-                r.troe = (alpha, t3, t1)
-        reactions.append(r)
+    reactions = [_make_reaction(rng, pairs[idx], kinds[idx], species, h298, species)
+                 for idx in order]
     if majors is None:
         majors = species[:4]
     return Mechanism(elements, species, comp, thermo, reactions, seed, list(majors))
+
+
+def _make_reaction(rng, pair, kind, species, h298, eff_species):
+    """One reaction with the recipe's random rate parameters.  It is written in
+    its exothermic direction: the random activation energy then belongs to the
+    downhill direction and the reverse (from Keq) carries Ea + dH, as in
+    physical mechanisms.  Third-body efficiencies go to 1-4 species drawn from
+    eff_species."""
+    a, b = pair
+    if h298[list(b)].sum() > h298[list(a)].sum():
+        a, b = b, a
+    A = 10.0 ** rng.uniform(8.0, 14.0)
+    beta = rng.uniform(-1.0, 2.0)
+    Ea = rng.uniform(0.0, 40.0) * 1000.0
+    r = Reaction([species[i] for i in a], [species[i] for i in b], kind, A, beta, Ea)
+    if kind in ("tb", "falloff"):
+        ne = int(rng.integers(1, 5))
+        for j in rng.choice(len(eff_species), size=min(ne, len(eff_species)), replace=False):
+            r.eff[eff_species[j]] = float(rng.uniform(0.5, 3.0))
+    if kind == "falloff":
+        r.low = (A * 10.0 ** rng.uniform(3.0, 7.0), rng.uniform(-2.0, 1.0),
+                 rng.uniform(0.0, 40.0) * 1000.0)
+        alpha = rng.uniform(0.1, 0.9)
+        t3 = rng.uniform(50.0, 500.0)
+        t1 = rng.uniform(500.0, 5000.0)
+        if rng.random() < 0.5:
+            r.troe = (alpha, t3, t1, rng.uniform(1000.0, 10000.0))
+        else:
+            r.troe = (alpha, t3, t1)
+    return r
+
+
+# ----------------------------------------------------------------------------
+# Large mechanisms: a core of small species and families of large ones
+# ----------------------------------------------------------------------------
+# The pool of small species through which large species react (abstractors,
+# leaving groups and fragments), as in the fuel-specific parts of real large
+# mechanisms (alkyl + O2, H abstraction by H/O/OH/HO2/CH3, beta scission).
+_POOL = ["H", "H2", "O", "O2", "OH", "H2O", "HO2", "H2O2", "CH3", "CH4", "CO", "CO2",
+         "CH2O", "HCO", "CH3O", "C2H5", "C2H4", "C2H6", "NC3H7", "C3H6"]
+
+
+def _family_members(rng, n_c, size, fam):
+    """Compositions C{n}H{h}O{o} of one family of large species (isomers of one
+    carbon skeleton): the alkane, alkyl radicals and the olefin first, then
+    random members of the lattice 2n-3 <= h <= 2n+2, 0 <= o <= 4, with up to
+    three isomers of a composition."""
+    comps = [(n_c, 2 * n_c + 2, 0), (n_c, 2 * n_c + 1, 0), (n_c, 2 * n_c + 1, 0),
+             (n_c, 2 * n_c, 0)]
+    lattice = [(n_c, h, o) for h in range(2 * n_c - 3, 2 * n_c + 3) for o in range(0, 5)]
+    while len(comps) < size:
+        c = lattice[int(rng.integers(len(lattice)))]
+        if comps.count(c) < 3:
+            comps.append(c)
+    out, seen = [], {}
+    for c in comps:
+        k = seen.get(c, 0)
+        seen[c] = k + 1
+        formula = f"C{c[0]}H{c[1]}" + (f"O{c[2]}" if c[2] > 1 else ("O" if c[2] == 1 else ""))
+        out.append((f"{formula}-{fam:03d}{chr(65 + k)}", {"C": c[0], "H": c[1], "O": c[2]}))
+    return out
+
+
+def generate_families(n_species: int, n_reactions: int, seed: int, n_core: int = 120,
+                      core_reactions: int = 600, frac_third_body: float = 0.25,
+                      frac_falloff: float = 0.10, n_fuels: int = 4,
+                      family_size=(24, 40), carbon=(5, 16),
+                      class_weights=(0.45, 0.25, 0.10, 0.20)) -> Mechanism:
+    """Seeded synthetic large mechanism with the structure of real ones: a core
+    of small species (the known table, then random formulas of at most 3 C)
+    whose reactions are sampled like the small mechanisms', and families of
+    large C/H/O species, each the isomer lattice of one carbon skeleton.  A
+    family member reacts only with members of its own family (H transfer
+    A + X <=> B + Y and decomposition A <=> B + Y with X, Y from the radical
+    pool, isomerisation A <=> B); a family's two alkyl radicals also split into
+    a pool fragment and the alkane, an alkyl or the olefin of the first family
+    of a smaller carbon count (beta scission down a ladder of skeletons).
+    Elimination therefore fills families and the core, not the whole matrix.  Rate parameters, kinds and
+    thermo follow the module recipe; third-body efficiencies go to core
+    species.  The majors are the core's eight base species and the alkanes of
+    the n_fuels largest families."""
+    rng = np.random.default_rng(seed)
+    elements = ["C", "H", "O", "N", "AR"]
+    core = list(_C2_BASE)
+    for s in _species_pool(elements):
+        if len(core) >= n_core:
+            break
+        if s not in core:
+            core.append(s)
+    core += _extra_species(rng, elements, n_core - len(core), set(core), cmax=3)
+    # families until the species count is reached
+    fams, n_left, fam = [], n_species - len(core), 0
+    while n_left > 0:
+        size = int(rng.integers(family_size[0], family_size[1] + 1))
+        if n_left - size < family_size[0]:
+            size = n_left
+        n_c = int(rng.integers(carbon[0], carbon[1] + 1))
+        fams.append((n_c, _family_members(rng, n_c, size, fam)))
+        n_left -= size
+        fam += 1
+    species = list(core)
+    comp = [parse_formula(x) for x in core]
+    comp = [{e: c[e] for e in elements if e in c} for c in comp]
+    members = []
+    for n_c, mem in fams:
+        idx = []
+        for name, c in mem:
+            idx.append(len(species))
+            species.append(name)
+            comp.append(c)
+        members.append(idx)
+    thermo = [_thermo_for(rng, x, c) for x, c in zip(species, comp)]
+    h298 = np.array([_h298(lo) for lo, _ in thermo])
+    evec = np.array([[c.get(e, 0) for e in elements] for c in comp], dtype=np.int64)
+    key = [tuple(v) for v in evec]
+    pool = [species.index(x) for x in _POOL]
+    single, diff = {}, {}
+    for y in pool:
+        single.setdefault(key[y], []).append(y)
+        for x in pool:
+            if x != y:
+                d = tuple(evec[y] - evec[x])
+                diff.setdefault(d, []).append((x, y))
+    # the first family of each carbon count is the ladder that larger
+    # families' alkyl radicals scission into (alkane, alkyls, olefin)
+    ladder = {}
+    for f, (n_c, _) in enumerate(fams):
+        ladder.setdefault(n_c, f)
+    by_key = {}
+    for f, idx in enumerate(members):
+        for i in idx:
+            by_key.setdefault(key[i], []).append((f, i))
+    # core reactions: element-balanced sides sampled over the core
+    reactive = [i for i, x in enumerate(core) if x not in ("N2", "AR")]
+    n_fo_c = int(round(frac_falloff * core_reactions))
+    n_tb_c = int(round(frac_third_body * core_reactions))
+    major_core = [species.index(x) for x in _C2_BASE]
+    pairs, kinds = _sample_pairs(rng, evec[:len(core)], reactive,
+                                 core_reactions - n_fo_c - n_tb_c, n_tb_c, n_fo_c, major_core)
+    seen = {frozenset((a, b)) for a, b in pairs}
+    # family reactions: candidates by class, sampled per family in proportion
+    # to its size; every member takes part in at least one reaction
+    n_fam_rx = n_reactions - core_reactions
+    n_fam_sp = sum(len(m) for m in members)
+    n_fo = int(round(frac_falloff * n_reactions)) - n_fo_c
+    n_tb = int(round(frac_third_body * n_reactions)) - n_tb_c
+    fam_pairs, fam_dis = [], []  # (reac, prod), dissociation/association class?
+    for f, idx in enumerate(members):
+        n_c = fams[f][0]
+        target = int(round(n_fam_rx * len(idx) / n_fam_sp)) if f < len(members) - 1 else \
+            n_fam_rx - len(fam_pairs)
+        cands = {"transfer": [], "dissoc": [], "isomer": [], "scission": []}
+        for a in idx:
+            for b in idx:
+                if a == b:
+                    continue
+                d = tuple(evec[a] - evec[b])
+                if not any(d):
+                    if a < b:
+                        cands["isomer"].append(((a,), (b,)))
+                    continue
+                for y in single.get(d, ()):
+                    cands["dissoc"].append(((a,), (b, y)))
+                if a < b:
+                    for x, y in diff.get(d, ()):
+                        cands["transfer"].append(((a, x), (b, y)))
+            if a not in idx[1:3]:
+                continue
+            for y in pool:
+                k = tuple(evec[a] - evec[y])
+                for g, b in by_key.get(k, ()):
+                    if ladder.get(fams[g][0]) == g and fams[g][0] < n_c and \
+                            b in members[g][:4]:
+                        cands["scission"].append(((a,), (b, y)))
+        weights = dict(zip(("transfer", "dissoc", "isomer", "scission"), class_weights))
+        order = {c: list(rng.permutation(len(v))) for c, v in cands.items()}
+        chosen = []
+
+        def take(cls, must=None):
+            lst, perm = cands[cls], order[cls]
+            for pos, ci in enumerate(perm):
+                r = lst[ci]
+                if must is not None and must not in r[0] + r[1]:
+                    continue
+                sig = frozenset(r)
+                if sig in seen:
+                    continue
+                seen.add(sig)
+                perm.pop(pos)
+                chosen.append((r, cls in ("dissoc", "scission")))
+                return True
+            return False
+
+        for a in rng.permutation(idx):
+            if any(a in r[0] + r[1] for r, _ in chosen):
+                continue
+            for cls in rng.permutation(list(weights)):
+                if take(str(cls), int(a)):
+                    break
+        classes = list(weights)
+        p = np.array([weights[c] for c in classes])
+        while len(chosen) < target:
+            live = [i for i, c in enumerate(classes) if order[c]]
+            if not live:
+                raise ValueError(f"family {f}: candidates exhausted")
+            q = p[live] / p[live].sum()
+            take(classes[live[int(rng.choice(len(live), p=q))]])
+        for r, dis in chosen:
+            fam_pairs.append(r)
+            fam_dis.append(dis)
+    # kinds: third-body and fall-off among the association/dissociation class
+    dis_idx = [i for i, d in enumerate(fam_dis) if d]
+    if len(dis_idx) < n_tb + n_fo:
+        raise ValueError("too few dissociation reactions for the third-body/fall-off mix")
+    pick = rng.permutation(dis_idx)
+    fam_kinds = ["elem"] * len(fam_pairs)
+    for i in pick[:n_tb]:
+        fam_kinds[i] = "tb"
+    for i in pick[n_tb:n_tb + n_fo]:
+        fam_kinds[i] = "falloff"
+    pairs = list(pairs) + fam_pairs
+    kinds = list(kinds) + fam_kinds
+    for (a, b) in pairs:
+        assert np.array_equal(evec[list(a)].sum(axis=0), evec[list(b)].sum(axis=0))
+    order_rx = rng.permutation(len(pairs))
+    eff_species = list(core)
+    reactions = [_make_reaction(rng, pairs[i], kinds[i], species, h298, eff_species)
+                 for i in order_rx]
+    fuels = [species[members[f][0]] for f in
+             sorted(range(len(fams)), key=lambda f: (-fams[f][0], f))[:n_fuels]]
+    return Mechanism(elements, species, comp, thermo, reactions, seed,
+                     list(_C2_BASE) + fuels)
 
 
 # ----------------------------------------------------------------------------
@@ -448,7 +654,8 @@ WORKLOADS = ("10sp", "53sp", "160sp", "2000sp")
 def workload_mechanism(name: str, seed: int | None = None) -> Mechanism:
     """Seeded synthetic mechanism of a named workload: "10sp" (H/O/N, 27 reactions),
     "53sp" (C/H/O/N/Ar, 325 reactions), "160sp" (C/H/O/N/Ar, 900 reactions) or
-    "2000sp" (C/H/O/N/Ar, 8,000 reactions, sampled element-balanced sides)."""
+    "2000sp" (C/H/O/N/Ar, 8,000 reactions: a 120-species core and families of
+    large species, generate_families)."""
     if name == "10sp":
         base = ["H2", "O2", "H2O", "OH", "H", "O", "HO2", "H2O2", "O3", "N2"]
         return generate(10, 27, seed if seed is not None else 1001, ("H", "O", "N"),
@@ -469,9 +676,7 @@ def workload_mechanism(name: str, seed: int | None = None) -> Mechanism:
                         ("C", "H", "O", "N", "AR"), 0.30, 0.10, inert=("N2", "AR"),
                         majors=_C2_BASE + ["C2H6", "CH3OH"], base=_C2_BASE)
     if name == "2000sp":
-        return generate(2000, 8000, seed if seed is not None else 2001,
-                        ("C", "H", "O", "N", "AR"), 0.25, 0.10, inert=("N2", "AR"),
-                        majors=_C2_BASE + ["C2H6", "CH3OH", "C2H4", "C3H8"], base=_C2_BASE)
+        return generate_families(2000, 8000, seed if seed is not None else 2001)
     raise ValueError(f"unknown workload {name}")
 
 
