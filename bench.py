@@ -105,7 +105,7 @@ def cpu_baseline(mech, therm, st, dt, rtol, atol, seconds):
     # an evenly spaced sample keeps both temperature modes represented
     idx = np.linspace(0, len(st.T) - 1, n).astype(int)
     t0 = time.perf_counter()
-    _, _, stats = o.integrate(st.T[idx], st.P[idx], st.Y[idx], dt, opts, nthreads=threads)
+    To, Yo, stats = o.integrate(st.T[idx], st.P[idx], st.Y[idx], dt, opts, nthreads=threads)
     el = time.perf_counter() - t0
     # one thread: a smaller evenly spaced sample (about a third of the time)
     n1 = int(max(4, min(len(st.T), n * (seconds / 3.0) / (el * threads))))
@@ -114,6 +114,7 @@ def cpu_baseline(mech, therm, st, dt, rtol, atol, seconds):
     o.integrate(st.T[idx1], st.P[idx1], st.Y[idx1], dt, opts, nthreads=1)
     el1 = time.perf_counter() - t0
     return {"value": n / el, "unit": "reactors/s", "cores": threads, "kind": "port",
+            "_idx": idx, "_end": (To, Yo),
             "sample": f"{n} evenly spaced reactors of the same {len(st.T):,}-reactor batch, "
                       f"{el:.1f} s on {threads} threads (OpenMP, one reactor per thread)",
             "one_core": {"value": n1 / el1, "unit": "reactors/s",
@@ -149,8 +150,8 @@ def main():
     st = mechgen.workload_states(wl, mg, n, seed=2)
     dt, rtol, atol = mechgen.WORKLOAD_DT[wl], 1.0e-6, 1.0e-12
     nsp = m.nsp
-    has_plain = nsp <= 63  # above 63 species the integrator runs the FP32 method only
-    sparse = nsp > 191     # above 191 species: the sparse path
+    sparse = nsp > 191     # above 191 species: the sparse path (FP32 LU only)
+    has_plain = not sparse
     sp_nnz = sp_lev = sp_flops = 0
     if sparse:
         sp_nnz, sp_lev, sp_flops = m.sparse_plan_info()
@@ -269,11 +270,30 @@ def main():
                                  "multiply_adds_per_factorisation": sp_flops,
                                  "note": "ops = FP64 RHS and Jacobian + FP32 LU and solves"}
     rate = cpu = None
-    if not sparse:
-        rate = rate_kernel(m, n, nsp, T0, P0, Y0, sp, stream, W)
+    rate = rate_kernel(m, n, nsp, T0, P0, Y0, sp, stream, W)
+    vs_oracle = None
     if not a.no_cpu and not sparse:
         cpu = cpu_baseline(mech, therm, st, dt, rtol, atol, a.cpu_seconds)
         log(f"cpu baseline {cpu['value']:.1f} reactors/s on {cpu['cores']} threads")
+        # end states of both methods against the FP64 oracle on the same sample
+        idx = cpu.pop("_idx")
+        To, Yo = cpu.pop("_end")
+
+        def errors(R):
+            rel = np.abs(R["Y"][idx] - Yo) / np.maximum(np.abs(Yo), atol / rtol)
+            relT = np.abs(R["T"][idx] - To) / To
+            u = np.maximum(np.max(np.abs(R["Y"][idx] - Yo) / (rtol * np.abs(Yo) + atol), axis=1),
+                           relT / rtol)
+            return {"max_rel_err_Y": float(rel.max()), "rms_rel_err_Y": float(np.sqrt(np.mean(rel**2))),
+                    "max_rel_err_T": float(relT.max()), "rms_rel_err_T": float(np.sqrt(np.mean(relT**2))),
+                    "tol_units_p50_p99_max": [round(float(np.percentile(u, q)), 3) for q in (50, 99, 100)]}
+
+        vs_oracle = {"reactors": int(len(idx)),
+                     "note": "relative errors with |Y| floored at atol/rtol; tolerance units "
+                             "|y - y_oracle| / (rtol |y_oracle| + atol)",
+                     "chosen": errors(chosen)}
+        if plain:
+            vs_oracle["plain_fp64"] = errors(plain)
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "reactors/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(chosen["ms_per_step"], 3),
@@ -302,6 +322,7 @@ def main():
         "plain_fp64_one_ulp_T_sensitivity_tol_units": sensitivity,
         "roofline": roofline,
         "rate_kernel": rate,
+        "end_state_error_vs_fp64_oracle": vs_oracle,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -326,7 +347,11 @@ def rate_kernel(m, n, nsp, T0, P0, Y0, sp, stream, W):
     rate_ms = e0.elapsed_time(e1) / reps
     rbytes = workcount.rate_bytes_per_reactor(nsp, m.nstep) * n
     rate_gbps = rbytes / (rate_ms / 1e3) / 1e9
-    rate = {"kernel": "rates_kernel<false>", "bound": "hbm", "achieved": round(rate_gbps, 1),
+    # one wavefront per reactor while its rate arrays fit in a quarter of LDS,
+    # else one workgroup per reactor (the kernels' own rule)
+    block = 8 * 4 * (3 * nsp + 2 + m.nstep + 1) > 160 * 1024
+    rate = {"kernel": "rates_block_kernel<false>" if block else "rates_kernel<false>",
+            "bound": "hbm", "achieved": round(rate_gbps, 1),
             "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(rate_gbps / HBM_PEAK_GBPS, 4),
             "bytes_per_launch": int(rbytes), "kernel_ms": round(rate_ms, 4),
             "fp64_ops_per_launch": int(W.rhs_ops() * n)}

@@ -30,6 +30,22 @@ def mech_paths(name):
     return os.path.join(MECHS, name + ".mech"), os.path.join(MECHS, name + ".therm")
 
 
+def plugin_solver_options(p):
+    """The linear-solver options a plugin handle integrates with (the
+    reference's defaults unless set), as keyword arguments of
+    capi.default_options, so that a device-path call reproduces the plugin."""
+    kw = {}
+    for k in ("dense", "analytic", "iterative"):
+        rc, v = p.get_int(k)
+        assert rc == 0, k
+        kw[k] = v
+    for k in ("eps_lin", "preconditioner_threshold"):
+        rc, v = p.get_double(k)
+        assert rc == 0, k
+        kw[k] = v
+    return kw
+
+
 def load_golden(name):
     with np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False) as z:
         return {k: z[k] for k in z.files}

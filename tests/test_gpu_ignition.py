@@ -19,7 +19,7 @@ root finding, through zrk_integrate_device_aux and the plugin.
 import numpy as np
 import pytest
 
-from conftest import mech_paths
+from conftest import mech_paths, plugin_solver_options
 
 pytestmark = pytest.mark.gpu
 
@@ -123,7 +123,8 @@ def test_plugin_ignition_time_field(torch_gpu):
     assert p.solve(0, 0.0, 1e-4, T, P, Y) == 0
     m = capi.Mech(*mech_paths("synth_10sp"))
     Td, Yd, sd, td = _run(torch, m, st, 1e-4, capi.default_options(
-        rel_tol=1e-6, abs_tol=1e-12, delta_temperature_ignition=DT_IGN))
+        rel_tol=1e-6, abs_tol=1e-12, delta_temperature_ignition=DT_IGN,
+        **plugin_solver_options(p)))
     np.testing.assert_array_equal(ign, td)
     np.testing.assert_array_equal(cost, sd)
     np.testing.assert_array_equal(T, Td)

@@ -8,7 +8,7 @@ seconds per reactor, FAILED_SOLVE with the input state kept on failure.
 import numpy as np
 import pytest
 
-from conftest import mech_paths
+from conftest import mech_paths, plugin_solver_options
 
 pytestmark = pytest.mark.gpu
 
@@ -46,7 +46,8 @@ def test_plugin_solve_matches_device_path(torch_gpu, cvol):
     dT, dP, dY = t(st.T), t(st.P), t(st.Y)
     steps = torch.zeros(n, dtype=torch.int32, device=dev)
     m.integrate(n, 1e-4, dT, dP, dY, nsp,
-                capi.default_options(rel_tol=1e-6, abs_tol=1e-12, constant_volume=cvol), steps)
+                capi.default_options(rel_tol=1e-6, abs_tol=1e-12, constant_volume=cvol,
+                                     **plugin_solver_options(p)), steps)
     torch.cuda.synchronize()
     np.testing.assert_array_equal(T, dT.cpu().numpy())
     np.testing.assert_array_equal(Y, dY.cpu().numpy())

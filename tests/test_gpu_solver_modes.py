@@ -45,9 +45,20 @@ MODES = {
 }
 
 
-def _compare(Tg, Yg, steps, To, Yo, ost, Tp, Yp, pst, f32, label):
-    gpu = _units(Yg, Tg, Yo, To, RTOL, ATOL)
-    ulp = _units(Yp, Tp, Yo, To, RTOL, ATOL)
+def _abstol_dens(o, T, P, Y, atol):
+    """The species absolute tolerances that abstol_dens = 1 gives each reactor:
+    atol / (rho0 / MW_j 1e-3), rho0 the initial density
+    (reactor_nvector_serial.cpp:107-120)."""
+    mw = o.tables.mw
+    rho0 = P / (8.314462618e3 * T) / np.sum(Y / mw, axis=1)
+    return atol * mw[None, :] * 1.0e3 / rho0[:, None]
+
+
+def _compare(Tg, Yg, steps, To, Yo, ost, Tp, Yp, pst, f32, label, atol=ATOL):
+    """atol: the absolute tolerance of the run (a scalar, or per reactor and
+    species), so that differences are counted in the run's own units."""
+    gpu = _units(Yg, Tg, Yo, To, RTOL, atol)
+    ulp = _units(Yp, Tp, Yo, To, RTOL, atol)
     q = [50, 90, 100]
     gq, uq = np.percentile(gpu, q), np.percentile(ulp, q)
     print(f"{label}: GPU-oracle tolerance units p50/p90/max {gq.round(2)}; oracle 1-ulp "
@@ -100,7 +111,8 @@ def test_teams_solver_modes(torch_gpu, oracle_mod, mode):
     oo = oracle_mod.default_opts(rtol=RTOL, atol=ATOL, **kw)
     To, Yo, ost = o.integrate(st.T, st.P, st.Y, dt, oo)
     Tp, Yp, pst = o.integrate(np.nextafter(st.T, np.inf), st.P, st.Y, dt, oo)
-    _compare(Tg, Yg, steps, To, Yo, ost, Tp, Yp, pst, 1, f"dme {mode}")
+    _compare(Tg, Yg, steps, To, Yo, ost, Tp, Yp, pst, 1, f"dme {mode}",
+             atol=_abstol_dens(o, st.T, st.P, st.Y, ATOL))
 
 
 @pytest.mark.parametrize("cvol", [0, 1])
@@ -123,7 +135,8 @@ def test_abstol_dens(torch_gpu, oracle_mod, cvol):
     plain = o.integrate(st.T, st.P, st.Y, 1e-4,
                         oracle_mod.default_opts(rtol=RTOL, atol=1e-12, constant_volume=cvol))
     assert ost["nsteps"].mean() < plain[2]["nsteps"].mean()
-    _compare(Tg, Yg, steps, To, Yo, ost, Tp, Yp, pst, 0, f"abstol_dens cvol={cvol}")
+    _compare(Tg, Yg, steps, To, Yo, ost, Tp, Yp, pst, 0, f"abstol_dens cvol={cvol}",
+             atol=_abstol_dens(o, st.T, st.P, st.Y, 1e-12))
 
 
 def _tester_plugin(capi, name, opts):

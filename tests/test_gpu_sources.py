@@ -10,7 +10,7 @@ the device path's results bit for bit.
 import numpy as np
 import pytest
 
-from conftest import mech_paths
+from conftest import mech_paths, plugin_solver_options
 
 pytestmark = pytest.mark.gpu
 
@@ -137,7 +137,8 @@ def test_plugin_aux_sources_match_device_path(torch_gpu):
     dev = torch.device("cuda:0")
     t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
     dT, dP, dY = t(st.T), t(st.P), t(st.Y)
-    m.integrate(n, 1e-4, dT, dP, dY, nsp, capi.default_options(rel_tol=RTOL, abs_tol=ATOL),
+    m.integrate(n, 1e-4, dT, dP, dY, nsp, capi.default_options(rel_tol=RTOL, abs_tol=ATOL,
+                                                               **plugin_solver_options(p)),
                 dpdt=t(dpdt), esrc=t(esrc), ysrc=t(ysrc))
     torch.cuda.synchronize()
     np.testing.assert_array_equal(T, dT.cpu().numpy())

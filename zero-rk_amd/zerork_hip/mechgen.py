@@ -410,17 +410,22 @@ def generate(n_species: int, n_reactions: int, seed: int, elements=("H", "O", "N
     return Mechanism(elements, species, comp, thermo, reactions, seed, list(majors))
 
 
-def _make_reaction(rng, pair, kind, species, h298, eff_species):
+def _make_reaction(rng, pair, kind, species, h298, eff_species, bounded=False):
     """One reaction with the recipe's random rate parameters.  It is written in
     its exothermic direction: the random activation energy then belongs to the
     downhill direction and the reverse (from Keq) carries Ea + dH, as in
     physical mechanisms.  Third-body efficiencies go to 1-4 species drawn from
-    eff_species."""
+    eff_species.  bounded: the 10^U[8,14] range applies to A (1000 K)^beta,
+    the temperature factor at combustion temperatures, instead of A alone
+    (with beta up to 2 an unbounded A reaches 1e20 at 1000 K, far above the
+    gas-kinetic limit)."""
     a, b = pair
     if h298[list(b)].sum() > h298[list(a)].sum():
         a, b = b, a
     A = 10.0 ** rng.uniform(8.0, 14.0)
     beta = rng.uniform(-1.0, 2.0)
+    if bounded:
+        A /= 1000.0 ** beta
     Ea = rng.uniform(0.0, 40.0) * 1000.0
     r = Reaction([species[i] for i in a], [species[i] for i in b], kind, A, beta, Ea)
     if kind in ("tb", "falloff"):
@@ -485,9 +490,10 @@ def generate_families(n_species: int, n_reactions: int, seed: int, n_core: int =
     pool, isomerisation A <=> B); a family's two alkyl radicals also split into
     a pool fragment and the alkane, an alkyl or the olefin of the first family
     of a smaller carbon count (beta scission down a ladder of skeletons).
-    Elimination therefore fills families and the core, not the whole matrix.  Rate parameters, kinds and
-    thermo follow the module recipe; third-body efficiencies go to core
-    species.  The majors are the core's eight base species and the alkanes of
+    Elimination therefore fills families and the core, not the whole matrix.
+    Kinds and thermo follow the module recipe, rate parameters too except
+    that A (1000 K)^beta, not A, is drawn from 10^U[8,14] (_make_reaction,
+    bounded); third-body efficiencies go to core species.  The majors are the core's eight base species and the alkanes of
     the n_fuels largest families."""
     rng = np.random.default_rng(seed)
     elements = ["C", "H", "O", "N", "AR"]
@@ -634,8 +640,8 @@ def generate_families(n_species: int, n_reactions: int, seed: int, n_core: int =
         assert np.array_equal(evec[list(a)].sum(axis=0), evec[list(b)].sum(axis=0))
     order_rx = rng.permutation(len(pairs))
     eff_species = list(core)
-    reactions = [_make_reaction(rng, pairs[i], kinds[i], species, h298, eff_species)
-                 for i in order_rx]
+    reactions = [_make_reaction(rng, pairs[i], kinds[i], species, h298, eff_species,
+                                bounded=True) for i in order_rx]
     fuels = [species[members[f][0]] for f in
              sorted(range(len(fams)), key=lambda f: (-fams[f][0], f))[:n_fuels]]
     return Mechanism(elements, species, comp, thermo, reactions, seed,
