@@ -132,8 +132,10 @@ ZRK_EXPORT int zrk_jacobian_device(zrk_mech m, const zrk_bdf_options* o, int n, 
 /* Integrate n reactors over [0, dt]; T and Y updated in place, and P at
  * constant volume (P = Ru T / (v Wmix) of the end state).  nsteps[r] (optional) = CVode one-step calls, negative on failure
  * (the plugin's COST output); stats[r*8] (optional) = nst nfe nje nsetups nni
- * ncfn netf status.  nsp <= 63, or nsp <= 191 with fp32_preconditioner = 1
- * (-3 otherwise); -2 if the mechanism does not fit.  Returns 0 or an error
+ * ncfn netf status.  Up to 63 species one wavefront per reactor, up to 191
+ * a workgroup of 2-3 wavefronts (the FP32 Newton matrix in LDS, the FP64 one
+ * in device memory), above that the sparse path (fp32_preconditioner = 1
+ * only, -3 otherwise); -2 if the mechanism does not fit.  Returns 0 or an error
  * code; the number of failed reactors is available from nsteps. */
 ZRK_EXPORT int zrk_integrate_device(zrk_mech m, const zrk_bdf_options* o, int n, double dt,
                                     double* T, double* P, double* Y, int stride,

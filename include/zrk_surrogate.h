@@ -4,8 +4,8 @@
  * reference integrates every reactor with CVODE).  They implement a small
  * MLP that proposes a reactor's end state, trained and evaluated with
  * BF16-input / FP32-accumulate MFMA kernels, an FP64 acceptance test of each
- * proposal, and the plain BDF integration of the rejected reactors (each
- * with its own dt).  Device pointers and a HIP stream, as in zrk_device.h.
+ * proposed trajectory, and the plain BDF integration of the rejected
+ * reactors (each with its own dt).  Device pointers and a HIP stream, as in zrk_device.h.
  * bf16 arrays are passed as void* (16-bit elements).  Return value: 0, a
  * negative code on invalid arguments, or a positive hipError_t.
  */
@@ -60,21 +60,31 @@ ZRK_EXPORT int zrk_mlp_adam(long n, float* w, const float* g, float* m, float* v
 /* y (bf16) = x (f32), round to nearest even. */
 ZRK_EXPORT int zrk_to_bf16(long n, const float* x, void* y, void* stream);
 
-/* FP64 acceptance test of proposed end states (T1, Y1) of the constant-
- * pressure step [0, dt] from (T0, Y0) at pressure P.  With newton > 0 the
- * proposal is first refined in place by that many Newton iterations on the
- * trapezoidal equation y - y0 - dt/2 (f(y0) + f(y)) = 0 (iteration matrix
- * I - dt/2 J at the proposal).  accept[r] = 1 when the trapezoidal implicit
- * residual y1 - y0 - dt/2 (f(y0) + f(y1)) and the error estimate
- * dt/2 (f(y1) - f(y0)) both have weighted RMS norm <= 1 in the integrator's
- * weights 1/(rel_tol |y1| + abs_tol).  res[2r], res[2r+1] (optional) receive
- * the two norms.  Uses the handle's workspace: calls on one handle must not
+/* FP64 acceptance test of proposed trajectories of the constant-pressure
+ * step [0, dt] from (T0, Y0) at pressure P (-4 with constant_volume set).
+ * The proposal gives the state at the nodes t_k = tau[k-1] dt, k = 1..nodes
+ * (3 <= nodes <= 8, tau a device array, increasing, tau[nodes-1] = 1):
+ * Tp[(k-1)*n + r], Yp[((k-1)*n + r)*stride + j].  Interval k (from t_{k-1},
+ * t_0 = 0 with the initial state) is the backward Euler equation (k = 1) or
+ * the trapezoidal one (k > 1), solved by at most max_newton Newton
+ * iterations that start from the proposal at t_k, with the iteration matrix
+ * I - c J (c = h or h/2, J the reference Jacobian at that proposal); it
+ * converges when the correction's weighted RMS norm is at most 0.1.  The
+ * converged nodes' local errors are estimated from divided differences of f
+ * over the nodes after t_0 ((h^2/2) f[t_1, t_2] for the first interval,
+ * (h^3/12) 2 f[t_a, t_b, t_c] for the others).  accept[r] = 1 when every
+ * interval converged and the componentwise sum of |local errors| has weighted
+ * RMS norm <= 1 in the integrator's weights 1 / (rel_tol |y| + abs_tol); the
+ * converged end state then replaces the last node's proposal.
+ * res[4r..4r+3] (optional): most Newton iterations of an interval, largest
+ * first correction, error norm, first failed interval (0: none, -1: the RHS
+ * failed).  Uses the handle's workspace: calls on one handle must not
  * overlap.  Requires nsp <= 63. */
 ZRK_EXPORT int zrk_surrogate_accept(zrk_mech m, const zrk_bdf_options* o, int n,
                                     const double* dt, double dt_all, const double* T0,
-                                    const double* P, const double* Y0, double* T1, double* Y1,
-                                    int stride, int newton, int* accept, double* res,
-                                    void* stream);
+                                    const double* P, const double* Y0, int nodes,
+                                    const double* tau, double* Tp, double* Yp, int stride,
+                                    int max_newton, int* accept, double* res, void* stream);
 
 #ifdef __cplusplus
 }

@@ -12,12 +12,16 @@ through zrk_integrate_device.
   count, sum(Y), the end pressure at constant volume, and accuracy against
   the LSODA referee on a subset (no worse than the oracle's own distance
   from it, plus the allowance the FP32 dense path has).
-* 2,000 species / 8,000 reactions (the large workload): every reactor
-  succeeds, sum(Y) = 1, the adiabatic constant-pressure invariant (the
-  mixture's mass enthalpy keeps its initial value to the integration
-  tolerance), and identical reactors give identical results.  The oracle
-  needs minutes per reactor at this size, so it is compared here through
-  these size-independent properties only.
+* 2,000 species / 8,000 reactions, the large workload at its full size
+  (4,096 reactors over dt = 1e-4 s): four reactors whose oracle end states
+  are committed (tests/golden/sparse_2000sp_fixture.npz, written by
+  make_sparse_fixture.py with the oracle's own one-ulp sensitivity beside
+  them; the oracle needs about a minute per reactor here) sit at four
+  places in the batch and meet the FP32 bar above against them; over the
+  whole batch every reactor succeeds, sum(Y) = 1, the adiabatic
+  constant-pressure invariant holds (the mixture's mass enthalpy keeps its
+  initial value to the integration tolerance), and the fixture reactors
+  integrated alone give bitwise the batch's results.
 * Options the path does not take are refused with the documented codes.
 * The drop-in plugin takes the sparse path for the 2,000-species mechanism by
   itself, with the device entry point's results bit for bit.
@@ -88,39 +92,52 @@ def _mass_enthalpy(o, T, Y):
     return np.sum(h * RU * T[:, None] / mw * Y, axis=1)
 
 
-def test_sparse_path_2000_species(torch_gpu, oracle_mod):
+def test_sparse_path_2000_species_workload(torch_gpu, oracle_mod):
     from zerork_hip import capi, mechgen
+    from conftest import GOLDEN
+    import os
     name = "synth_2000sp"
     m = capi.Mech(*mech_paths(name))
     nnz, lev, flops = m.sparse_plan_info()
-    print(f"case sparse 2000sp: LU entries {nnz}, levels {lev}, multiply-adds {flops}",
+    print(f"case sparse 2000sp workload: LU entries {nnz}, levels {lev}, multiply-adds {flops}",
           flush=True)
-    o = oracle_mod.Oracle(*mech_paths(name))
-    # a short interval: about 100 steps per reactor (a bounded single-reactor
-    # run takes 457 steps, 73 factorisations and 7.8 s over 1e-5 s)
-    n = 8
-    st = mechgen.workload_states("2000sp", mechgen.workload_mechanism("2000sp"), n, seed=62)
-    st.T[n // 2:] = st.T[:n // 2]
-    st.P[n // 2:] = st.P[:n // 2]
-    st.Y[n // 2:] = st.Y[:n // 2]
-    dt = 1.0e-6
+    with np.load(os.path.join(GOLDEN, "sparse_2000sp_fixture.npz"), allow_pickle=False) as z:
+        fx = {k: z[k] for k in z.files}
+    n = mechgen.WORKLOAD_NR["2000sp"]
+    dt = float(fx["dt"])
+    st = mechgen.workload_states("2000sp", mechgen.workload_mechanism("2000sp"), n, seed=2)
+    at = np.array([0, n // 3, (2 * n) // 3, n - 1])
+    st.T[at], st.P[at], st.Y[at] = fx["T0"], fx["P0"], fx["Y0"]
     opts = capi.default_options(rel_tol=RTOL, abs_tol=ATOL, constant_volume=0,
                                 fp32_preconditioner=1)
     t0 = time.perf_counter()
     Tg, Yg, steps, stats = _gpu_integrate(torch_gpu, m, st.T, st.P, st.Y, dt, opts)
-    print(f"2000sp: {time.perf_counter() - t0:.1f} s; steps mean {steps.mean():.1f} max "
-          f"{steps.max()}; setups {stats[:, 3].mean():.1f}; T {st.T[:3]} -> {Tg[:3]}",
-          flush=True)
+    el = time.perf_counter() - t0
+    print(f"2000sp: {n} reactors in {el:.1f} s; steps mean {steps.mean():.1f} max "
+          f"{steps.max()}; setups {stats[:, 3].mean():.1f}", flush=True)
     assert np.all(steps > 0) and np.all(stats[:, 7] == 0)
-    np.testing.assert_array_equal(Tg[n // 2:], Tg[:n // 2])
-    np.testing.assert_array_equal(Yg[n // 2:], Yg[:n // 2])
+    # the fixture reactors against the oracle, in tolerance units
+    gpu = _units(Yg[at], Tg[at], fx["Y"], fx["T"], RTOL, ATOL)
+    ulp = _units(fx["Y_ulp"], fx["T_ulp"], fx["Y"], fx["T"], RTOL, ATOL)
+    print(f"2000sp fixture: GPU-oracle tolerance units {gpu.round(2)}; oracle 1-ulp "
+          f"{ulp.round(2)}; steps gpu {steps[at]} oracle {fx['nsteps']}", flush=True)
+    assert np.all(gpu <= 4.0 * np.maximum(ulp, 1.0) + 50.0)
+    assert abs(steps[at].mean() / fx["nsteps"].mean() - 1.0) < 0.1
+    # the same reactors alone: bitwise the batch's results
+    Ta, Ya, sa, _ = _gpu_integrate(torch_gpu, m, fx["T0"], fx["P0"], fx["Y0"], dt, opts)
+    np.testing.assert_array_equal(Ta, Tg[at])
+    np.testing.assert_array_equal(Ya, Yg[at])
+    np.testing.assert_array_equal(sa, steps[at])
+    # whole batch: sum(Y) and the adiabatic constant-pressure invariant
     assert np.all(np.abs(Yg.sum(axis=1) - 1.0) <= 1e-8)
-    h0, h1 = _mass_enthalpy(o, st.T, st.Y), _mass_enthalpy(o, Tg, Yg)
-    cp = np.array([np.sum(o.thermo(t)[0] * RU / o.tables.mw * y) for t, y in zip(Tg, Yg)])
-    # enthalpy drift in kelvin-equivalents against the tolerance on T
+    o = oracle_mod.Oracle(*mech_paths(name))
+    sub = np.linspace(0, n - 1, 256).astype(int)
+    h0, h1 = _mass_enthalpy(o, st.T[sub], st.Y[sub]), _mass_enthalpy(o, Tg[sub], Yg[sub])
+    cp = np.array([np.sum(o.thermo(t)[0] * RU / o.tables.mw * y)
+                   for t, y in zip(Tg[sub], Yg[sub])])
     drift = np.abs(h1 - h0) / cp
-    print(f"2000sp: enthalpy drift / cp max {drift.max():.3e} K", flush=True)
-    assert np.all(drift <= 50.0 * (RTOL * Tg + ATOL))
+    print(f"2000sp: enthalpy drift / cp max {drift.max():.3e} K (256 reactors)", flush=True)
+    assert np.all(drift <= 50.0 * (RTOL * Tg[sub] + ATOL))
 
 
 def test_sparse_path_refuses_unsupported(torch_gpu):

@@ -360,8 +360,11 @@ __device__ __forceinline__ bool gj_invert_lds(const Team& team, MT* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// Blocked Gauss-Jordan inverse (no pivoting) on FP32 MFMA for a team of W
-// wavefronts: the N x N matrix A in LDS, column-major with column stride js
+// Blocked Gauss-Jordan inverse (no pivoting) on MFMA for a team of W
+// wavefronts (FP32 tiles for the FP32 Newton matrix, FP64 tiles for the FP64
+// one): the N x N matrix A (in LDS, or for FP64 teams in the team's slot of
+// device memory, which stays in the last-level cache), column-major with
+// column stride js
 // (element (r, c) at A[c*js + r]), is inverted in place in 16 x 16 tiles.  The
 // padding to a multiple of 16 is the identity and is never stored (tile
 // loads make it up, stores beyond N are dropped).  Block step kb with the
@@ -371,41 +374,72 @@ __device__ __forceinline__ bool gj_invert_lds(const Team& team, MT* __restrict__
 //   other tiles    A[I,J] <- A[I,J] - A[I,K] A[K,J]      (I != K, J != K)
 //                  A[I,K] <- -A[I,K] D^-1                (I != K)
 // with the tiles of each phase spread over the wavefronts and a barrier
-// between phases.  v_mfma_f32_16x16x4f32 operand layouts (one 16 x 16 block):
+// between phases.  v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64 operand
+// layouts (one 16 x 16 block):
 //   A[i][k] (16 x 4): lane l holds i = l % 16, k = l / 16
 //   B[k][j] (4 x 16): lane l holds k = l / 16, j = l % 16
-//   C[i][j] (16 x 16): register v of lane l holds i = 4 (l / 16) + v, j = l % 16
-// flag: one float of LDS for the zero-pivot result.  Returns false on a zero
+//   C[i][j] (16 x 16): register v of lane l holds j = l % 16 and i = 4 (l / 16) + v
+//                      (FP32) or i = l / 16 + 4 v (FP64)
+// flag: one element of LDS for the zero-pivot result.  Returns false on a zero
 // pivot of a diagonal-tile inversion.
 // ---------------------------------------------------------------------------
-typedef float f32x4 __attribute__((ext_vector_type(4)));
+// 16 x 16 x 4 MFMA by element type: v_mfma_f32_16x16x4f32 for the FP32
+// Newton matrices, v_mfma_f64_16x16x4f64 for the FP64 ones.  Both take one A
+// element (row lane % 16, k lane / 16) and one B element (k lane / 16, column
+// lane % 16) per lane and accumulate the 16 x 16 tile as 4 elements per lane
+// in column lane % 16, but in different rows (row(lk, v), lk = lane / 16):
+// the FP32 instruction holds rows 4 lk + v, the FP64 one rows lk + 4 v
+// (checked on gfx950 against a host product).
+template <typename T>
+struct Mfma16;
+template <>
+struct Mfma16<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mac(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
+};
+template <>
+struct Mfma16<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mac(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
+};
 
-__device__ __forceinline__ float tile_elem(const float* __restrict__ A, int js, int N, int r,
-                                           int c) {
-  return (r < N && c < N) ? A[c * js + r] : (r == c ? 1.0f : 0.0f);
+template <typename T>
+__device__ __forceinline__ T tile_elem(const T* __restrict__ A, int js, int N, int r, int c) {
+  return (r < N && c < N) ? A[c * js + r] : (r == c ? T(1) : T(0));
 }
+
+// reciprocal of a pivot: the FP32 inverse only preconditions the Newton
+// iteration (its residual and convergence test stay in FP64), so the
+// hardware reciprocal (1 ulp) serves; the FP64 inverse divides exactly
+__device__ __forceinline__ float pivot_recip(float p) { return __builtin_amdgcn_rcpf(p); }
+__device__ __forceinline__ double pivot_recip(double p) { return 1.0 / p; }
 
 // In-register Gauss-Jordan inverse of a 16 x 16 matrix held one row per lane
 // (lanes 0..15; other lanes compute values that are not used), with the
 // deferred row scaling of gj_invert.
-__device__ __forceinline__ bool gj_invert16(float (&d)[16]) {
+template <typename T>
+__device__ __forceinline__ bool gj_invert16(T (&d)[16]) {
   const int lane = lane_id();
   bool ok = true;
-  float s = 1.0f;
+  T s = T(1);
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
-    const float piv = bcast(d[k], k);
-    if (piv == 0.0f) ok = false;
-    // hardware reciprocal (1 ulp): this inverse only preconditions the Newton
-    // iteration, whose residual and convergence test stay in FP64
-    const float rp = __builtin_amdgcn_rcpf(piv);
+    const T piv = bcast(d[k], k);
+    if (piv == T(0)) ok = false;
+    const T rp = pivot_recip(piv);
     const bool me = (lane == k);
-    const float g = me ? 0.0f : d[k] * rp;
+    const T g = me ? T(0) : d[k] * rp;
     if (me) s = rp;
 #pragma unroll
     for (int c = 0; c < 16; ++c)
-      if (c != k) d[c] = fmaf(-g, bcast(d[c], k), d[c]);
-    d[k] = me ? 1.0f : -g;
+      if (c != k) d[c] = fma(-g, bcast(d[c], k), d[c]);
+    d[k] = me ? T(1) : -g;
   }
 #pragma unroll
   for (int c = 0; c < 16; ++c) d[c] *= s;
@@ -416,44 +450,47 @@ __device__ __forceinline__ bool gj_invert16(float (&d)[16]) {
 // columns J0..): every access of a lane is in its column J0 + (lane % 16).
 // full: the tiles lie inside the N x N matrix (uniform); otherwise elements
 // beyond N read as the identity and are not stored.
-template <bool kFull>
-__device__ __forceinline__ void mfma_tile_update(float* __restrict__ A, int js, int N, int I0,
-                                                 int J0, int K0, const float (&a)[4]) {
+template <bool kFull, typename T>
+__device__ __forceinline__ void mfma_tile_update(T* __restrict__ A, int js, int N, int I0,
+                                                 int J0, int K0, const T (&a)[4]) {
   const int lane = lane_id();
   const int li = lane & 15, lk = lane >> 4;
   const int c = J0 + li;
-  float* __restrict__ col = A + c * js;
-  f32x4 acc;
-  float b[4];
+  T* __restrict__ col = A + c * js;
+  typename Mfma16<T>::v4 acc;
+  T b[4];
 #pragma unroll
-  for (int v = 0; v < 4; ++v)
-    acc[v] = kFull ? col[I0 + 4 * lk + v] : tile_elem(A, js, N, I0 + 4 * lk + v, c);
+  for (int v = 0; v < 4; ++v) {
+    const int r = I0 + Mfma16<T>::row(lk, v);
+    acc[v] = kFull ? col[r] : tile_elem(A, js, N, r, c);
+  }
 #pragma unroll
   for (int s4 = 0; s4 < 4; ++s4)
     b[s4] = kFull ? col[K0 + 4 * s4 + lk] : tile_elem(A, js, N, K0 + 4 * s4 + lk, c);
 #pragma unroll
-  for (int s4 = 0; s4 < 4; ++s4) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s4], b[s4], acc, 0, 0, 0);
+  for (int s4 = 0; s4 < 4; ++s4) acc = Mfma16<T>::mac(a[s4], b[s4], acc);
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    const int r = I0 + 4 * lk + v;
+    const int r = I0 + Mfma16<T>::row(lk, v);
     if (kFull || (r < N && c < N)) col[r] = acc[v];
   }
 }
 
 // Two full-tile updates (columns J0.. and J1..) interleaved: all loads first,
 // then the two independent MFMA chains, then the stores.
-__device__ __forceinline__ void mfma_tile_update2(float* __restrict__ A, int js, int I0, int J0,
-                                                  int J1, int K0, const float (&a)[4]) {
+template <typename T>
+__device__ __forceinline__ void mfma_tile_update2(T* __restrict__ A, int js, int I0, int J0,
+                                                  int J1, int K0, const T (&a)[4]) {
   const int lane = lane_id();
   const int li = lane & 15, lk = lane >> 4;
-  float* __restrict__ c0 = A + (J0 + li) * js;
-  float* __restrict__ c1 = A + (J1 + li) * js;
-  f32x4 x0, x1;
-  float b0[4], b1[4];
+  T* __restrict__ c0 = A + (J0 + li) * js;
+  T* __restrict__ c1 = A + (J1 + li) * js;
+  typename Mfma16<T>::v4 x0, x1;
+  T b0[4], b1[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    x0[v] = c0[I0 + 4 * lk + v];
-    x1[v] = c1[I0 + 4 * lk + v];
+    x0[v] = c0[I0 + Mfma16<T>::row(lk, v)];
+    x1[v] = c1[I0 + Mfma16<T>::row(lk, v)];
   }
 #pragma unroll
   for (int s4 = 0; s4 < 4; ++s4) {
@@ -462,62 +499,63 @@ __device__ __forceinline__ void mfma_tile_update2(float* __restrict__ A, int js,
   }
 #pragma unroll
   for (int s4 = 0; s4 < 4; ++s4) {
-    x0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s4], b0[s4], x0, 0, 0, 0);
-    x1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s4], b1[s4], x1, 0, 0, 0);
+    x0 = Mfma16<T>::mac(a[s4], b0[s4], x0);
+    x1 = Mfma16<T>::mac(a[s4], b1[s4], x1);
   }
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    c0[I0 + 4 * lk + v] = x0[v];
-    c1[I0 + 4 * lk + v] = x1[v];
+    c0[I0 + Mfma16<T>::row(lk, v)] = x0[v];
+    c1[I0 + Mfma16<T>::row(lk, v)] = x1[v];
   }
 }
 
 // C (tile at rows R0.., columns J0..) = a * (tile at rows K0.., columns J0..)
-template <bool kFull>
-__device__ __forceinline__ void mfma_tile_product(float* __restrict__ A, int js, int N, int R0,
-                                                  int J0, int K0, const float (&a)[4]) {
+template <bool kFull, typename T>
+__device__ __forceinline__ void mfma_tile_product(T* __restrict__ A, int js, int N, int R0,
+                                                  int J0, int K0, const T (&a)[4]) {
   const int lane = lane_id();
   const int li = lane & 15, lk = lane >> 4;
   const int c = J0 + li;
-  float* __restrict__ col = A + c * js;
-  float b[4];
+  T* __restrict__ col = A + c * js;
+  T b[4];
 #pragma unroll
   for (int s4 = 0; s4 < 4; ++s4)
     b[s4] = kFull ? col[K0 + 4 * s4 + lk] : tile_elem(A, js, N, K0 + 4 * s4 + lk, c);
-  f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  typename Mfma16<T>::v4 acc = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-  for (int s4 = 0; s4 < 4; ++s4) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s4], b[s4], acc, 0, 0, 0);
+  for (int s4 = 0; s4 < 4; ++s4) acc = Mfma16<T>::mac(a[s4], b[s4], acc);
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    const int r = R0 + 4 * lk + v;
+    const int r = R0 + Mfma16<T>::row(lk, v);
     if (kFull || (r < N && c < N)) col[r] = acc[v];
   }
 }
 
 // the A operand of tile (rows I0.., columns K0..), times sign
-__device__ __forceinline__ void mfma_a_operand(const float* __restrict__ A, int js, int N, int I0,
-                                               int K0, float sign, float (&a)[4]) {
+template <typename T>
+__device__ __forceinline__ void mfma_a_operand(const T* __restrict__ A, int js, int N, int I0,
+                                               int K0, T sign, T (&a)[4]) {
   const int lane = lane_id();
   const int li = lane & 15, lk = lane >> 4;
 #pragma unroll
   for (int s4 = 0; s4 < 4; ++s4) a[s4] = sign * tile_elem(A, js, N, I0 + li, K0 + 4 * s4 + lk);
 }
 
-template <class Team>
-__device__ __forceinline__ bool gj_invert_mfma(const Team& team, float* __restrict__ A, int js,
-                                               int N, float* __restrict__ flag) {
+template <class Team, typename T>
+__device__ __forceinline__ bool gj_invert_mfma(const Team& team, T* __restrict__ A, int js,
+                                               int N, T* __restrict__ flag) {
   constexpr int kWaves = Team::kThreads / kWave;
   const int wave = team.tid() / kWave;
   const int lane = lane_id();
   const int li = lane & 15;
   const int nb = (N + 15) >> 4;
   const int nfull = N >> 4;  // tiles 0..nfull-1 lie inside the matrix
-  if (team.tid() == 0) *flag = 1.0f;
+  if (team.tid() == 0) *flag = T(1);
   team.sync();
   for (int kb = 0; kb < nb; ++kb) {
     const int K0 = kb * 16;
     if (wave == 0) {
-      float d[16];
+      T d[16];
 #pragma unroll
       for (int c = 0; c < 16; ++c) d[c] = tile_elem(A, js, N, K0 + li, K0 + c);
       const bool ok = gj_invert16(d);
@@ -526,12 +564,12 @@ __device__ __forceinline__ bool gj_invert_mfma(const Team& team, float* __restri
         for (int c = 0; c < 16; ++c)
           if (K0 + c < N) A[(K0 + c) * js + K0 + li] = d[c];
       }
-      if (!ok && lane == 0) *flag = 0.0f;
+      if (!ok && lane == 0) *flag = T(0);
     }
     team.sync();
     // row tiles: A[K,J] <- D^-1 A[K,J]
-    float dk[4];
-    mfma_a_operand(A, js, N, K0, K0, 1.0f, dk);
+    T dk[4];
+    mfma_a_operand(A, js, N, K0, K0, T(1), dk);
     for (int jb = wave; jb < nb; jb += kWaves) {
       if (jb == kb) continue;
       if (kb < nfull && jb < nfull)
@@ -544,8 +582,8 @@ __device__ __forceinline__ bool gj_invert_mfma(const Team& team, float* __restri
     for (int ib = wave; ib < nb; ib += kWaves) {
       if (ib == kb) continue;
       const int I0 = ib * 16;
-      float ak[4];
-      mfma_a_operand(A, js, N, I0, K0, -1.0f, ak);
+      T ak[4];
+      mfma_a_operand(A, js, N, I0, K0, T(-1), ak);
       const bool rin = ib < nfull && kb < nfull;
       // full tiles in pairs (skipping column block kb), then the rest one by one
       int jb = 0;
@@ -573,7 +611,7 @@ __device__ __forceinline__ bool gj_invert_mfma(const Team& team, float* __restri
     }
     team.sync();
   }
-  return *flag != 0.0f;
+  return *flag != T(0);
 }
 
 }  // namespace zrk

@@ -277,12 +277,14 @@ bdf_kernel(ZRK_BDF_KERNEL_ARGS) {
 }
 #undef ZRK_BDF_KERNEL_ARGS
 
-// W wavefronts per reactor (64 <= nsp < 64W), FP32 preconditioner.  LDS
-// (bdf_team_lds_bytes): CvUni[W] (one per wavefront), Mlds[(nsp+1)*(64W+1)]
-// float, prow[64W], rvec[64W] float, t2l[64W], hl[64W], red[W+1] double,
-// then C[nsp+1], G[nsp+1], rop[nstep+1] doubles.  savedJ slot: slot_doubles
-// doubles per workgroup.
-template <int W, bool kExt, bool kIter>
+// W wavefronts per reactor (64 <= nsp < 64W).  LDS (bdf_lds_bytes):
+// CvUni[W] (one per wavefront), for an FP32 matrix Mlds[(nsp+1)*(64W+1)]
+// float, then prow[64W], rvec[64W] of MT, t2l[64W], hl[64W], red[W+1]
+// double, then C[nsp+1], G[nsp+1], rop[nstep+1] doubles.  Slot of
+// slot_doubles doubles per workgroup: the saved Jacobian (MT, column stride
+// 64W+1), then for an FP64 matrix the Newton matrix and its inverse (same
+// layout).
+template <typename MT, int W, bool kExt, bool kIter>
 __global__ void __launch_bounds__(kWave * W)
 bdf_team_kernel(DevMech m, BdfOpts o, int n, double dt, const double* __restrict__ dts,
                 double* __restrict__ Tio,
@@ -292,13 +294,18 @@ bdf_team_kernel(DevMech m, BdfOpts o, int n, double dt, const double* __restrict
                 double* __restrict__ jscratch, size_t slot_doubles) {
   extern __shared__ double lds[];
   constexpr int L = kWave * W;
-  using CV = Cvode<float, TeamBlock<W, kExt>, kIter>;
+  using CV = Cvode<MT, TeamBlock<W, kExt>, kIter>;
   const int nsp = m.nsp;
   CV cv(reinterpret_cast<CvUni*>(lds) + threadIdx.x / kWave);
   char* p = reinterpret_cast<char*>(lds) + W * sizeof(CvUni);
-  cv.Mlds = reinterpret_cast<float*>(p);
-  p += ((size_t)(nsp + 1) * CV::kJS * sizeof(float) + 15) & ~(size_t)15;
-  cv.prow = reinterpret_cast<float*>(p);
+  MT* slot = reinterpret_cast<MT*>(jscratch + (size_t)blockIdx.x * slot_doubles);
+  if constexpr (std::is_same<MT, float>::value) {
+    cv.Mlds = reinterpret_cast<MT*>(p);
+    p += ((size_t)(nsp + 1) * CV::kJS * sizeof(float) + 15) & ~(size_t)15;
+  } else {
+    cv.Mlds = slot + (size_t)(nsp + 1) * CV::kJS;
+  }
+  cv.prow = reinterpret_cast<MT*>(p);
   cv.rvec = cv.prow + L;
   cv.t2l = reinterpret_cast<double*>(cv.rvec + L);
   cv.hl = cv.t2l + L;
@@ -307,7 +314,7 @@ bdf_team_kernel(DevMech m, BdfOpts o, int n, double dt, const double* __restrict
   double* rl = red + ((W + 2) & ~1);
   cv.s = rhs_lds_at(rl, nsp);
   cv.kry = rl + 2 * (nsp + 1) + m.nstep + 1;  // after the RHS arrays (kIter kernels)
-  cv.savedJ = reinterpret_cast<float*>(jscratch + (size_t)blockIdx.x * slot_doubles);
+  cv.savedJ = slot;
   bdf_driver(cv, m, o, n, dt, dts, Tio, Pio, Yio, stride, nsteps_out, stats_out, order, counter);
 }
 
@@ -336,20 +343,26 @@ hipError_t launch_bdf_wave(const BdfLaunch& a, bool ext, bool f32) {
   });
 }
 
-template <bool kIter>
-hipError_t launch_bdf_team(const BdfLaunch& a, bool ext, int W, size_t slot) {
+template <typename MT, bool kIter>
+hipError_t launch_bdf_team_mt(const BdfLaunch& a, bool ext, int W, size_t slot) {
   return with_ext(ext, [&](auto e) {
     constexpr bool E = decltype(e)::value;
     if (W == 2)
-      hipLaunchKernelGGL((bdf_team_kernel<2, E, kIter>), dim3(a.grid), dim3(2 * kWave), a.lds,
-                         a.st, a.m, a.o, a.n, a.dt, a.dts, a.T, a.P, a.Y, a.stride, a.nsteps,
-                         a.stats, a.order, a.counter, a.jscratch, slot);
+      hipLaunchKernelGGL((bdf_team_kernel<MT, 2, E, kIter>), dim3(a.grid), dim3(2 * kWave),
+                         a.lds, a.st, a.m, a.o, a.n, a.dt, a.dts, a.T, a.P, a.Y, a.stride,
+                         a.nsteps, a.stats, a.order, a.counter, a.jscratch, slot);
     else
-      hipLaunchKernelGGL((bdf_team_kernel<3, E, kIter>), dim3(a.grid), dim3(3 * kWave), a.lds,
-                         a.st, a.m, a.o, a.n, a.dt, a.dts, a.T, a.P, a.Y, a.stride, a.nsteps,
-                         a.stats, a.order, a.counter, a.jscratch, slot);
+      hipLaunchKernelGGL((bdf_team_kernel<MT, 3, E, kIter>), dim3(a.grid), dim3(3 * kWave),
+                         a.lds, a.st, a.m, a.o, a.n, a.dt, a.dts, a.T, a.P, a.Y, a.stride,
+                         a.nsteps, a.stats, a.order, a.counter, a.jscratch, slot);
     return hipGetLastError();
   });
+}
+
+template <bool kIter>
+hipError_t launch_bdf_team(const BdfLaunch& a, bool ext, int W, size_t slot, bool f32) {
+  return f32 ? launch_bdf_team_mt<float, kIter>(a, ext, W, slot)
+             : launch_bdf_team_mt<double, kIter>(a, ext, W, slot);
 }
 
 template <bool kIter>
@@ -364,16 +377,22 @@ int resident_bdf_wave(bool ext, bool f32, size_t lds) {
   return r;
 }
 
-template <bool kIter>
-int resident_bdf_team(bool ext, int W, size_t lds) {
+template <typename MT, bool kIter>
+int resident_bdf_team_mt(bool ext, int W, size_t lds) {
   int r = 0;
   (void)with_ext(ext, [&](auto e) {
     constexpr bool E = decltype(e)::value;
-    r = W == 2 ? resident_blocks(bdf_team_kernel<2, E, kIter>, 2 * kWave, lds)
-               : resident_blocks(bdf_team_kernel<3, E, kIter>, 3 * kWave, lds);
+    r = W == 2 ? resident_blocks(bdf_team_kernel<MT, 2, E, kIter>, 2 * kWave, lds)
+               : resident_blocks(bdf_team_kernel<MT, 3, E, kIter>, 3 * kWave, lds);
     return hipSuccess;
   });
   return r;
+}
+
+template <bool kIter>
+int resident_bdf_team(bool ext, int W, size_t lds, bool f32) {
+  return f32 ? resident_bdf_team_mt<float, kIter>(ext, W, lds)
+             : resident_bdf_team_mt<double, kIter>(ext, W, lds);
 }
 
 }  // namespace zrk

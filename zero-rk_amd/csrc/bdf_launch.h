@@ -39,11 +39,11 @@ struct BdfLaunch {
 template <bool kIter>
 hipError_t launch_bdf_wave(const BdfLaunch& a, bool ext, bool f32);
 template <bool kIter>
-hipError_t launch_bdf_team(const BdfLaunch& a, bool ext, int W, size_t slot);
+hipError_t launch_bdf_team(const BdfLaunch& a, bool ext, int W, size_t slot, bool f32);
 template <bool kIter>
 int resident_bdf_wave(bool ext, bool f32, size_t lds);
 template <bool kIter>
-int resident_bdf_team(bool ext, int W, size_t lds);
+int resident_bdf_team(bool ext, int W, size_t lds, bool f32);
 
 inline size_t max_lds_per_block() {
   int dev = 0, v = 0;

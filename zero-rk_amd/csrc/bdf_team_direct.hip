@@ -1,7 +1,7 @@
-// Integrator instantiations: teams of 2 and 3 wavefronts (FP32 Newton matrices), with the direct linear solver.
+// Integrator instantiations: teams of 2 and 3 wavefronts (FP32 and FP64 Newton matrices), with the direct linear solver.
 #include "bdf_kernels.h"
 
 namespace zrk {
-template hipError_t launch_bdf_team<false>(const BdfLaunch&, bool, int, size_t);
-template int resident_bdf_team<false>(bool, int, size_t);
+template hipError_t launch_bdf_team<false>(const BdfLaunch&, bool, int, size_t, bool);
+template int resident_bdf_team<false>(bool, int, size_t, bool);
 }  // namespace zrk

@@ -380,9 +380,8 @@ zrk_bdf_options solve_options(zerork_handle h, int nsp) {
   o.max_steps = h->iopt["max_steps"];
   o.cvode_num_retries = h->iopt["cvode_num_retries"];
   o.constant_volume = h->iopt["constant_volume"] != 0;
-  // mechanisms above 63 species run on multi-wavefront teams, which keep the
-  // Newton matrix in FP32 only
-  o.fp32_preconditioner = h->iopt["fp32_preconditioner"] != 0 || nsp > 63;
+  // mechanisms above 191 species take the sparse path, whose LU is FP32 only
+  o.fp32_preconditioner = h->iopt["fp32_preconditioner"] != 0 || nsp > 191;
   o.delta_temperature_ignition = h->dopt["delta_temperature_ignition"];
   o.stop_after_ignition = h->iopt["stop_after_ignition"] != 0;
   // the linear solver (solver_cvode.cpp:93-214) and tolerances
@@ -393,6 +392,15 @@ zrk_bdf_options solve_options(zerork_handle h, int nsp) {
   o.iterative = h->iopt["iterative"];
   o.eps_lin = h->dopt["eps_lin"];
   o.preconditioner_threshold = h->dopt["preconditioner_threshold"];
+  if (nsp > 191) {
+    // the sparse path solves the Newton system directly with the static
+    // sparse LU of I - gamma J and the analytic Jacobian (the reference's
+    // sparse direct solve); the Krylov and difference-quotient options apply
+    // to the dense paths only
+    o.dense = 1;
+    o.iterative = 0;
+    o.analytic = 1;
+  }
   return o;
 }
 
@@ -530,7 +538,7 @@ zerork_status_t solve(zerork_handle h, int n_cycle, double dt, int n, double* T,
   for (int k = 0; k < n; ++k)
     if (steps[k] < 0) failed.push_back(k);
   int n_resolved = 0;
-  if (!failed.empty() && o.fp32_preconditioner && nsp <= 63) {
+  if (!failed.empty() && o.fp32_preconditioner && nsp <= 191) {
     zrk_bdf_options o64 = o;
     o64.fp32_preconditioner = 0;
     ok = hipMemcpyAsync(h->dorder, failed.data(), sizeof(int) * failed.size(),

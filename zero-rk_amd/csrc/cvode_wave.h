@@ -58,20 +58,25 @@ static_assert(sizeof(CvUni) % 16 == 0, "CvUni keeps the LDS layout 16-byte align
 // (double: the plain FP64 method; float: the FP32 preconditioner, with the
 // residual, error test and state update still in FP64).
 // Team: TeamWave (one wavefront per reactor, nsp <= 63) or TeamBlock<W> (W
-// wavefronts per reactor, nsp < 64W, FP32 matrices only).
-// kIter: the Krylov solver (SPGMR, the plugin's iterative option) is compiled
-// in; its instantiations are used exactly when BdfOpts::krylov is set.
+// wavefronts per reactor, nsp < 64W; an FP32 matrix lives in LDS, an FP64
+// one in the team's slot of device memory).
+// kIter: the alternate solver modes are compiled in -- the Krylov solver
+// (SPGMR, the plugin's iterative option) and the difference-quotient
+// Jacobians (analytic = 0); these instantiations are used exactly when
+// BdfOpts::krylov is set or the Jacobian is not the analytic one, so the
+// default direct analytic kernels carry none of that code (their register
+// allocation, and their speed, stay those of the direct solver alone).
 template <typename MT, class Team = TeamWave, bool kIter = false>
 struct Cvode {
   static constexpr int kL = Team::kThreads;        // state slots = matrix row registers
   static constexpr bool kWaveTeam = (kL == kWave);
-  // teams invert their FP32 Newton matrix with the blocked MFMA Gauss-Jordan
-  // (build with -DZRK_TEAM_MFMA_INVERSE=0 for the element-wise LDS version)
+  // teams invert their Newton matrix with the blocked MFMA Gauss-Jordan, FP32
+  // or FP64 tiles (build with -DZRK_TEAM_MFMA_INVERSE=0 for the element-wise
+  // version)
 #ifndef ZRK_TEAM_MFMA_INVERSE
 #define ZRK_TEAM_MFMA_INVERSE 1
 #endif
-  static constexpr bool kUseMfmaInverse =
-      !kWaveTeam && ZRK_TEAM_MFMA_INVERSE && std::is_same<MT, float>::value;
+  static constexpr bool kUseMfmaInverse = !kWaveTeam && ZRK_TEAM_MFMA_INVERSE;
   // one wavefront, FP32 matrix: the same blocked MFMA inverse on the compact
   // N x N matrix in LDS (-DZRK_WAVE_MFMA_INVERSE=0: the in-register inverse)
 #ifndef ZRK_WAVE_MFMA_INVERSE
@@ -90,8 +95,9 @@ struct Cvode {
   RhsLds s;
   BdfOpts opt;
   MT* savedJ;      // this team's saved Jacobian in HBM (column stride kJS)
-  MT* Mlds;        // LDS, column-major: Jacobian workspace, then the inverse of the
-                   // Newton matrix; stride nsp + 1 for one wavefront, kJS for teams
+  MT* Mlds;        // column-major: Jacobian workspace, then the inverse of the
+                   // Newton matrix; stride nsp + 1 for one wavefront, kJS for teams;
+                   // in LDS, except for FP64 teams (the team's device slot)
   MT* prow;        // LDS [kL]: published pivot row (teams)
   MT* rvec;        // LDS [kL]: right-hand side of a solve (teams)
   double* t2l;     // LDS [kL]: Jacobian column factors
@@ -478,7 +484,7 @@ struct Cvode {
       nje++;
       nstlj = nst;
       ZRK_TIC(t0);
-      if (opt.jac == kJacAnalytic) {
+      if (!kIter || opt.jac == kJacAnalytic) {
         inv_rho_last = jacobian_lds<MT, Team::kExt>(mech, s, P, v0, opt, ypred, fpred,
                                                     inv_rho_last, Mlds, N, savedJ, t2l,
                                                     sources(tn), solveT);
@@ -548,7 +554,7 @@ struct Cvode {
       nje++;
       nstlj = nst;
       ZRK_TIC(t0);
-      if (opt.jac == kJacAnalytic) {
+      if (!kIter || opt.jac == kJacAnalytic) {
         inv_rho_last = jacobian_team<MT>(team, mech, s, P, v0, opt, ypred, fpred, inv_rho_last,
                                          Mlds, savedJ, kJS, t2l, hl, sources(tn), solveT);
         copy = false;
@@ -585,7 +591,7 @@ struct Cvode {
   // with |v| > threshold.  Rows and columns < nsp + 1.
   __device__ __forceinline__ double newton_entry(double v, int row, int col) const {
     if (row == col) return v + 1.0;
-    if (opt.pfull) return v;
+    if (!kIter || opt.pfull) return v;
     const bool in = (mech.jpat[row * 3 + (col >> 6)] >> (col & 63)) & 1ull;
     return (in && fabs(v) > opt.pthr) ? v : 0.0;
   }
@@ -887,7 +893,7 @@ struct Cvode {
         for (;;) {
           nni++;
           double b = -delta;
-          if constexpr (kIter) {
+          if (kIter && opt.krylov) {
             // cvLsSolve with SPGMR: no rescaling for a lagged gamma
             int lflag;
             b = lsolve_krylov(b, curiter, lflag);

@@ -54,6 +54,10 @@ enum JacKind : int {
   kJacReactorDQ = 2,  // the reactor's divided differences (:816-835)
 };
 
+// whether these options need the integrator's alternate-mode kernels (SPGMR
+// or a difference-quotient Jacobian); the direct analytic solve does not
+inline bool bdf_alt_modes(const BdfOpts& o) { return o.krylov != 0 || o.jac != kJacAnalytic; }
+
 enum RecFlags : int {
   kRecRevArr = 1,      // explicit reverse Arrhenius (REV)
   kRecRevKeq = 2,      // reverse from equilibrium constant

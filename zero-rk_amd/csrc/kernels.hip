@@ -355,14 +355,18 @@ int bdf_team_width(int nsp) {
 }
 
 bool bdf_team_ok(int nsp, bool f32, int W) {
+  (void)f32;  // teams have both FP32 (LDS) and FP64 (device slot) Newton matrices
   if (W == 1) return nsp <= kNP - 1;
-  return f32 && (W == 2 || W == 3) && nsp + 1 <= kWave * W;
+  return (W == 2 || W == 3) && nsp + 1 <= kWave * W;
 }
 
-size_t bdf_slot_doubles(const DevMech& m, int W) {
+size_t bdf_slot_doubles(const DevMech& m, int W, bool f32) {
   if (W <= 1) return (size_t)kNP * kNP;
-  const size_t floats = (size_t)(m.nsp + 1) * (kWave * W + 1);
-  return ((floats + 1) / 2 + 31) & ~(size_t)31;  // 256-byte aligned slots
+  // the saved Jacobian, column stride 64W + 1; FP64 teams add the Newton
+  // matrix (inverted in place) with the same layout
+  const size_t elems = (size_t)(m.nsp + 1) * (kWave * W + 1);
+  const size_t d = f32 ? (elems + 1) / 2 : 2 * elems;
+  return (d + 31) & ~(size_t)31;  // 256-byte aligned slots
 }
 
 size_t bdf_lds_bytes(const DevMech& m, bool f32, int W, bool iter) {
@@ -371,8 +375,9 @@ size_t bdf_lds_bytes(const DevMech& m, bool f32, int W, bool iter) {
     return sizeof(CvUni) + wave_matrix_bytes(m.nsp, f32 ? sizeof(float) : sizeof(double)) +
            sizeof(double) * kNP + rhs_lds_bytes(m) + kry;
   const size_t L = (size_t)kWave * W;
-  const size_t mat = ((size_t)(m.nsp + 1) * (L + 1) * sizeof(float) + 15) & ~(size_t)15;
-  return W * sizeof(CvUni) + mat + 2 * L * sizeof(float) + 2 * L * sizeof(double) +
+  const size_t mt = f32 ? sizeof(float) : sizeof(double);
+  const size_t mat = f32 ? ((size_t)(m.nsp + 1) * (L + 1) * sizeof(float) + 15) & ~(size_t)15 : 0;
+  return W * sizeof(CvUni) + mat + 2 * L * mt + 2 * L * sizeof(double) +
          (size_t)((W + 2) & ~1) * sizeof(double) + rhs_lds_bytes(m) + kry;
 }
 
@@ -383,7 +388,8 @@ int bdf_max_resident_teams(const DevMech& m, bool f32, int W, bool iter) {
   const bool ext = m.n_ext > 0;
   if (W == 1) return iter ? resident_bdf_wave<true>(ext, f32, lds)
                           : resident_bdf_wave<false>(ext, f32, lds);
-  return iter ? resident_bdf_team<true>(ext, W, lds) : resident_bdf_team<false>(ext, W, lds);
+  return iter ? resident_bdf_team<true>(ext, W, lds, f32)
+              : resident_bdf_team<false>(ext, W, lds, f32);
 }
 
 hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, const double* dts,
@@ -392,7 +398,7 @@ hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, cons
                       int W, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   const bool f32 = o.fp32_preconditioner != 0;
-  const bool iter = o.krylov != 0;
+  const bool iter = bdf_alt_modes(o);
   if (!bdf_team_ok(m.nsp, f32, W)) return hipErrorInvalidValue;
   const size_t lds = bdf_lds_bytes(m, f32, W, iter);
   if (lds > max_lds_per_block()) return hipErrorInvalidValue;
@@ -402,8 +408,9 @@ hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, cons
   const BdfLaunch a{m, o, n, dt, dts, T, P, Y, stride, nsteps, stats, order, counter, jscratch,
                     grid, lds, st};
   if (W == 1) return iter ? launch_bdf_wave<true>(a, ext, f32) : launch_bdf_wave<false>(a, ext, f32);
-  const size_t slot = bdf_slot_doubles(m, W);
-  return iter ? launch_bdf_team<true>(a, ext, W, slot) : launch_bdf_team<false>(a, ext, W, slot);
+  const size_t slot = bdf_slot_doubles(m, W, f32);
+  return iter ? launch_bdf_team<true>(a, ext, W, slot, f32)
+              : launch_bdf_team<false>(a, ext, W, slot, f32);
 }
 
 }  // namespace zrk

@@ -20,12 +20,13 @@ hipError_t launch_jacobian(const DevMech& m, const BdfOpts& o, int n, const doub
                            const double* P, const double* Y, int stride, double* J,
                            double* jscratch, int nblocks, hipStream_t st);
 // Smallest number of wavefronts per reactor of the integrator: 1 (nsp <= 63),
-// 2 or 3 (FP32 preconditioner only), 0 if the mechanism is too large.
+// 2 or 3 (nsp <= 191), 0 if the mechanism is too large.
 int bdf_team_width(int nsp);
 // whether a team of W wavefronts can integrate this mechanism
 bool bdf_team_ok(int nsp, bool f32, int W);
-// doubles of saved-Jacobian workspace per resident team of W wavefronts
-size_t bdf_slot_doubles(const DevMech& m, int W);
+// doubles of device workspace per resident team of W wavefronts (the saved
+// Jacobian; for FP64 teams also the Newton matrix)
+size_t bdf_slot_doubles(const DevMech& m, int W, bool f32);
 // iter: the Krylov solver's kernels (their LDS holds its uniform block too)
 size_t bdf_lds_bytes(const DevMech& m, bool f32, int W, bool iter = false);
 // teams the persistent integrator keeps resident (0: does not fit)
@@ -69,7 +70,8 @@ hipError_t launch_adam(long n, float* w, const float* g, float* m, float* v, uns
 hipError_t launch_to_bf16(long n, const float* x, unsigned short* y, hipStream_t st);
 hipError_t launch_accept(const DevMech& m, const BdfOpts& o, int n, const double* dts,
                          double dt_all, const double* T0, const double* P, const double* Y0,
-                         double* T1, double* Y1, int stride, int newton, int* accept,
-                         double* res, double* jscratch, int nblocks, hipStream_t st);
+                         int nodes, const double* tau, double* Tp, double* Yp, int stride,
+                         int max_newton, int* accept, double* res, double* jscratch,
+                         int nblocks, hipStream_t st);
 
 }  // namespace zrk

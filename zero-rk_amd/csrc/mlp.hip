@@ -9,7 +9,7 @@
 //   act_backward dZ = dY * gelu'(Z) (f32 and bf16)
 //   col_sum      db = sum over rows of dZ
 //   adam         FP32 master weights, BF16 copies
-//   accept       FP64 acceptance test of a proposed end state (one wavefront
+//   accept       FP64 acceptance test of a proposed trajectory (one wavefront
 //                per reactor, the integrator's RHS)
 #include <hip/hip_runtime.h>
 
@@ -156,106 +156,190 @@ __global__ void to_bf16_kernel(long n, const float* __restrict__ x, unsigned sho
 }
 
 // ---------------------------------------------------------------------------
-// Acceptance of a proposed end state y1 = (T1, Y1) for the step [0, dt] from
-// y0 (one wavefront per reactor, constant pressure), all in FP64.
-// With newton > 0 the proposal is first refined by that many Newton
-// iterations on the trapezoidal equation G(y) = y - y0 - dt/2 (f(y0) + f(y))
-// = 0, with the iteration matrix I - dt/2 J built once at the proposal (the
-// reference Jacobian, Gauss-Jordan inverse), and the refined state is written
-// back to T1/Y1.  Then, in the integrator's weights w = 1/(rtol |y1| + atol):
-//   implicit residual   r = y1 - y0 - dt/2 (f(y0) + f(y1))
-//   error estimate      e = dt/2 (f(y1) - f(y0))       (trapezoid - backward Euler)
-// accept[r] = 1 when ||r||_wrms <= 1 and ||e||_wrms <= 1, else 0;
-// res[2r], res[2r+1] (optional) = the two norms.
-// LDS: Ml[kNP*kNP], prow[kNP], rvec[kNP], t2l[kNP], then C, G, rop.
+// Acceptance of a proposed trajectory of one reactor step [0, dt] (one
+// wavefront per reactor, constant pressure, FP64).  The proposal gives the
+// state at the nodes t_k = tau_k dt (k = 1..m, tau increasing, tau_m = 1):
+// Tp[(k-1)*n + r] and Yp[((k-1)*n + r)*stride + j].  Interval k, from t_{k-1}
+// (t_0 = 0, the initial state) to t_k, is the implicit equation
+//   G(y) = y - y_{k-1} - h f(y)                      (k = 1, backward Euler)
+//   G(y) = y - y_{k-1} - h/2 (f(y_{k-1}) + f(y))     (k > 1, trapezoidal rule)
+// solved by Newton iterations that start from the proposal at t_k, with the
+// iteration matrix I - c J (c = h or h/2, J the reference Jacobian at that
+// proposal): at most max_newton iterations, converged when the correction's
+// weighted RMS norm is at most kAcceptConv.  A proposal too far from the
+// solution for that many iterations fails the test.  The local errors of the
+// converged nodes come from divided differences of f over the nodes after
+// t_0 (the initial state may lie off the slow manifold that the first
+// interval's backward Euler step damps away):
+//   backward Euler   (h^2 / 2) f[t_1, t_2]
+//   trapezoidal      (h^3 / 12) 2 f[t_a, t_b, t_c], three consecutive nodes
+// The step is accepted when every interval converged and the componentwise
+// sum of |local errors| has weighted RMS norm at most 1, in the integrator's
+// weights 1 / (rtol |y_m| + atol).  An accepted reactor's converged end state
+// is written to the last node's arrays.  res[4r..4r+3] (optional): most
+// Newton iterations of an interval, largest first correction, error norm,
+// first failed interval (0: none, -1: the RHS failed).
+// LDS: Ml[kNP*kNP], rvec[kNP], t2l[kNP], then C, G, rop.
 // ---------------------------------------------------------------------------
+constexpr int kAcceptMaxNodes = 8;
+constexpr double kAcceptConv = 0.1;
+
 template <bool kExt>
 __global__ void __launch_bounds__(kWave)
 accept_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ dts, double dt_all,
               const double* __restrict__ T0, const double* __restrict__ P,
-              const double* __restrict__ Y0, double* __restrict__ T1, double* __restrict__ Y1,
-              int stride, int newton, int* __restrict__ accept, double* __restrict__ res,
+              const double* __restrict__ Y0, int nodes, const double* __restrict__ tau,
+              double* __restrict__ Tp, double* __restrict__ Yp, int stride, int max_newton,
+              int* __restrict__ accept, double* __restrict__ res,
               double* __restrict__ jscratch) {
   extern __shared__ double lds[];
   const int lane = lane_id();
   const int nsp = m.nsp;
   const int N = nsp + 1;
   double* Ml = lds;
-  double* prow = Ml + kNP * kNP;
-  double* rvec = prow + kNP;
+  double* rvec = Ml + kNP * kNP;
   double* t2l = rvec + kNP;
   double* rl = t2l + kNP;
   RhsLds s = rhs_lds_at(rl, nsp);
   double* saved = jscratch + (size_t)blockIdx.x * kNP * kNP;
   const bool act = lane <= nsp;
+  const double inv = 1.0 / (double)N;
+  const int rot = gj_rotation(N);  // register c of the inverse holds column (c + rot) mod 64
   for (int r = blockIdx.x; r < n; r += gridDim.x) {
     const double dt = dts ? dts[r] : dt_all;
     const double Pr = P[r];
-    double y0 = 0.0, y1 = 0.0;
-    if (lane < nsp) {
-      y0 = Y0[(size_t)r * stride + lane];
-      y1 = Y1[(size_t)r * stride + lane];
-    }
-    if (lane == nsp) {
-      y0 = T0[r] / o.Tref;
-      y1 = T1[r] / o.Tref;
-    }
-    double f0, f1;
+    double y0 = 0.0;
+    if (lane < nsp) y0 = Y0[(size_t)r * stride + lane];
+    if (lane == nsp) y0 = T0[r] / o.Tref;
+    double fk[kAcceptMaxNodes], yk[kAcceptMaxNodes];
     RhsAux aux;
-    int bad = rhs64<kExt>(m, s, Pr, 0.0, o.Tref, o.step_limiter, y0, f0, aux);
-    if (newton > 0 && !bad) {
+    double yprev = y0, fprev = 0.0;
+    int bad = rhs64<kExt>(m, s, Pr, 0.0, o.Tref, o.step_limiter, y0, fprev, aux);
+    int failed = 0, most_it = 0;
+    double first_max = 0.0;
+#pragma unroll
+    for (int k = 0; k < kAcceptMaxNodes; ++k) {
+      fk[k] = 0.0;
+      yk[k] = 0.0;
+      if (k >= nodes || bad || failed) continue;
+      const size_t row = (size_t)k * n + r;
+      double y = 0.0;
+      if (lane < nsp) y = Yp[row * stride + lane];
+      if (lane == nsp) y = Tp[row] / o.Tref;
+      const double h = (tau[k] - (k > 0 ? tau[k - 1] : 0.0)) * dt;
+      const double c = k == 0 ? h : 0.5 * h;
+      // Newton matrix I - c J at the proposal, inverted in registers
       double fy;
-      bad = rhs64<kExt>(m, s, Pr, 0.0, o.Tref, o.step_limiter, y1, fy, aux);
-      if (!bad) {
-        jacobian_lds<double, kExt>(m, s, Pr, 0.0, o, y1, fy, aux.inv_rho, Ml, kNP, saved, t2l);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        double R[kNP];
-        const double hg = -0.5 * dt;
+      bad = rhs64<kExt>(m, s, Pr, 0.0, o.Tref, o.step_limiter, y, fy, aux);
+      if (bad) continue;
+      jacobian_lds<double, kExt>(m, s, Pr, 0.0, o, y, fy, aux.inv_rho, Ml, kNP, saved, t2l);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      double R[kNP];
 #pragma unroll
-        for (int c = 0; c < kNP; ++c) {
-          double v = (c <= nsp && lane <= nsp) ? hg * Ml[c * kNP + lane] : 0.0;
-          if (c == lane) v += 1.0;
-          R[c] = v;
-        }
-        const bool ok = gj_invert<double>(R, N);
-#pragma unroll
-        for (int c = 0; c < kNP; ++c) Ml[c * kNP + lane] = R[c];
-        if (!ok) bad = 1;
+      for (int col = 0; col < kNP; ++col) {
+        double v = (col <= nsp && lane <= nsp) ? -c * Ml[col * kNP + lane] : 0.0;
+        if (col == lane) v += 1.0;
+        R[col] = v;
       }
-      for (int it = 0; it < newton && !bad; ++it) {
-        if (it > 0) bad = rhs64<kExt>(m, s, Pr, 0.0, o.Tref, o.step_limiter, y1, fy, aux);
-        if (bad) break;
-        const double G = act ? y1 - y0 - 0.5 * dt * (f0 + fy) : 0.0;
+      const bool ok = gj_invert<double>(R, N);
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int col = 0; col < kNP; ++col) Ml[col * kNP + lane] = R[col];
+      if (!ok) {
+        failed = k + 1;
+        continue;
+      }
+      const double w = act ? 1.0 / (o.rtol * fabs(y) + o.atol) : 0.0;
+      bool conv = false;
+      int it = 0;
+      for (; it < max_newton && !conv && !bad; ++it) {
+        if (it > 0) bad = rhs64<kExt>(m, s, Pr, 0.0, o.Tref, o.step_limiter, y, fy, aux);
+        if (bad) continue;
+        const double G = !act ? 0.0
+                         : (k == 0 ? y - yprev - h * fy : y - yprev - 0.5 * h * (fprev + fy));
         rvec[lane] = G;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         double d = 0.0;
-        const int rot = gj_rotation(N);  // register c held logical column (c + rot) mod 64
 #pragma unroll 8
-        for (int c = 0; c < kNP; ++c) d = fma(Ml[c * kNP + lane], rvec[(c + rot) & (kNP - 1)], d);
+        for (int col = 0; col < kNP; ++col)
+          d = fma(Ml[col * kNP + lane], rvec[(col + rot) & (kNP - 1)], d);
         __builtin_amdgcn_wave_barrier();
-        if (act) y1 -= d;
+        if (act) y -= d;
+        const double dn = sqrt(wave_sum(act ? d * d * w * w : 0.0) * inv);
+        if (it == 0) first_max = fmax(first_max, dn);
+        conv = dn <= kAcceptConv;
       }
+      if (bad) continue;
+      most_it = max(most_it, it);
+      if (!conv) {
+        failed = k + 1;
+        continue;
+      }
+      bad = rhs64<kExt>(m, s, Pr, 0.0, o.Tref, o.step_limiter, y, fy, aux);
+      yk[k] = y;
+      fk[k] = fy;
+      yprev = y;
+      fprev = fy;
     }
-    if (!bad) bad = rhs64<kExt>(m, s, Pr, 0.0, o.Tref, o.step_limiter, y1, f1, aux);
-    const double w = act ? 1.0 / (o.rtol * fabs(y1) + o.atol) : 0.0;
-    const double rr = bad ? 0.0 : (y1 - y0 - 0.5 * dt * (f0 + f1)) * w;
-    const double ee = bad ? 0.0 : 0.5 * dt * (f1 - f0) * w;
-    const double inv = 1.0 / (double)N;
-    const double rn = sqrt(wave_sum(act ? rr * rr : 0.0) * inv);
-    const double en = sqrt(wave_sum(act ? ee * ee : 0.0) * inv);
-    if (newton > 0) {
-      if (lane < nsp) Y1[(size_t)r * stride + lane] = y1;
-      if (lane == nsp) T1[r] = y1 * o.Tref;
+    // local error estimates of the converged nodes
+    double en = INFINITY;
+    double yend = 0.0;
+    if (!bad && !failed) {
+      double tk[kAcceptMaxNodes];
+#pragma unroll
+      for (int k = 0; k < kAcceptMaxNodes; ++k) tk[k] = k < nodes ? tau[k] * dt : 0.0;
+      // backward Euler on [0, t_1]: (h^2 / 2) y'', with y'' from f[t_1, t_2]
+      double e = fabs(0.5 * tk[0] * tk[0] * (fk[1] - fk[0]) / (tk[1] - tk[0]));
+#pragma unroll
+      for (int k = 1; k < kAcceptMaxNodes; ++k) {
+        if (k >= nodes) continue;
+        // trapezoidal rule on [t_{k-1}, t_k] (0-based nodes): (h^3 / 12) times
+        // the third derivative 2 f[t_a, t_b, t_c] over the three consecutive
+        // nodes ending at k (the first three for k = 1)
+        const int a = k >= 2 ? k - 2 : 0;
+        const double h = tk[k] - tk[k - 1];
+        double fa = 0.0, fb = 0.0, fc = 0.0, ta = 0.0, tb = 0.0, tc = 0.0;
+#pragma unroll
+        for (int q = 0; q < kAcceptMaxNodes; ++q) {
+          if (q == a) {
+            fa = fk[q];
+            ta = tk[q];
+          }
+          if (q == a + 1) {
+            fb = fk[q];
+            tb = tk[q];
+          }
+          if (q == a + 2) {
+            fc = fk[q];
+            tc = tk[q];
+          }
+        }
+        const double dd = ((fc - fb) / (tc - tb) - (fb - fa) / (tb - ta)) / (tc - ta);
+        e += fabs(h * h * h / 12.0 * 2.0 * dd);
+      }
+#pragma unroll
+      for (int k = 0; k < kAcceptMaxNodes; ++k)
+        if (k == nodes - 1) yend = yk[k];
+      const double w = act ? 1.0 / (o.rtol * fabs(yend) + o.atol) : 0.0;
+      en = sqrt(wave_sum(act ? e * e * w * w : 0.0) * inv);
+    }
+    const bool acc = !bad && !failed && en <= 1.0;
+    if (acc) {
+      const size_t row = (size_t)(nodes - 1) * n + r;
+      if (lane < nsp) Yp[row * stride + lane] = yend;
+      if (lane == nsp) Tp[row] = yend * o.Tref;
     }
     if (lane == 0) {
-      accept[r] = (!bad && rn <= 1.0 && en <= 1.0) ? 1 : 0;
+      accept[r] = acc ? 1 : 0;
       if (res) {
-        res[2 * (size_t)r] = bad ? INFINITY : rn;
-        res[2 * (size_t)r + 1] = bad ? INFINITY : en;
+        res[4 * (size_t)r] = (double)most_it;
+        res[4 * (size_t)r + 1] = first_max;
+        res[4 * (size_t)r + 2] = en;
+        res[4 * (size_t)r + 3] = bad ? -1.0 : (double)failed;
       }
     }
     __builtin_amdgcn_wave_barrier();
@@ -328,18 +412,21 @@ hipError_t launch_to_bf16(long n, const float* x, unsigned short* y, hipStream_t
 
 hipError_t launch_accept(const DevMech& m, const BdfOpts& o, int n, const double* dts,
                          double dt_all, const double* T0, const double* P, const double* Y0,
-                         double* T1, double* Y1, int stride, int newton, int* accept,
-                         double* res, double* jscratch, int nblocks, hipStream_t st) {
+                         int nodes, const double* tau, double* Tp, double* Yp, int stride,
+                         int max_newton, int* accept, double* res, double* jscratch,
+                         int nblocks, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  if (m.nsp > kNP - 1) return hipErrorInvalidValue;
+  if (m.nsp > kNP - 1 || nodes < 3 || nodes > kAcceptMaxNodes) return hipErrorInvalidValue;
   const int grid = n < nblocks ? n : nblocks;
-  const size_t lds = sizeof(double) * (size_t)(kNP * kNP + 3 * kNP) + rhs_lds_bytes(m);
+  const size_t lds = sizeof(double) * (size_t)(kNP * kNP + 2 * kNP) + rhs_lds_bytes(m);
   if (m.n_ext > 0)
     hipLaunchKernelGGL((accept_kernel<true>), dim3(grid), dim3(kWave), lds, st, m, o, n, dts,
-                       dt_all, T0, P, Y0, T1, Y1, stride, newton, accept, res, jscratch);
+                       dt_all, T0, P, Y0, nodes, tau, Tp, Yp, stride, max_newton, accept, res,
+                       jscratch);
   else
     hipLaunchKernelGGL((accept_kernel<false>), dim3(grid), dim3(kWave), lds, st, m, o, n, dts,
-                       dt_all, T0, P, Y0, T1, Y1, stride, newton, accept, res, jscratch);
+                       dt_all, T0, P, Y0, nodes, tau, Tp, Yp, stride, max_newton, accept, res,
+                       jscratch);
   return hipGetLastError();
 }
 

@@ -77,19 +77,21 @@ int zrk_to_bf16(long n, const float* x, void* y, void* stream) {
 
 int zrk_surrogate_accept(zrk_mech m, const zrk_bdf_options* o, int n, const double* dt,
                          double dt_all, const double* T0, const double* P, const double* Y0,
-                         double* T1, double* Y1, int stride, int newton, int* accept, double* res,
-                         void* stream) {
-  if (!m || !o || n < 0 || !T0 || !P || !Y0 || !T1 || !Y1 || !accept) return -1;
+                         int nodes, const double* tau, double* Tp, double* Yp, int stride,
+                         int max_newton, int* accept, double* res, void* stream) {
+  if (!m || !o || n < 0 || !T0 || !P || !Y0 || !tau || !Tp || !Yp || !accept) return -1;
   if (!dt && !(dt_all > 0.0)) return -1;
+  if (nodes < 3 || nodes > 8 || max_newton < 1) return -1;
   auto* h = reinterpret_cast<MechHandle*>(m);
   if (stride < h->host.nsp) return -1;
   if (h->host.nsp > zrk::kNP - 1) return -2;
-  if (newton < 0) return -1;
+  // the test integrates the constant-pressure reactor without source terms
+  if (o->constant_volume) return -4;
   if (int rc = zrk::ensure_device(h)) return rc;
   const int nb = std::max(1, std::min(n, 2048));
   if (int rc = zrk::ensure_workspace(h, (size_t)nb * zrk::kNP * zrk::kNP)) return rc;
-  return rc_of(zrk::launch_accept(h->dev, zrk::to_bdf_opts(*o), n, dt, dt_all, T0, P, Y0, T1, Y1,
-                                  stride, newton, accept, res, h->ws_jac, nb,
+  return rc_of(zrk::launch_accept(h->dev, zrk::to_bdf_opts(*o), n, dt, dt_all, T0, P, Y0, nodes,
+                                  tau, Tp, Yp, stride, max_newton, accept, res, h->ws_jac, nb,
                                   (hipStream_t)stream));
 }
 

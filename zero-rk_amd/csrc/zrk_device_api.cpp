@@ -4,6 +4,7 @@
 #include <cstdio>
 
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -552,9 +553,24 @@ int integrate_sparse(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
   const size_t per = sparse_slot_bytes(h->dev, h->sp_dev) / sizeof(double);
   if (int rc = ensure_workspace(h, (size_t)nb * per)) return rc;
   BdfOpts bo = to_bdf_opts(o);
-  const hipError_t e = launch_bdf_sparse(h->dev, h->sp_dev, bo, n, dt, dts, T, P, Y, stride,
-                                         nsteps, stats, order, h->ws_counter, h->ws_jac, nb, st);
-  return e == hipSuccess ? 0 : (int)e;
+  // bounded launches: each one integrates at most four rounds of the resident
+  // workgroups, so a large batch of a large mechanism is a few launches of
+  // seconds rather than one of minutes (reactors are independent; with an
+  // order array the chunks are consecutive ranges of it)
+  const int chunk = 4 * nb;
+  for (int off = 0; off < n; off += chunk) {
+    const int k = std::min(chunk, n - off);
+    const hipError_t e =
+        order ? launch_bdf_sparse(h->dev, h->sp_dev, bo, k, dt, dts, T, P, Y, stride, nsteps,
+                                  stats, order + off, h->ws_counter, h->ws_jac, nb, st)
+              : launch_bdf_sparse(h->dev, h->sp_dev, bo, k, dt, dts ? dts + off : nullptr,
+                                  T + off, P + off, Y + (size_t)off * stride, stride,
+                                  nsteps ? nsteps + off : nullptr,
+                                  stats ? stats + (size_t)off * 8 : nullptr, nullptr,
+                                  h->ws_counter, h->ws_jac, nb, st);
+    if (e != hipSuccess) return (int)e;
+  }
+  return 0;
 }
 
 }  // namespace
@@ -570,11 +586,14 @@ int team_width_for(MechHandle* h, bool f32, bool iter) {
 int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
                      const double* dts, double* T, double* P, double* Y, int stride, int* nsteps, int* stats,
                      const int* order, hipStream_t st, const double* dpdt, const double* esrc,
-                     const double* ysrc, double* ign_time, const double* tdelta, int always_T) {
+                     const double* ysrc, double* ign_time, const double* tdelta, int always_T,
+                     const ExtWorkspace* ws) {
   if (n < 0 || !T || !P || !Y || stride < h->host.nsp || (!dts && !(dt > 0.0))) return -1;
   BdfOpts bo = to_bdf_opts(o);
   const int W0 = bdf_team_width(h->host.nsp);
+  if (ws && !h->dev_blob) return -6;  // no upload here: nothing may be allocated
   if (W0 == 0 || h->force_sparse) {  // above 191 species: the sparse path
+    if (ws) return -4;
     // the sparse path has the direct analytic solver with solved temperature
     // and plain absolute tolerances only
     if (bo.krylov || bo.jac != kJacAnalytic || bo.abstol_dens || (!always_T && tdelta))
@@ -584,16 +603,26 @@ int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
     return integrate_sparse(h, o, n, dt, dts, T, P, Y, stride, nsteps, stats, order, st, dpdt,
                             esrc, ysrc);
   }
-  if (W0 > 1 && !o.fp32_preconditioner) return -3;     // nsp > 63: FP32 preconditioner only
   if (n == 0) return 0;
   if (int rc = ensure_device(h)) return rc;
   const int f32 = o.fp32_preconditioner != 0;
-  const int it = bo.krylov ? 1 : 0;
+  const int it = bdf_alt_modes(bo) ? 1 : 0;
   const int W = team_width_for(h, f32, it);
   if (W == 0) return -2;                                // does not fit in LDS
-  const int nw = std::max(1, std::min(h->resident[f32][it][W], n));
-  int rc = ensure_workspace(h, (size_t)nw * bdf_slot_doubles(h->dev, W));
-  if (rc) return rc;
+  const size_t slot = bdf_slot_doubles(h->dev, W, f32);
+  int nw = std::max(1, std::min(h->resident[f32][it][W], n));
+  double* jac = nullptr;
+  unsigned int* counter = nullptr;
+  if (ws) {  // the caller's scratch: no allocation
+    if (!ws->jac || !ws->counter || ws->doubles < slot) return -7;
+    nw = std::min(nw, (int)std::min<size_t>(ws->doubles / slot, (size_t)INT32_MAX));
+    jac = ws->jac;
+    counter = ws->counter;
+  } else {
+    if (int rc = ensure_workspace(h, (size_t)nw * slot)) return rc;
+    jac = h->ws_jac;
+    counter = h->ws_counter;
+  }
   bo.dpdt = dpdt;
   bo.esrc = esrc;
   bo.ysrc = ysrc;
@@ -601,7 +630,7 @@ int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
   bo.always_T = always_T ? 1 : 0;
   bo.tdelta = always_T ? nullptr : tdelta;
   hipError_t e = launch_bdf(h->dev, bo, n, dt, dts, T, P, Y, stride, nsteps, stats, order,
-                            h->ws_counter, h->ws_jac, nw, W, st);
+                            counter, jac, nw, W, st);
   return e == hipSuccess ? 0 : (int)e;
 }
 
@@ -854,7 +883,7 @@ int zrk_integrator_resident_waves(zrk_mech m, const zrk_bdf_options* o) {
     if (!f32 || zrk::ensure_sparse(h)) return 0;
     return h->sp_resident * zrk::kSparseWaves;
   }
-  const int it = (o && !(o->dense == 1 && o->iterative == 0)) ? 1 : 0;
+  const int it = (o && zrk::bdf_alt_modes(zrk::to_bdf_opts(*o))) ? 1 : 0;
   const int W = zrk::team_width_for(h, f32, it);
   return W ? h->resident[f32][it][W] * W : 0;
 }

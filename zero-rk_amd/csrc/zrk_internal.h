@@ -46,16 +46,27 @@ int ensure_workspace(MechHandle* h, size_t doubles);
 // builds and uploads the sparse plan once (0 on success)
 int ensure_sparse(MechHandle* h);
 BdfOpts to_bdf_opts(const zrk_bdf_options& o);
+// Integrator scratch supplied by the caller instead of the handle's own
+// allocations: `doubles` doubles of per-team slots and a 256-byte work counter.
+// With it integrate_device allocates nothing: the device image must already
+// exist (-6 otherwise), the dense paths only (-4 for the sparse path), and the
+// persistent grid shrinks to the slots that fit (-7 if not even one does).
+struct ExtWorkspace {
+  double* jac;
+  size_t doubles;
+  unsigned int* counter;
+};
 // dpdt[n], esrc[n], ysrc[n*stride]: optional device arrays of source terms;
 // ign_time[n]: optional ignition-time output (with o.delta_temperature_ignition > 0)
 // tdelta[n] (optional, with always_T = 0): the manager's temperature change
 // of each reactor's previous solve; a reactor whose entry is not above 0 is
 // integrated at fixed temperature (zerork_reactor_manager.cpp:1109-1113)
+// ws (optional): caller-supplied scratch, see ExtWorkspace
 int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
                      const double* dts, double* T, double* P, double* Y, int stride, int* nsteps, int* stats,
                      const int* order, hipStream_t st, const double* dpdt = nullptr,
                      const double* esrc = nullptr, const double* ysrc = nullptr,
                      double* ign_time = nullptr, const double* tdelta = nullptr,
-                     int always_T = 1);
+                     int always_T = 1, const ExtWorkspace* ws = nullptr);
 
 }  // namespace zrk

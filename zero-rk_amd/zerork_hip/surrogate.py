@@ -7,10 +7,16 @@ which span many decades, as well conditioned as major ones).  Forward,
 backward and Adam run in the library's own BF16-input / FP32-accumulate
 MFMA kernels (include/zrk_surrogate.h); PyTorch only holds the device arrays.
 
-`solve()` is the surrogate-accelerated step: every reactor gets the
-network's proposal, an FP64 acceptance test (trapezoidal implicit residual
-and error estimate in the integrator's weights) keeps the proposals that pass,
-and the plain BDF integrates the others with their own dt.
+`solve()` is the surrogate-accelerated step: the network proposes each
+reactor's state at the nodes t_k = tau_k dt of its step (one forward pass per
+node, dt replaced by t_k in the features), an FP64 acceptance test keeps the
+trajectories that pass (backward Euler on the first interval, the
+trapezoidal rule on the others, each solved by at most a few Newton
+iterations started from the proposal, and a local-error estimate within the
+integrator's tolerance; zrk_surrogate_accept), and the plain BDF integrates
+the others with their own dt.  The network-free proposal "hold" (every node
+at the initial state) goes through the same test, which measures what the
+network gains.
 """
 from __future__ import annotations
 
@@ -36,8 +42,8 @@ def _lib():
         L.zrk_mlp_col_sum.argtypes = [I, I, P, P, P]
         L.zrk_mlp_adam.argtypes = [Lg, P, P, P, P, P, F, F, F, F, I, P]
         L.zrk_to_bf16.argtypes = [Lg, P, P, P]
-        L.zrk_surrogate_accept.argtypes = [P, C.POINTER(capi.BdfOptions), I, P, D, P, P, P, P, P,
-                                           I, I, P, P, P]
+        L.zrk_surrogate_accept.argtypes = [P, C.POINTER(capi.BdfOptions), I, P, D, P, P, P, I, P,
+                                           P, P, I, I, P, P, P]
         for f in ("zrk_integrate_device_select", "zrk_mlp_features", "zrk_gemm_bf16",
                   "zrk_mlp_bias_act", "zrk_mlp_act_backward", "zrk_mlp_col_sum", "zrk_mlp_adam",
                   "zrk_to_bf16", "zrk_surrogate_accept"):
@@ -239,8 +245,12 @@ class Surrogate:
         return losses
 
     # -- proposals and the accelerated step -------------------------------------
+    # node fractions of the step: the first interval is short (its backward
+    # Euler step absorbs the fast initial transient), the others grow
+    TAU = (1.0e-3, 4.0e-3, 0.015, 0.05, 0.13, 0.3, 0.6, 1.0)
+
     def propose(self, T, P, Y, dt):
-        """Proposed end states (T1, Y1) (device f64) for per-reactor dt."""
+        """Proposed states (T1, Y1) (device f64) after per-reactor times dt."""
         t = self.torch
         out = self.forward(self.features(T, P, Y, dt))
         z = out[:, :self.nout].double() * t.as_tensor(self.tsig, device=self.device) + \
@@ -250,37 +260,55 @@ class Surrogate:
         T1 = T * t.exp(z[:, self.nsp])
         return T1.contiguous(), Y1.contiguous()
 
-    def accept(self, T0, P, Y0, T1, Y1, dt, opts, newton=0):
-        """FP64 acceptance (T1/Y1 refined in place when newton > 0)."""
+    def propose_trajectory(self, T, P, Y, dt, tau=TAU):
+        """The proposal at every node t_k = tau_k dt: (Tp [m n], Yp [m n, nsp])."""
+        t = self.torch
+        Ts, Ys = [], []
+        for f in tau:
+            T1, Y1 = self.propose(T, P, Y, dt * f)
+            Ts.append(T1)
+            Ys.append(Y1)
+        return t.cat(Ts).contiguous(), t.cat(Ys).contiguous()
+
+    def accept(self, T0, P, Y0, Tp, Yp, dt, opts, tau=TAU, max_newton=3):
+        """FP64 acceptance of proposed trajectories (zrk_surrogate_accept);
+        returns (accept int32 [n], res [n, 4]) and writes an accepted
+        reactor's converged end state over its last node."""
         t = self.torch
         n = T0.numel()
         acc = t.empty(n, dtype=t.int32, device=self.device)
-        res = t.empty(2 * n, dtype=t.float64, device=self.device)
+        res = t.empty(4 * n, dtype=t.float64, device=self.device)
+        tv = t.as_tensor(tau, dtype=t.float64, device=self.device)
         import ctypes as C
-        _chk(_lib().zrk_surrogate_accept(self.mech.h, C.byref(opts), n, _p(dt), 0.0, _p(T0), _p(P), _p(Y0),
-                                         _p(T1), _p(Y1), self.nsp, newton, _p(acc), _p(res), 0),
+        _chk(_lib().zrk_surrogate_accept(self.mech.h, C.byref(opts), n, _p(dt), 0.0, _p(T0), _p(P),
+                                         _p(Y0), len(tau), _p(tv), _p(Tp), _p(Yp), self.nsp,
+                                         max_newton, _p(acc), _p(res), 0),
              "zrk_surrogate_accept")
-        return acc, res.view(n, 2)
+        return acc, res.view(n, 4)
 
-    def solve(self, T, P, Y, dt, opts, proposal="mlp", newton=2):
+    def solve(self, T, P, Y, dt, opts, proposal="mlp", tau=TAU, max_newton=3):
         """Surrogate-accelerated step, in place on T and Y.  proposal: "mlp" or
-        "hold" (y1 = y0, the same test without the network).  Returns the
-        accepted mask (device int32) and the BDF step counts of the rest."""
+        "hold" (every node at the initial state, the same test without the
+        network).  Returns the accepted mask (device int32), the BDF step
+        counts of the rest and the acceptance test's per-reactor results
+        (res [n, 4], see accept)."""
         t = self.torch
+        n = T.numel()
+        m = len(tau)
         if proposal == "mlp":
-            T1, Y1 = self.propose(T, P, Y, dt)
+            Tp, Yp = self.propose_trajectory(T, P, Y, dt, tau)
         else:
-            T1, Y1 = T.clone(), Y.clone()
-        acc, _ = self.accept(T, P, Y, T1, Y1, dt, opts, newton)
+            Tp, Yp = T.repeat(m).contiguous(), Y.repeat(m, 1).contiguous()
+        acc, res = self.accept(T, P, Y, Tp, Yp, dt, opts, tau, max_newton)
         rej = t.nonzero(acc == 0).flatten().to(t.int32).contiguous()
         keep = acc.bool()
-        T[keep] = T1[keep]
-        Y[keep] = Y1[keep]
-        steps = t.zeros(T.numel(), dtype=t.int32, device=self.device)
+        T[keep] = Tp[(m - 1) * n:][keep]
+        Y[keep] = Yp[(m - 1) * n:][keep]
+        steps = t.zeros(n, dtype=t.int32, device=self.device)
         if rej.numel():
             import ctypes as C
             _chk(_lib().zrk_integrate_device_select(self.mech.h, C.byref(opts), rej.numel(), _p(rej),
                                                     _p(dt), _p(T), _p(P), _p(Y), self.nsp,
                                                     _p(steps), None, 0),
                  "zrk_integrate_device_select")
-        return acc, steps
+        return acc, steps, res
