@@ -164,6 +164,23 @@ ZRK_EXPORT int zrk_integrate_device_aux(zrk_mech m, const zrk_bdf_options* o, in
                                         const double* ysrc, double* ignition_time, int* nsteps,
                                         int* stats, void* stream);
 
+/* zrk_integrate_device without any device allocation (no counterpart in the
+ * reference): for callers that cannot allocate -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Same
+ * arguments, kernels and results as zrk_integrate_device.  The integrator's
+ * scratch (work counter and saved-Jacobian slots) is a 64 MiB device array
+ * of the library, one per GPU, held until the process ends; the persistent
+ * grid is capped at the slots that fit in it.  Every call on a GPU shares
+ * that array, so calls that overlap on one GPU (other streams, other
+ * threads) race: serialise them.  Dense paths only (up to 191 species): -4
+ * for a mechanism on the sparse path; -6 if the handle's device image does
+ * not exist yet (it is built by the first call of any other device entry
+ * point, e.g. zrk_integrator_resident_waves); -7 if one reactor's slot does
+ * not fit in the array. */
+ZRK_EXPORT int zrk_integrate_device_noalloc(zrk_mech m, const zrk_bdf_options* o, int n,
+                                            double dt, double* T, double* P, double* Y,
+                                            int stride, int* nsteps, int* stats, void* stream);
+
 /* Autotuner (no counterpart in the reference): times the integrator on the
  * given device batch (copies of T, P, Y; inputs unchanged) with each team
  * size that fits the mechanism and these options (1 wavefront per reactor,

@@ -42,7 +42,7 @@ DEVICE_SYMBOLS = [
     "zrk_jacobian_device", "zrk_integrate_device", "zrk_integrate_device_aux",
     "zrk_integrator_resident_waves",
     "zrk_autotune_integrator", "zrk_integrator_set_team_width", "zrk_integrator_set_sparse",
-    "zrk_sparse_plan_info",
+    "zrk_sparse_plan_info", "zrk_integrate_device_noalloc",
 ]
 
 
@@ -122,6 +122,9 @@ def _declare(L):
     L.zrk_integrate_device_aux.argtypes = [P, C.POINTER(BdfOptions), I, D, P, P, P, I, P, P, P,
                                            P, P, P, P]
     L.zrk_integrate_device_aux.restype = I
+    L.zrk_integrate_device_noalloc.argtypes = [P, C.POINTER(BdfOptions), I, D, P, P, P, I, P,
+                                               P, P]
+    L.zrk_integrate_device_noalloc.restype = I
     L.zrk_integrator_resident_waves.argtypes = [P, P]
     L.zrk_autotune_integrator.argtypes = [P, C.POINTER(BdfOptions), I, D, P, P, P, I, P, P, P]
     L.zrk_autotune_integrator.restype = I
