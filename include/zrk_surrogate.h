@@ -67,19 +67,23 @@ ZRK_EXPORT int zrk_to_bf16(long n, const float* x, void* y, void* stream);
  * Tp[(k-1)*n + r], Yp[((k-1)*n + r)*stride + j].  Interval k (from t_{k-1},
  * t_0 = 0 with the initial state) is the backward Euler equation (k = 1) or
  * the trapezoidal one (k > 1), solved by at most max_newton Newton
- * iterations that start from the proposal at t_k, with the iteration matrix
- * I - c J (c = h or h/2, J the reference Jacobian at that proposal); it
- * converges when the correction's weighted RMS norm is at most 0.1.  The
- * converged nodes' local errors are estimated from divided differences of f
- * over the nodes after t_0 ((h^2/2) f[t_1, t_2] for the first interval,
- * (h^3/12) 2 f[t_a, t_b, t_c] for the others).  accept[r] = 1 when every
- * interval converged and the componentwise sum of |local errors| has weighted
- * RMS norm <= 1 in the integrator's weights 1 / (rel_tol |y| + abs_tol); the
- * converged end state then replaces the last node's proposal.
+ * iterations with the iteration matrix I - c J (c = h or h/2, J the reference
+ * Jacobian at the start point); it converges when the correction's weighted
+ * RMS norm is at most 0.1.  The start point is the proposal at t_k, or the
+ * converged state at t_{k-1} when its residual -h f(y_{k-1}) has the smaller
+ * weighted norm.  The converged nodes' local errors are estimated from
+ * divided differences of f over the nodes after t_0 ((h^2/2) f[t_1, t_2] for
+ * the first interval, (h^3/12) 2 f[t_a, t_b, t_c] for the others), each
+ * weighted with the weights 1 / (rel_tol |y| + abs_tol) of the node that
+ * completes it, and summed componentwise.  accept[r] = 1 when every interval
+ * converged and that sum has weighted RMS norm <= 1 (the test stops at the
+ * first node where it exceeds 1); the converged end state then replaces the
+ * last node's proposal.
  * res[4r..4r+3] (optional): most Newton iterations of an interval, largest
- * first correction, error norm, first failed interval (0: none, -1: the RHS
- * failed).  Uses the handle's workspace: calls on one handle must not
- * overlap.  Requires nsp <= 63. */
+ * first correction, the error norm where the test stopped (infinite after a
+ * failed interval), first interval whose Newton iterations failed (0: none,
+ * -1: the RHS failed).  Uses the handle's workspace: calls on one handle must
+ * not overlap.  Requires nsp <= 63. */
 ZRK_EXPORT int zrk_surrogate_accept(zrk_mech m, const zrk_bdf_options* o, int n,
                                     const double* dt, double dt_all, const double* T0,
                                     const double* P, const double* Y0, int nodes,

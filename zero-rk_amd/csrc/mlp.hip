@@ -91,6 +91,108 @@ gemm_bf16_kernel(int M, int N, int K, const unsigned short* __restrict__ A, long
   }
 }
 
+// LDS-tiled form of gemm_bf16_kernel for the network's shapes: a workgroup of
+// 4 wavefronts computes a 64 x 64 block of C, each wavefront a 32 x 32 quarter
+// (2 x 2 MFMA tiles).  K is walked 32 at a time: the A and B tiles are read
+// from HBM with 16-byte loads along whichever of their dimensions is
+// contiguous (kAk: A[m, k] has k contiguous, else m; kBk: B[k, n] has k
+// contiguous, else n) and stored k-contiguous in LDS (As[m][k], Bs[n][k], rows
+// padded to 40 elements), so every MFMA operand is one 8-byte LDS read.
+// Requires N % 64 == 0, the contiguous dimension of each operand a multiple
+// of 8 elements, the other stride a multiple of 8 and 16-byte aligned bases
+// (launch_gemm_bf16 checks this and otherwise uses gemm_bf16_kernel).  Rows
+// of A past M and k past K read as zero.
+constexpr int kGemmBM = 64, kGemmBN = 64, kGemmBK = 32, kGemmLd = kGemmBK + 8;
+
+template <bool kAk, bool kBk>
+__global__ void __launch_bounds__(256)
+gemm_bf16_tiled_kernel(int M, int N, int K, const unsigned short* __restrict__ A, long sAm,
+                       long sAk, const unsigned short* __restrict__ B, long sBk, long sBn,
+                       float* __restrict__ C, long sCm, long sCn, int accumulate) {
+  __shared__ __align__(16) unsigned short As[kGemmBM * kGemmLd];
+  __shared__ __align__(16) unsigned short Bs[kGemmBN * kGemmLd];
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const long m0 = (long)blockIdx.x * kGemmBM;
+  const int n0 = blockIdx.y * kGemmBN;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k0 = 0; k0 < K; k0 += kGemmBK) {
+    // A tile (64 m x 32 k): 256 chunks of 8 elements, one per thread
+    {
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (kAk) {
+        const int mm = t >> 2, c = (t & 3) * 8;
+        const long gm = m0 + mm;
+        const int gk = k0 + c;
+        if (gm < M && gk < K) v = *reinterpret_cast<const uint4*>(A + gm * sAm + gk);
+        *reinterpret_cast<uint4*>(As + mm * kGemmLd + c) = v;
+      } else {
+        const int kk = t >> 3, c = (t & 7) * 8;
+        const long gm = m0 + c;
+        const int gk = k0 + kk;
+        if (gm < M && gk < K) v = *reinterpret_cast<const uint4*>(A + (long)gk * sAk + gm);
+        const unsigned short* e = reinterpret_cast<const unsigned short*>(&v);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) As[(c + j) * kGemmLd + kk] = e[j];
+      }
+    }
+    // B tile (32 k x 64 n)
+    {
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (kBk) {
+        const int nn = t >> 2, c = (t & 3) * 8;
+        const int gk = k0 + c;
+        if (gk < K) v = *reinterpret_cast<const uint4*>(B + (long)(n0 + nn) * sBn + gk);
+        *reinterpret_cast<uint4*>(Bs + nn * kGemmLd + c) = v;
+      } else {
+        const int kk = t >> 3, c = (t & 7) * 8;
+        const int gk = k0 + kk;
+        if (gk < K) v = *reinterpret_cast<const uint4*>(B + (long)gk * sBk + n0 + c);
+        const unsigned short* e = reinterpret_cast<const unsigned short*>(&v);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) Bs[(c + j) * kGemmLd + kk] = e[j];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < kGemmBK; ks += 16) {
+      const int kq = ks + (lane >> 4) * 4;
+      bf16x4 a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+        a[i] = *reinterpret_cast<const bf16x4*>(As + (wm + 16 * i + (lane & 15)) * kGemmLd + kq);
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        b[j] = *reinterpret_cast<const bf16x4*>(Bs + (wn + 16 * j + (lane & 15)) * kGemmLd + kq);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = n0 + wn + 16 * j + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const long row = m0 + wm + 16 * i + (lane >> 4) * 4 + v;
+        if (row < M) {
+          float* c = C + row * sCm + (long)col * sCn;
+          *c = accumulate ? *c + acc[i][j][v] : acc[i][j][v];
+        }
+      }
+    }
+}
+
 __device__ __forceinline__ float gelu(float z) {
   return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
 }
@@ -163,22 +265,27 @@ __global__ void to_bf16_kernel(long n, const float* __restrict__ x, unsigned sho
 // (t_0 = 0, the initial state) to t_k, is the implicit equation
 //   G(y) = y - y_{k-1} - h f(y)                      (k = 1, backward Euler)
 //   G(y) = y - y_{k-1} - h/2 (f(y_{k-1}) + f(y))     (k > 1, trapezoidal rule)
-// solved by Newton iterations that start from the proposal at t_k, with the
-// iteration matrix I - c J (c = h or h/2, J the reference Jacobian at that
-// proposal): at most max_newton iterations, converged when the correction's
-// weighted RMS norm is at most kAcceptConv.  A proposal too far from the
-// solution for that many iterations fails the test.  The local errors of the
-// converged nodes come from divided differences of f over the nodes after
-// t_0 (the initial state may lie off the slow manifold that the first
-// interval's backward Euler step damps away):
+// solved by Newton iterations with the iteration matrix I - c J (c = h or
+// h/2, J the reference Jacobian at the start point): at most max_newton
+// iterations, converged when the correction's weighted RMS norm is at most
+// kAcceptConv.  The start point is the proposal at t_k, or the converged state
+// at t_{k-1} when its residual (-h f(y_{k-1})) has the smaller weighted norm
+// (weights at y_{k-1}); a start too far from the solution for that many
+// iterations fails the test.  Local errors come from divided differences of f
+// over the converged nodes after t_0 (the initial state may lie off the slow
+// manifold that the first interval's backward Euler step damps away):
 //   backward Euler   (h^2 / 2) f[t_1, t_2]
 //   trapezoidal      (h^3 / 12) 2 f[t_a, t_b, t_c], three consecutive nodes
-// The step is accepted when every interval converged and the componentwise
-// sum of |local errors| has weighted RMS norm at most 1, in the integrator's
-// weights 1 / (rtol |y_m| + atol).  An accepted reactor's converged end state
-// is written to the last node's arrays.  res[4r..4r+3] (optional): most
-// Newton iterations of an interval, largest first correction, error norm,
-// first failed interval (0: none, -1: the RHS failed).
+// Each interval's |local error| is weighted with the weights
+// 1 / (rtol |y| + atol) of the node that completes its divided difference and
+// summed componentwise; the step is accepted when every interval converged
+// and that sum has weighted RMS norm at most 1.  The sum only grows, so the
+// test stops at the first node where its norm exceeds 1.  An accepted
+// reactor's converged end state is written to the last node's arrays.
+// res[4r..4r+3] (optional): most Newton iterations of an interval, largest
+// first correction, the error norm where the test stopped (infinite if an
+// interval failed first), first interval whose Newton iterations failed
+// (0: none, -1: the RHS failed).
 // LDS: Ml[kNP*kNP], rvec[kNP], t2l[kNP], then C, G, rop.
 // ---------------------------------------------------------------------------
 constexpr int kAcceptMaxNodes = 8;
@@ -211,27 +318,38 @@ accept_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ dts, doubl
     double y0 = 0.0;
     if (lane < nsp) y0 = Y0[(size_t)r * stride + lane];
     if (lane == nsp) y0 = T0[r] / o.Tref;
-    double fk[kAcceptMaxNodes], yk[kAcceptMaxNodes];
     RhsAux aux;
     double yprev = y0, fprev = 0.0;
     int bad = rhs64<kExt>(m, s, Pr, 0.0, o.Tref, o.step_limiter, y0, fprev, aux);
     int failed = 0, most_it = 0;
     double first_max = 0.0;
-#pragma unroll
-    for (int k = 0; k < kAcceptMaxNodes; ++k) {
-      fk[k] = 0.0;
-      yk[k] = 0.0;
-      if (k >= nodes || bad || failed) continue;
+    double esum = 0.0;  // this lane's weighted |local errors| summed so far
+    double en = 0.0;
+    bool stopped = false;
+    // f and t at the two converged nodes before node k (sliding window)
+    double fm2 = 0.0, fm1 = 0.0, tm2 = 0.0, tm1 = 0.0;
+    for (int k = 0; k < nodes && !bad && !failed && !stopped; ++k) {
       const size_t row = (size_t)k * n + r;
       double y = 0.0;
       if (lane < nsp) y = Yp[row * stride + lane];
       if (lane == nsp) y = Tp[row] / o.Tref;
-      const double h = (tau[k] - (k > 0 ? tau[k - 1] : 0.0)) * dt;
+      const double tk = tau[k] * dt;
+      const double h = tk - (k > 0 ? tm1 : 0.0);
       const double c = k == 0 ? h : 0.5 * h;
-      // Newton matrix I - c J at the proposal, inverted in registers
+      // the start point: the proposal, or y_{k-1} if its residual is smaller
       double fy;
-      bad = rhs64<kExt>(m, s, Pr, 0.0, o.Tref, o.step_limiter, y, fy, aux);
-      if (bad) continue;
+      const int badp = rhs64<kExt>(m, s, Pr, 0.0, o.Tref, o.step_limiter, y, fy, aux);
+      const double wp = act ? 1.0 / (o.rtol * fabs(yprev) + o.atol) : 0.0;
+      const double gp = (!act || badp) ? 0.0
+                        : (k == 0 ? y - yprev - h * fy : y - yprev - 0.5 * h * (fprev + fy));
+      const double rp = badp ? INFINITY : wave_sum(gp * gp * wp * wp);
+      const double rq = wave_sum(act ? h * h * fprev * fprev * wp * wp : 0.0);
+      if (rq < rp) {
+        y = yprev;
+        bad = rhs64<kExt>(m, s, Pr, 0.0, o.Tref, o.step_limiter, y, fy, aux);
+        if (bad) continue;
+      }
+      // Newton matrix I - c J at the start point, inverted in registers
       jacobian_lds<double, kExt>(m, s, Pr, 0.0, o, y, fy, aux.inv_rho, Ml, kNP, saved, t2l);
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -280,58 +398,38 @@ accept_kernel(DevMech m, BdfOpts o, int n, const double* __restrict__ dts, doubl
         continue;
       }
       bad = rhs64<kExt>(m, s, Pr, 0.0, o.Tref, o.step_limiter, y, fy, aux);
-      yk[k] = y;
-      fk[k] = fy;
+      if (bad) continue;
       yprev = y;
       fprev = fy;
-    }
-    // local error estimates of the converged nodes
-    double en = INFINITY;
-    double yend = 0.0;
-    if (!bad && !failed) {
-      double tk[kAcceptMaxNodes];
-#pragma unroll
-      for (int k = 0; k < kAcceptMaxNodes; ++k) tk[k] = k < nodes ? tau[k] * dt : 0.0;
-      // backward Euler on [0, t_1]: (h^2 / 2) y'', with y'' from f[t_1, t_2]
-      double e = fabs(0.5 * tk[0] * tk[0] * (fk[1] - fk[0]) / (tk[1] - tk[0]));
-#pragma unroll
-      for (int k = 1; k < kAcceptMaxNodes; ++k) {
-        if (k >= nodes) continue;
-        // trapezoidal rule on [t_{k-1}, t_k] (0-based nodes): (h^3 / 12) times
-        // the third derivative 2 f[t_a, t_b, t_c] over the three consecutive
-        // nodes ending at k (the first three for k = 1)
-        const int a = k >= 2 ? k - 2 : 0;
-        const double h = tk[k] - tk[k - 1];
-        double fa = 0.0, fb = 0.0, fc = 0.0, ta = 0.0, tb = 0.0, tc = 0.0;
-#pragma unroll
-        for (int q = 0; q < kAcceptMaxNodes; ++q) {
-          if (q == a) {
-            fa = fk[q];
-            ta = tk[q];
-          }
-          if (q == a + 1) {
-            fb = fk[q];
-            tb = tk[q];
-          }
-          if (q == a + 2) {
-            fc = fk[q];
-            tc = tk[q];
-          }
+      // the local errors that node k completes, in the weights at node k
+      double e = 0.0;
+      if (k == 1) {  // backward Euler on [0, t_1]: (h^2 / 2) y'' with y'' = f[t_1, t_2]
+        e = fabs(0.5 * tm1 * tm1 * (fy - fm1) / (tk - tm1));
+      } else if (k >= 2) {
+        // trapezoidal intervals: (h^3 / 12) 2 f[t_a, t_b, t_c] over the three
+        // consecutive nodes ending at k; the first three nodes also serve the
+        // trapezoidal interval before, which has no node of its own before it
+        const double dd = ((fy - fm1) / (tk - tm1) - (fm1 - fm2) / (tm1 - tm2)) / (tk - tm2);
+        e = fabs(h * h * h / 12.0 * 2.0 * dd);
+        if (k == 2) {
+          const double h1 = tm1 - tm2;
+          e += fabs(h1 * h1 * h1 / 12.0 * 2.0 * dd);
         }
-        const double dd = ((fc - fb) / (tc - tb) - (fb - fa) / (tb - ta)) / (tc - ta);
-        e += fabs(h * h * h / 12.0 * 2.0 * dd);
       }
-#pragma unroll
-      for (int k = 0; k < kAcceptMaxNodes; ++k)
-        if (k == nodes - 1) yend = yk[k];
-      const double w = act ? 1.0 / (o.rtol * fabs(yend) + o.atol) : 0.0;
-      en = sqrt(wave_sum(act ? e * e * w * w : 0.0) * inv);
+      fm2 = fm1;
+      fm1 = fy;
+      tm2 = tm1;
+      tm1 = tk;
+      esum += act ? e / (o.rtol * fabs(y) + o.atol) : 0.0;
+      en = sqrt(wave_sum(esum * esum) * inv);
+      if (en > 1.0) stopped = true;
     }
+    if (bad || failed) en = INFINITY;
     const bool acc = !bad && !failed && en <= 1.0;
     if (acc) {
       const size_t row = (size_t)(nodes - 1) * n + r;
-      if (lane < nsp) Yp[row * stride + lane] = yend;
-      if (lane == nsp) Tp[row] = yend * o.Tref;
+      if (lane < nsp) Yp[row * stride + lane] = yprev;
+      if (lane == nsp) Tp[row] = yprev * o.Tref;
     }
     if (lane == 0) {
       accept[r] = acc ? 1 : 0;
@@ -367,6 +465,29 @@ hipError_t launch_gemm_bf16(int M, int N, int K, const unsigned short* A, long s
                             const unsigned short* B, long sBk, long sBn, float* C, long sCm,
                             long sCn, int accumulate, hipStream_t st) {
   if (M <= 0 || N <= 0) return hipSuccess;
+  // the tiled kernel where its layout conditions hold (16-byte loads along
+  // the contiguous dimension of each operand), else one wavefront per tile
+  const auto al16 = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
+  const bool aK = sAk == 1 && sAm % 8 == 0 && K % 8 == 0;
+  const bool aM = sAm == 1 && sAk % 8 == 0 && M % 8 == 0;
+  const bool bK = sBk == 1 && sBn % 8 == 0 && K % 8 == 0;
+  const bool bN = sBn == 1 && sBk % 8 == 0;
+  if (N % kGemmBN == 0 && K > 0 && (aK || aM) && (bK || bN) && al16(A) && al16(B)) {
+    const dim3 grid((unsigned)((M + kGemmBM - 1) / kGemmBM), (unsigned)(N / kGemmBN));
+    if (aK && bN)
+      hipLaunchKernelGGL((gemm_bf16_tiled_kernel<true, false>), grid, dim3(256), 0, st, M, N, K, A,
+                         sAm, sAk, B, sBk, sBn, C, sCm, sCn, accumulate);
+    else if (aK)
+      hipLaunchKernelGGL((gemm_bf16_tiled_kernel<true, true>), grid, dim3(256), 0, st, M, N, K, A,
+                         sAm, sAk, B, sBk, sBn, C, sCm, sCn, accumulate);
+    else if (bN)
+      hipLaunchKernelGGL((gemm_bf16_tiled_kernel<false, false>), grid, dim3(256), 0, st, M, N, K,
+                         A, sAm, sAk, B, sBk, sBn, C, sCm, sCn, accumulate);
+    else
+      hipLaunchKernelGGL((gemm_bf16_tiled_kernel<false, true>), grid, dim3(256), 0, st, M, N, K, A,
+                         sAm, sAk, B, sBk, sBn, C, sCm, sCn, accumulate);
+    return hipGetLastError();
+  }
   const long tiles = (long)((M + 15) / 16) * ((N + 15) / 16);
   hipLaunchKernelGGL(gemm_bf16_kernel, dim3(blocks_for(tiles * kWave, 256)), dim3(256), 0, st, M,
                      N, K, A, sAm, sAk, B, sBk, sBn, C, sCm, sCn, accumulate);
