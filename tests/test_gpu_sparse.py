@@ -101,11 +101,11 @@ def test_sparse_path_2000_species_workload(torch_gpu, oracle_mod):
     the adiabatic enthalpy invariant.  The step cap is 1,500 CVode steps with
     no retry: a reactor that needs more ends with CV_TOO_MUCH_WORK and keeps
     its input state, as the plugin's solver does (solver_cvode.cpp:228-244);
-    at most 1 % of the batch may: 23 of these 4,096 reactors need more than
-    1,500 steps or stop at a Newton convergence failure (the others at most
-    812 steps), and so does the FP64 oracle on the one checked (reactor 397:
-    1,500 of 1,500 steps, 6 convergence failures); a step takes about 20 ms
-    on one workgroup."""
+    at most 1 % of the batch may: about 25 of these 4,096 reactors (which
+    ones moves with the round-off of the workgroup's reductions) need more
+    than 1,500 steps or stop at a Newton convergence failure, the others at
+    most about 950, and so does the FP64 oracle on the one checked (reactor
+    397: 1,500 of 1,500 steps, 6 convergence failures)."""
     from zerork_hip import capi, mechgen
     from conftest import GOLDEN
     import os
@@ -123,17 +123,8 @@ def test_sparse_path_2000_species_workload(torch_gpu, oracle_mod):
     st.T[at], st.P[at], st.Y[at] = fx["T0"], fx["P0"], fx["Y0"]
     opts = capi.default_options(rel_tol=RTOL, abs_tol=ATOL, constant_volume=0,
                                 fp32_preconditioner=1, max_steps=1500, cvode_num_retries=1)
-    # the batch in four calls of 1,024 reactors, each reporting when it ends
     t0 = time.perf_counter()
-    parts = []
-    for c0 in range(0, n, 1024):
-        sl = slice(c0, min(n, c0 + 1024))
-        parts.append(_gpu_integrate(torch_gpu, m, st.T[sl], st.P[sl], st.Y[sl], dt, opts))
-        s_ = parts[-1][2]
-        print(f"2000sp: reactors {sl.start}..{sl.stop - 1} done at {time.perf_counter() - t0:.1f} s; "
-              f"steps mean {s_[s_ > 0].mean():.1f} max {np.abs(s_).max()}; failed "
-              f"{int(np.sum(s_ <= 0))}", flush=True)
-    Tg, Yg, steps, stats = (np.concatenate([q[k] for q in parts]) for k in range(4))
+    Tg, Yg, steps, stats = _gpu_integrate(torch_gpu, m, st.T, st.P, st.Y, dt, opts)
     el = time.perf_counter() - t0
     ok = steps > 0
     print(f"2000sp: {n} reactors in {el:.1f} s; steps mean {steps[ok].mean():.1f} max "

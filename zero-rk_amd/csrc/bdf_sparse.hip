@@ -950,7 +950,9 @@ __global__ void sp_counter_clear(unsigned int* p) {
   if (threadIdx.x == 0) p[0] = 0u;
 }
 
-__global__ void __launch_bounds__(kSpThreads)
+// at least four wavefronts per SIMD (128 VGPRs): four workgroups per CU,
+// which the LDS rows (four of N floats per workgroup) also allow
+__global__ void __launch_bounds__(kSpThreads, 4)
 bdf_sparse_kernel(DevMech m, DevSparse S, BdfOpts o, int n, double dt_all,
                   const double* __restrict__ dts, double* __restrict__ Tio,
                   double* __restrict__ Pio, double* __restrict__ Yio, int stride,

@@ -37,8 +37,12 @@ hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, cons
                       int W, hipStream_t st);
 
 // sparse path (bdf_sparse.hip): one reactor per workgroup of kSparseWaves
-// wavefronts, any mechanism size
-constexpr int kSparseWaves = 8;
+// wavefronts, any mechanism size.  Four wavefronts with four workgroups per
+// CU (128 VGPRs) gave the 2,000-species workload 123 reactors/s, against 98
+// for two workgroups of eight wavefronts, 89 for eight of two and 60 for one
+// workgroup of eight: its BDF steps are latency chains, so more reactors in
+// flight per CU win over more threads per reactor
+constexpr int kSparseWaves = 4;
 struct DevSparse;
 size_t sparse_slot_bytes(const DevMech& m, const DevSparse& S);
 size_t sparse_lds_bytes(const DevMech& m);
