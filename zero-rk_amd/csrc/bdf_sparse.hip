@@ -51,7 +51,12 @@ struct SpCv {
   DevMech m;
   DevSparse S;
   BdfOpts o;
-  int tid, wave, lane, nsp, N;
+  int nsp, N;
+  // per-lane and per-wavefront indices (the object itself is shared by the
+  // wavefront's lanes in LDS)
+  __device__ __forceinline__ int tid() const { return (int)threadIdx.x; }
+  __device__ __forceinline__ int wave() const { return (int)threadIdx.x / kWave; }
+  __device__ __forceinline__ int lane() const { return lane_id(); }
   float* w;  // this wavefront's dense LU row (LDS, N floats, zero between rows)
   double *zn[kSpQmax + 1], *ewt, *acor, *tempv, *ftemp, *y, *delta, *t1, *t2, *t3, *x;
   double *C, *G, *H, *CPM, *rop, *J;
@@ -94,7 +99,7 @@ struct SpCv {
 
   __device__ __forceinline__ double wrms(const double* v) {
     double s = 0.0;
-    for (int i = tid; i < N; i += kSpThreads) {
+    for (int i = tid(); i < N; i += kSpThreads) {
       const double p = v[i] * ewt[i];
       s += p * p;
     }
@@ -112,7 +117,7 @@ struct SpCv {
     double inv_rho = v0;
     if (!(v0 > 0.0)) {
       double part = 0.0;
-      for (int j = tid; j < nsp; j += kSpThreads) part += yy[j] * m.inv_mw[j];
+      for (int j = tid(); j < nsp; j += kSpThreads) part += yy[j] * m.inv_mw[j];
       const double wmix = 1.0 / team.sum(part);
       inv_rho = 1.0 / (P * wmix / (kRu * T));
     }
@@ -120,7 +125,7 @@ struct SpCv {
     const double vd = inv_rho, rho_c = 1.0 / vd;
     const TPow p = make_tpow(T);
     double cps = 0.0, cs = 0.0;
-    for (int j = tid; j < nsp; j += kSpThreads) {
+    for (int j = tid(); j < nsp; j += kSpThreads) {
       double cp, hh, g;
       thermo_one(m.thermo + 16 * j, p, cp, hh, g);
       if (cvol) {
@@ -136,16 +141,16 @@ struct SpCv {
       cps += cpm * yy[j];
       cs += Cj;
     }
-    if (tid == 0) {
+    if (tid() == 0) {
       C[nsp] = 1.0;
       G[nsp] = 0.0;
     }
     const double cp_mix = team.sum(cps);
     const double Csum = team.sum(cs);  // the sums end with a barrier: C, G visible
-    team_rates<kSpThreads, false>(tid, m, C, G, rop, p, Csum, o.step_limiter);
+    team_rates<kSpThreads, false>(tid(), m, C, G, rop, p, Csum, o.step_limiter);
     team.sync();
     double e = 0.0;
-    for (int j = tid; j < nsp; j += kSpThreads) {
+    for (int j = tid(); j < nsp; j += kSpThreads) {
       double cr, de;
       species_cd<false>(m, rop, j, cr, de);
       const double net = cr - de;
@@ -153,7 +158,7 @@ struct SpCv {
       e += H[j] * net * vd;
     }
     const double esum = team.sum(e);
-    if (tid == nsp % kSpThreads) yd[nsp] = -esum * kRu * yT / cp_mix;
+    if (tid() == nsp % kSpThreads) yd[nsp] = -esum * kRu * yT / cp_mix;
     inv_rho_last = inv_rho;
     cp_last = cp_mix;
     return 0;
@@ -168,16 +173,16 @@ struct SpCv {
   __device__ void jacobian(const double* yp, const double* fp) {
     const double mmf = o.min_mass_fraction;
     team.sync();
-    for (int i = tid; i < nsp; i += kSpThreads) {
+    for (int i = tid(); i < nsp; i += kSpThreads) {
       double v = yp[i];
       if (fabs(v) < mmf) v = (v < 0.0) ? -mmf : mmf;
       t1[i] = v;
       t2[i] = m.mw[i] / v * inv_rho_last;
     }
-    if (tid == nsp % kSpThreads) t1[nsp] = yp[nsp];
+    if (tid() == nsp % kSpThreads) t1[nsp] = yp[nsp];
     (void)rhs(t1, t3);
     team.sync();
-    for (int p = tid; p < S.nnz; p += kSpThreads) {
+    for (int p = tid(); p < S.nnz; p += kSpThreads) {
       const int r = S.pos_row[p], c = S.pos_col[p];
       double acc = 0.0;
       if (r < nsp && c < nsp) {
@@ -195,7 +200,7 @@ struct SpCv {
     const double RuTemp = kRu * yp[nsp];
     const double mult = -1.0 / cp_last;
     const double tdot = t3[nsp];
-    for (int j = tid; j < nsp; j += kSpThreads) {
+    for (int j = tid(); j < nsp; j += kSpThreads) {
       double acc = 0.0;
       for (int k = S.cp_ptr[j]; k < S.cp_ptr[j + 1]; ++k) {
         const int p = S.cp_pos[k];
@@ -213,13 +218,13 @@ struct SpCv {
     dT = tT - yT;
     const double inv = 1.0 / dT;
     team.sync();
-    for (int i = tid; i < nsp; i += kSpThreads) t1[i] = yp[i];
-    if (tid == nsp % kSpThreads) t1[nsp] = tT;
+    for (int i = tid(); i < nsp; i += kSpThreads) t1[i] = yp[i];
+    if (tid() == nsp % kSpThreads) t1[nsp] = tT;
     (void)rhs(t1, t3);
     team.sync();
-    for (int k = tid; k < N; k += kSpThreads) J[S.tcol_pos[k]] = (t3[k] - fp[k]) * inv;
+    for (int k = tid(); k < N; k += kSpThreads) J[S.tcol_pos[k]] = (t3[k] - fp[k]) * inv;
     // concentration -> mass-fraction columns
-    for (int p = tid; p < S.nnz; p += kSpThreads) {
+    for (int p = tid(); p < S.nnz; p += kSpThreads) {
       const int r = S.pos_row[p], c = S.pos_col[p];
       if (r < nsp && c < nsp) J[p] *= m.mw[r] * (1.0 / m.mw[c]);
     }
@@ -230,7 +235,7 @@ struct SpCv {
   // the lower-part columns below kend in ascending order
   __device__ __forceinline__ void eliminate_row(int i, int kend) {
     const int b = S.rp[i], e = S.rp[i + 1];
-    for (int p = b + lane; p < e; p += kWave) w[S.ci[p]] = lu[p];
+    for (int p = b + lane(); p < e; p += kWave) w[S.ci[p]] = lu[p];
     wave_sync();
     for (int p = b; p < e; ++p) {
       const int k = S.ci[p];
@@ -238,8 +243,8 @@ struct SpCv {
       const int dk = S.dpos[k], ek = S.rp[k + 1];
       const float lk = w[k] / lu[dk];
       wave_sync();
-      if (lane == 0) w[k] = lk;
-      for (int q2 = dk + 1 + lane; q2 < ek; q2 += kWave) w[S.ci[q2]] -= lk * lu[q2];
+      if (lane() == 0) w[k] = lk;
+      for (int q2 = dk + 1 + lane(); q2 < ek; q2 += kWave) w[S.ci[q2]] -= lk * lu[q2];
       wave_sync();
     }
   }
@@ -253,11 +258,11 @@ struct SpCv {
     const int sp = S.split, nd = N - sp;
     bool ok = true;
     for (int lev = 0; lev < S.nlev_l; ++lev) {
-      for (int idx = S.ll_ptr[lev] + wave; idx < S.ll_ptr[lev + 1]; idx += kSpWaves) {
+      for (int idx = S.ll_ptr[lev] + wave(); idx < S.ll_ptr[lev + 1]; idx += kSpWaves) {
         const int i = S.ll_rows[idx];
         eliminate_row(i, i);
         if (w[i] == 0.0f) ok = false;
-        for (int p = S.rp[i] + lane; p < S.rp[i + 1]; p += kWave) {
+        for (int p = S.rp[i] + lane(); p < S.rp[i + 1]; p += kWave) {
           const int c = S.ci[p];
           lu[p] = w[c];
           w[c] = 0.0f;
@@ -266,16 +271,16 @@ struct SpCv {
       }
       team.sync();
     }
-    for (int i = sp + wave; i < N; i += kSpWaves) {
+    for (int i = sp + wave(); i < N; i += kSpWaves) {
       eliminate_row(i, sp);
-      for (int p = S.rp[i] + lane; p < S.rp[i + 1]; p += kWave) {
+      for (int p = S.rp[i] + lane(); p < S.rp[i + 1]; p += kWave) {
         const int c = S.ci[p];
         if (c < sp) lu[p] = w[c];
       }
       float* Di = D + (size_t)(i - sp) * nd;
-      for (int c = sp + lane; c < N; c += kWave) Di[c - sp] = w[c];
+      for (int c = sp + lane(); c < N; c += kWave) Di[c - sp] = w[c];
       wave_sync();
-      for (int p = S.rp[i] + lane; p < S.rp[i + 1]; p += kWave) w[S.ci[p]] = 0.0f;
+      for (int p = S.rp[i] + lane(); p < S.rp[i + 1]; p += kWave) w[S.ci[p]] = 0.0f;
       wave_sync();
     }
     team.sync();
@@ -283,19 +288,19 @@ struct SpCv {
       const float* Dk = D + (size_t)k * nd;
       const float piv = Dk[k];
       if (piv == 0.0f) ok = false;
-      for (int i = k + 1 + wave; i < nd; i += kSpWaves) {
+      for (int i = k + 1 + wave(); i < nd; i += kSpWaves) {
         float* Di = D + (size_t)i * nd;
         const float a = Di[k];
         if (a == 0.0f) continue;
         const float li = a / piv;
-        for (int j = k + 1 + lane; j < nd; j += kWave) Di[j] -= li * Dk[j];
-        if (lane == 0) Di[k] = li;
+        for (int j = k + 1 + lane(); j < nd; j += kWave) Di[j] -= li * Dk[j];
+        if (lane() == 0) Di[k] = li;
       }
       team.sync();
     }
-    for (int i = sp + wave; i < N; i += kSpWaves) {
+    for (int i = sp + wave(); i < N; i += kSpWaves) {
       const float* Di = D + (size_t)(i - sp) * nd;
-      for (int p = S.rp[i] + lane; p < S.rp[i + 1]; p += kWave) {
+      for (int p = S.rp[i] + lane(); p < S.rp[i + 1]; p += kWave) {
         const int c = S.ci[p];
         if (c >= sp) lu[p] = Di[c - sp];
       }
@@ -311,85 +316,85 @@ struct SpCv {
   __device__ void solve(double* b) {
     const int sp = S.split, nd = N - sp;
     team.sync();
-    for (int i = tid; i < N; i += kSpThreads) x[i] = b[S.perm[i]];
+    for (int i = tid(); i < N; i += kSpThreads) x[i] = b[S.perm[i]];
     team.sync();
     for (int lev = 0; lev < S.nlev_l; ++lev) {
-      for (int idx = S.ll_ptr[lev] + wave; idx < S.ll_ptr[lev + 1]; idx += kSpWaves) {
+      for (int idx = S.ll_ptr[lev] + wave(); idx < S.ll_ptr[lev + 1]; idx += kSpWaves) {
         const int i = S.ll_rows[idx];
         double s = 0.0;
-        for (int p = S.rp[i] + lane; p < S.dpos[i]; p += kWave) s += (double)lu[p] * x[S.ci[p]];
+        for (int p = S.rp[i] + lane(); p < S.dpos[i]; p += kWave) s += (double)lu[p] * x[S.ci[p]];
         s = wave_sum(s);
-        if (lane == 0) x[i] -= s;
+        if (lane() == 0) x[i] -= s;
       }
       team.sync();
     }
-    for (int i = sp + wave; i < N; i += kSpWaves) {
+    for (int i = sp + wave(); i < N; i += kSpWaves) {
       double s = 0.0;
-      for (int p = S.rp[i] + lane; p < S.rp[i + 1]; p += kWave)
+      for (int p = S.rp[i] + lane(); p < S.rp[i + 1]; p += kWave)
         if (S.ci[p] < sp) s += (double)lu[p] * x[S.ci[p]];
       s = wave_sum(s);
-      if (lane == 0) x[i] -= s;
+      if (lane() == 0) x[i] -= s;
     }
     team.sync();
     double* xb = x + sp;
     for (int kb = 0; kb < nd; kb += kWave) {  // forward, unit lower
       const int kn = min(kWave, nd - kb);
-      if (wave == 0) {
-        double xv = lane < kn ? xb[kb + lane] : 0.0;
+      if (wave() == 0) {
+        double xv = lane() < kn ? xb[kb + lane()] : 0.0;
         for (int c = 0; c < kn; ++c) {
           const double xc = __shfl(xv, c);
-          if (lane > c && lane < kn) xv -= (double)D[(size_t)(kb + lane) * nd + kb + c] * xc;
+          if (lane() > c && lane() < kn) xv -= (double)D[(size_t)(kb + lane()) * nd + kb + c] * xc;
         }
-        if (lane < kn) xb[kb + lane] = xv;
+        if (lane() < kn) xb[kb + lane()] = xv;
       }
       team.sync();
-      const double xl = lane < kn ? xb[kb + lane] : 0.0;
-      for (int j = kb + kn + wave; j < nd; j += kSpWaves) {
-        double s = lane < kn ? (double)D[(size_t)j * nd + kb + lane] * xl : 0.0;
+      const double xl = lane() < kn ? xb[kb + lane()] : 0.0;
+      for (int j = kb + kn + wave(); j < nd; j += kSpWaves) {
+        double s = lane() < kn ? (double)D[(size_t)j * nd + kb + lane()] * xl : 0.0;
         s = wave_sum(s);
-        if (lane == 0) xb[j] -= s;
+        if (lane() == 0) xb[j] -= s;
       }
       team.sync();
     }
     for (int kb = ((nd - 1) / kWave) * kWave; nd > 0 && kb >= 0; kb -= kWave) {  // backward
       const int kn = min(kWave, nd - kb);
-      if (wave == 0) {
-        double xv = lane < kn ? xb[kb + lane] : 0.0;
+      if (wave() == 0) {
+        double xv = lane() < kn ? xb[kb + lane()] : 0.0;
         for (int c = kn - 1; c >= 0; --c) {
-          if (lane == c) xv /= (double)D[(size_t)(kb + c) * nd + kb + c];
+          if (lane() == c) xv /= (double)D[(size_t)(kb + c) * nd + kb + c];
           const double xc = __shfl(xv, c);
-          if (lane < c) xv -= (double)D[(size_t)(kb + lane) * nd + kb + c] * xc;
+          if (lane() < c) xv -= (double)D[(size_t)(kb + lane()) * nd + kb + c] * xc;
         }
-        if (lane < kn) xb[kb + lane] = xv;
+        if (lane() < kn) xb[kb + lane()] = xv;
       }
       team.sync();
-      const double xl = lane < kn ? xb[kb + lane] : 0.0;
-      for (int j = wave; j < kb; j += kSpWaves) {
-        double s = lane < kn ? (double)D[(size_t)j * nd + kb + lane] * xl : 0.0;
+      const double xl = lane() < kn ? xb[kb + lane()] : 0.0;
+      for (int j = wave(); j < kb; j += kSpWaves) {
+        double s = lane() < kn ? (double)D[(size_t)j * nd + kb + lane()] * xl : 0.0;
         s = wave_sum(s);
-        if (lane == 0) xb[j] -= s;
+        if (lane() == 0) xb[j] -= s;
       }
       team.sync();
     }
     for (int lev = 0; lev < S.nlev_lb; ++lev) {
-      for (int idx = S.lbl_ptr[lev] + wave; idx < S.lbl_ptr[lev + 1]; idx += kSpWaves) {
+      for (int idx = S.lbl_ptr[lev] + wave(); idx < S.lbl_ptr[lev + 1]; idx += kSpWaves) {
         const int i = S.lbl_rows[idx];
         double s = 0.0;
-        for (int p = S.dpos[i] + 1 + lane; p < S.rp[i + 1]; p += kWave)
+        for (int p = S.dpos[i] + 1 + lane(); p < S.rp[i + 1]; p += kWave)
           s += (double)lu[p] * x[S.ci[p]];
         s = wave_sum(s);
-        if (lane == 0) x[i] = (x[i] - s) / (double)lu[S.dpos[i]];
+        if (lane() == 0) x[i] = (x[i] - s) / (double)lu[S.dpos[i]];
       }
       team.sync();
     }
-    for (int i = tid; i < N; i += kSpThreads) b[S.perm[i]] = x[i];
+    for (int i = tid(); i < N; i += kSpThreads) b[S.perm[i]] = x[i];
     team.sync();
   }
 
   // ---- CVODE (cvode.c) ----
   __device__ bool set_ewt(const double* yy) {
     double bad = 0.0;
-    for (int i = tid; i < N; i += kSpThreads) {
+    for (int i = tid(); i < N; i += kSpThreads) {
       const double v = rtol * fabs(yy[i]) + atol;
       if (v <= 0.0) bad = 1.0;
       else ewt[i] = 1.0 / v;
@@ -399,7 +404,7 @@ struct SpCv {
 
   __device__ void reinit(double t0, const double* y0) {
     if (y0 != zn[0])
-      for (int i = tid; i < N; i += kSpThreads) zn[0][i] = y0[i];
+      for (int i = tid(); i < N; i += kSpThreads) zn[0][i] = y0[i];
     tn = t0;
     q = 1;
     L = 2;
@@ -427,7 +432,7 @@ struct SpCv {
 
   __device__ double upper_bound_h0(double tdist) {
     double hub_inv = 0.0;
-    for (int i = tid; i < N; i += kSpThreads) {
+    for (int i = tid(); i < N; i += kSpThreads) {
       const double a = kHubFactor * fabs(zn[0][i]) + 1.0 / ewt[i];
       const double b = fabs(zn[1][i]) / a;
       if (b > hub_inv) hub_inv = b;
@@ -439,9 +444,9 @@ struct SpCv {
   }
 
   __device__ int ydd_norm(double hg, double& yddnrm) {
-    for (int i = tid; i < N; i += kSpThreads) y[i] = hg * zn[1][i] + zn[0][i];
+    for (int i = tid(); i < N; i += kSpThreads) y[i] = hg * zn[1][i] + zn[0][i];
     if (f(y, tempv) > 0) return 1;
-    for (int i = tid; i < N; i += kSpThreads) tempv[i] = (tempv[i] - zn[1][i]) * (1.0 / hg);
+    for (int i = tid(); i < N; i += kSpThreads) tempv[i] = (tempv[i] - zn[1][i]) * (1.0 / hg);
     yddnrm = wrms(tempv);
     return 0;
   }
@@ -495,7 +500,7 @@ struct SpCv {
   __device__ void rescale() {
     double fac = eta;
     for (int j = 1; j <= q; ++j) {
-      for (int i = tid; i < N; i += kSpThreads) zn[j][i] *= fac;
+      for (int i = tid(); i < N; i += kSpThreads) zn[j][i] *= fac;
       fac *= eta;
     }
     h = hscale * eta;
@@ -523,9 +528,9 @@ struct SpCv {
     const double A1 = (-alpha0 - alpha1) / prod;
     double* zL = zn[L];
     const double* za = zn[indx_acor];
-    for (int i = tid; i < N; i += kSpThreads) zL[i] = A1 * za[i];
+    for (int i = tid(); i < N; i += kSpThreads) zL[i] = A1 * za[i];
     for (int j = 2; j <= q; ++j)
-      for (int i = tid; i < N; i += kSpThreads) zn[j][i] += l[j] * zL[i];
+      for (int i = tid(); i < N; i += kSpThreads) zn[j][i] += l[j] * zL[i];
   }
 
   __device__ void decrease_bdf() {
@@ -538,7 +543,7 @@ struct SpCv {
       for (int i = j + 2; i >= 2; --i) l[i] = l[i] * xi + l[i - 1];
     }
     for (int j = 2; j < q; ++j)
-      for (int i = tid; i < N; i += kSpThreads) zn[j][i] = -l[j] * zn[q][i] + zn[j][i];
+      for (int i = tid(); i < N; i += kSpThreads) zn[j][i] = -l[j] * zn[q][i] + zn[j][i];
   }
 
   __device__ void adjust_order(int dq) {
@@ -561,14 +566,14 @@ struct SpCv {
     tn += h;
     for (int k = 1; k <= q; ++k)
       for (int j = q; j >= k; --j)
-        for (int i = tid; i < N; i += kSpThreads) zn[j - 1][i] += zn[j][i];
+        for (int i = tid(); i < N; i += kSpThreads) zn[j - 1][i] += zn[j][i];
   }
 
   __device__ void restore(double saved_t) {
     tn = saved_t;
     for (int k = 1; k <= q; ++k)
       for (int j = q; j >= k; --j)
-        for (int i = tid; i < N; i += kSpThreads) zn[j - 1][i] -= zn[j][i];
+        for (int i = tid(); i < N; i += kSpThreads) zn[j - 1][i] -= zn[j][i];
   }
 
   __device__ void set_tq(double hsum, double alpha0, double alpha0_hat, double xi_inv,
@@ -637,7 +642,7 @@ struct SpCv {
     } else {
       jcur = 0;
     }
-    for (int p = tid; p < S.nnz; p += kSpThreads) {
+    for (int p = tid(); p < S.nnz; p += kSpThreads) {
       double v = J[p] * (-gamma);
       if (S.pos_row[p] == S.pos_col[p]) v += 1.0;
       lu[p] = (float)v;
@@ -658,9 +663,9 @@ struct SpCv {
   }
 
   __device__ int nls_residual(const double* ycor, double* res) {
-    for (int i = tid; i < N; i += kSpThreads) y[i] = zn[0][i] + ycor[i];
+    for (int i = tid(); i < N; i += kSpThreads) y[i] = zn[0][i] + ycor[i];
     if (f(y, ftemp) > 0) return kRhsRecvr;
-    for (int i = tid; i < N; i += kSpThreads) {
+    for (int i = tid(); i < N; i += kSpThreads) {
       double r = rl1 * zn[1][i] + ycor[i];
       res[i] = -gamma * ftemp[i] + r;
     }
@@ -691,13 +696,13 @@ struct SpCv {
         int curiter = 0;
         for (;;) {
           nni++;
-          for (int i = tid; i < N; i += kSpThreads) delta[i] = -delta[i];
+          for (int i = tid(); i < N; i += kSpThreads) delta[i] = -delta[i];
           solve(delta);
           if (gamrat != 1.0) {
             const double s = 2.0 / (1.0 + gamrat);
-            for (int i = tid; i < N; i += kSpThreads) delta[i] *= s;
+            for (int i = tid(); i < N; i += kSpThreads) delta[i] *= s;
           }
-          for (int i = tid; i < N; i += kSpThreads) ycor[i] += delta[i];
+          for (int i = tid(); i < N; i += kSpThreads) ycor[i] += delta[i];
           rv = nls_convtest(curiter, ycor, delta, tol);
           if (rv == kNlsSuccess) {
             jcur = 0;
@@ -716,7 +721,7 @@ struct SpCv {
       if (rv > 0 && !jcur) {
         callLSetup = true;
         jbad = true;
-        for (int i = tid; i < N; i += kSpThreads) ycor[i] = 0.0;
+        for (int i = tid(); i < N; i += kSpThreads) ycor[i] = 0.0;
         continue;
       }
       return rv;
@@ -727,11 +732,11 @@ struct SpCv {
     convfail = (nflag == kFirstCall || nflag == kPrevErrFail) ? kNoFailures : kFailOther;
     const bool callSetup = (nflag == kPrevConvFail) || (nflag == kPrevErrFail) || (nst == 0) ||
                            (nst >= nstlp + kMsbp) || (fabs(gamrat - 1.0) > kDgmaxLsetup);
-    for (int i = tid; i < N; i += kSpThreads) acor[i] = 0.0;
+    for (int i = tid(); i < N; i += kSpThreads) acor[i] = 0.0;
     acnrmcur = 0;
     const int rv = newton(acor, tq[4], callSetup);
     if (rv != kNlsSuccess) return rv;
-    for (int i = tid; i < N; i += kSpThreads) y[i] = zn[0][i] + acor[i];
+    for (int i = tid(); i < N; i += kSpThreads) y[i] = zn[0][i] + acor[i];
     if (!acnrmcur) acnrm = wrms(acor);
     jcur = 0;
     return kCvSuccess;
@@ -785,7 +790,7 @@ struct SpCv {
     qwait = kLongWait;
     nscon = 0;
     if (f(zn[0], tempv) > 0) return kCvReptdRhsErr;
-    for (int i = tid; i < N; i += kSpThreads) zn[1][i] = h * tempv[i];
+    for (int i = tid(); i < N; i += kSpThreads) zn[1][i] = h * tempv[i];
     return kTryAgain;
   }
 
@@ -798,10 +803,10 @@ struct SpCv {
     if (q == 1 && nst > 1) tau[2] = tau[1];
     tau[1] = h;
     for (int j = 0; j <= q; ++j)
-      for (int i = tid; i < N; i += kSpThreads) zn[j][i] += l[j] * acor[i];
+      for (int i = tid(); i < N; i += kSpThreads) zn[j][i] += l[j] * acor[i];
     qwait--;
     if (qwait == 1 && q != kSpQmax) {
-      for (int i = tid; i < N; i += kSpThreads) zn[kSpQmax][i] = acor[i];
+      for (int i = tid(); i < N; i += kSpThreads) zn[kSpQmax][i] = acor[i];
       saved_tq5 = tq[5];
       indx_acor = kSpQmax;
     }
@@ -835,7 +840,7 @@ struct SpCv {
       double hr = 1.0;
       for (int i = 0; i < L; ++i) hr *= h / tau[2];
       const double cquot = (tq[5] / saved_tq5) * hr;
-      for (int i = tid; i < N; i += kSpThreads) tempv[i] = -cquot * zn[kSpQmax][i] + acor[i];
+      for (int i = tid(); i < N; i += kSpThreads) tempv[i] = -cquot * zn[kSpQmax][i] + acor[i];
       const double dup = wrms(tempv) * tq[3];
       etaqp1 = 1.0 / (pow(kBias3 * dup, 1.0 / (L + 1)) + kAddon);
     }
@@ -858,7 +863,7 @@ struct SpCv {
     } else {
       eta = etaqp1;
       qprime = q + 1;
-      for (int i = tid; i < N; i += kSpThreads) zn[kSpQmax][i] = acor[i];
+      for (int i = tid(); i < N; i += kSpThreads) zn[kSpQmax][i] = acor[i];
     }
   }
 
@@ -904,7 +909,7 @@ struct SpCv {
     complete_step();
     prepare_next_step(dsm);
     etamax = (nst <= kSmallNst) ? kEtamx2 : kEtamx3;
-    for (int i = tid; i < N; i += kSpThreads) acor[i] *= tq[2];
+    for (int i = tid(); i < N; i += kSpThreads) acor[i] *= tq[2];
     return kCvSuccess;
   }
 
@@ -918,7 +923,7 @@ struct SpCv {
       hscale = h;
       h0u = h;
       hprime = h;
-      for (int i = tid; i < N; i += kSpThreads) zn[1][i] *= h;
+      for (int i = tid(); i < N; i += kSpThreads) zn[1][i] *= h;
     } else {
       if (!set_ewt(zn[0])) return kCvIllInput;
     }
@@ -937,14 +942,19 @@ struct SpCv {
   // CVodeGetDky(t, 0) into out
   __device__ void dky0(double t, double* out) {
     const double s = (t - tn) / h;
-    for (int i = tid; i < N; i += kSpThreads) out[i] = 0.0;
+    for (int i = tid(); i < N; i += kSpThreads) out[i] = 0.0;
     for (int j = q; j >= 0; --j) {
       double c = 1.0;
       for (int i = 0; i < j; ++i) c *= s;
-      for (int i = tid; i < N; i += kSpThreads) out[i] += c * zn[j][i];
+      for (int i = tid(); i < N; i += kSpThreads) out[i] += c * zn[j][i];
     }
   }
 };
+
+// doubles of LDS holding one SpCv per wavefront (16-byte multiple)
+__host__ __device__ constexpr size_t sp_object_doubles() {
+  return ((sizeof(SpCv) * kSpWaves + 15) & ~(size_t)15) / sizeof(double);
+}
 
 __global__ void sp_counter_clear(unsigned int* p) {
   if (threadIdx.x == 0) p[0] = 0u;
@@ -960,23 +970,27 @@ bdf_sparse_kernel(DevMech m, DevSparse S, BdfOpts o, int n, double dt_all,
                   const int* __restrict__ order, unsigned int* __restrict__ counter,
                   double* __restrict__ ws, size_t slot_doubles) {
   extern __shared__ double lds[];
-  SpCv c;  // scalars are uniform: every thread computes the same values
-  c.team.red = lds;
+  // LDS: this wavefront's integrator object (every member is uniform: the
+  // lanes compute and store the same values; kept here rather than in
+  // registers or private memory, which its size would otherwise spill to),
+  // the team's reduction slots, then a dense FP32 row of N per wavefront
+  SpCv& c = reinterpret_cast<SpCv*>(lds)[threadIdx.x / kWave];
+  double* tail = lds + sp_object_doubles();
+  c.team.red = tail;
   c.m = m;
   c.S = S;
   c.o = o;
-  c.tid = (int)threadIdx.x;
-  c.wave = c.tid / kWave;
-  c.lane = lane_id();
   c.nsp = m.nsp;
   c.N = m.nsp + 1;
-  c.w = reinterpret_cast<float*>(lds + ((kSpWaves + 2) & ~1)) + (size_t)c.wave * c.N;
-  for (int i = c.lane; i < c.N; i += kWave) c.w[i] = 0.0f;
+  c.w = reinterpret_cast<float*>(tail + ((kSpWaves + 2) & ~1)) + (size_t)c.wave() * c.N;
+  wave_sync();
+  for (int i = c.lane(); i < c.N; i += kWave) c.w[i] = 0.0f;
   wave_sync();
   c.bind(ws + (size_t)blockIdx.x * slot_doubles);
   c.nlscoef = o.nlscoef;
   c.hmax_inv = o.hmax_inv;
-  const int nsp = m.nsp, N = c.N, tid = c.tid;
+  wave_sync();
+  const int nsp = m.nsp, N = c.N, tid = c.tid();
   for (;;) {
     unsigned int r = 0;
     if (tid == 0) r = atomicAdd(counter, 1u);
@@ -1057,7 +1071,7 @@ size_t sparse_slot_bytes(const DevMech& m, const DevSparse& S) {
 }
 
 size_t sparse_lds_bytes(const DevMech& m) {
-  return sizeof(double) * (size_t)((kSpWaves + 2) & ~1) +
+  return sizeof(double) * (sp_object_doubles() + (size_t)((kSpWaves + 2) & ~1)) +
          ((sizeof(float) * (size_t)kSpWaves * (m.nsp + 1) + 15) & ~(size_t)15);
 }
 
