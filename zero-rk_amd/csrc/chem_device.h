@@ -534,7 +534,8 @@ __device__ __forceinline__ void species_cd(const DevMech& m, const double* __res
       c += e.x * rop[(int)e.y];
     }
   }
-  const int nd = m.gd_blocks, nc = m.gc_blocks;
+  const int2 gn = m.g_nb[jb];  // this group's blocks in use (padding beyond)
+  const int nd = gn.x, nc = gn.y;
   const int nb = nd > nc ? nd : nc;
   uint4 wd = nd > 0 ? gd[0] : make_uint4(0, 0, 0, 0);
   uint4 wc = nc > 0 ? gc[0] : make_uint4(0, 0, 0, 0);

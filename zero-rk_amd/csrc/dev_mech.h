@@ -120,6 +120,9 @@ struct DevMech {
   // species j = 64*jb + (j&63), in step order; padding index nstep (rate 0)
   const uint4* g_d;
   const uint4* g_c;
+  // per 64-species group: {destroy, create} blocks in use (at most gd_blocks,
+  // gc_blocks); the entries beyond are padding
+  const int2* g_nb;
   // Jacobian species-block terms by row (nsp <= 191): jr[b*S + row] (S = 64,
   // 128 or 192: the integrator's team size) holds 4
   // terms, each bits 0-15 step, 16-23 column (0xFF = padding), bit 31 set for

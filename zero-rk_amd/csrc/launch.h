@@ -41,8 +41,7 @@ hipError_t launch_bdf(const DevMech& m, const BdfOpts& o, int n, double dt, cons
 // CU (128 VGPRs) gave the 2,000-species workload 123 reactors/s, against 98
 // for two workgroups of eight wavefronts, 89 for eight of two and 60 for one
 // workgroup of eight: its BDF steps are latency chains, so more reactors in
-// flight per CU win over more threads per reactor (with the integrator's
-// state moved to LDS since, 178 reactors/s)
+// flight per CU win over more threads per reactor
 constexpr int kSparseWaves = 4;
 struct DevSparse;
 size_t sparse_slot_bytes(const DevMech& m, const DevSparse& S);

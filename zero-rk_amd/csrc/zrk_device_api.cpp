@@ -195,6 +195,17 @@ int build_device_mech(MechHandle* h, std::string* err) {
     return g;
   };
   const std::vector<uint16_t> gd = pad_gather(dl, gdb), gc = pad_gather(cl, gcb);
+  // the blocks each 64-species group actually uses ({destroy, create}): a
+  // wavefront's gather loop stops there (the padding beyond adds the zero
+  // rate, so stopping early changes no sum); the common length is set by the
+  // most connected species of all, 15 to 60 times the typical one
+  std::vector<int32_t> gnb(2 * (size_t)nsb, 0);
+  for (int j = 0; j < nsp; ++j) {
+    int32_t& bd = gnb[2 * (size_t)(j / 64)];
+    int32_t& bc = gnb[2 * (size_t)(j / 64) + 1];
+    bd = std::max(bd, (int32_t)((dl[j].size() + 7) / 8));
+    bc = std::max(bc, (int32_t)((cl[j].size() + 7) / 8));
+  }
   // non-integer destruction/creation terms by species, in step order and
   // within a step in reaction-text order (non_integer_reaction_network.cpp:
   // 295-333; perf_net.cpp:250-253 adds them before the integer terms)
@@ -325,7 +336,8 @@ int build_device_mech(MechHandle* h, std::string* err) {
                o_fo = b.add(fo), o_gd = b.add(gd), o_gc = b.add(gc), o_jr = b.add(jrt),
                o_ext = b.add(ext), o_pl = b.add(plog_t), o_nsp = b.add(ni_sp),
                o_npw = b.add(ni_pow), o_ndp = b.add(nig_dptr), o_ncp = b.add(nig_cptr),
-               o_nig = b.add(nig), o_njt = b.add(ni_jt), o_jpat = b.add(jpat);
+               o_nig = b.add(nig), o_njt = b.add(ni_jt), o_jpat = b.add(jpat),
+               o_gnb = b.add(gnb);
   void* base = nullptr;
   hipError_t e = hipMalloc(&base, b.bytes.size());
   if (e != hipSuccess) {
@@ -361,6 +373,7 @@ int build_device_mech(MechHandle* h, std::string* err) {
   d.r_fo = (const double*)(p + o_fo);
   d.g_d = (const uint4*)(p + o_gd);
   d.g_c = (const uint4*)(p + o_gc);
+  d.g_nb = (const int2*)(p + o_gnb);
   d.jr = (const uint4*)(p + o_jr);
   d.n_ext = n_ext;
   d.r_ext = (const int4*)(p + o_ext);
