@@ -27,6 +27,7 @@ namespace zrk {
 
 constexpr int kSpWaves = kSparseWaves;
 constexpr int kSpThreads = kSpWaves * kWave;
+static_assert(kSpWaves >= 2, "the solves' N doubles live in the kSpWaves LDS rows of N floats");
 constexpr int kSpQmax = 5;
 using SpTeam = TeamBlock<kSpWaves, false>;
 
@@ -39,6 +40,18 @@ __host__ __device__ inline size_t sparse_slot_doubles(int nsp, int nstep, int nn
                    ((size_t)nd * nd + 1) / 2 + 1;
   return (d + 31) & ~(size_t)31;
 }
+
+// -DZRK_PHASE_CYCLES: per-reactor cycle counts of the phases (RHS, Jacobian,
+// Newton-matrix setup and factorisation, solves, whole reactor; then the
+// factorisation's leading levels, trailing rows and dense block) in stats
+// slots 8..15 of a 20-int layout, as the dense integrators report theirs
+#ifdef ZRK_PHASE_CYCLES
+#define SP_TIC(t) const long long t = (long long)__builtin_amdgcn_s_memtime()
+#define SP_TOC(t, i) cyc[i] += (long long)__builtin_amdgcn_s_memtime() - t
+#else
+#define SP_TIC(t)
+#define SP_TOC(t, i)
+#endif
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -58,6 +71,8 @@ struct SpCv {
   __device__ __forceinline__ int wave() const { return (int)threadIdx.x / kWave; }
   __device__ __forceinline__ int lane() const { return lane_id(); }
   float* w;  // this wavefront's dense LU row (LDS, N floats, zero between rows)
+  double* xs;  // the solves' work vector (N doubles): the LDS of the rows w, which
+               // only the factorisation uses; it zeroes its row on entry
   double *zn[kSpQmax + 1], *ewt, *acor, *tempv, *ftemp, *y, *delta, *t1, *t2, *t3, *x;
   double *C, *G, *H, *CPM, *rop, *J;
   float* lu;
@@ -72,6 +87,9 @@ struct SpCv {
   int q, qprime, next_q, qwait, L, qu, indx_acor;
   int nst, nstlp, nstlj, nfe, nje, nsetups, nni, ncfn, netf, nscon;
   int convfail, jcur, acnrmcur;
+#ifdef ZRK_PHASE_CYCLES
+  long long cyc[8];
+#endif
 
   __device__ void bind(double* slot) {
     double* p = slot;
@@ -166,7 +184,10 @@ struct SpCv {
 
   __device__ __forceinline__ int f(const double* yy, double* yd) {
     nfe++;
-    return rhs(yy, yd);
+    SP_TIC(t0);
+    const int rv = rhs(yy, yd);
+    SP_TOC(t0, 0);
+    return rv;
   }
 
   // the reference's Jacobian at (yp, fp) into J at the plan's positions
@@ -232,20 +253,57 @@ struct SpCv {
   }
 
   // row i of the LU by one wavefront (up-looking): scatter into w, eliminate
-  // the lower-part columns below kend in ascending order
+  // the lower-part columns below kend in ascending order.  The eliminations
+  // form a chain (each multiplier reads w after the previous updates), but
+  // what they read of the finished rows k (pivot, and each lane's first
+  // entry of row k's upper part) does not depend on the chain: it is loaded
+  // kPre columns at a time before the chain runs through them, so the chain
+  // waits for memory once per kPre columns instead of once per column.
+  static constexpr int kPre = 8;
   __device__ __forceinline__ void eliminate_row(int i, int kend) {
     const int b = S.rp[i], e = S.rp[i + 1];
     for (int p = b + lane(); p < e; p += kWave) w[S.ci[p]] = lu[p];
     wave_sync();
-    for (int p = b; p < e; ++p) {
-      const int k = S.ci[p];
-      if (k >= kend) break;
-      const int dk = S.dpos[k], ek = S.rp[k + 1];
-      const float lk = w[k] / lu[dk];
-      wave_sync();
-      if (lane() == 0) w[k] = lk;
-      for (int q2 = dk + 1 + lane(); q2 < ek; q2 += kWave) w[S.ci[q2]] -= lk * lu[q2];
-      wave_sync();
+    for (int p0 = b; p0 < e; p0 += kPre) {
+      int kk[kPre], dkk[kPre], ekk[kPre], uc[kPre];
+      float pv[kPre], uv[kPre];
+#pragma unroll
+      for (int j = 0; j < kPre; ++j) {
+        const int p = p0 + j;
+        const int k = p < e ? S.ci[p] : kend;
+        kk[j] = k < kend ? k : -1;
+        dkk[j] = ekk[j] = uc[j] = 0;
+        pv[j] = uv[j] = 0.0f;
+        if (k < kend) {
+          const int dk = S.dpos[k], ek = S.rp[k + 1], q2 = dk + 1 + lane();
+          dkk[j] = dk;
+          ekk[j] = ek;
+          pv[j] = lu[dk];
+          if (q2 < ek) {
+            uv[j] = lu[q2];
+            uc[j] = S.ci[q2];
+          }
+        }
+      }
+      bool done = false;
+#pragma unroll
+      for (int j = 0; j < kPre; ++j) {
+        if (!done) {
+          const int k = kk[j];
+          if (k < 0) {
+            done = true;
+          } else {
+            const float lk = w[k] / pv[j];
+            wave_sync();
+            if (lane() == 0) w[k] = lk;
+            if (dkk[j] + 1 + lane() < ekk[j]) w[uc[j]] -= lk * uv[j];
+            for (int q2 = dkk[j] + 1 + kWave + lane(); q2 < ekk[j]; q2 += kWave)
+              w[S.ci[q2]] -= lk * lu[q2];
+            wave_sync();
+          }
+        }
+      }
+      if (done) break;
     }
   }
 
@@ -257,6 +315,9 @@ struct SpCv {
   __device__ bool factor() {
     const int sp = S.split, nd = N - sp;
     bool ok = true;
+    for (int i = lane(); i < N; i += kWave) w[i] = 0.0f;  // the solves reuse this LDS
+    wave_sync();
+    SP_TIC(f0);
     for (int lev = 0; lev < S.nlev_l; ++lev) {
       for (int idx = S.ll_ptr[lev] + wave(); idx < S.ll_ptr[lev + 1]; idx += kSpWaves) {
         const int i = S.ll_rows[idx];
@@ -271,6 +332,8 @@ struct SpCv {
       }
       team.sync();
     }
+    SP_TOC(f0, 5);
+    SP_TIC(f1);
     for (int i = sp + wave(); i < N; i += kSpWaves) {
       eliminate_row(i, sp);
       for (int p = S.rp[i] + lane(); p < S.rp[i + 1]; p += kWave) {
@@ -284,6 +347,8 @@ struct SpCv {
       wave_sync();
     }
     team.sync();
+    SP_TOC(f1, 6);
+    SP_TIC(f2);
     for (int k = 0; k < nd; ++k) {
       const float* Dk = D + (size_t)k * nd;
       const float piv = Dk[k];
@@ -306,6 +371,7 @@ struct SpCv {
       }
     }
     team.sync();
+    SP_TOC(f2, 7);
     return team.max(ok ? 0.0 : 1.0) == 0.0;
   }
 
@@ -316,27 +382,27 @@ struct SpCv {
   __device__ void solve(double* b) {
     const int sp = S.split, nd = N - sp;
     team.sync();
-    for (int i = tid(); i < N; i += kSpThreads) x[i] = b[S.perm[i]];
+    for (int i = tid(); i < N; i += kSpThreads) xs[i] = b[S.perm[i]];
     team.sync();
     for (int lev = 0; lev < S.nlev_l; ++lev) {
       for (int idx = S.ll_ptr[lev] + wave(); idx < S.ll_ptr[lev + 1]; idx += kSpWaves) {
         const int i = S.ll_rows[idx];
         double s = 0.0;
-        for (int p = S.rp[i] + lane(); p < S.dpos[i]; p += kWave) s += (double)lu[p] * x[S.ci[p]];
+        for (int p = S.rp[i] + lane(); p < S.dpos[i]; p += kWave) s += (double)lu[p] * xs[S.ci[p]];
         s = wave_sum(s);
-        if (lane() == 0) x[i] -= s;
+        if (lane() == 0) xs[i] -= s;
       }
       team.sync();
     }
     for (int i = sp + wave(); i < N; i += kSpWaves) {
       double s = 0.0;
       for (int p = S.rp[i] + lane(); p < S.rp[i + 1]; p += kWave)
-        if (S.ci[p] < sp) s += (double)lu[p] * x[S.ci[p]];
+        if (S.ci[p] < sp) s += (double)lu[p] * xs[S.ci[p]];
       s = wave_sum(s);
-      if (lane() == 0) x[i] -= s;
+      if (lane() == 0) xs[i] -= s;
     }
     team.sync();
-    double* xb = x + sp;
+    double* xb = xs + sp;
     for (int kb = 0; kb < nd; kb += kWave) {  // forward, unit lower
       const int kn = min(kWave, nd - kb);
       if (wave() == 0) {
@@ -381,13 +447,13 @@ struct SpCv {
         const int i = S.lbl_rows[idx];
         double s = 0.0;
         for (int p = S.dpos[i] + 1 + lane(); p < S.rp[i + 1]; p += kWave)
-          s += (double)lu[p] * x[S.ci[p]];
+          s += (double)lu[p] * xs[S.ci[p]];
         s = wave_sum(s);
-        if (lane() == 0) x[i] = (x[i] - s) / (double)lu[S.dpos[i]];
+        if (lane() == 0) xs[i] = (xs[i] - s) / (double)lu[S.dpos[i]];
       }
       team.sync();
     }
-    for (int i = tid(); i < N; i += kSpThreads) b[S.perm[i]] = x[i];
+    for (int i = tid(); i < N; i += kSpThreads) b[S.perm[i]] = xs[i];
     team.sync();
   }
 
@@ -638,17 +704,22 @@ struct SpCv {
       jcur = 1;
       nje++;
       nstlj = nst;
+      SP_TIC(t0);
       jacobian(y, ftemp);
+      SP_TOC(t0, 1);
     } else {
       jcur = 0;
     }
+    SP_TIC(t1);
     for (int p = tid(); p < S.nnz; p += kSpThreads) {
       double v = J[p] * (-gamma);
       if (S.pos_row[p] == S.pos_col[p]) v += 1.0;
       lu[p] = (float)v;
     }
     team.sync();
-    return factor() ? 0 : 1;
+    const bool ok = factor();
+    SP_TOC(t1, 2);
+    return ok ? 0 : 1;
   }
 
   __device__ int nls_lsetup(bool jbad) {
@@ -697,7 +768,9 @@ struct SpCv {
         for (;;) {
           nni++;
           for (int i = tid(); i < N; i += kSpThreads) delta[i] = -delta[i];
+          SP_TIC(t0);
           solve(delta);
+          SP_TOC(t0, 3);
           if (gamrat != 1.0) {
             const double s = 2.0 / (1.0 + gamrat);
             for (int i = tid(); i < N; i += kSpThreads) delta[i] *= s;
@@ -983,6 +1056,7 @@ bdf_sparse_kernel(DevMech m, DevSparse S, BdfOpts o, int n, double dt_all,
   c.nsp = m.nsp;
   c.N = m.nsp + 1;
   c.w = reinterpret_cast<float*>(tail + ((kSpWaves + 2) & ~1)) + (size_t)c.wave() * c.N;
+  c.xs = tail + ((kSpWaves + 2) & ~1);  // kSpWaves rows of N floats hold N doubles
   wave_sync();
   for (int i = c.lane(); i < c.N; i += kWave) c.w[i] = 0.0f;
   wave_sync();
@@ -1015,6 +1089,10 @@ bdf_sparse_kernel(DevMech m, DevSparse S, BdfOpts o, int n, double dt_all,
     c.rtol = o.rtol;
     c.inv_rho_last = 0.0;
     c.team.sync();
+#ifdef ZRK_PHASE_CYCLES
+    for (int k = 0; k < 8; ++k) c.cyc[k] = 0;
+    const long long tr0 = (long long)__builtin_amdgcn_s_memtime();
+#endif
     c.reinit(0.0, y0);
     int nsteps = 0, num_tries = 0, flag = kCvSuccess;
     int tot[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -1054,11 +1132,20 @@ bdf_sparse_kernel(DevMech m, DevSparse S, BdfOpts o, int n, double dt_all,
     } else {
       nsteps = (nsteps <= 0) ? -1 : -nsteps;
     }
+#ifdef ZRK_PHASE_CYCLES
+    c.cyc[4] = (long long)__builtin_amdgcn_s_memtime() - tr0;
+#endif
     if (tid == 0) {
       if (nsteps_out) nsteps_out[r] = nsteps;
       if (stats_out) {
+#ifdef ZRK_PHASE_CYCLES
+        for (int k = 0; k < 7; ++k) stats_out[(size_t)r * 20 + k] = tot[k];
+        stats_out[(size_t)r * 20 + 7] = flag;
+        for (int k = 0; k < 8; ++k) stats_out[(size_t)r * 20 + 8 + k] = (int)(c.cyc[k] >> 4);
+#else
         for (int k = 0; k < 7; ++k) stats_out[(size_t)r * 8 + k] = tot[k];
         stats_out[(size_t)r * 8 + 7] = flag;
+#endif
       }
     }
     c.team.sync();
