@@ -285,11 +285,13 @@ def test_surrogate_step_53sp(torch_gpu):
     implicit-residual and error test, the rest integrated by the plain BDF.
     Against the network-free proposal (every node at the initial state) under
     the same test, the network's trajectories must start the test's Newton
-    iterations with smaller corrections; the convergence and acceptance of
-    both are reported (the error test, which only near-frozen reactors pass
-    at rtol 1e-6, decides it).  Accepted end
-    states must agree with the plain BDF's within the integrator's own scale
-    of differences, and rejected reactors are the plain BDF's."""
+    iterations with smaller corrections.  Acceptance is decided by the error
+    test, which at rtol 1e-6 only near-frozen reactors pass: the integrator's
+    own states at the nodes, the proposal no network can improve on, must be
+    accepted within a point of "hold", and the network's acceptance too.
+    Accepted end states must agree with the plain BDF's within the
+    integrator's own scale of differences, and rejected reactors are the
+    plain BDF's."""
     from zerork_hip import capi, mechgen, surrogate
     torch = torch_gpu
     m = capi.Mech(*mech_paths("synth_53sp"))
@@ -328,3 +330,17 @@ def test_surrogate_step_53sp(torch_gpu):
         if a.any():
             assert np.percentile(units[a], 99) <= 100.0
     assert first["mlp"] < first["hold"], first
+    # the perfect proposal: the plain integrator's states at the nodes
+    Tn, Yn = [], []
+    for f in surrogate.Surrogate.TAU:
+        Tk, Pk, Yk = t(st.T), t(st.P), t(st.Y)
+        _select(torch, surrogate, m, opts, Tk, Pk, Yk, t(dt * f))
+        Tn.append(Tk)
+        Yn.append(Yk)
+    acc, res = s.accept(t(st.T), t(st.P), t(st.Y), torch.cat(Tn).contiguous(),
+                        torch.cat(Yn).contiguous(), t(dt), opts)
+    torch.cuda.synchronize()
+    out["bdf nodes"] = float(acc.cpu().numpy().mean())
+    print(f"bdf nodes: accepted {100 * out['bdf nodes']:.1f} %", flush=True)
+    assert abs(out["bdf nodes"] - out["hold"]) <= 0.01, out
+    assert abs(out["mlp"] - out["hold"]) <= 0.01, out
