@@ -93,6 +93,10 @@ zerork_status_t ZERORK_CFD_PLUGIN_EXPORTS zerork_reactor_get_double_option(
 zerork_status_t ZERORK_CFD_PLUGIN_EXPORTS zerork_reactor_get_string_option(
     const char* option_name_chr, char** option_value, int string_length, zerork_handle handle);
 
+/* Called for a one-reactor solve with load_balance 0, once per successful
+ * CVode call (solver_cvode.cpp:247-257), after the GPU solve over step
+ * records the integrator kept; a non-zero return ends the integration at
+ * that call. */
 zerork_status_t ZERORK_CFD_PLUGIN_EXPORTS zerork_reactor_set_callback_fn(
     zerork_callback_fn fn, void* cb_fn_data, zerork_handle handle);
 

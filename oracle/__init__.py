@@ -79,6 +79,11 @@ class Opts(C.Structure):
                 ("preconditioner_threshold", C.c_double)]
 
 
+# the plugin's step callback type (zerork_cfd_plugin.h:44)
+STEP_CB = C.CFUNCTYPE(C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                      C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p)
+
+
 class Stats(C.Structure):
     _fields_ = [("nsteps", C.c_int), ("nst", C.c_int), ("nfe", C.c_int), ("nje", C.c_int),
                 ("nsetups", C.c_int), ("nni", C.c_int), ("ncfn", C.c_int), ("netf", C.c_int),
@@ -98,6 +103,8 @@ def _declare(L):
     L.zrko_integrate_batch.restype = C.c_int
     L.zrko_integrate_batch_aux.argtypes = [P, P, C.c_int, C.c_double, P, P, P, C.c_int, P,
                                            C.c_int, P, P, P, P]
+    L.zrko_integrate_one_cb.argtypes = [P, P, C.c_double, P, P, P, P, STEP_CB, C.c_void_p]
+    L.zrko_integrate_one_cb.restype = C.c_int
     L.zrko_integrate_batch_aux.restype = C.c_int
     L.zrko_rhs_src.argtypes = [P, P, C.c_double, P, C.c_double, P, C.c_double, C.c_double, P, P]
     L.zrko_rhs_src.restype = C.c_int
@@ -220,6 +227,32 @@ class Oracle:
         flag = self.L.zrko_rhs_src(self.ptr(), C.byref(o), P0, _p(y0), t, _p(y), dpdt, esrc,
                                    _p(ys) if ys is not None else None, _p(yd))
         return yd, flag
+
+    def integrate_steps(self, T, P, Y, dt, opts: Opts | None = None, stop_at=None):
+        """One reactor with the plugin's step callback (solver_cvode.cpp:247-257):
+        returns (T_out, P_out, Y_out, stats, records), records a list of
+        (CVode calls, t, t - t_prev, y, ydot) per successful CVode call, y and
+        ydot of the solved variables; with stop_at = k the callback returns 1
+        on record k, which ends the integration there."""
+        o = opts or default_opts()
+        nsp = self.nsp
+        T = np.array([T], dtype=np.float64).reshape(1)
+        P = np.array([P], dtype=np.float64).reshape(1)
+        Y = np.array(Y, dtype=np.float64, copy=True).reshape(nsp)
+        nvar = nsp + 1 if o.solve_temperature else nsp
+        recs = []
+
+        def cb(rid, nsteps, t, dtk, y, ydot, data):
+            recs.append((nsteps, t, dtk, np.ctypeslib.as_array(y, (nvar,)).copy(),
+                         np.ctypeslib.as_array(ydot, (nvar,)).copy()))
+            return 1 if stop_at is not None and len(recs) - 1 == stop_at else 0
+
+        fn = STEP_CB(cb)
+        st = Stats()
+        self.L.zrko_integrate_one_cb(self.ptr(), C.byref(o), dt, _p(T), _p(P), _p(Y),
+                                     C.cast(C.pointer(st), C.c_void_p), fn, None)
+        stats = {f: getattr(st, f) for f, _ in Stats._fields_}
+        return T[0], P[0], Y, stats, recs
 
     def integrate(self, T, P, Y, dt, opts: Opts | None = None, nthreads=0, dpdt=None,
                   esrc=None, ysrc=None):

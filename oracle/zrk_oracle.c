@@ -1606,6 +1606,42 @@ static void cv_dky0(cvmem* cv, double t, double* out) {
   }
 }
 
+/* CVodeGetDky(t, 1): the sum over j = q..1 of j s^(j-1) zn[j], times 1/h */
+static void cv_dky1(cvmem* cv, double t, double* out) {
+  const double s = (t - cv->tn) / cv->h;
+  for (int i = 0; i < cv->N; ++i) out[i] = 0.0;
+  for (int j = cv->q; j >= 1; --j) {
+    double c = (double)j;
+    for (int i = 0; i < j - 1; ++i) c *= s;
+    for (int i = 0; i < cv->N; ++i) out[i] += c * cv->zn[j][i];
+  }
+  const double hinv = 1.0 / cv->h;
+  for (int i = 0; i < cv->N; ++i) out[i] *= hinv;
+}
+
+/* The plugin's step callback after a successful CVode call
+ * (solver_cvode.cpp:247-257): the state CVode returned (zn[0] at tn, else the
+ * interpolant at tc), CVodeGetDky(tc, 1), the time since the previous call;
+ * returns the callback's value (non-zero ends the integration). */
+typedef struct {
+  zrko_step_cb fn;
+  void* data;
+  double tprev;
+  double* y;
+  double* ydot;
+} stepcb;
+
+static int step_callback(cvmem* cv, stepcb* cb, long nsteps, double tc) {
+  if (tc == cv->tn)
+    memcpy(cb->y, cv->zn[0], sizeof(double) * (size_t)cv->N);
+  else
+    cv_dky0(cv, tc, cb->y);
+  cv_dky1(cv, tc, cb->ydot);
+  const int r = cb->fn(0, (int)nsteps, tc, tc - cb->tprev, cb->y, cb->ydot, cb->data);
+  cb->tprev = tc;
+  return r;
+}
+
 /* CvodeSolver::Integrate (solver_cvode.cpp:51-299): one-step loop, retries,
  * Dky interpolation at min(tcurr, end_time). */
 /* ------------------------------------------------------------------------- */
@@ -1740,7 +1776,7 @@ static char* jacobian_pattern(const zrko_mech* m, int N) {
 
 static int integrate_one(const zrko_mech* m, const zrko_opts* o, double* P, double dt,
                          double* y, zrko_stats* st, double dpdt, double esrc,
-                         const double* ysrc, double T0, double* ign_time) {
+                         const double* ysrc, double T0, double* ign_time, stepcb* cb) {
   const int nsp = m->nsp;
   const int Nf = nsp + 1;                            /* storage of every vector */
   const int N = o->solve_temperature ? nsp + 1 : nsp; /* CVODE's state size */
@@ -1821,12 +1857,15 @@ static int integrate_one(const zrko_mech* m, const zrko_opts* o, double* P, doub
         if (nroots < ROOT_MAX_RETURNS && root_check3(&cv, &rs, &troot)) {
           ++nroots;
           irfnd = 1;
-          tcurr = rs.tret = troot_last = troot;
+          tcurr = rs.tret = troot;
+          if (cb && step_callback(&cv, cb, nsteps, tcurr) != 0) break;
+          troot_last = troot;
           if (o->stop_after_ignition) break;
           continue;
         }
         irfnd = 0;
         tcurr = rs.tret = cv.tn;
+        if (cb && step_callback(&cv, cb, nsteps, tcurr) != 0) break;
         continue;
       }
     }
@@ -1839,6 +1878,7 @@ static int integrate_one(const zrko_mech* m, const zrko_opts* o, double* P, doub
       cv.rtol *= o->retry_rtol_adj;
       nst_total += cv.nst; nfe_total += cv.nfe; nje_total += cv.nje; nset_total += cv.nsetups;
       nni_total += cv.nni; ncfn_total += cv.ncfn; netf_total += cv.netf;
+      if (cb) cb->tprev = tcurr;
       if (num_tries == o->num_retries) break;
       double* y0 = cv.tempv; /* yout = zn[0] at tn */
       memcpy(y0, cv.zn[0], sizeof(double) * (size_t)N);
@@ -1850,13 +1890,22 @@ static int integrate_one(const zrko_mech* m, const zrko_opts* o, double* P, doub
         root_init(&cv, &rs);
         irfnd = 0;
       }
-    } else if (ign) {
-      rs.tret = cv.tn;
-      double troot;
-      if (nroots < ROOT_MAX_RETURNS && root_check3(&cv, &rs, &troot)) { /* before tn */
-        ++nroots;
-        irfnd = 1;
-        tcurr = rs.tret = troot_last = troot;
+    } else {
+      int rootret = 0;
+      if (ign) {
+        rs.tret = cv.tn;
+        double troot;
+        if (nroots < ROOT_MAX_RETURNS && root_check3(&cv, &rs, &troot)) { /* before tn */
+          ++nroots;
+          irfnd = 1;
+          tcurr = rs.tret = troot;
+          rootret = 1;
+        }
+      }
+      /* the callback comes before the root time is kept (solver_cvode.cpp:247-264) */
+      if (cb && step_callback(&cv, cb, nsteps, tcurr) != 0) break;
+      if (rootret) {
+        troot_last = tcurr;
         if (o->stop_after_ignition) break;
       }
     }
@@ -1927,11 +1976,31 @@ int zrko_integrate_batch_aux(const zrko_mech* m, const zrko_opts* o, int n, doub
     int flag = integrate_one(m, o, &P[k], dt, y, stats ? stats + k : NULL,
                              dpdt ? dpdt[k] : 0.0, esrc ? esrc[k] : 0.0,
                              ysrc ? ysrc + (size_t)k * stride : NULL, T[k],
-                             ign_time ? ign_time + k : NULL);
+                             ign_time ? ign_time + k : NULL, NULL);
     for (int j = 0; j < nsp; ++j) mf[(size_t)k * stride + j] = y[j];
     T[k] = y[nsp] * o->Tref;
     if (flag < 0) nfail++;
     free(y);
   }
   return nfail;
+}
+
+int zrko_integrate_one_cb(const zrko_mech* m, const zrko_opts* o, double dt, double* T,
+                          double* P, double* mf, zrko_stats* stats, zrko_step_cb fn,
+                          void* data) {
+  const int nsp = m->nsp;
+  double* y = (double*)malloc(sizeof(double) * (size_t)(3 * (nsp + 1)));
+  for (int j = 0; j < nsp; ++j) y[j] = mf[j];
+  y[nsp] = *T / o->Tref;
+  stepcb cb;
+  cb.fn = fn;
+  cb.data = data;
+  cb.tprev = 0.0;
+  cb.y = y + (nsp + 1);
+  cb.ydot = y + 2 * (nsp + 1);
+  const int flag = integrate_one(m, o, P, dt, y, stats, 0.0, 0.0, NULL, *T, NULL, fn ? &cb : NULL);
+  for (int j = 0; j < nsp; ++j) mf[j] = y[j];
+  *T = y[nsp] * o->Tref;
+  free(y);
+  return flag;
 }

@@ -158,6 +158,18 @@ int zrko_integrate_batch_aux(const zrko_mech* m, const zrko_opts* o, int n, doub
                              double* T, double* P, double* mf, int stride,
                              zrko_stats* stats, int nthreads, const double* dpdt,
                              const double* esrc, const double* ysrc, double* ign_time);
+/* The plugin's step callback (zerork_cfd_plugin.h:44): reactor id, CVode
+ * calls so far, t, t - t_prev, state, its time derivative; non-zero ends the
+ * integration. */
+typedef int (*zrko_step_cb)(int id, int nsteps, double t, double dt, const double* y,
+                            const double* ydot, void* data);
+/* One reactor (T, P, mf[nsp] updated in place) with fn called after every
+ * successful CVode call as CvodeSolver::Integrate calls it
+ * (solver_cvode.cpp:228-272); a non-zero return ends the integration there
+ * with the state CVodeGetDky(min(t, dt), 0). */
+int zrko_integrate_one_cb(const zrko_mech* m, const zrko_opts* o, double dt, double* T,
+                          double* P, double* mf, zrko_stats* stats, zrko_step_cb fn,
+                          void* data);
 /* RHS at time t of the reactor initialised at (P0, y0) with sources. */
 int zrko_rhs_src(const zrko_mech* m, const zrko_opts* o, double P0, const double* y0,
                  double t, const double* y, double dpdt, double esrc, const double* ysrc,

@@ -120,6 +120,31 @@ __device__ __forceinline__ void bdf_driver(Cvode<MT, Team, kIter>& cv, const Dev
     int nsteps = 0, num_tries = 0, flag = kCvSuccess;
     int tot[7] = {0, 0, 0, 0, 0, 0, 0};
     double tcurr = 0.0;
+    // the step record of a one-reactor batch after each successful CVode
+    // call: the state CVode returned (zn[0] at tn, else the interpolant) and
+    // CVodeGetDky(t, 1); t_prev follows every call, failed ones included
+    // (solver_cvode.cpp:247-265)
+    const bool tr = o.trace != nullptr && r == 0;
+    int ntr = 0;
+    double tprev = 0.0;
+    auto record = [&](double tc) {
+      if (ntr < o.trace_cap) {
+        double* rec = o.trace + (size_t)ntr * (2 * nsp + 5);
+        const double yv = (tc == cv.tn) ? cv.zn[0] : cv.dky0(tc);
+        const double yd = cv.dky1(tc);
+        if (lane == 0) {
+          rec[0] = (double)nsteps;
+          rec[1] = tc;
+          rec[2] = tc - tprev;
+        }
+        if (lane < cv.nvar) {
+          rec[3 + lane] = yv;
+          rec[4 + nsp + lane] = yd;
+        }
+      }
+      ++ntr;
+      tprev = tc;
+    };
     // one CVode call per iteration, CV_ONE_STEP (solver_cvode.cpp:228-266)
     while (tcurr < dt) {
       if (ign && cv.nst > 0) {
@@ -134,11 +159,13 @@ __device__ __forceinline__ void bdf_driver(Cvode<MT, Team, kIter>& cv, const Dev
             ++nroots;
             irfnd = true;
             tcurr = cv.rt_tret = troot_last = troot;
+            if (tr) record(tcurr);
             if (o.stop_after_ignition) break;
             continue;
           }
           irfnd = false;
           tcurr = cv.rt_tret = cv.tn;
+          if (tr) record(tcurr);
           continue;
         }
       }
@@ -151,23 +178,30 @@ __device__ __forceinline__ void bdf_driver(Cvode<MT, Team, kIter>& cv, const Dev
         cv.rtol *= o.retry_rtol_adj;
         tot[0] += cv.nst; tot[1] += cv.nfe; tot[2] += cv.nje; tot[3] += cv.nsetups;
         tot[4] += cv.nni; tot[5] += cv.ncfn; tot[6] += cv.netf;
+        tprev = tcurr;
         if (num_tries == o.num_retries) break;
         cv.reinit(tcurr, cv.zn[0]);
         if (ign) {
           cv.root_init();
           irfnd = false;
         }
-      } else if (ign) {
-        cv.rt_tret = cv.tn;
-        double troot;
-        if (nroots < kMaxRootReturns && cv.root_check3(troot)) {  // returned before tn
-          ++nroots;
-          irfnd = true;
-          tcurr = cv.rt_tret = troot_last = troot;
-          if (o.stop_after_ignition) break;
+      } else {
+        bool stop = false;
+        if (ign) {
+          cv.rt_tret = cv.tn;
+          double troot;
+          if (nroots < kMaxRootReturns && cv.root_check3(troot)) {  // returned before tn
+            ++nroots;
+            irfnd = true;
+            tcurr = cv.rt_tret = troot_last = troot;
+            stop = o.stop_after_ignition != 0;
+          }
         }
+        if (tr) record(tcurr);
+        if (stop) break;
       }
     }
+    if (tr && lane == 0) *o.trace_n = ntr;
     // failed reactors keep their input state (zerork_reactor_manager.cpp:1123-1139)
     if (flag == kCvSuccess) {
       if (ign && o.ign_time && lane == 0) o.ign_time[r] = troot_last;

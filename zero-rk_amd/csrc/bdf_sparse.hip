@@ -1097,6 +1097,12 @@ bdf_sparse_kernel(DevMech m, DevSparse S, BdfOpts o, int n, double dt_all,
     int nsteps = 0, num_tries = 0, flag = kCvSuccess;
     int tot[7] = {0, 0, 0, 0, 0, 0, 0};
     double tcurr = 0.0;
+    // the step record of a one-reactor batch after each successful CVode
+    // call (solver_cvode.cpp:247-265): zn[0] at tn and CVodeGetDky(tn, 1) =
+    // zn[1] / h; t_prev follows every call, failed ones included
+    const bool tr = o.trace != nullptr && r == 0;
+    int ntr = 0;
+    double tprev = 0.0;
     // one CVode call per iteration, CV_ONE_STEP (solver_cvode.cpp:228-266)
     while (tcurr < dt) {
       flag = c.one_step(dt, tcurr);
@@ -1108,11 +1114,29 @@ bdf_sparse_kernel(DevMech m, DevSparse S, BdfOpts o, int n, double dt_all,
         c.rtol *= o.retry_rtol_adj;
         tot[0] += c.nst; tot[1] += c.nfe; tot[2] += c.nje; tot[3] += c.nsetups;
         tot[4] += c.nni; tot[5] += c.ncfn; tot[6] += c.netf;
+        tprev = tcurr;
         if (num_tries == o.num_retries) break;
         c.team.sync();
         c.reinit(tcurr, c.zn[0]);
+      } else if (tr) {
+        if (ntr < o.trace_cap) {
+          double* rec = o.trace + (size_t)ntr * (2 * nsp + 5);
+          if (tid == 0) {
+            rec[0] = (double)nsteps;
+            rec[1] = tcurr;
+            rec[2] = tcurr - tprev;
+          }
+          const double hinv = 1.0 / c.h;
+          for (int i = tid; i < N; i += kSpThreads) {
+            rec[3 + i] = c.zn[0][i];
+            rec[4 + nsp + i] = c.zn[1][i] * hinv;
+          }
+        }
+        ++ntr;
+        tprev = tcurr;
       }
     }
+    if (tr && tid == 0) *o.trace_n = ntr;
     if (flag == kCvSuccess) {
       double* yout = c.tempv;
       c.dky0(tcurr < dt ? tcurr : dt, yout);

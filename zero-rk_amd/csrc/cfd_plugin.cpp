@@ -24,8 +24,14 @@
 // CPU re-solve of a failed GPU batch, :1021-1034, 1090-1147); reactors that
 // still fail keep their input state and report a negative cost, and the call
 // returns ZERORK_STATUS_FAILED_SOLVE (:1123-1139).  Each solve appends a line
-// to the reactor_timing_log when output_performance_log is set.  No C++
-// exception or exit() crosses the ABI.
+// to the reactor_timing_log when output_performance_log is set.  A step
+// callback is called, as the manager sets it on the CPU solver, for a
+// one-reactor solve with load_balance = 0 (:1082-1084): the integrator
+// records every successful CVode call of that reactor on the device, and the
+// callback is called for each record in order after the solve; a non-zero
+// return ends the integration at that record, whose state is then the
+// result (solver_cvode.cpp:247-257).  No C++ exception or exit() crosses the
+// ABI.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -70,6 +76,9 @@ struct zerork_handle_impl {
   int* dorder = nullptr;
   double* dsrc_n = nullptr;  // dpdt, e_src, ignition times (3n)
   double* dysrc = nullptr;   // y_src (n*stride)
+  double* dtrace = nullptr;  // step records of a one-reactor solve with a callback
+  int* dtrace_n = nullptr;
+  size_t cap_trace = 0;      // records
   size_t cap_n = 0, cap_y = 0, cap_src_n = 0, cap_src_y = 0;
   int n_calls = 0;           // solves so far (names the reactor dump files)
   // the manager's own cost and temperature-change arrays when the caller
@@ -134,11 +143,16 @@ struct zerork_handle_impl {
     if (dsrc_n) (void)hipFree(dsrc_n);
     if (dysrc) (void)hipFree(dysrc);
     if (dtdelta) (void)hipFree(dtdelta);
+    if (dtrace) (void)hipFree(dtrace);
+    if (dtrace_n) (void)hipFree(dtrace_n);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
 
 namespace {
+
+// the universal gas constant, J/kmol/K (src/zerork/constants.h:10)
+constexpr double kRuJ = 8.314462618e3;
 
 zerork_status_t load_mechanism(zerork_handle h) {
   auto m = std::make_unique<zrk::MechHandle>();
@@ -404,6 +418,56 @@ zrk_bdf_options solve_options(zerork_handle h, int nsp) {
   return o;
 }
 
+// The step callback over the records of a one-reactor solve
+// (solver_cvode.cpp:247-257): cb(reactor id, CVode calls, t, t - t_prev, y,
+// ydot) for each successful CVode call in order; a non-zero return ends the
+// integration there, as a success: the reactor's state is that record's and
+// its cost that record's call count, the pressure the end pressure of that
+// state (GetState at the solve's dt, reactor_constant_volume_cpu.cpp:44-63,
+// reactor_constant_pressure_cpu.cpp:44-66); a stop on the call that passed dt
+// leaves the solve's end state, CVodeGetDky at dt (solver_cvode.cpp:268-270).
+bool replay_callback(zerork_handle h, const zrk_bdf_options& o, int nsp, int rec_len, double dt,
+                     double T_in, double P_in, const double* Y_in, bool solve_T, bool srcs,
+                     double* T, double* P, double* mf, int* steps) {
+  int nrec = 0;
+  if (hipMemcpy(&nrec, h->dtrace_n, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+    return false;
+  std::vector<double> rec((size_t)nrec * rec_len);
+  if (nrec > 0 && hipMemcpy(rec.data(), h->dtrace, sizeof(double) * rec.size(),
+                            hipMemcpyDeviceToHost) != hipSuccess)
+    return false;
+  const int id = h->ids ? h->ids[0] : 0;
+  for (int k = 0; k < nrec; ++k) {
+    const double* r = rec.data() + (size_t)k * rec_len;
+    if (h->cb(id, (int)r[0], r[1], r[2], r + 3, r + 4 + nsp, h->cb_data) == 0) continue;
+    // stopped on the call that passed dt: the state is the interpolant at
+    // dt, the solve's own end state
+    if (r[1] >= dt) break;
+    const double* y = r + 3;
+    std::copy(y, y + nsp, mf);
+    const double Tref = o.reference_temperature;
+    *T = solve_T ? y[nsp] * Tref : (T_in / Tref) * Tref;
+    *steps = (int)r[0];
+    const double dpdt = h->dpdt ? h->dpdt[0] : 0.0;
+    if (o.constant_volume || (srcs && h->y_src)) {
+      // the volume of the input state, P = Ru T / (v Wmix) (+ dpdt dt at
+      // constant volume)
+      const std::vector<double>& mw = h->mech->host.mw;
+      double mws0 = 0.0, mws = 0.0;
+      for (int j = 0; j < nsp; ++j) {
+        mws0 += Y_in[j] * (1.0 / mw[j]);
+        mws += y[j] * (1.0 / mw[j]);
+      }
+      const double v = 1.0 / (P_in * (1.0 / mws0) / (kRuJ * T_in));
+      *P = kRuJ * *T / (v * (1.0 / mws)) + (o.constant_volume ? dpdt * dt : 0.0);
+    } else if (srcs) {
+      *P = P_in + dpdt * dt;
+    }
+    break;
+  }
+  return true;
+}
+
 zerork_status_t solve(zerork_handle h, int n_cycle, double dt, int n, double* T, double* P,
                       double* mf) {
   if (n < 0 || !T || !P || (n > 0 && !mf)) return ZERORK_STATUS_INVALID_POINTER;
@@ -519,39 +583,88 @@ zerork_status_t solve(zerork_handle h, int n_cycle, double dt, int n, double* T,
   }
   if (!ok) return ZERORK_STATUS_UNKNOWN_ERROR;
   zrk_bdf_options o = solve_options(h, nsp);
-  const auto t0 = std::chrono::steady_clock::now();
-  int rc_dev = zrk::integrate_device(m, o, n, dt, nullptr, h->dT, h->dP, h->dY, stride,
-                                     h->dsteps, nullptr, sorted ? h->dorder : nullptr, st, ddpdt,
-                                     desrc, dysrc, dign, always_T ? nullptr : h->dtdelta,
-                                     always_T);
-  if (rc_dev != 0) return ZERORK_STATUS_UNKNOWN_ERROR;
+  // the step callback: one reactor, load_balance 0 (zerork_reactor_manager.cpp:1082-1084)
+  const bool use_cb = h->cb != nullptr && h->iopt["load_balance"] == 0 && n == 1;
+  const int rec_len = 2 * nsp + 5;
+  zrk::StepTrace trace{nullptr, 0, nullptr};
+  std::vector<double> mf0;
+  if (use_cb) {
+    // records for max_steps calls per try (plus root returns) of up to
+    // cvode_num_retries tries, within 256 MiB; a solve that makes more is
+    // run again from its inputs with room for all of them
+    const size_t tries = std::max(1, o.cvode_num_retries);
+    const size_t want = std::min<size_t>(((size_t)std::max(o.max_steps, 1) + 136) * tries,
+                                         std::max<size_t>(1, (256u << 20) / (8u * rec_len)));
+    if (!h->dtrace_n && !grow(&h->dtrace_n, 1)) return ZERORK_STATUS_UNKNOWN_ERROR;
+    if (want > h->cap_trace) {
+      if (!grow(&h->dtrace, want * rec_len)) return ZERORK_STATUS_UNKNOWN_ERROR;
+      h->cap_trace = want;
+    }
+    mf0.assign(mf, mf + ny);
+  }
   std::vector<int> steps(n);
-  ok = hipMemcpyAsync(steps.data(), h->dsteps, sizeof(int) * n, hipMemcpyDeviceToHost, st) ==
-       hipSuccess;
-  if (hipStreamSynchronize(st) != hipSuccess || !ok) return ZERORK_STATUS_UNKNOWN_ERROR;
-  // a reactor whose batched integration failed is solved again on its own,
-  // as the manager re-solves a failed GPU batch on the CPU
-  // (zerork_reactor_manager.cpp:1021-1034, 1090-1147): here with the
-  // reference's own arithmetic, the FP64 Newton matrix, when the batch used
-  // the FP32 one (failed reactors kept their input state)
-  std::vector<int> failed;
-  for (int k = 0; k < n; ++k)
-    if (steps[k] < 0) failed.push_back(k);
   int n_resolved = 0;
-  if (!failed.empty() && o.fp32_preconditioner && nsp <= 191) {
-    zrk_bdf_options o64 = o;
-    o64.fp32_preconditioner = 0;
-    ok = hipMemcpyAsync(h->dorder, failed.data(), sizeof(int) * failed.size(),
-                        hipMemcpyHostToDevice, st) == hipSuccess;
-    if (!ok) return ZERORK_STATUS_UNKNOWN_ERROR;
-    rc_dev = zrk::integrate_device(m, o64, (int)failed.size(), dt, nullptr, h->dT, h->dP, h->dY,
-                                   stride, h->dsteps, nullptr, h->dorder, st, ddpdt, desrc,
-                                   dysrc, dign, always_T ? nullptr : h->dtdelta, always_T);
+  int rc_dev = 0;
+  std::chrono::steady_clock::time_point t0;
+  for (;;) {
+    trace = zrk::StepTrace{h->dtrace, (int)std::min<size_t>(h->cap_trace, INT32_MAX), h->dtrace_n};
+    t0 = std::chrono::steady_clock::now();
+    rc_dev = zrk::integrate_device(m, o, n, dt, nullptr, h->dT, h->dP, h->dY, stride, h->dsteps,
+                                   nullptr, sorted ? h->dorder : nullptr, st, ddpdt, desrc, dysrc,
+                                   dign, always_T ? nullptr : h->dtdelta, always_T, nullptr,
+                                   use_cb ? &trace : nullptr);
     if (rc_dev != 0) return ZERORK_STATUS_UNKNOWN_ERROR;
     ok = hipMemcpyAsync(steps.data(), h->dsteps, sizeof(int) * n, hipMemcpyDeviceToHost, st) ==
          hipSuccess;
     if (hipStreamSynchronize(st) != hipSuccess || !ok) return ZERORK_STATUS_UNKNOWN_ERROR;
-    n_resolved = (int)failed.size();
+    // a reactor whose batched integration failed is solved again on its own,
+    // as the manager re-solves a failed GPU batch on the CPU
+    // (zerork_reactor_manager.cpp:1021-1034, 1090-1147): here with the
+    // reference's own arithmetic, the FP64 Newton matrix, when the batch used
+    // the FP32 one (failed reactors kept their input state); the step records
+    // are then the re-solve's
+    std::vector<int> failed;
+    for (int k = 0; k < n; ++k)
+      if (steps[k] < 0) failed.push_back(k);
+    n_resolved = 0;
+    if (!failed.empty() && o.fp32_preconditioner && nsp <= 191) {
+      zrk_bdf_options o64 = o;
+      o64.fp32_preconditioner = 0;
+      ok = hipMemcpyAsync(h->dorder, failed.data(), sizeof(int) * failed.size(),
+                          hipMemcpyHostToDevice, st) == hipSuccess;
+      if (!ok) return ZERORK_STATUS_UNKNOWN_ERROR;
+      rc_dev = zrk::integrate_device(m, o64, (int)failed.size(), dt, nullptr, h->dT, h->dP,
+                                     h->dY, stride, h->dsteps, nullptr, h->dorder, st, ddpdt,
+                                     desrc, dysrc, dign, always_T ? nullptr : h->dtdelta,
+                                     always_T, nullptr, use_cb ? &trace : nullptr);
+      if (rc_dev != 0) return ZERORK_STATUS_UNKNOWN_ERROR;
+      ok = hipMemcpyAsync(steps.data(), h->dsteps, sizeof(int) * n, hipMemcpyDeviceToHost,
+                          st) == hipSuccess;
+      if (hipStreamSynchronize(st) != hipSuccess || !ok) return ZERORK_STATUS_UNKNOWN_ERROR;
+      n_resolved = (int)failed.size();
+    }
+    if (!use_cb) break;
+    int nrec = 0;
+    ok = hipMemcpy(&nrec, h->dtrace_n, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess;
+    if (!ok) return ZERORK_STATUS_UNKNOWN_ERROR;
+    if ((size_t)nrec <= h->cap_trace) break;
+    // more records than room: the same solve again (it is deterministic)
+    // from the inputs, with room for all of them
+    if (!grow(&h->dtrace, (size_t)nrec * rec_len)) return ZERORK_STATUS_UNKNOWN_ERROR;
+    h->cap_trace = (size_t)nrec;
+    ok = hipMemcpyAsync(h->dT, T0.data(), sizeof(double) * n, hipMemcpyHostToDevice, st) ==
+         hipSuccess;
+    ok = ok && hipMemcpyAsync(h->dP, P0.data(), sizeof(double) * n, hipMemcpyHostToDevice,
+                              st) == hipSuccess;
+    ok = ok && hipMemcpyAsync(h->dY, mf0.data(), sizeof(double) * ny, hipMemcpyHostToDevice,
+                              st) == hipSuccess;
+    if (sorted)
+      ok = ok && hipMemcpyAsync(h->dorder, order.data(), sizeof(int) * n, hipMemcpyHostToDevice,
+                                st) == hipSuccess;
+    if (ign)
+      ok = ok && hipMemcpyAsync(dign, h->ign, sizeof(double) * n, hipMemcpyHostToDevice, st) ==
+                     hipSuccess;
+    if (!ok) return ZERORK_STATUS_UNKNOWN_ERROR;
   }
   ok = hipMemcpyAsync(T, h->dT, sizeof(double) * n, hipMemcpyDeviceToHost, st) == hipSuccess;
   ok = ok && hipMemcpyAsync(mf, h->dY, sizeof(double) * ny, hipMemcpyDeviceToHost, st) == hipSuccess;
@@ -561,6 +674,11 @@ zerork_status_t solve(zerork_handle h, int n_cycle, double dt, int n, double* T,
     ok = ok && hipMemcpyAsync(h->ign, dign, sizeof(double) * n, hipMemcpyDeviceToHost, st) ==
                    hipSuccess;
   if (hipStreamSynchronize(st) != hipSuccess || !ok) return ZERORK_STATUS_UNKNOWN_ERROR;
+  if (use_cb) {
+    if (!replay_callback(h, o, nsp, rec_len, dt, T0[0], P0[0], mf0.data(), always_T || tdelta[0] > 0.0,
+                         srcs, T, P, mf, &steps[0]))
+      return ZERORK_STATUS_UNKNOWN_ERROR;
+  }
   const double secs =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   zerork_status_t flag = ZERORK_STATUS_SUCCESS;

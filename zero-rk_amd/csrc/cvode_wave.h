@@ -1248,6 +1248,22 @@ struct Cvode {
     rt_glo = g;
   }
 
+  // CVodeGetDky(t, 1): sum over j = q..1 of j s^(j-1) zn[j], times 1/h
+  __device__ __forceinline__ double dky1(double t) const {
+    const double sv = (t - tn) / h;
+    double out = 0.0;
+#pragma unroll
+    for (int j = kQmax; j >= 1; --j) {
+      if (j <= q) {
+        double c = (double)j;
+#pragma unroll
+        for (int i = 0; i < j - 1; ++i) c *= sv;
+        out += c * zn[j];
+      }
+    }
+    return out * (1.0 / h);
+  }
+
   // CVodeGetDky(t, 0)
   __device__ __forceinline__ double dky0(double t) const {
     const double sv = (t - tn) / h;

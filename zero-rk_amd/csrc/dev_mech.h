@@ -46,6 +46,14 @@ struct BdfOpts {
   // above 0 keeps its temperature (not a state variable)
   int always_T = 1;
   const double* tdelta = nullptr;
+  // per-step record of a one-reactor batch (the plugin's step callback,
+  // solver_cvode.cpp:247-257): after each successful CVode call record k =
+  // {CVode calls so far, t, t - t_prev, y[nsp+1], ydot[nsp+1]} (y and ydot
+  // CVodeGetDky(t, 0 and 1), nvar entries used) at trace + k * (2 nsp + 5);
+  // records past trace_cap are counted, not written; *trace_n = records
+  double* trace = nullptr;
+  int trace_cap = 0;
+  int* trace_n = nullptr;
 };
 
 enum JacKind : int {

@@ -549,7 +549,8 @@ namespace {
 int integrate_sparse(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
                      const double* dts, double* T, double* P, double* Y, int stride,
                      int* nsteps, int* stats, const int* order, hipStream_t st,
-                     const double* dpdt, const double* esrc, const double* ysrc) {
+                     const double* dpdt, const double* esrc, const double* ysrc,
+                     const StepTrace* trace) {
   if (!o.fp32_preconditioner) return -3;
   // source terms, the ignition root and PLOG / non-integer / SRI records are
   // not on this path
@@ -566,6 +567,11 @@ int integrate_sparse(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
   const size_t per = sparse_slot_bytes(h->dev, h->sp_dev) / sizeof(double);
   if (int rc = ensure_workspace(h, (size_t)nb * per)) return rc;
   BdfOpts bo = to_bdf_opts(o);
+  if (trace) {
+    bo.trace = trace->buf;
+    bo.trace_cap = trace->cap;
+    bo.trace_n = trace->count;
+  }
   // bounded launches: each one integrates at most four rounds of the resident
   // workgroups, so a large batch of a large mechanism is a few launches of
   // seconds rather than one of minutes (reactors are independent; with an
@@ -600,8 +606,9 @@ int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
                      const double* dts, double* T, double* P, double* Y, int stride, int* nsteps, int* stats,
                      const int* order, hipStream_t st, const double* dpdt, const double* esrc,
                      const double* ysrc, double* ign_time, const double* tdelta, int always_T,
-                     const ExtWorkspace* ws) {
+                     const ExtWorkspace* ws, const StepTrace* trace) {
   if (n < 0 || !T || !P || !Y || stride < h->host.nsp || (!dts && !(dt > 0.0))) return -1;
+  if (trace && (n != 1 || !trace->buf || !trace->count || trace->cap < 0)) return -1;
   BdfOpts bo = to_bdf_opts(o);
   const int W0 = bdf_team_width(h->host.nsp);
   if (ws && !h->dev_blob) return -6;  // no upload here: nothing may be allocated
@@ -614,7 +621,7 @@ int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
     if (n > 0)
       if (int rc = ensure_device(h)) return rc;
     return integrate_sparse(h, o, n, dt, dts, T, P, Y, stride, nsteps, stats, order, st, dpdt,
-                            esrc, ysrc);
+                            esrc, ysrc, trace);
   }
   if (n == 0) return 0;
   if (int rc = ensure_device(h)) return rc;
@@ -642,6 +649,11 @@ int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
   bo.ign_time = bo.ign_dT > 0.0 ? ign_time : nullptr;
   bo.always_T = always_T ? 1 : 0;
   bo.tdelta = always_T ? nullptr : tdelta;
+  if (trace) {
+    bo.trace = trace->buf;
+    bo.trace_cap = trace->cap;
+    bo.trace_n = trace->count;
+  }
   hipError_t e = launch_bdf(h->dev, bo, n, dt, dts, T, P, Y, stride, nsteps, stats, order,
                             counter, jac, nw, W, st);
   return e == hipSuccess ? 0 : (int)e;

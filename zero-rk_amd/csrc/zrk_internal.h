@@ -56,17 +56,27 @@ struct ExtWorkspace {
   size_t doubles;
   unsigned int* counter;
 };
+// Step records of a one-reactor batch (the plugin's step callback): a device
+// buffer of cap records of 2 nsp + 5 doubles and a device int that receives
+// the number of successful CVode calls (see BdfOpts::trace).
+struct StepTrace {
+  double* buf;
+  int cap;
+  int* count;
+};
 // dpdt[n], esrc[n], ysrc[n*stride]: optional device arrays of source terms;
 // ign_time[n]: optional ignition-time output (with o.delta_temperature_ignition > 0)
 // tdelta[n] (optional, with always_T = 0): the manager's temperature change
 // of each reactor's previous solve; a reactor whose entry is not above 0 is
 // integrated at fixed temperature (zerork_reactor_manager.cpp:1109-1113)
 // ws (optional): caller-supplied scratch, see ExtWorkspace
+// trace (optional, n = 1 only): step records, see StepTrace
 int integrate_device(MechHandle* h, const zrk_bdf_options& o, int n, double dt,
                      const double* dts, double* T, double* P, double* Y, int stride, int* nsteps, int* stats,
                      const int* order, hipStream_t st, const double* dpdt = nullptr,
                      const double* esrc = nullptr, const double* ysrc = nullptr,
                      double* ign_time = nullptr, const double* tdelta = nullptr,
-                     int always_T = 1, const ExtWorkspace* ws = nullptr);
+                     int always_T = 1, const ExtWorkspace* ws = nullptr,
+                     const StepTrace* trace = nullptr);
 
 }  // namespace zrk

@@ -327,6 +327,11 @@ class Mech:
         return lib().zrk_integrator_resident_waves(self.h, C.byref(opts) if opts else None)
 
 
+# the step callback type (zerork_cfd_plugin.h:44)
+STEP_CB = C.CFUNCTYPE(C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                      C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p)
+
+
 class Plugin:
     """Thin Python view of the plugin C API (mirrors zerork_cfd_plugin_tester)."""
 
@@ -375,6 +380,18 @@ class Plugin:
         p = C.c_char_p(C.addressof(buf))
         rc = self.L.zerork_reactor_get_string_option(k.encode(), C.byref(p), n, self.h)
         return rc, buf.value.decode()
+
+    def set_callback(self, fn, data=None):
+        """zerork_reactor_set_callback_fn with a Python callable
+        fn(reactor_id, nsteps, t, dt, y, ydot, data) -> int (y and ydot are
+        double pointers), kept alive with the handle."""
+        cb = STEP_CB(fn)
+        self._keep.append(cb)
+        return self.L.zerork_reactor_set_callback_fn(C.cast(cb, C.c_void_p), data, self.h)
+
+    def set_reactor_ids(self, ids):
+        self._keep.append(ids)
+        return self.L.zerork_reactor_set_reactor_ids(ids.ctypes.data, self.h)
 
     def set_aux(self, field, arr):
         self._keep.append(arr)
