@@ -383,12 +383,42 @@ struct SpCv {
     const int sp = S.split, nd = N - sp;
     team.sync();
     for (int i = tid(); i < N; i += kSpThreads) xs[i] = b[S.perm[i]];
+    // the level-set sweeps are chains of levels; the factor entries a
+    // wavefront needs at the next level do not depend on this one, so the
+    // first 64 of its first row there are loaded before this level's
+    // barrier (the sums keep their order: the same bits)
+    int nx_i = -1, nx_c = 0;
+    float nx_v = 0.0f;
+    auto next_lower = [&](int lev) {
+      nx_i = -1;
+      if (lev >= S.nlev_l) return;
+      const int idx = S.ll_ptr[lev] + wave();
+      if (idx >= S.ll_ptr[lev + 1]) return;
+      nx_i = S.ll_rows[idx];
+      const int p = S.rp[nx_i] + lane();
+      nx_v = 0.0f;
+      nx_c = 0;
+      if (p < S.dpos[nx_i]) {
+        nx_v = lu[p];
+        nx_c = S.ci[p];
+      }
+    };
+    next_lower(0);
     team.sync();
     for (int lev = 0; lev < S.nlev_l; ++lev) {
+      const int i0 = nx_i, c0 = nx_c;
+      const float v0 = nx_v;
+      next_lower(lev + 1);
       for (int idx = S.ll_ptr[lev] + wave(); idx < S.ll_ptr[lev + 1]; idx += kSpWaves) {
-        const int i = S.ll_rows[idx];
+        const bool first = idx == S.ll_ptr[lev] + wave();
+        const int i = first ? i0 : S.ll_rows[idx];
         double s = 0.0;
-        for (int p = S.rp[i] + lane(); p < S.dpos[i]; p += kWave) s += (double)lu[p] * xs[S.ci[p]];
+        int p = S.rp[i] + lane();
+        if (first) {
+          if (p < S.dpos[i]) s += (double)v0 * xs[c0];
+          p += kWave;
+        }
+        for (; p < S.dpos[i]; p += kWave) s += (double)lu[p] * xs[S.ci[p]];
         s = wave_sum(s);
         if (lane() == 0) xs[i] -= s;
       }
@@ -442,14 +472,43 @@ struct SpCv {
       }
       team.sync();
     }
+    // backward over the leading rows, with the next level's first row of
+    // each wavefront loaded ahead as in the forward sweep
+    float nx_d = 1.0f;
+    auto next_upper = [&](int lev) {
+      nx_i = -1;
+      if (lev >= S.nlev_lb) return;
+      const int idx = S.lbl_ptr[lev] + wave();
+      if (idx >= S.lbl_ptr[lev + 1]) return;
+      nx_i = S.lbl_rows[idx];
+      const int dp = S.dpos[nx_i];
+      const int p = dp + 1 + lane();
+      nx_v = 0.0f;
+      nx_c = 0;
+      nx_d = lu[dp];
+      if (p < S.rp[nx_i + 1]) {
+        nx_v = lu[p];
+        nx_c = S.ci[p];
+      }
+    };
+    next_upper(0);
     for (int lev = 0; lev < S.nlev_lb; ++lev) {
+      const int i0 = nx_i, c0 = nx_c;
+      const float v0 = nx_v, d0 = nx_d;
+      next_upper(lev + 1);
       for (int idx = S.lbl_ptr[lev] + wave(); idx < S.lbl_ptr[lev + 1]; idx += kSpWaves) {
-        const int i = S.lbl_rows[idx];
+        const bool first = idx == S.lbl_ptr[lev] + wave();
+        const int i = first ? i0 : S.lbl_rows[idx];
         double s = 0.0;
-        for (int p = S.dpos[i] + 1 + lane(); p < S.rp[i + 1]; p += kWave)
-          s += (double)lu[p] * xs[S.ci[p]];
+        int p = S.dpos[i] + 1 + lane();
+        if (first) {
+          if (p < S.rp[i + 1]) s += (double)v0 * xs[c0];
+          p += kWave;
+        }
+        for (; p < S.rp[i + 1]; p += kWave) s += (double)lu[p] * xs[S.ci[p]];
         s = wave_sum(s);
-        if (lane() == 0) xs[i] = (xs[i] - s) / (double)lu[S.dpos[i]];
+        const float piv = first ? d0 : lu[S.dpos[i]];
+        if (lane() == 0) xs[i] = (xs[i] - s) / (double)piv;
       }
       team.sync();
     }
