@@ -432,6 +432,8 @@ bool replay_callback(zerork_handle h, const zrk_bdf_options& o, int nsp, int rec
   int nrec = 0;
   if (hipMemcpy(&nrec, h->dtrace_n, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
     return false;
+  // the solve was run until every record had room (it is deterministic)
+  if (nrec < 0 || (size_t)nrec > h->cap_trace) return false;
   std::vector<double> rec((size_t)nrec * rec_len);
   if (nrec > 0 && hipMemcpy(rec.data(), h->dtrace, sizeof(double) * rec.size(),
                             hipMemcpyDeviceToHost) != hipSuccess)
