@@ -425,8 +425,24 @@ struct SpCv {
       team.sync();
     }
     for (int i = sp + wave(); i < N; i += kSpWaves) {
+      // the hub rows hold hundreds of entries: four chunks of 64 are loaded
+      // together before they are summed (in the same order: the same bits)
       double s = 0.0;
-      for (int p = S.rp[i] + lane(); p < S.rp[i + 1]; p += kWave)
+      const int e = S.rp[i + 1];
+      int p = S.rp[i] + lane();
+      for (; p + 3 * kWave < e; p += 4 * kWave) {
+        int c[4];
+        float v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          c[u] = S.ci[p + u * kWave];
+          v[u] = lu[p + u * kWave];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (c[u] < sp) s += (double)v[u] * xs[c[u]];
+      }
+      for (; p < e; p += kWave)
         if (S.ci[p] < sp) s += (double)lu[p] * xs[S.ci[p]];
       s = wave_sum(s);
       if (lane() == 0) xs[i] -= s;
