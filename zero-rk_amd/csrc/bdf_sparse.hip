@@ -262,7 +262,22 @@ struct SpCv {
   static constexpr int kPre = 8;
   __device__ __forceinline__ void eliminate_row(int i, int kend) {
     const int b = S.rp[i], e = S.rp[i + 1];
-    for (int p = b + lane(); p < e; p += kWave) w[S.ci[p]] = lu[p];
+    {
+      // a hub row's hundreds of entries: four chunks of 64 loaded together
+      int p = b + lane();
+      for (; p + 3 * kWave < e; p += 4 * kWave) {
+        int c[4];
+        float v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          c[u] = S.ci[p + u * kWave];
+          v[u] = lu[p + u * kWave];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) w[c[u]] = v[u];
+      }
+      for (; p < e; p += kWave) w[S.ci[p]] = lu[p];
+    }
     wave_sync();
     for (int p0 = b; p0 < e; p0 += kPre) {
       int kk[kPre], dkk[kPre], ekk[kPre], uc[kPre];
@@ -336,9 +351,21 @@ struct SpCv {
     SP_TIC(f1);
     for (int i = sp + wave(); i < N; i += kSpWaves) {
       eliminate_row(i, sp);
-      for (int p = S.rp[i] + lane(); p < S.rp[i + 1]; p += kWave) {
-        const int c = S.ci[p];
-        if (c < sp) lu[p] = w[c];
+      {
+        const int e = S.rp[i + 1];
+        int p = S.rp[i] + lane();
+        for (; p + 3 * kWave < e; p += 4 * kWave) {
+          int c[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) c[u] = S.ci[p + u * kWave];
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (c[u] < sp) lu[p + u * kWave] = w[c[u]];
+        }
+        for (; p < e; p += kWave) {
+          const int c = S.ci[p];
+          if (c < sp) lu[p] = w[c];
+        }
       }
       float* Di = D + (size_t)(i - sp) * nd;
       for (int c = sp + lane(); c < N; c += kWave) Di[c - sp] = w[c];
