@@ -370,7 +370,18 @@ struct SpCv {
       float* Di = D + (size_t)(i - sp) * nd;
       for (int c = sp + lane(); c < N; c += kWave) Di[c - sp] = w[c];
       wave_sync();
-      for (int p = S.rp[i] + lane(); p < S.rp[i + 1]; p += kWave) w[S.ci[p]] = 0.0f;
+      {
+        const int e = S.rp[i + 1];
+        int p = S.rp[i] + lane();
+        for (; p + 3 * kWave < e; p += 4 * kWave) {
+          int c[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) c[u] = S.ci[p + u * kWave];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) w[c[u]] = 0.0f;
+        }
+        for (; p < e; p += kWave) w[S.ci[p]] = 0.0f;
+      }
       wave_sync();
     }
     team.sync();
@@ -488,7 +499,20 @@ struct SpCv {
       }
       team.sync();
       const double xl = lane() < kn ? xb[kb + lane()] : 0.0;
-      for (int j = kb + kn + wave(); j < nd; j += kSpWaves) {
+      int j = kb + kn + wave();
+      for (; j + kSpWaves < nd; j += 2 * kSpWaves) {  // two rows' loads together
+        const float d0 = lane() < kn ? D[(size_t)j * nd + kb + lane()] : 0.0f;
+        const float d1 = lane() < kn ? D[(size_t)(j + kSpWaves) * nd + kb + lane()] : 0.0f;
+        double s0 = lane() < kn ? (double)d0 * xl : 0.0;
+        double s1 = lane() < kn ? (double)d1 * xl : 0.0;
+        s0 = wave_sum(s0);
+        s1 = wave_sum(s1);
+        if (lane() == 0) {
+          xb[j] -= s0;
+          xb[j + kSpWaves] -= s1;
+        }
+      }
+      for (; j < nd; j += kSpWaves) {
         double s = lane() < kn ? (double)D[(size_t)j * nd + kb + lane()] * xl : 0.0;
         s = wave_sum(s);
         if (lane() == 0) xb[j] -= s;
@@ -508,7 +532,20 @@ struct SpCv {
       }
       team.sync();
       const double xl = lane() < kn ? xb[kb + lane()] : 0.0;
-      for (int j = wave(); j < kb; j += kSpWaves) {
+      int j = wave();
+      for (; j + kSpWaves < kb; j += 2 * kSpWaves) {  // two rows' loads together
+        const float d0 = lane() < kn ? D[(size_t)j * nd + kb + lane()] : 0.0f;
+        const float d1 = lane() < kn ? D[(size_t)(j + kSpWaves) * nd + kb + lane()] : 0.0f;
+        double s0 = lane() < kn ? (double)d0 * xl : 0.0;
+        double s1 = lane() < kn ? (double)d1 * xl : 0.0;
+        s0 = wave_sum(s0);
+        s1 = wave_sum(s1);
+        if (lane() == 0) {
+          xb[j] -= s0;
+          xb[j + kSpWaves] -= s1;
+        }
+      }
+      for (; j < kb; j += kSpWaves) {
         double s = lane() < kn ? (double)D[(size_t)j * nd + kb + lane()] * xl : 0.0;
         s = wave_sum(s);
         if (lane() == 0) xb[j] -= s;
